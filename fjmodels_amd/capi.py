@@ -1,0 +1,254 @@
+"""ctypes binding of libfjmodels.so (include/fjmodels.h).
+
+Plumbing for tests and bench.py: every call goes through the C ABI into the
+gfx950 kernels.  There is no CPU fallback; a missing library or device raises.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libfjmodels.so")
+
+_dp = C.POINTER(C.c_double)
+
+FJ_QUAD_LOG22, FJ_QUAD_TANHSINH22 = 0, 1
+FJ_POISSON_QUADPACK, FJ_POISSON_BLOCKSCAN = 0, 1
+FJ_PREC_FP64, FJ_PREC_MIXED = 0, 1
+GUESS = {"Hernquist": 0, "Isochrone": 1, "NFW": 2, "NFWext": 3}
+SLOPES = {"Hernquist": (1.667, 5.0), "Isochrone": (0.0, 5.0), "NFW": (1.667, 3.0)}
+
+# Every symbol include/fjmodels.h declares.
+EXPORTS = [
+    "fj_ctx_create", "fj_ctx_destroy", "fj_last_error", "fj_set_grid", "fj_set_quad_rule",
+    "fj_set_precision", "fj_set_potential", "fj_get_potential", "fj_set_delta_table",
+    "fj_get_delta_table", "fj_set_df", "fj_guess_rhl", "fj_poisson", "fj_moments", "fj_rhofDF",
+    "fj_project_legendre", "fj_compute_new_phi", "fj_tabulate_delta", "fj_virial",
+    "fj_eval_potential", "fj_actions", "fj_get_stats",
+]
+
+
+class DFParams(C.Structure):
+    _fields_ = [("A", C.c_double), ("B", C.c_double),
+                ("dphi_h", C.c_double), ("dz_h", C.c_double),
+                ("dphi_g", C.c_double), ("dz_g", C.c_double),
+                ("mass", C.c_double), ("J0", C.c_double), ("chi", C.c_double),
+                ("k", C.c_double), ("rot_form", C.c_int)]
+
+
+class Stats(C.Structure):
+    _fields_ = ([(n, C.c_double) for n in ("ms_points", "ms_setup", "ms_integrals", "ms_overflow",
+                                           "ms_moments", "ms_project", "ms_poisson", "ms_total")] +
+                [(n, C.c_longlong) for n in ("points", "nodes", "bound_nodes", "tasks", "qk_calls",
+                                             "overflow_tasks", "brent_iters")] +
+                [("integral_lanes", C.c_int), ("setup_lanes", C.c_int)])
+
+
+def df_params(model="Hernquist", dphi_h=0.55, dz_h=0.55, dphi_g=1.0, dz_g=1.0,
+              mass=1.0, r0=1.0, chi=0.0, k=None, rot_form=0, A=None, B=None):
+    """setDF (df/DF.cpp:36-63): slopes from the model type, J0 = sqrt(M r0)."""
+    a, b = SLOPES.get(model, (A, B))
+    if A is not None:
+        a = A
+    if B is not None:
+        b = B
+    if k is None:
+        k = 0.5 if rot_form == 1 else 1.0
+    return DFParams(a, b, dphi_h, dz_h, dphi_g, dz_g, mass, float(np.sqrt(mass * r0)), chi, k, rot_form)
+
+
+def load():
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"{LIB_PATH} is not built; run __graft_entry__.build()")
+    lib = C.CDLL(LIB_PATH)
+    v, i, d = C.c_void_p, C.c_int, C.c_double
+    sig = {
+        "fj_ctx_create": [i, C.POINTER(v)],
+        "fj_ctx_destroy": [v],
+        "fj_set_grid": [v, i, _dp, i, i, i],
+        "fj_set_quad_rule": [v, i],
+        "fj_set_precision": [v, i],
+        "fj_set_potential": [v, _dp, _dp, _dp],
+        "fj_get_potential": [v, _dp, _dp, _dp],
+        "fj_set_delta_table": [v, _dp, _dp],
+        "fj_get_delta_table": [v, _dp, _dp],
+        "fj_set_df": [v, C.POINTER(DFParams)],
+        "fj_guess_rhl": [v, i, d, d, d, _dp],
+        "fj_poisson": [v, i, _dp, _dp, _dp, _dp],
+        "fj_moments": [v] + [_dp] * 6,
+        "fj_rhofDF": [v, i, _dp, _dp, _dp] + [_dp] * 6,
+        "fj_project_legendre": [v] + [_dp] * 12,
+        "fj_compute_new_phi": [v, i] + [_dp] * 9,
+        "fj_tabulate_delta": [v, _dp, _dp],
+        "fj_virial": [v] + [_dp] * 6,
+        "fj_eval_potential": [v, i, _dp, _dp, _dp, _dp, _dp],
+        "fj_actions": [v, i] + [_dp] * 8,
+        "fj_get_stats": [v, C.POINTER(Stats)],
+    }
+    for name, args in sig.items():
+        f = getattr(lib, name)
+        f.restype = i
+        f.argtypes = args
+    lib.fj_last_error.restype = C.c_char_p
+    lib.fj_last_error.argtypes = [v]
+    return lib
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        _lib = load()
+    return _lib
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data_as(_dp)
+
+
+def _a(x):
+    return np.ascontiguousarray(x, dtype=np.float64)
+
+
+class FJError(RuntimeError):
+    pass
+
+
+class Context:
+    """One fj_ctx on a HIP device (the reference's Grid.h/DF.cpp globals)."""
+
+    def __init__(self, device=0):
+        self.lib = lib()
+        h = C.c_void_p()
+        rc = self.lib.fj_ctx_create(device, C.byref(h))
+        if rc:
+            raise FJError(f"fj_ctx_create failed ({rc}): no HIP device {device}?")
+        self.h = h
+        self.nr = self.ntheta = self.npoly = self.ngrid = 0
+
+    def close(self):
+        if self.h:
+            self.lib.fj_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _chk(self, rc, what):
+        if rc:
+            raise FJError(f"{what}: {self.lib.fj_last_error(self.h).decode()} (code {rc})")
+
+    def set_grid(self, ar, ntheta, npoly, ngrid=30):
+        self.ar = _a(ar)
+        self.nr, self.ntheta, self.npoly, self.ngrid = len(self.ar), ntheta, npoly, ngrid
+        self._chk(self.lib.fj_set_grid(self.h, self.nr, _p(self.ar), ntheta, npoly, ngrid), "fj_set_grid")
+
+    def set_quad(self, preset):
+        self._chk(self.lib.fj_set_quad_rule(self.h, preset), "fj_set_quad_rule")
+
+    def set_precision(self, mode):
+        self._chk(self.lib.fj_set_precision(self.h, mode), "fj_set_precision")
+
+    def set_potential(self, phil, Pr, Pr2):
+        a, b, c = _a(phil), _a(Pr), _a(Pr2)
+        self._chk(self.lib.fj_set_potential(self.h, _p(a), _p(b), _p(c)), "fj_set_potential")
+
+    def potential(self):
+        s = (self.nr, self.npoly)
+        a, b, c = np.zeros(s), np.zeros(s), np.zeros(s)
+        self._chk(self.lib.fj_get_potential(self.h, _p(a), _p(b), _p(c)), "fj_get_potential")
+        return a, b, c
+
+    def set_delta(self, E, D):
+        E, D = _a(E), _a(D)
+        self._chk(self.lib.fj_set_delta_table(self.h, _p(E), _p(D)), "fj_set_delta_table")
+
+    def delta(self):
+        E, D = np.zeros(self.ngrid), np.zeros(self.ngrid)
+        self._chk(self.lib.fj_get_delta_table(self.h, _p(E), _p(D)), "fj_get_delta_table")
+        return E, D
+
+    def set_df(self, df):
+        self._chk(self.lib.fj_set_df(self.h, C.byref(df)), "fj_set_df")
+
+    def guess_rhl(self, model="Hernquist", mass=1.0, r0=1.0, q=1.0):
+        out = np.zeros((self.nr, self.npoly))
+        self._chk(self.lib.fj_guess_rhl(self.h, GUESS[model], mass, r0, q, _p(out)), "fj_guess_rhl")
+        return out
+
+    def poisson(self, rhl, mode=FJ_POISSON_QUADPACK):
+        rhl = _a(rhl)
+        s = (self.nr, self.npoly)
+        a, b, c = np.zeros(s), np.zeros(s), np.zeros(s)
+        self._chk(self.lib.fj_poisson(self.h, mode, _p(rhl), _p(a), _p(b), _p(c)), "fj_poisson")
+        return a, b, c
+
+    def moments(self):
+        s = (self.nr, self.ntheta)
+        o = [np.zeros(s) for _ in range(6)]
+        self._chk(self.lib.fj_moments(self.h, *[_p(x) for x in o]), "fj_moments")
+        return o
+
+    def rhofDF(self, R, z):
+        R, z = _a(R), _a(z)
+        n = len(R)
+        out6 = np.zeros((n, 6))
+        o = [np.zeros(n) for _ in range(6)]
+        self._chk(self.lib.fj_rhofDF(self.h, n, _p(R), _p(z), _p(out6), *[_p(x) for x in o]), "fj_rhofDF")
+        return out6, o
+
+    def project(self, rho, vrot, sigR, sigp, sigz, sigRz):
+        ins = [_a(x) for x in (rho, vrot, sigR, sigp, sigz, sigRz)]
+        s = (self.nr, self.npoly)
+        o = [np.zeros(s) for _ in range(6)]
+        self._chk(self.lib.fj_project_legendre(self.h, *[_p(x) for x in ins], *[_p(x) for x in o]),
+                  "fj_project_legendre")
+        return dict(zip(["rhl", "vrotl", "sigRl", "sigpl", "sigzl", "sigRzl"], o))
+
+    def compute_new_phi(self, mode=FJ_POISSON_QUADPACK):
+        s = (self.nr, self.npoly)
+        o = [np.zeros(s) for _ in range(9)]
+        # C order: rhl, sigRl, sigpl, sigzl, sigRzl, vrotl, philP, PrP, Pr2P
+        self._chk(self.lib.fj_compute_new_phi(self.h, mode, *[_p(x) for x in o]), "fj_compute_new_phi")
+        return dict(zip(["rhl", "sigRl", "sigpl", "sigzl", "sigRzl", "vrotl", "phil", "Pr", "Pr2"], o))
+
+    def tabulate_delta(self):
+        E, D = np.zeros(self.ngrid), np.zeros(self.ngrid)
+        self._chk(self.lib.fj_tabulate_delta(self.h, _p(E), _p(D)), "fj_tabulate_delta")
+        return E, D
+
+    def virial(self, rhl, vrotl, sigRl, sigpl, sigzl):
+        a = [_a(x) for x in (rhl, vrotl, sigRl, sigpl, sigzl)]
+        out = np.zeros(11)
+        self._chk(self.lib.fj_virial(self.h, *[_p(x) for x in a], _p(out)), "fj_virial")
+        return out
+
+    def eval_potential(self, R, z, grad=True):
+        R, z = _a(R), _a(z)
+        n = len(R)
+        phi = np.zeros(n)
+        dR = np.zeros(n) if grad else None
+        dz = np.zeros(n) if grad else None
+        self._chk(self.lib.fj_eval_potential(self.h, n, _p(R), _p(z), _p(phi), _p(dR), _p(dz)),
+                  "fj_eval_potential")
+        return (phi, dR, dz) if grad else phi
+
+    def actions(self, R, z, vR, vphi, vz, Delta=None):
+        R, z, vR, vphi, vz = (_a(x) for x in (R, z, vR, vphi, vz))
+        n = len(R)
+        D = None if Delta is None else _a(Delta)
+        Jr, Jz = np.zeros(n), np.zeros(n)
+        self._chk(self.lib.fj_actions(self.h, n, _p(R), _p(z), _p(vR), _p(vphi), _p(vz), _p(D),
+                                      _p(Jr), _p(Jz)), "fj_actions")
+        return Jr, Jz
+
+    def stats(self):
+        s = Stats()
+        self._chk(self.lib.fj_get_stats(self.h, C.byref(s)), "fj_get_stats")
+        return {n: getattr(s, n) for n, _ in Stats._fields_}

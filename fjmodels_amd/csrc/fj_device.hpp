@@ -1,0 +1,741 @@
+// fj_device.hpp -- device building blocks of the fJmodels SCF path on gfx950.
+//
+// Potential evaluation (pot/Potential.cpp:206-276, include/UtilsLeg.h:43-158)
+// is fused: one radial bracket search and one pass over the Legendre
+// coefficients give Phi, dPhi/dR and dPhi/dz together, with the derivative
+// recurrences run alongside instead of re-running legend() per order.
+// Root finding (include/Utils.h:161-194) and the adaptive quadratures
+// (include/Utils.h:200-212, pot/Potential.cpp:161-206) are written as
+// steppers so that a kernel can drive many of them in lock step.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace fj {
+
+// include/Utils.h:49-60
+constexpr double EPSABSROOTS = 1e-8;
+constexpr double EPSRELROOTS = 1e-6;
+constexpr double EPSABSINTEG = 1e-5;
+constexpr double EPSRELINTEG = 1e-3;
+constexpr double PI = 3.141592653589793;
+constexpr double PIH = 1.570796326794897;
+constexpr double TPI = 6.283185307179586;
+constexpr double FPI = 12.56637061435917;
+constexpr double TINY = 1e-12;
+constexpr double DBL_EPS = 2.220446049250313e-16;
+constexpr double DBL_MINV = 2.2250738585072014e-308;
+constexpr double DBL_MAXV = 1.7976931348623157e+308;
+constexpr int MAXPOLY = 32;
+
+// Recurrence coefficients, filled by the host once per npoly.
+//   evenlegend: P2k from P2k-2, P2k-4 (UtilsLeg.h:68-79):
+//     pol[np] = (-pol[np-2]*ea[np] + pol[np-1]*(c2 - eb[np])) * eg[np]
+//   legend: P_i = la[i]*c*P_{i-1} - lb[i]*P_{i-2} (UtilsLeg.h:43-50)
+struct LegCoef {
+    double ea[MAXPOLY], eb[MAXPOLY], eg[MAXPOLY];
+    double la[2 * MAXPOLY], lb[2 * MAXPOLY];
+};
+
+// Potential tables as the node kernels see them (LDS or global).
+struct PotView {
+    int nr, npoly;
+    const double *ar;    // [nr]
+    const double *phil;  // [nr][npoly]
+    const double *Pr;    // [nr][npoly]
+    const LegCoef *lc;
+};
+
+// ---------------------------------------------------------------------------
+// Radial bracket + interpolation weights (UtilsLeg.h:96-158, Utils.h:130-146).
+// r is already clamped to >= ar[0] where the caller requires it.
+// ---------------------------------------------------------------------------
+struct Interp {
+    int bot, top, thr;
+    double f1, f2, f3;
+    bool quad, kep;
+    double q;   // ar[nr-1]/r on the Keplerian branch
+};
+
+__device__ __forceinline__ Interp radial_interp(const PotView &T, double r) {
+    Interp I;
+    const int nr = T.nr;
+    const double *ar = T.ar;
+    I.kep = (r >= ar[nr - 1]);
+    I.quad = false;
+    I.bot = 0; I.top = 0; I.thr = 0; I.f1 = 0; I.f2 = 0; I.f3 = 0; I.q = 0;
+    if (I.kep) {
+        I.q = ar[nr - 1] / r;
+        return I;
+    }
+    int top = nr - 1, bot = 0;
+    // (xm[top]-x)*(x-xm[m]) >= 0 is x >= xm[m] for x inside [xm[0], xm[top]]
+    while (top - bot > 1) {
+        int m = (top + bot) >> 1;
+        if (r >= ar[m]) bot = m; else top = m;
+    }
+    I.bot = bot; I.top = top;
+    const double db = r - ar[bot];
+    I.f1 = db / (ar[top] - ar[bot]);
+    if (top < 10) {
+        I.quad = true;
+        I.thr = (I.f1 < 0.5 && bot > 0) ? bot - 1 : top + 1;
+        const double dt = r - ar[top];
+        I.f2 = dt * db / ((ar[I.thr] - ar[top]) * (ar[I.thr] - ar[bot]));
+        I.f3 = (ar[I.thr] - ar[bot]) / (ar[top] - ar[bot]);
+    }
+    return I;
+}
+
+// Interpolated coefficient np of table tab (interior branch).
+__device__ __forceinline__ double interp_coef(const Interp &I, const double *tab, int npoly, int np) {
+    const double t = tab[I.top * npoly + np], b = tab[I.bot * npoly + np];
+    double v = I.f1 * t + (1 - I.f1) * b;
+    if (I.quad) {
+        const double h = tab[I.thr * npoly + np];
+        v += I.f2 * (h - b - I.f3 * (t - b));
+    }
+    return v;
+}
+
+// Phi(R,z): Potential::operator() (Potential.cpp:206-219).
+__device__ __forceinline__ double pot_phi(const PotView &T, double R, double z) {
+    const double r = sqrt(R * R + z * z);
+    const double c = z / r;
+    const double rc = fmax(r, T.ar[0]);
+    const Interp I = radial_interp(T, rc);
+    const int np_ = T.npoly;
+    const double c2 = c * c;
+    double phi, pm2 = 1.0, pm1 = 1.5 * c2 - .5;
+    if (I.kep) {
+        const double q2 = I.q * I.q;
+        double qk = I.q;
+        const double *last = T.phil + (T.nr - 1) * np_;
+        phi = last[0] * qk;
+        if (np_ > 1) { qk *= q2; phi += last[1] * qk * pm1; }
+        for (int np = 2; np < np_; np++) {
+            const double p = (-pm2 * T.lc->ea[np] + pm1 * (c2 - T.lc->eb[np])) * T.lc->eg[np];
+            qk *= q2;
+            phi += last[np] * qk * p;
+            pm2 = pm1; pm1 = p;
+        }
+    } else {
+        phi = interp_coef(I, T.phil, np_, 0);
+        if (np_ > 1) phi += interp_coef(I, T.phil, np_, 1) * pm1;
+        for (int np = 2; np < np_; np++) {
+            const double p = (-pm2 * T.lc->ea[np] + pm1 * (c2 - T.lc->eb[np])) * T.lc->eg[np];
+            phi += interp_coef(I, T.phil, np_, np) * p;
+            pm2 = pm1; pm1 = p;
+        }
+    }
+    return phi;
+}
+
+// Phi, dPhi/dR, dPhi/dz at one point: Potential::operator(), dR, dz
+// (Potential.cpp:206-219, 224-244, 261-276).  dr() evaluates P_l at
+// cos(acos(c)); c is used directly here (the two agree to rounding).
+__device__ __forceinline__ double pot_phi_grad(const PotView &T, double R, double z, double *dR, double *dz) {
+    const double rr = sqrt(R * R + z * z);
+    const double cphi = z / rr;
+    const double r = fmax(rr, T.ar[0]);
+    const double s = R / r;
+    double c = z / r;
+    if (c == 1) c -= 1e-8;
+    const Interp I = radial_interp(T, r);
+    const int np_ = T.npoly;
+    const double c2 = c * c;
+    const double inv1mc2 = 1.0 / (1 - c2);
+    // evenlegend(c) for dr(); legend recurrence for dlegend (dtheta)
+    double pm2 = 1.0, pm1 = 1.5 * c2 - .5;
+    // (P_{2np-2}, P_{2np-3}) of the full recurrence; la[1] = 1, lb[1] = 0 gives P_1 = c
+    double lm1 = 1.0, lm2 = 0.0;
+    double dth = 0, g1 = 0, phi = 0;
+    const double *last = T.phil + (T.nr - 1) * np_;
+    const double q2 = I.q * I.q;
+    double qk = I.q;
+    for (int np = 0; np < np_; np++) {
+        double ph, dph;
+        if (I.kep) {
+            ph = last[np] * qk;
+            dph = -(2 * np + 1) * ph / r;
+            qk *= q2;
+        } else {
+            ph = interp_coef(I, T.phil, np_, np);
+            dph = interp_coef(I, T.Pr, np_, np);
+        }
+        double pe;
+        if (np == 0) pe = 1.0;
+        else if (np == 1) pe = pm1;
+        else {
+            pe = (-pm2 * T.lc->ea[np] + pm1 * (c2 - T.lc->eb[np])) * T.lc->eg[np];
+            pm2 = pm1; pm1 = pe;
+        }
+        // dP_{2np}/dc = n (P_{n-1} - c P_n)/(1-c^2), n = 2np
+        double dpl = 0;
+        if (np > 0) {
+            const int i1 = 2 * np - 1, i2 = 2 * np;
+            const double p1 = T.lc->la[i1] * c * lm1 - T.lc->lb[i1] * lm2;
+            const double p2 = T.lc->la[i2] * c * p1 - T.lc->lb[i2] * lm1;
+            lm2 = p1; lm1 = p2;
+            const double n = (double)i2;
+            dpl = (n * p1 - n * c * p2) * inv1mc2;
+        }
+        dth += ph * dpl;
+        g1 += dph * pe;
+        phi = (np == 0) ? ph : phi + ph * pe;
+    }
+    if (cphi != c) {
+        // Phi uses c = z/|x| with the unclamped radius (only differs near r < ar[0] or c = 1)
+        const double cc2 = cphi * cphi;
+        double a2 = 1.0, a1 = 1.5 * cc2 - .5;
+        qk = I.q;
+        for (int np = 0; np < np_; np++) {
+            double ph;
+            if (I.kep) { ph = last[np] * qk; qk *= q2; }
+            else ph = interp_coef(I, T.phil, np_, np);
+            double pe;
+            if (np == 0) pe = 1.0;
+            else if (np == 1) pe = a1;
+            else {
+                pe = (-a2 * T.lc->ea[np] + a1 * (cc2 - T.lc->eb[np])) * T.lc->eg[np];
+                a2 = a1; a1 = pe;
+            }
+            phi = (np == 0) ? ph : phi + ph * pe;
+        }
+    }
+    dth *= -sqrt(1 - c2);
+    *dR = dth * (c / r) + g1 * s;
+    *dz = dth * (-s / r) + g1 * c;
+    return phi;
+}
+
+// ---------------------------------------------------------------------------
+// Brent root finder in GSL iterate form (Brent 1973) driven as
+// wrp_GSLroots (Utils.h:161-194), with F(low), F(high) supplied.
+// ---------------------------------------------------------------------------
+struct Brent {
+    double a, b, c, d, e, fa, fb, fc;
+    double lo, hi, root;
+    int cont;    // 1 while the wrapper loop continues
+    int iters;
+};
+
+// Returns true if the wrapper answers without iterating (|F| < 5e-4 at an end,
+// or a bracket already narrower than the tolerance, which returns 0 as there).
+__device__ __forceinline__ bool brent_start(Brent &s, double low, double high, double flow, double fhigh) {
+    s.iters = 0;
+    if (fabs(flow) < 5e-4) { s.root = low; s.cont = 0; return true; }
+    if (fabs(fhigh) < 5e-4) { s.root = high; s.cont = 0; return true; }
+    s.a = low; s.fa = flow; s.b = high; s.fb = fhigh; s.c = high; s.fc = fhigh;
+    s.d = high - low; s.e = high - low;
+    s.lo = low; s.hi = high; s.root = 0;
+    s.cont = (high - low > EPSABSROOTS) ? 1 : 0;
+    return !s.cont;
+}
+
+// Computes the next abscissa to evaluate; returns false if this iterate
+// terminates without a new evaluation (fb == 0 or |m| <= tol).
+__device__ __forceinline__ bool brent_propose(Brent &s, double &xnew) {
+    double a = s.a, b = s.b, c = s.c, fa = s.fa, fb = s.fb, fc = s.fc, d = s.d, e = s.e;
+    int ac_equal = 0;
+    if ((fb < 0 && fc < 0) || (fb > 0 && fc > 0)) { ac_equal = 1; c = a; fc = fa; d = b - a; e = b - a; }
+    if (fabs(fc) < fabs(fb)) { ac_equal = 1; a = b; b = c; c = a; fa = fb; fb = fc; fc = fa; }
+    const double tol = 0.5 * DBL_EPS * fabs(b);
+    const double m = 0.5 * (c - b);
+    if (fb == 0) {
+        s.a = a; s.b = b; s.c = c; s.fa = fa; s.fb = fb; s.fc = fc; s.d = d; s.e = e;
+        s.root = b; s.lo = b; s.hi = b;
+        return false;
+    }
+    if (fabs(m) <= tol) {
+        s.a = a; s.b = b; s.c = c; s.fa = fa; s.fb = fb; s.fc = fc; s.d = d; s.e = e;
+        s.root = b;
+        if (b < c) { s.lo = b; s.hi = c; } else { s.lo = c; s.hi = b; }
+        return false;
+    }
+    if (fabs(e) < tol || fabs(fa) <= fabs(fb)) { d = m; e = m; }
+    else {
+        double p, q, r, ss = fb / fa;
+        if (ac_equal) { p = 2 * m * ss; q = 1 - ss; }
+        else {
+            q = fa / fc; r = fb / fc;
+            p = ss * (2 * m * q * (q - r) - (b - a) * (r - 1));
+            q = (q - 1) * (r - 1) * (ss - 1);
+        }
+        if (p > 0) q = -q; else p = -p;
+        if (2 * p < fmin(3 * m * q - fabs(tol * q), fabs(e * q))) { e = d; d = p / q; }
+        else { d = m; e = m; }
+    }
+    a = b; fa = fb;
+    if (fabs(d) > tol) b += d; else b += (m > 0 ? +tol : -tol);
+    s.a = a; s.b = b; s.c = c; s.fa = fa; s.d = d; s.e = e; s.fc = fc;
+    xnew = b;
+    return true;
+}
+
+__device__ __forceinline__ void brent_accept(Brent &s, double fb) {
+    s.fb = fb;
+    s.root = s.b;
+    double c = s.c;
+    if ((fb < 0 && s.fc < 0) || (fb > 0 && s.fc > 0)) c = s.a;
+    if (s.b < c) { s.lo = s.b; s.hi = c; } else { s.lo = c; s.hi = s.b; }
+}
+
+// gsl_root_test_interval + the wrapper's width test: decides whether the
+// wrapper loop runs again.
+__device__ __forceinline__ void brent_after(Brent &s) {
+    s.iters++;
+    double lo = s.lo, hi = s.hi, min_abs;
+    if ((lo > 0.0 && hi > 0.0) || (lo < 0.0 && hi < 0.0)) min_abs = fmin(fabs(lo), fabs(hi));
+    else min_abs = 0;
+    const double tol = EPSABSROOTS + EPSRELROOTS * min_abs;
+    const int cont = fabs(hi - lo) < tol ? 0 : 1;
+    s.cont = (cont || hi - lo > EPSABSROOTS) ? 1 : 0;
+    if (s.iters > 400) s.cont = 0;
+}
+
+// Full root find for a functor F (one lane).
+template <class F>
+__device__ __forceinline__ double brent_solve(F &f, double low, double high, double flow, double fhigh, int *nit = nullptr) {
+    Brent s;
+    if (brent_start(s, low, high, flow, fhigh)) { if (nit) *nit = 0; return s.root; }
+    while (s.cont) {
+        double x;
+        if (brent_propose(s, x)) brent_accept(s, f(x));
+        brent_after(s);
+    }
+    if (nit) *nit = s.iters;
+    return s.root;
+}
+
+// ---------------------------------------------------------------------------
+// QUADPACK Gauss-Kronrod rules (dqk21, dqk15)
+// ---------------------------------------------------------------------------
+__constant__ static const double kXGK21[11] = {
+    0.995657163025808080735527280689003, 0.973906528517171720077964012084452,
+    0.930157491355708226001207180059508, 0.865063366688984510732096688423493,
+    0.780817726586416897063717578345042, 0.679409568299024406234327365114874,
+    0.562757134668604683339000099272694, 0.433395394129247190799265943165784,
+    0.294392862701460198131126603103866, 0.148874338981631210884826001129720, 0.0};
+__constant__ static const double kWGK21[11] = {
+    0.011694638867371874278064396062192, 0.032558162307964727478818972459390,
+    0.054755896574351996031381300244580, 0.075039674810919952767043140916190,
+    0.093125454583697605535065465083366, 0.109387158802297641899210590325805,
+    0.123491976262065851077208980178525, 0.134709217311473325928054001771707,
+    0.142775938577060080797094273138717, 0.147739104901338491374841515972068,
+    0.149445554002916905664936468389821};
+__constant__ static const double kWG10[5] = {
+    0.066671344308688137593568809893332, 0.149451349150580593145776339657697,
+    0.219086362515982043995534934228163, 0.269266719309996355091226921569469,
+    0.295524224714752870173892994651338};
+__constant__ static const double kXGK15[8] = {
+    0.991455371120812639206854697526329, 0.949107912342758524526189684047851,
+    0.864864423359769072789712788640926, 0.741531185599394439863864773280788,
+    0.586087235467691130294144845693013, 0.405845151377397166906606412076961,
+    0.207784955007898467600689403773245, 0.0};
+__constant__ static const double kWGK15[8] = {
+    0.022935322010529224963732008058970, 0.063092092629978553290700663189204,
+    0.104790010322250183839876322541518, 0.140653259715525918745189590510238,
+    0.169004726639267902826583426598550, 0.190350578064785409913256402421014,
+    0.204432940075298892414161999234649, 0.209482141084727828012999174891714};
+__constant__ static const double kWG7[4] = {
+    0.129484966168869693270611432679082, 0.279705391489276667901467771423780,
+    0.381830050505118944950369775488975, 0.417959183673469387755102040816327};
+
+struct QKOut { double result, abserr, resabs, resasc; };
+
+__device__ __forceinline__ double qk_rescale(double err, double resabs, double resasc) {
+    err = fabs(err);
+    if (resasc != 0 && err != 0) {
+        const double t = 200 * err / resasc;
+        const double scale = t * sqrt(t);   // (200 err/resasc)^1.5
+        err = scale < 1 ? resasc * scale : resasc;
+    }
+    if (resabs > DBL_MINV / (50 * DBL_EPS)) {
+        const double min_err = 50 * DBL_EPS * resabs;
+        if (min_err > err) err = min_err;
+    }
+    return err;
+}
+
+// Kronrod rule with NK abscissae >= 0 (11: QK21, 8: QK15).
+template <int NK, class F>
+__device__ __forceinline__ QKOut qk_rule(F &f, double a, double b) {
+    const double *xgk = NK == 11 ? kXGK21 : kXGK15;
+    const double *wgk = NK == 11 ? kWGK21 : kWGK15;
+    const double *wg = NK == 11 ? kWG10 : kWG7;
+    double fv1[NK], fv2[NK];
+    const double center = 0.5 * (a + b), half = 0.5 * (b - a), ahalf = fabs(half);
+    const double fc = f(center);
+    double rg = 0, rk = fc * wgk[NK - 1], rabs = fabs(rk);
+    if (NK % 2 == 0) rg = fc * wg[NK / 2 - 1];
+#pragma unroll
+    for (int j = 0; j < (NK - 1) / 2; j++) {
+        const int jtw = j * 2 + 1;
+        const double absc = half * xgk[jtw];
+        const double f1 = f(center - absc), f2 = f(center + absc), fs = f1 + f2;
+        fv1[jtw] = f1; fv2[jtw] = f2;
+        rg += wg[j] * fs;
+        rk += wgk[jtw] * fs;
+        rabs += wgk[jtw] * (fabs(f1) + fabs(f2));
+    }
+#pragma unroll
+    for (int j = 0; j < NK / 2; j++) {
+        const int jtwm1 = j * 2;
+        const double absc = half * xgk[jtwm1];
+        const double f1 = f(center - absc), f2 = f(center + absc);
+        fv1[jtwm1] = f1; fv2[jtwm1] = f2;
+        rk += wgk[jtwm1] * (f1 + f2);
+        rabs += wgk[jtwm1] * (fabs(f1) + fabs(f2));
+    }
+    const double mean = rk * 0.5;
+    double rasc = wgk[NK - 1] * fabs(fc - mean);
+#pragma unroll
+    for (int j = 0; j < NK - 1; j++) rasc += wgk[j] * (fabs(fv1[j] - mean) + fabs(fv2[j] - mean));
+    const double err = (rk - rg) * half;
+    QKOut o;
+    o.result = rk * half;
+    o.resabs = rabs * ahalf;
+    o.resasc = rasc * ahalf;
+    o.abserr = qk_rescale(err, o.resabs, o.resasc);
+    return o;
+}
+
+// ---------------------------------------------------------------------------
+// QUADPACK dqagse as a stepper: the caller evaluates one Kronrod rule on
+// [next_a, next_b] and hands the result to qags_consume().  Lists are
+// 1-based arrays of length cap+1 in a per-lane workspace.
+// ---------------------------------------------------------------------------
+struct QagsWork {
+    double *alist, *blist, *rlist, *elist;  // [cap+1]
+    int *iord;                              // [cap+1]
+    double *rlist2;                         // [56]
+    double *res3la;                         // [4]
+};
+
+struct Qags {
+    // inputs
+    double a0, b0, epsabs, epsrel;
+    int limit;
+    // stepping state: 0 first rule, 1 left half pending, 2 right half pending, 3 done
+    int state;
+    double next_a, next_b;
+    // dqagse locals
+    double area, area1, error1, defab1, a1, b1, a2, b2, defabs, dres, erlarg, erlast, errbnd, errmax,
+        errsum, ertest, small, correc, result, abserr;
+    int ier, ierro, iroff1, iroff2, iroff3, ksgn, ktmin, last, maxerr, nres, nrmax, numrl2, extrap, noext;
+    int overflow;  // workspace capacity reached before limit
+};
+
+__device__ void qags_begin(Qags &q, double a, double b, double epsabs, double epsrel, int limit);
+__device__ void qags_consume(Qags &q, const QagsWork &w, int cap, const QKOut &r);
+
+template <int NK, class F>
+__device__ __forceinline__ double qags_solve(F &f, double a, double b, double epsabs, double epsrel, int limit,
+                                             const QagsWork &w, int cap, int *ier = nullptr, int *nqk = nullptr) {
+    Qags q;
+    qags_begin(q, a, b, epsabs, epsrel, limit);
+    int n = 0;
+    while (q.state != 3) {
+        QKOut r = qk_rule<NK>(f, q.next_a, q.next_b);
+        n++;
+        qags_consume(q, w, cap, r);
+    }
+    if (ier) *ier = q.overflow ? -1 : q.ier;
+    if (nqk) *nqk = n;
+    return q.result;
+}
+
+}  // namespace fj
+
+namespace fj {
+
+// QUADPACK dqpsrt (1-based lists).
+__device__ __forceinline__ void qpsrt(int limit, int last, int &maxerr, double &ermax, const double *elist,
+                                      int *iord, int &nrmax) {
+    int i, j, k, ido, ibeg, jbnd, isucc, jupbn;
+    double errmax, errmin;
+    if (last <= 2) { iord[1] = 1; iord[2] = 2; goto L90; }
+    errmax = elist[maxerr];
+    if (nrmax != 1) {
+        ido = nrmax - 1;
+        for (i = 1; i <= ido; i++) {
+            isucc = iord[nrmax - 1];
+            if (errmax <= elist[isucc]) break;
+            iord[nrmax] = isucc;
+            nrmax--;
+        }
+    }
+    jupbn = last;
+    if (last > (limit / 2 + 2)) jupbn = limit + 3 - last;
+    errmin = elist[last];
+    jbnd = jupbn - 1;
+    ibeg = nrmax + 1;
+    if (ibeg <= jbnd) {
+        for (i = ibeg; i <= jbnd; i++) {
+            isucc = iord[i];
+            if (errmax >= elist[isucc]) goto L60;
+            iord[i - 1] = isucc;
+        }
+    }
+    iord[jbnd] = maxerr;
+    iord[jupbn] = last;
+    goto L90;
+L60:
+    iord[i - 1] = maxerr;
+    k = jbnd;
+    for (j = i; j <= jbnd; j++) {
+        isucc = iord[k];
+        if (errmin < elist[isucc]) { iord[k + 1] = last; goto L90; }
+        iord[k + 1] = isucc;
+        k--;
+    }
+    iord[i] = last;
+L90:
+    maxerr = iord[nrmax];
+    ermax = elist[maxerr];
+}
+
+// QUADPACK dqelg (epsilon algorithm), epstab and res3la 1-based.
+__device__ __forceinline__ void qelg(int &n, double *epstab, double &result, double &abserr, double *res3la, int &nres) {
+    const double epmach = DBL_EPS, oflow = DBL_MAXV;
+    nres++;
+    abserr = oflow;
+    result = epstab[n];
+    if (n >= 3) {
+        const int limexp = 50;
+        epstab[n + 2] = epstab[n];
+        const int newelm = (n - 1) / 2;
+        epstab[n] = oflow;
+        const int num = n;
+        int k1 = n;
+        for (int i = 1; i <= newelm; i++) {
+            const int k2 = k1 - 1, k3 = k1 - 2;
+            double res = epstab[k1 + 2];
+            const double e0 = epstab[k3], e1 = epstab[k2], e2 = res;
+            const double e1abs = fabs(e1), delta2 = e2 - e1, err2 = fabs(delta2);
+            const double tol2 = fmax(fabs(e2), e1abs) * epmach;
+            const double delta3 = e1 - e0, err3 = fabs(delta3);
+            const double tol3 = fmax(e1abs, fabs(e0)) * epmach;
+            if (!(err2 > tol2 || err3 > tol3)) {
+                result = res;
+                abserr = err2 + err3;
+                abserr = fmax(abserr, 0.5e+01 * epmach * fabs(result));
+                return;
+            }
+            const double e3 = epstab[k1];
+            epstab[k1] = e1;
+            const double delta1 = e1 - e3, err1 = fabs(delta1);
+            const double tol1 = fmax(e1abs, fabs(e3)) * epmach;
+            bool shrink = (err1 <= tol1 || err2 <= tol2 || err3 <= tol3);
+            double ss = 0;
+            if (!shrink) {
+                ss = 1.0 / delta1 + 1.0 / delta2 - 1.0 / delta3;
+                if (!(fabs(ss * e1) > 0.1e-03)) shrink = true;
+            }
+            if (shrink) { n = i + i - 1; break; }
+            res = e1 + 1.0 / ss;
+            epstab[k1] = res;
+            k1 = k1 - 2;
+            const double error = err2 + fabs(res - e2) + err3;
+            if (!(error > abserr)) { abserr = error; result = res; }
+        }
+        if (n == limexp) n = 2 * (limexp / 2) - 1;
+        int ib = ((num / 2) * 2 == num) ? 2 : 1;
+        const int ie = newelm + 1;
+        for (int i = 1; i <= ie; i++) { epstab[ib] = epstab[ib + 2]; ib += 2; }
+        if (num != n) {
+            int indx = num - n + 1;
+            for (int i = 1; i <= n; i++) { epstab[i] = epstab[indx]; indx++; }
+        }
+        if (nres >= 4) {
+            abserr = fabs(result - res3la[3]) + fabs(result - res3la[2]) + fabs(result - res3la[1]);
+            res3la[1] = res3la[2]; res3la[2] = res3la[3]; res3la[3] = result;
+        } else {
+            res3la[nres] = result;
+            abserr = oflow;
+        }
+    }
+    abserr = fmax(abserr, 0.5e+01 * epmach * fabs(result));
+}
+
+__device__ __forceinline__ void qags_begin(Qags &q, double a, double b, double epsabs, double epsrel, int limit) {
+    q.a0 = a; q.b0 = b; q.epsabs = epsabs; q.epsrel = epsrel; q.limit = limit;
+    q.state = 0; q.next_a = a; q.next_b = b;
+    q.ier = 0; q.last = 0; q.result = 0; q.abserr = 0; q.overflow = 0;
+    if (epsabs <= 0 && epsrel < fmax(50 * DBL_EPS, 0.5e-28)) { q.ier = 6; q.state = 3; }
+}
+
+// Sets up the bisection of interval maxerr for loop index q.last.
+__device__ __forceinline__ void qags_start_iter(Qags &q, const QagsWork &w, int cap) {
+    if (q.last > cap) { q.overflow = 1; q.state = 3; return; }
+    q.a1 = w.alist[q.maxerr];
+    q.b1 = 0.5 * (w.alist[q.maxerr] + w.blist[q.maxerr]);
+    q.a2 = q.b1;
+    q.b2 = w.blist[q.maxerr];
+    q.erlast = q.errmax;
+    q.next_a = q.a1; q.next_b = q.b1;
+    q.state = 1;
+}
+
+// dqagse labels 100..140
+__device__ __forceinline__ void qags_finish(Qags &q, const QagsWork &w, int from115) {
+    const double oflow = DBL_MAXV;
+    int go115 = from115, go130 = 0;
+    if (!go115) {
+        if (q.abserr == oflow) go115 = 1;
+        else if (q.ier + q.ierro != 0) {
+            if (q.ierro == 3) q.abserr = q.abserr + q.correc;
+            if (q.ier == 0) q.ier = 3;
+            if (q.result != 0.0 && q.area != 0.0) {
+                if (q.abserr / fabs(q.result) > q.errsum / fabs(q.area)) go115 = 1;
+            } else if (q.abserr > q.errsum) go115 = 1;
+            else if (q.area == 0.0) go130 = 1;
+        }
+        if (!go115 && !go130) {
+            if (q.ksgn == -1 && fmax(fabs(q.result), fabs(q.area)) <= q.defabs * 0.1e-01) go130 = 1;
+            else if (0.1e-01 > (q.result / q.area) || (q.result / q.area) > 0.1e+03 || q.errsum > fabs(q.area))
+                q.ier = 6;
+        }
+    }
+    if (go115) {
+        double s = 0.0;
+        for (int k = 1; k <= q.last; k++) s = s + w.rlist[k];
+        q.result = s;
+        q.abserr = q.errsum;
+    }
+    if (q.ier > 2) q.ier = q.ier - 1;
+    q.state = 3;
+}
+
+__device__ __forceinline__ void qags_consume(Qags &q, const QagsWork &w, int cap, const QKOut &r) {
+    const double epmach = DBL_EPS, uflow = DBL_MINV, oflow = DBL_MAXV;
+    if (q.state == 0) {
+        q.result = r.result; q.abserr = r.abserr; q.defabs = r.resabs;
+        const double resabs = r.resasc;
+        q.dres = fabs(q.result);
+        q.errbnd = fmax(q.epsabs, q.epsrel * q.dres);
+        q.last = 1;
+        w.alist[1] = q.a0; w.blist[1] = q.b0;
+        w.rlist[1] = q.result; w.elist[1] = q.abserr; w.iord[1] = 1;
+        if (q.abserr <= 1.0e+02 * epmach * q.defabs && q.abserr > q.errbnd) q.ier = 2;
+        if (q.limit == 1) q.ier = 1;
+        if (q.ier != 0 || (q.abserr <= q.errbnd && q.abserr != resabs) || q.abserr == 0.0) { q.state = 3; return; }
+        w.rlist2[1] = q.result;
+        q.errmax = q.abserr; q.maxerr = 1; q.area = q.result; q.errsum = q.abserr; q.abserr = oflow;
+        q.nrmax = 1; q.nres = 0; q.numrl2 = 2; q.ktmin = 0; q.extrap = 0; q.noext = 0;
+        q.ierro = 0; q.iroff1 = 0; q.iroff2 = 0; q.iroff3 = 0;
+        q.erlarg = 0; q.ertest = 0; q.small = 0; q.correc = 0;
+        q.ksgn = -1;
+        if (q.dres >= (1.0 - 0.5e+02 * epmach) * q.defabs) q.ksgn = 1;
+        q.last = 2;
+        qags_start_iter(q, w, cap);
+        return;
+    }
+    if (q.state == 1) {
+        q.area1 = r.result; q.error1 = r.abserr; q.defab1 = r.resasc;
+        q.next_a = q.a2; q.next_b = q.b2;
+        q.state = 2;
+        return;
+    }
+    // state 2: both halves known -- body of the dqagse loop for index last
+    const int last = q.last, limit = q.limit;
+    const double area2 = r.result, error2 = r.abserr, defab2 = r.resasc;
+    const double area1 = q.area1, error1 = q.error1, a1 = q.a1, b1 = q.b1, a2 = q.a2, b2 = q.b2;
+    const double area12 = area1 + area2;
+    const double erro12 = error1 + error2;
+    q.errsum = q.errsum + erro12 - q.errmax;
+    q.area = q.area + area12 - w.rlist[q.maxerr];
+    if (!(q.defab1 == error1 || defab2 == error2)) {
+        if (!(fabs(w.rlist[q.maxerr] - area12) > 0.1e-04 * fabs(area12) || erro12 < 0.99 * q.errmax)) {
+            if (q.extrap) q.iroff2++;
+            else q.iroff1++;
+        }
+        if (last > 10 && erro12 > q.errmax) q.iroff3++;
+    }
+    w.rlist[q.maxerr] = area1;
+    w.rlist[last] = area2;
+    q.errbnd = fmax(q.epsabs, q.epsrel * fabs(q.area));
+    if (q.iroff1 + q.iroff2 >= 10 || q.iroff3 >= 20) q.ier = 2;
+    if (q.iroff2 >= 5) q.ierro = 3;
+    if (last == limit) q.ier = 1;
+    if (fmax(fabs(a1), fabs(b2)) <= (1.0 + 0.1e+03 * epmach) * (fabs(a2) + 0.1e+04 * uflow)) q.ier = 4;
+    if (error2 > error1) {
+        w.alist[q.maxerr] = a2;
+        w.alist[last] = a1;
+        w.blist[last] = b1;
+        w.rlist[q.maxerr] = area2;
+        w.rlist[last] = area1;
+        w.elist[q.maxerr] = error2;
+        w.elist[last] = error1;
+    } else {
+        w.alist[last] = a2;
+        w.blist[q.maxerr] = b1;
+        w.blist[last] = b2;
+        w.elist[q.maxerr] = error1;
+        w.elist[last] = error2;
+    }
+    qpsrt(limit, last, q.maxerr, q.errmax, w.elist, w.iord, q.nrmax);
+    if (q.errsum <= q.errbnd) { qags_finish(q, w, 1); return; }
+    if (q.ier != 0) { qags_finish(q, w, 0); return; }
+    bool next = false;
+    if (last == 2) {
+        q.small = fabs(q.b0 - q.a0) * 0.375;
+        q.erlarg = q.errsum;
+        q.ertest = q.errbnd;
+        w.rlist2[2] = q.area;
+        next = true;
+    } else if (q.noext) {
+        next = true;
+    } else {
+        q.erlarg = q.erlarg - q.erlast;
+        if (fabs(b1 - a1) > q.small) q.erlarg = q.erlarg + erro12;
+        if (!q.extrap) {
+            if (fabs(w.blist[q.maxerr] - w.alist[q.maxerr]) > q.small) next = true;
+            else { q.extrap = 1; q.nrmax = 2; }
+        }
+        if (!next) {
+            if (!(q.ierro == 3 || q.erlarg <= q.ertest)) {
+                const int id = q.nrmax;
+                int jupbnd = last;
+                if (last > (2 + limit / 2)) jupbnd = limit + 3 - last;
+                for (int k = id; k <= jupbnd; k++) {
+                    q.maxerr = w.iord[q.nrmax];
+                    q.errmax = w.elist[q.maxerr];
+                    if (fabs(w.blist[q.maxerr] - w.alist[q.maxerr]) > q.small) { next = true; break; }
+                    q.nrmax++;
+                }
+            }
+        }
+        if (!next) {
+            q.numrl2++;
+            w.rlist2[q.numrl2] = q.area;
+            double reseps, abseps;
+            qelg(q.numrl2, w.rlist2, reseps, abseps, w.res3la, q.nres);
+            q.ktmin++;
+            if (q.ktmin > 5 && q.abserr < 0.1e-02 * q.errsum) q.ier = 5;
+            if (abseps < q.abserr) {
+                q.ktmin = 0;
+                q.abserr = abseps;
+                q.result = reseps;
+                q.correc = q.erlarg;
+                q.ertest = fmax(q.epsabs, q.epsrel * fabs(reseps));
+                if (q.abserr <= q.ertest) { qags_finish(q, w, 0); return; }
+            }
+            if (q.numrl2 == 1) q.noext = 1;
+            if (q.ier == 5) { qags_finish(q, w, 0); return; }
+            q.maxerr = w.iord[1];
+            q.errmax = w.elist[q.maxerr];
+            q.nrmax = 1;
+            q.extrap = 0;
+            q.small = q.small * 0.5;
+            q.erlarg = q.errsum;
+            next = true;
+        }
+    }
+    q.last = last + 1;
+    if (q.last > limit) { q.last = limit; qags_finish(q, w, 0); return; }
+    qags_start_iter(q, w, cap);
+}
+
+}  // namespace fj

@@ -1,0 +1,1882 @@
+// fj_kernels.hip -- gfx950 kernels of the fJmodels SCF iteration and the
+// extern "C" layer declared in include/fjmodels.h.
+//
+// Node phase of rhofDF (integ/Integ.cpp:150-163) for a batch of points:
+//   k_points    per point: Phi(R,z) and the escape speed Ve
+//   k_setup     per velocity node: H, Deltafn, spheroidal coordinates, Getu0,
+//               turning points (uvOrb.cpp:15-36,109-224); appends the two
+//               action integrals of every bound node to task lists
+//   k_integrals persistent: every lane evaluates one Gauss-Kronrod rule per
+//               step for its own QAGS task and takes a new task when done,
+//               so the expensive evaluations stay in lock step
+//   k_moments   per point: f(J) at every node and the 6 weighted sums
+//               (GaussQuad.cpp:160-180) reduced in a fixed order
+// Grid phase: k_project (potLeg.cpp:68 + UtilsLeg.h:200-253), k_poisson
+// (Potential.cpp:110-200), k_guess (Potential.cpp:63-80), k_delta
+// (delta/tabulate.cpp + delta/Delta.cpp), k_virial (libs/Stats.cpp).
+#include <hip/hip_runtime.h>
+
+#include <math.h>
+#include <stdarg.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <vector>
+
+#include "../../include/fjmodels.h"
+#include "fj_device.hpp"
+
+using namespace fj;
+
+namespace {
+
+constexpr int NQ1 = 22;                 // nodes per velocity axis (QUADORD)
+constexpr int NQ = NQ1 * NQ1 * NQ1;     // 10,648 nodes per point
+constexpr int QCAP = 48;                // QAGS intervals kept per lane before overflow
+constexpr int QCAP_FULL = 1000;         // the reference's workspace limit (Utils.h:55)
+
+// include/GaussPts.h:76-136
+__constant__ double c_xpt11[22] = {
+    -1., -0.50118723, -0.25118864, -0.12589254, -0.06309573, -0.03162278, -0.01584893,
+    -0.00794328, -0.00398107, -0.00199526, -0.001, 0.001, 0.00199526, 0.00398107,
+    0.00794328, 0.01584893, 0.03162278, 0.06309573, 0.12589254, 0.25118864, 0.50118723, 1.};
+__constant__ double c_xw11[22] = {
+    0.49881277, 0.37440568, 0.18764735, 0.09404645, 0.04713488, 0.0236234, 0.01183975,
+    0.00593393, 0.00297401, 0.00149054, 0.00149763, 0.00149763, 0.00149054, 0.00297401,
+    0.00593393, 0.01183975, 0.0236234, 0.04713488, 0.09404645, 0.18764735, 0.37440568,
+    0.49881277};
+__constant__ double c_xpt01[22] = {
+    1.12614037692e-05, 8.09053908976e-05, 0.000425204351858, 0.00172603419131,
+    0.0056596260047, 0.015541463911, 0.0367576002085, 0.0764489202523,
+    0.141870454058, 0.237263531493, 0.360233468529, 0.5,
+    0.639766531471, 0.762736468507, 0.858129545942, 0.923551079748,
+    0.963242399792, 0.984458536089, 0.994340373995, 0.998273965809,
+    0.999574795648, 0.999919094609};
+__constant__ double c_xw01[22] = {
+    2.42000467048e-05, 0.000146089994545, 0.000647128630152, 0.00222235986647,
+    0.00618889854184, 0.0144761668119, 0.0291728616842, 0.0514732311361,
+    0.0801433368009, 0.110279557896, 0.133823374104, 0.142799666072,
+    0.133823374104, 0.110279557896, 0.0801433368009, 0.0514732311361,
+    0.0291728616842, 0.0144761668119, 0.00618889854184, 0.00222235986647,
+    0.000647128630152, 0.000146089994545};
+
+// ---------------------------------------------------------------------------
+// Tables in LDS
+// ---------------------------------------------------------------------------
+struct TabArgs {
+    int nr, npoly, ngrid;
+    const double *ar, *phil, *Pr, *E, *D;
+    const LegCoef *lc;
+};
+
+__host__ __device__ inline size_t tab_lds_doubles(int nr, int npoly, int ngrid) {
+    return (size_t)nr + 2 * (size_t)nr * npoly + 2 * (size_t)ngrid + sizeof(LegCoef) / sizeof(double);
+}
+
+struct LdsTabs {
+    PotView T;
+    const double *E, *D;
+    int ngrid;
+};
+
+__device__ inline LdsTabs stage_tables(const TabArgs &a, double *smem) {
+    const int nr = a.nr, np = a.npoly, ng = a.ngrid;
+    double *ar = smem, *phil = ar + nr, *Pr = phil + nr * np, *E = Pr + nr * np, *D = E + ng;
+    double *lc = D + ng;
+    const int nlc = sizeof(LegCoef) / sizeof(double);
+    for (int i = threadIdx.x; i < nr; i += blockDim.x) ar[i] = a.ar[i];
+    for (int i = threadIdx.x; i < nr * np; i += blockDim.x) { phil[i] = a.phil[i]; Pr[i] = a.Pr[i]; }
+    for (int i = threadIdx.x; i < ng; i += blockDim.x) { E[i] = a.E[i]; D[i] = a.D[i]; }
+    const double *lsrc = reinterpret_cast<const double *>(a.lc);
+    for (int i = threadIdx.x; i < nlc; i += blockDim.x) lc[i] = lsrc[i];
+    __syncthreads();
+    LdsTabs t;
+    t.T.nr = nr; t.T.npoly = np; t.T.ar = ar; t.T.phil = phil; t.T.Pr = Pr;
+    t.T.lc = reinterpret_cast<const LegCoef *>(lc);
+    t.E = E; t.D = D; t.ngrid = ng;
+    return t;
+}
+
+// include/Utils.h:149-155 linterp (x strictly inside uses the bisection)
+__device__ inline double d_linterp(const double *xm, const double *ym, int np, double x) {
+    if (x <= xm[0]) return ym[0];
+    if (x >= xm[np - 1]) return ym[np - 1];
+    int top = np - 1, bot = 0;
+    while (top - bot > 1) {
+        int m = (top + bot) >> 1;
+        if (x >= xm[m]) bot = m; else top = m;
+    }
+    return ym[bot] + (x - xm[bot]) / (xm[top] - xm[bot]) * (ym[top] - ym[bot]);
+}
+
+// delta/tabulate.cpp:40-44
+__device__ inline double d_deltafn(const double *Eg, const double *Dg, int n, double E) {
+    if (E < Eg[0]) return sqrt(fmax(.000001, Dg[0]));
+    else if (E < Eg[n - 1]) return sqrt(fmax(.000001, d_linterp(Eg, Dg, n, E)));
+    return sqrt(fmax(.000001, Dg[n - 1]));
+}
+
+// ---------------------------------------------------------------------------
+// Orbit functions of uvOrb (uvOrb.cpp:42-83) with per-node constants
+// ---------------------------------------------------------------------------
+struct NodeConst {
+    double Delta, E, Lzsq, sv, cv, sv2, sinv, cosv;  // u-side
+    double shuX, chuX, shu2;                         // v-side: sinh/cosh(X.u), X.shu2
+    double sh1sq, K1, K2, K3, invsh1sq;              // uturnfn: Er*ch1sq, (sh1sq+sv2)*Phiu1, .5Lzsq/D^2, 1/sh1sq
+    double Iv, KV;                                   // vturnfn: Iv, (1+shu2)*Phiv(PIH)
+};
+
+struct DUturn {   // duturnfn (uvOrb.cpp:74-79)
+    const PotView *T;
+    const NodeConst *c;
+    long long *nev;
+    __device__ double operator()(double u) const {
+        const double shu = sinh(u), chu = cosh(u), shu2 = shu * shu;
+        const double D = c->Delta;
+        double dR, dz;
+        const double Phi0 = pot_phi_grad(*T, D * shu * c->sinv, D * chu * c->cosv, &dR, &dz);
+        (*nev)++;
+        const double dPhidu = D * (dR * chu * c->sv + dz * shu * c->cv);
+        return 2 * (c->E - Phi0) * shu * chu - (shu2 + c->sv2) * dPhidu + c->Lzsq * chu / (D * D * shu2 * shu);
+    }
+};
+
+struct Uturn {    // uturnfn (uvOrb.cpp:70-73) with dU (61-63)
+    const PotView *T;
+    const NodeConst *c;
+    __device__ double operator()(double u) const {
+        const double shu = sinh(u), chu = cosh(u), shu2 = shu * shu;
+        const double D = c->Delta;
+        const double Phi = pot_phi(*T, D * shu * c->sinv, D * chu * c->cosv);
+        const double dU = (shu2 + c->sv2) * Phi - c->K2;
+        return c->E * (shu2 - c->sh1sq) + c->K1 - dU - c->K3 * (1 / shu2 - c->invsh1sq);
+    }
+};
+
+struct Vturn {    // vturnfn (uvOrb.cpp:49-52) with dV (46-48)
+    const PotView *T;
+    const NodeConst *c;
+    __device__ double operator()(double v) const {
+        double s, cs;
+        sincos(v, &s, &cs);
+        const double sv2 = s * s;
+        const double D = c->Delta;
+        const double Phi = pot_phi(*T, D * c->shuX * s, D * c->chuX * cs);
+        const double dV = c->KV - (c->shu2 + sv2) * Phi;
+        return c->E * sv2 + c->Iv + dV - .5 * c->Lzsq / (D * D * sv2);
+    }
+};
+
+// Action integrands Juint / Jvint (uvOrb.cpp:53-56, 80-83)
+struct JInt {
+    const PotView *T;
+    const NodeConst *c;
+    double ubar, Deltau, vturn;
+    int type;   // 0: Ju, 1: Jv
+    __device__ double operator()(double psi) const {
+        double sp, cp;
+        sincos(psi, &sp, &cp);
+        double F;
+        const double D = c->Delta;
+        if (type == 0) {
+            const double u = ubar - Deltau * cp;
+            const double shu = sinh(u), chu = cosh(u), shu2 = shu * shu;
+            const double Phi = pot_phi(*T, D * shu * c->sinv, D * chu * c->cosv);
+            const double dU = (shu2 + c->sv2) * Phi - c->K2;
+            F = c->E * (shu2 - c->sh1sq) + c->K1 - dU - c->K3 * (1 / shu2 - c->invsh1sq);
+        } else {
+            const double v = PIH - (PIH - vturn) * cp;
+            double s, cs;
+            sincos(v, &s, &cs);
+            const double sv2 = s * s;
+            const double Phi = pot_phi(*T, D * c->shuX * s, D * c->chuX * cs);
+            const double dV = c->KV - (c->shu2 + sv2) * Phi;
+            F = c->E * sv2 + c->Iv + dV - .5 * c->Lzsq / (D * D * sv2);
+        }
+        return sqrt(fmax(0., F)) * sp;
+    }
+};
+
+// Node record between the kernels (structure of arrays).
+enum { NF_DELTA, NF_E, NF_LZSQ, NF_SV, NF_CV, NF_SV2, NF_SINV, NF_COSV, NF_SHUX, NF_CHUX, NF_SHU2,
+       NF_SH1SQ, NF_K1, NF_K2, NF_K3, NF_INVSH1, NF_IV, NF_KV, NF_UBAR, NF_DELTAU, NF_VTURN, NF_COUNT };
+
+struct NodeBuf {
+    double *f;       // [NF_COUNT][cap]
+    long long cap;
+    double *Jr, *Jz; // [cap]
+    int *flag;       // [cap]: 0 zero DF (H >= 0), 1 bound, 2 unbound (E >= 0)
+};
+
+__device__ inline void store_const(const NodeBuf &nb, long long i, const NodeConst &c, double ubar, double Deltau,
+                                   double vturn) {
+    double *f = nb.f;
+    const long long s = nb.cap;
+    f[NF_DELTA * s + i] = c.Delta; f[NF_E * s + i] = c.E; f[NF_LZSQ * s + i] = c.Lzsq;
+    f[NF_SV * s + i] = c.sv; f[NF_CV * s + i] = c.cv; f[NF_SV2 * s + i] = c.sv2;
+    f[NF_SINV * s + i] = c.sinv; f[NF_COSV * s + i] = c.cosv; f[NF_SHUX * s + i] = c.shuX;
+    f[NF_CHUX * s + i] = c.chuX; f[NF_SHU2 * s + i] = c.shu2; f[NF_SH1SQ * s + i] = c.sh1sq;
+    f[NF_K1 * s + i] = c.K1; f[NF_K2 * s + i] = c.K2; f[NF_K3 * s + i] = c.K3;
+    f[NF_INVSH1 * s + i] = c.invsh1sq; f[NF_IV * s + i] = c.Iv; f[NF_KV * s + i] = c.KV;
+    f[NF_UBAR * s + i] = ubar; f[NF_DELTAU * s + i] = Deltau; f[NF_VTURN * s + i] = vturn;
+}
+
+__device__ inline void load_const(const NodeBuf &nb, long long i, NodeConst &c, double &ubar, double &Deltau,
+                                  double &vturn) {
+    const double *f = nb.f;
+    const long long s = nb.cap;
+    c.Delta = f[NF_DELTA * s + i]; c.E = f[NF_E * s + i]; c.Lzsq = f[NF_LZSQ * s + i];
+    c.sv = f[NF_SV * s + i]; c.cv = f[NF_CV * s + i]; c.sv2 = f[NF_SV2 * s + i];
+    c.sinv = f[NF_SINV * s + i]; c.cosv = f[NF_COSV * s + i]; c.shuX = f[NF_SHUX * s + i];
+    c.chuX = f[NF_CHUX * s + i]; c.shu2 = f[NF_SHU2 * s + i]; c.sh1sq = f[NF_SH1SQ * s + i];
+    c.K1 = f[NF_K1 * s + i]; c.K2 = f[NF_K2 * s + i]; c.K3 = f[NF_K3 * s + i];
+    c.invsh1sq = f[NF_INVSH1 * s + i]; c.Iv = f[NF_IV * s + i]; c.KV = f[NF_KV * s + i];
+    ubar = f[NF_UBAR * s + i]; Deltau = f[NF_DELTAU * s + i]; vturn = f[NF_VTURN * s + i];
+}
+
+// Device counters
+struct Counters {
+    unsigned long long bound, tasks_u, tasks_v, qk, overflow, brent, grad;
+};
+
+// Wave-aggregated append to a task list.
+__device__ inline void wave_append(bool pred, unsigned int *counter, int *list, int value) {
+    const unsigned long long mask = __ballot(pred);
+    if (mask == 0) return;
+    const int lane = __lane_id();
+    const int leader = __ffsll((long long)mask) - 1;
+    unsigned int base = 0;
+    if (lane == leader) base = atomicAdd(counter, (unsigned int)__popcll(mask));
+    base = __shfl(base, leader);
+    if (pred) {
+        const unsigned long long lower = (lane == 0) ? 0ull : (mask & ((~0ull) >> (64 - lane)));
+        list[base + __popcll(lower)] = value;
+    }
+}
+
+struct PointArgs {
+    const double *R, *z;   // [npts]
+    double *Phi, *Ve;      // [npts]
+};
+
+// k_points: Phi_h = Phi(R,z); Ve = sqrt(-2(Phi_h - Phi(10 r_max, 10 r_max))) (Integ.cpp:152-153)
+__global__ void __launch_bounds__(256) k_points(TabArgs ta, PointArgs pa, int npts) {
+    extern __shared__ double smem[];
+    const LdsTabs L = stage_tables(ta, smem);
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= npts) return;
+    const double far = 10 * L.T.ar[L.T.nr - 1];
+    const double Phi_h = pot_phi(L.T, pa.R[i], pa.z[i]);
+    pa.Phi[i] = Phi_h;
+    pa.Ve[i] = sqrt(-2 * (Phi_h - pot_phi(L.T, far, far)));
+}
+
+struct SetupArgs {
+    const double *R, *z, *Phi, *Ve;
+    int p0;             // first point of this chunk
+    int npts;           // points in this chunk
+    int quad;           // FJ_QUAD_*
+    NodeBuf nb;
+    unsigned int *cnt_u, *cnt_v;
+    int *list_u, *list_v;
+    Counters *ctr;
+    // explicit phase-space points (fj_actions): node gid is point gid
+    const double *vR, *vphi, *vz, *Dx;
+};
+
+__device__ inline void node_velocity(int quad, int node, double Ve, double V[3], double &w) {
+    const int i = node / (NQ1 * NQ1), j = (node / NQ1) % NQ1, k = node % NQ1;
+    const double *xa = quad ? c_xpt01 : c_xpt11, *wa = quad ? c_xw01 : c_xw11;
+    V[0] = 1e-6 + Ve * xa[i];
+    V[1] = 1e-6 + Ve * c_xpt11[j];
+    V[2] = 1e-6 + Ve * xa[k];
+    w = wa[i] * c_xw11[j] * wa[k];
+}
+
+// k_setup: everything of df()/uvOrb before the two QAGS integrals.
+__global__ void __launch_bounds__(256) k_setup(TabArgs ta, SetupArgs sa) {
+    extern __shared__ double smem[];
+    const LdsTabs L = stage_tables(ta, smem);
+    const long long gid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const bool explicit_v = sa.vR != nullptr;
+    const long long nnodes = explicit_v ? (long long)sa.npts : (long long)sa.npts * NQ;
+    const bool valid = gid < nnodes;
+    bool want_u = false, want_v = false;
+    long long nev = 0;
+    int brent_it = 0;
+    if (valid) {
+        const int pl = explicit_v ? (int)gid : (int)(gid / NQ), node = explicit_v ? 0 : (int)(gid % NQ);
+        const int p = sa.p0 + pl;
+        const double R = sa.R[p], z = sa.z[p], Phi_h = sa.Phi[p], Ve = sa.Ve[p];
+        double V[3], w;
+        if (explicit_v) { V[0] = sa.vR[p]; V[1] = sa.vphi[p]; V[2] = sa.vz[p]; }
+        else node_velocity(sa.quad, node, Ve, V, w);
+        const double H = .5 * (V[0] * V[0] + V[1] * V[1] + V[2] * V[2]) + Phi_h;   // DF.cpp:83
+        int flag = 0;
+        double Jr = 0, Jz = 0;
+        if (H < 0 || explicit_v) {
+            NodeConst c;
+            const double Delta = (explicit_v && sa.Dx && sa.Dx[p] > 0) ? sa.Dx[p] : d_deltafn(L.E, L.D, L.ngrid, H);
+            // pt(Delta, x, p): Uvpt.cpp:11-24
+            const double Delta2 = Delta * Delta, R2 = R * R, z2 = z * z, r2 = R2 + z2;
+            const double shu2 = .5 * (r2 - Delta2 + sqrt((Delta2 - r2) * (Delta2 - r2) + 4 * Delta2 * R2)) / Delta2;
+            const double shu = sqrt(shu2), u = asinh(shu);
+            const double chu = sqrt(1 + shu2);
+            const double cv2 = .5 * (r2 + Delta2 - sqrt((Delta2 + r2) * (Delta2 + r2) - 4 * Delta2 * z2)) / Delta2;
+            const double cv = z < 0 ? -sqrt(cv2) : sqrt(cv2);
+            const double v = acos(cv);
+            const double sv = sin(v), sv2 = 1 - cv2;
+            const double pR = V[0], pz = V[2], p2 = pR * pR + pz * pz;
+            const double pu = Delta * (pR * chu * sv + pz * shu * cv);
+            const double pv = Delta * (pR * shu * cv - pz * chu * sv);
+            const double Lz = R * V[1];
+            c.Delta = Delta; c.Lzsq = Lz * Lz;
+            c.E = .5 * p2 + .5 * c.Lzsq / (R * R) + Phi_h;   // uvOrb.cpp:22
+            if (c.E >= 0 || isnan(c.E)) {
+                flag = 2; Jr = 1e31; Jz = 1e31;   // uvOrb.cpp:38-40
+            } else {
+                flag = 1;
+                c.sv = sv; c.cv = cv; c.sv2 = sv2; c.sinv = sv; c.cosv = cos(v); c.shu2 = shu2;
+                // Getu0(us = 58): uvOrb.cpp:109-129
+                DUturn du{&L.T, &c, &nev};
+                double utry = 58, ui = 58, uo = 58, dfui = 0, dfuo = 0;
+                double dut = du(utry);
+                int guard = 0;
+                if (dut > 0) {
+                    while (dut > 0 && guard++ < 4000) { ui = utry; dfui = dut; utry *= 1.2; dut = du(utry); }
+                    uo = utry; dfuo = dut;
+                } else {
+                    while (dut < 0 && guard++ < 4000) { uo = utry; dfuo = dut; utry *= .8; dut = du(utry); }
+                    ui = utry; dfui = dut;
+                }
+                int it = 0;
+                const double umid = brent_solve(du, ui, uo, dfui, dfuo, &it);
+                brent_it += it;
+                // second pass at umid: uvOrb.cpp:30-33
+                const double shm = sinh(umid), chm = cosh(umid);
+                c.sh1sq = shm * shm;
+                const double ch1sq = chm * chm;
+                const double Phiu1 = pot_phi(L.T, Delta * shm * c.sinv, Delta * chm * c.cosv);
+                c.shuX = sinh(u); c.chuX = cosh(u);
+                const double PhiX = pot_phi(L.T, Delta * c.shuX * c.sinv, Delta * c.chuX * c.cosv);
+                c.invsh1sq = 1 / c.sh1sq;
+                c.K2 = (c.sh1sq + sv2) * Phiu1;
+                const double dUX = (c.shuX * c.shuX + sv2) * PhiX - c.K2;
+                const double Er = -c.E * (shu2 - c.sh1sq) / ch1sq + .5 * (pu * pu) / (Delta2 * ch1sq) +
+                                  .5 * c.Lzsq / (Delta2 * ch1sq) * (1 / shu2 - c.invsh1sq) + dUX / ch1sq;
+                c.K1 = Er * ch1sq;
+                c.K3 = .5 * c.Lzsq / (Delta * Delta);
+                double sP, cP;
+                sincos(PIH, &sP, &cP);
+                const double PhivPIH = pot_phi(L.T, Delta * c.shuX * sP, Delta * c.chuX * cP);
+                c.KV = (1 + shu2) * PhivPIH;
+                const double dVX = c.KV - (shu2 + c.sinv * c.sinv) * PhiX;
+                c.Iv = .5 * (pv * pv) / Delta2 - c.E * sv2 + .5 * c.Lzsq / (Delta2 * sv2) - dVX;
+
+                // GetTurnu(1): uvOrb.cpp:131-201
+                Uturn U{&L.T, &c};
+                double ut_ = 1, ut = U(ut_), dutt = du(ut_), eps = .1;
+                double uinner = ut_, uouter = ut_, u1 = ut_, u2 = ut_, fu1 = ut, fu2 = ut, fuinner = ut, fuouter = ut;
+                bool noroot = false;
+                guard = 0;
+                if (ut < 0) {
+                    if (dutt > 0) {
+                        uinner = ut_; fuinner = ut;
+                        while (guard++ < 4000) {
+                            while (ut < 0 && dutt > 0 && guard++ < 4000) {
+                                uinner = ut_; fuinner = ut; ut_ *= (1 + eps);
+                                ut = U(ut_); dutt = du(ut_);
+                            }
+                            if (ut >= 0) { u1 = ut_; fu1 = ut; break; }
+                            ut_ /= (1 + eps); eps *= .2;
+                            if (eps < TINY) { uinner = uouter = ut_; noroot = true; break; }
+                            ut_ *= (1 + eps);
+                            ut = U(ut_); dutt = du(ut_);
+                        }
+                        if (!noroot) {
+                            while (ut > 0 && guard++ < 8000) { u2 = ut_; fu2 = ut; ut_ *= 1.1; ut = U(ut_); }
+                            uouter = ut_; fuouter = ut;
+                        }
+                    } else {
+                        while (guard++ < 4000) {
+                            while (ut < 0 && dutt < 0 && guard++ < 4000) {
+                                uouter = ut_; fuouter = ut; ut_ /= (1 + eps);
+                                ut = U(ut_); dutt = du(ut_);
+                            }
+                            if (ut >= 0) { u2 = ut_; fu2 = ut; break; }
+                            ut_ *= (1 + eps); eps *= .2;
+                            if (eps < TINY) { uinner = uouter = ut_; noroot = true; break; }
+                            ut_ /= (1 + eps);
+                            ut = U(ut_); dutt = du(ut_);
+                        }
+                        if (!noroot) {
+                            while (ut > 0 && guard++ < 8000) { u1 = ut_; fu1 = ut; ut_ *= .9; ut = U(ut_); }
+                            uinner = ut_; fuinner = ut;
+                        }
+                    }
+                } else {
+                    u1 = u2 = ut_; fu1 = fu2 = ut;
+                    while (ut > 0 && guard++ < 4000) { u2 = ut_; fu2 = ut; ut_ *= 1.1; ut = U(ut_); }
+                    uouter = ut_; fuouter = ut;
+                    // the inward walk starts where the outward one stopped; its first
+                    // pass pairs u1 = uouter with the stale fu1, whose true F is fuouter
+                    ut = fu1;
+                    bool first = true;
+                    while (ut > 0 && guard++ < 8000) {
+                        u1 = ut_; fu1 = first ? fuouter : ut; first = false;
+                        ut_ *= .9; ut = U(ut_);
+                    }
+                    uinner = ut_; fuinner = ut;
+                }
+                if (!noroot) {
+                    int i1 = 0, i2 = 0;
+                    const double a = brent_solve(U, uinner, u1, fuinner, fu1, &i1);
+                    const double b = brent_solve(U, u2, uouter, fu2, fuouter, &i2);
+                    uinner = a; uouter = b;
+                    brent_it += i1 + i2;
+                }
+                double ubar = 0, Deltau = 0, vturn = 0;
+                if (fabs(uouter - uinner) < TINY) Jr = 0;
+                else {
+                    ubar = .5 * (uinner + uouter);
+                    Deltau = .5 * (uouter - uinner);
+                    want_u = true;
+                }
+                // GetTurnv(.5*PIH): uvOrb.cpp:203-224
+                Vturn Vf{&L.T, &c};
+                {
+                    const double vtry = .5 * PIH;
+                    double vtop = vtry, vbot = vtry, ft, fb;
+                    const double f0 = Vf(vtry);
+                    bool done = false;
+                    guard = 0;
+                    if (f0 < 0) {
+                        fb = f0; ft = f0;
+                        while (fb < 0 && guard++ < 4000) {
+                            vbot = .5 * (PIH + vbot);
+                            if (PIH - vbot < TINY) { vturn = PIH; done = true; break; }
+                            fb = Vf(vbot);
+                        }
+                    } else {
+                        ft = f0; fb = f0;
+                        while (ft > 0 && guard++ < 4000) {
+                            vtop *= .5;
+                            if (vtop < TINY) { vturn = 0; done = true; break; }
+                            ft = Vf(vtop);
+                        }
+                    }
+                    if (!done) {
+                        int i3 = 0;
+                        if (!(vbot < vtop)) vturn = brent_solve(Vf, vtop, vbot, ft, fb, &i3);
+                        else vturn = brent_solve(Vf, vbot, vtop, fb, ft, &i3);
+                        brent_it += i3;
+                    }
+                }
+                if (fabs(vturn - PIH) < TINY) Jz = 0;
+                else want_v = true;
+                store_const(sa.nb, gid, c, ubar, Deltau, vturn);
+            }
+        }
+        sa.nb.flag[gid] = flag;
+        sa.nb.Jr[gid] = Jr;
+        sa.nb.Jz[gid] = Jz;
+    }
+    wave_append(want_u, sa.cnt_u, sa.list_u, (int)gid);
+    wave_append(want_v, sa.cnt_v, sa.list_v, (int)gid);
+    // counters (wave-aggregated)
+    const unsigned long long nb_ = __popcll(__ballot(valid && (want_u || want_v || nev > 0)));
+    if (__lane_id() == 0 && nb_) atomicAdd(&sa.ctr->bound, nb_);
+    if (valid && brent_it) atomicAdd(&sa.ctr->brent, (unsigned long long)brent_it);
+    if (valid && nev) atomicAdd(&sa.ctr->grad, (unsigned long long)nev);
+}
+
+struct IntegArgs {
+    NodeBuf nb;
+    const int *list_u, *list_v;
+    const unsigned int *cnt_u, *cnt_v;
+    unsigned int *next;           // task counter
+    int *ovf_list;                // overflow task ids (task index into u..v space)
+    unsigned int *ovf_cnt;
+    double *ws;                   // per-lane workspace
+    int cap;                      // intervals per lane
+    int ntask_override;           // >= 0: tasks are ovf_list[0..n)
+    Counters *ctr;
+};
+
+__device__ inline QagsWork lane_work(double *ws, int cap, long long lane) {
+    const long long per = 4 * (long long)(cap + 1) + (cap + 1) / 2 + 1 + 56 + 4;
+    double *base = ws + lane * per;
+    QagsWork w;
+    w.alist = base;
+    w.blist = base + (cap + 1);
+    w.rlist = base + 2 * (cap + 1);
+    w.elist = base + 3 * (cap + 1);
+    w.iord = reinterpret_cast<int *>(base + 4 * (cap + 1));
+    w.rlist2 = base + 4 * (cap + 1) + (cap + 1) / 2 + 1;
+    w.res3la = w.rlist2 + 56;
+    return w;
+}
+
+// k_integrals: persistent QAGS over the Ju and Jv task lists (Utils.h:200-212,
+// uvOrb.cpp:226-262).  Each loop step every busy lane applies one QK21 rule to
+// its own current interval; lanes that finish fetch the next task.
+__global__ void __launch_bounds__(256) k_integrals(TabArgs ta, IntegArgs ia) {
+    extern __shared__ double smem[];
+    const LdsTabs L = stage_tables(ta, smem);
+    const long long lane_g = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const QagsWork w = lane_work(ia.ws, ia.cap, lane_g);
+    const unsigned int nu = *ia.cnt_u, nv = *ia.cnt_v;
+    const unsigned int ntask = ia.ntask_override >= 0 ? (unsigned int)ia.ntask_override : nu + nv;
+    int task = -1;    // -1: needs a task, -2: queue drained
+    long long node = 0;
+    NodeConst c;
+    JInt f{&L.T, &c, 0, 0, 0, 0};
+    Qags q;
+    unsigned long long nqk = 0, novf = 0;
+    while (true) {
+        // fetch (wave-aggregated)
+        const bool need = (task == -1);
+        const unsigned long long mask = __ballot(need);
+        if (mask) {
+            const int lane = __lane_id();
+            const int leader = __ffsll((long long)mask) - 1;
+            unsigned int base = 0;
+            if (lane == leader) base = atomicAdd(ia.next, (unsigned int)__popcll(mask));
+            base = __shfl(base, leader);
+            if (need) {
+                const unsigned long long lower = (lane == 0) ? 0ull : (mask & ((~0ull) >> (64 - lane)));
+                const unsigned int t = base + (unsigned int)__popcll(lower);
+                if (t < ntask) {
+                    const unsigned int tt = ia.ntask_override >= 0 ? (unsigned int)ia.ovf_list[t] : t;
+                    task = (int)tt;
+                    f.type = tt < nu ? 0 : 1;
+                    node = f.type == 0 ? ia.list_u[tt] : ia.list_v[tt - nu];
+                    double ubar, Deltau, vturn;
+                    load_const(ia.nb, node, c, ubar, Deltau, vturn);
+                    f.ubar = ubar; f.Deltau = Deltau; f.vturn = vturn;
+                    qags_begin(q, 0, f.type == 0 ? PI : PIH, EPSABSINTEG, EPSRELINTEG, 1000);
+                } else {
+                    task = -2;
+                }
+            }
+        }
+        if (__ballot(task >= 0) == 0) break;
+        if (task >= 0) {
+            const QKOut r = qk_rule<11>(f, q.next_a, q.next_b);
+            nqk++;
+            qags_consume(q, w, ia.cap, r);
+            if (q.state == 3) {
+                if (q.overflow) {
+                    const unsigned int slot = atomicAdd(ia.ovf_cnt, 1u);
+                    ia.ovf_list[slot] = task;
+                    novf++;
+                } else {
+                    const double rtwo = 1.4142135623730951;   // sqrt(2) (uvOrb.cpp:13)
+                    if (f.type == 0)
+                        ia.nb.Jr[node] = rtwo * c.Delta * f.Deltau * q.result / PI;          // uvOrb.cpp:236
+                    else
+                        ia.nb.Jz[node] = 2 * rtwo * c.Delta * (PIH - f.vturn) * q.result / PI;  // uvOrb.cpp:254
+                }
+                task = -1;
+            }
+        }
+    }
+    if (nqk) atomicAdd(&ia.ctr->qk, nqk);
+    if (novf) atomicAdd(&ia.ctr->overflow, novf);
+}
+
+struct MomArgs {
+    const double *Phi, *Ve, *R;
+    int p0, npts, quad;
+    NodeBuf nb;
+    fj_df_params df;
+    double *out6;                       // [npts_total][6]
+    double *rho, *vrot, *sigR, *sigp, *sigz, *sigRz;   // [npts_total]
+};
+
+// df_hg (DF.cpp:65-76) and the rotation term (DF.cpp:114-119)
+__device__ inline double d_df(const fj_df_params &d, double Jr, double Jphi, double Jz) {
+    const double h = Jr + d.dphi_h * fabs(Jphi) + d.dz_h * Jz;
+    const double g = Jr + d.dphi_g * fabs(Jphi) + d.dz_g * Jz;
+    const double J03 = d.J0 * d.J0 * d.J0;
+    const double TPI3 = TPI * TPI * TPI;
+    const double even = d.mass / J03 * fmax(0., pow(1. + d.J0 / h, d.A) / (pow(1. + g / d.J0, d.B))) / TPI3;
+    if (d.chi == 0.) return even;
+    const double t = tanh(d.chi * Jphi / d.J0);
+    if (d.rot_form == 1) return (1. - d.k) * even + d.k * t * even;
+    return even + d.k * t * even;
+}
+
+// k_moments: one block per point; fixed-order reduction of the 6 sums.
+__global__ void __launch_bounds__(256) k_moments(MomArgs ma) {
+    __shared__ double red[6][256];
+    const int pl = blockIdx.x;
+    const int p = ma.p0 + pl;
+    const double R = ma.R[p], Ve = ma.Ve[p];
+    const double jacob = Ve * Ve * Ve;
+    double s[6] = {0, 0, 0, 0, 0, 0};
+    for (int node = threadIdx.x; node < NQ; node += blockDim.x) {
+        const long long gid = (long long)pl * NQ + node;
+        const int flag = ma.nb.flag[gid];
+        if (flag == 0) continue;   // H >= 0: df returns 0 (DF.cpp:84)
+        double V[3], w;
+        node_velocity(ma.quad, node, Ve, V, w);
+        const double Jphi = R * V[1];
+        const double DF = d_df(ma.df, ma.nb.Jr[gid], Jphi, ma.nb.Jz[gid]);
+        const double jd = jacob * DF;
+        s[0] += w * jd;
+        s[1] += w * (jd * V[1]);
+        s[2] += w * (jd * V[0] * V[0]);
+        s[3] += w * (jd * V[1] * V[1]);
+        s[4] += w * (jd * V[2] * V[2]);
+        s[5] += w * (jd * V[0] * V[2]);
+    }
+    for (int m = 0; m < 6; m++) red[m][threadIdx.x] = s[m];
+    __syncthreads();
+    for (int st = blockDim.x / 2; st > 0; st >>= 1) {
+        if ((int)threadIdx.x < st)
+            for (int m = 0; m < 6; m++) red[m][threadIdx.x] += red[m][threadIdx.x + st];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        double o[6];
+        for (int m = 0; m < 6; m++) { o[m] = red[m][0]; ma.out6[(long long)p * 6 + m] = o[m]; }
+        // Integ.cpp:160-161
+        ma.vrot[p] = o[1] / o[0];
+        ma.sigR[p] = sqrt(o[2] / o[0]);
+        ma.sigp[p] = sqrt(o[3] / o[0]);
+        ma.sigz[p] = sqrt(o[4] / o[0]);
+        ma.sigRz[p] = sqrt(fabs(o[5]) / o[0]);
+        ma.rho[p] = ma.quad ? 4 * o[0] : o[0];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Grid phase
+// ---------------------------------------------------------------------------
+// get_Ylm (UtilsLeg.h:200-253) with the rho[0][0] fix (potLeg.cpp:68).
+__global__ void k_project(int nr, int ng, int np_, const double *ar, const double *poly,
+                          const double *rho, const double *vrot, const double *sigR, const double *sigp,
+                          const double *sigz, const double *sigRz, double *rhL, double *vrotL, double *sigRL,
+                          double *sigpL, double *sigzL, double *sigRzL) {
+    const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= nr * np_) return;
+    const int n = idx / np_, np = idx % np_;
+    if (n == 0) {
+        if (np == 0) {
+            const double rho00 = (rho[2 * ng] - rho[1 * ng]) / (ar[2] - ar[1]) * (ar[0] - ar[1]) + rho[1 * ng];
+            rhL[0] = 2 * rho00; vrotL[0] = 2 * vrot[0]; sigRL[0] = 2 * sigR[0];
+            sigpL[0] = 2 * sigp[0]; sigzL[0] = 2 * sigz[0]; sigRzL[0] = 2 * sigRz[0];
+        } else {
+            rhL[np] = 0; vrotL[np] = 0; sigRL[np] = 0; sigpL[np] = 0; sigzL[np] = 0; sigRzL[np] = 0;
+        }
+        return;
+    }
+    double a = 0, b = 0, c = 0, d = 0, e = 0, f = 0;
+    for (int g = 0; g < ng; g++) {
+        const double wv = poly[np * ng + g];
+        const int i = n * ng + g;
+        a += wv * rho[i]; b += wv * vrot[i]; c += wv * sigR[i];
+        d += wv * sigp[i]; e += wv * sigz[i]; f += wv * sigRz[i];
+    }
+    rhL[idx] = a; vrotL[idx] = b; sigRL[idx] = c; sigpL[idx] = d; sigzL[idx] = e; sigRzL[idx] = f;
+}
+
+// Guess densities (pot/models.cpp:85-110)
+__device__ inline double d_rho_guess(int model, double mass, double r0, double q, double q2, double R, double z) {
+    const double m = sqrt(R * R + z * z / q2);
+    switch (model) {
+    case 0: return mass / (m / r0 * pow(r0 + m, 3)) / TPI;
+    case 1: {
+        const double a = sqrt(r0 * r0 + m * m);
+        return mass / FPI / (q * pow((a + r0) * a, 2) * a) * r0 * (r0 + 2 * a);
+    }
+    case 2: return mass / (m / r0 * pow(r0 + m, 2)) / FPI;
+    default: return .01 / (m / 25. * pow(25. + m, 1)) / FPI;
+    }
+}
+
+// computeGuessRhl (Potential.cpp:63-80)
+__global__ void k_guess(int nr, int ng, int np_, const double *ar, const double *ci, const double *si,
+                        const double *poly, int model, double mass, double r0, double q, double *rhl) {
+    const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= nr * np_) return;
+    const int n = idx / np_, np = idx % np_;
+    const double q2 = q * q;
+    double s = 0.;
+    for (int g = 0; g < ng; g++) s += poly[np * ng + g] * d_rho_guess(model, mass, r0, q, q2, ar[n] * si[g], ar[n] * ci[g]);
+    if (idx == 0) s = 2 * d_rho_guess(model, mass, r0, q, q2, ar[0] * si[0], ar[0] * ci[0]);
+    rhl[idx] = s;
+}
+
+// I_internal / I_external (Potential.cpp:83-101) on column np of rhl.
+struct IInt {
+    const double *ar, *rhl; int nr, np_, col, l;
+    __device__ double val(double x) const {
+        if (x <= ar[0]) return rhl[col];
+        if (x >= ar[nr - 1]) return rhl[(nr - 1) * np_ + col];
+        int top = nr - 1, bot = 0;
+        while (top - bot > 1) { int m = (top + bot) >> 1; if (x >= ar[m]) bot = m; else top = m; }
+        const double yb = rhl[bot * np_ + col], yt = rhl[top * np_ + col];
+        return yb + (x - ar[bot]) / (ar[top] - ar[bot]) * (yt - yb);
+    }
+    __device__ double operator()(double x) const { return pow(x, l + 2) * val(x); }
+};
+struct IExtT {   // QAGIU transform of I_external: x = a + (1-t)/t, (f(x)/t)/t
+    IInt in; double a;
+    __device__ double operator()(double t) const {
+        const double x = a + (1 - t) / t;
+        const int nr = in.nr;
+        double rhol;
+        if (x > in.ar[nr - 1]) rhol = fmax(0., in.rhl[(nr - 1) * in.np_ + in.col] * pow(in.ar[nr - 1] / x, 4));
+        else rhol = in.val(x);
+        const double y = rhol / pow(x, in.l - 1);
+        return (y / t) / t;
+    }
+};
+
+// computeInts + computePhil (Potential.cpp:110-200), one lane per (n, np).
+__global__ void k_poisson(int nr, int np_, const double *ar, const double *rhl, double *ws, int cap,
+                          double *philP, double *PrP, double *Pr2P, int *status) {
+    const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= nr * np_) return;
+    const int n = idx / np_, np = idx % np_;
+    const int l = 2 * np;
+    const QagsWork w = lane_work(ws, cap, idx);
+    IInt fi{ar, rhl, nr, np_, np, l};
+    const double Iint = qags_solve<11>(fi, 0., ar[n], 1e-4, 1e-3, 1000, w, cap);
+    IExtT fe{fi, ar[n]};
+    double epsabs = 1e-4, epsrel = 1e-3, Iext = 0;
+    int ier = 1, guard = 0;
+    while (ier != 0) {
+        Iext = qags_solve<8>(fe, 0.0, 1.0, epsabs, epsrel, 1000, w, cap, &ier);
+        epsabs *= 2; epsrel *= 2;
+        if (isnan(Iext) || ++guard > 200) { atomicOr(status, 1); break; }
+    }
+    const double G = 1, rn = ar[n];
+    philP[idx] = -TPI * G * (Iint / pow(rn, l + 1) + Iext * pow(rn, l));
+    PrP[idx] = -TPI * G * (-(l + 1) * Iint / pow(rn, l + 2) + l * Iext * pow(rn, l - 1));
+    Pr2P[idx] = -TPI * G * ((l + 2) * (l + 1) * Iint / pow(rn, l + 3) + (l - 1) * l * Iext * pow(rn, l - 2) -
+                            (2 * l + 1) * rhl[idx]);
+}
+
+// Potential evaluation at a batch of points (facade, tests).
+__global__ void __launch_bounds__(256) k_eval(TabArgs ta, int n, const double *R, const double *z, double *phi,
+                                              double *dR, double *dz) {
+    extern __shared__ double smem[];
+    const LdsTabs L = stage_tables(ta, smem);
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    if (dR) {
+        double a, b;
+        pot_phi_grad(L.T, R[i], z[i], &a, &b);
+        dR[i] = a; dz[i] = b;
+    }
+    phi[i] = pot_phi(L.T, R[i], z[i]);
+}
+
+// ---------------------------------------------------------------------------
+// Virial diagnostic: libs/Stats.cpp:12-41 with ev_dens (UtilsLeg.h:300-354).
+// One lane per midpoint cell; the sums run in the reference's order in one lane.
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) k_vir_terms(TabArgs ta, int ng, const double *ci, const double *wi,
+                                                   const double *rhl, const double *sigRl, const double *sigpl,
+                                                   const double *sigzl, const double *vrotl, double *terms) {
+    extern __shared__ double smem[];
+    const LdsTabs L = stage_tables(ta, smem);
+    const int nr = L.T.nr, np_ = L.T.npoly;
+    const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    const int ncell = (nr - 1) * ng;
+    if (idx >= ncell + nr) return;
+    const double *ar = L.T.ar;
+    if (idx >= ncell) {
+        // ev_dens(ar[i], 0) with the double-weight int_dens (UtilsLeg.h:285-298)
+        const int i = idx - ncell;
+        const double R = ar[i], z = 0;
+        const double rr = sqrt(R * R + z * z);
+        const double c = z / rr, c2 = c * c;
+        int top = nr - 1, bot = 0;
+        while (top - bot > 1) { int m = (top + bot) >> 1; if (rr >= ar[m]) bot = m; else top = m; }
+        const double f = (rr - ar[bot]) / (ar[top] - ar[bot]);
+        double dens = .5 * (f * rhl[top * np_] + (1 - f) * rhl[bot * np_]);
+        double pm2 = 1.0, pm1 = 1.5 * c2 - .5;
+        for (int np = 1; np < np_; np++) {
+            double pe = pm1;
+            if (np >= 2) { pe = (-pm2 * L.T.lc->ea[np] + pm1 * (c2 - L.T.lc->eb[np])) * L.T.lc->eg[np]; pm2 = pm1; pm1 = pe; }
+            const double dp = f * rhl[top * np_ + np] + (1 - f) * rhl[bot * np_ + np];
+            dens += .5 * (4 * np + 1) * dp * pe;
+        }
+        terms[5 * ncell + i] = 2 * TPI * ar[i] * ar[i] * dens;
+        return;
+    }
+    const int i = idx / ng + 1, j = idx % ng;
+    const double r = .5 * (ar[i - 1] + ar[i]), dr = ar[i] - ar[i - 1];
+    const double sij = sqrt(1 - ci[j] * ci[j]);
+    const double z = r * ci[j], R = r * sij;
+    const double rr = sqrt(R * R + z * z);
+    const double c = z / rr, c2 = c * c;
+    double dens = 0, Sigu = 0, Sigp = 0, Sigv = 0;
+    if (!(rr > ar[nr - 1])) {
+        int top = nr - 1, bot = 0;
+        while (top - bot > 1) { int m = (top + bot) >> 1; if (rr >= ar[m]) bot = m; else top = m; }
+        const float f = (rr - ar[bot]) / (ar[top] - ar[bot]);   // float weight as UtilsLeg.h:312
+        const float omf = 1 - f;
+        double pm2 = 1.0, pm1 = 1.5 * c2 - .5;
+        for (int np = 0; np < np_; np++) {
+            const int t = top * np_ + np, b = bot * np_ + np;
+            const double d0 = f * rhl[t] + omf * rhl[b];
+            const double s1 = f * sigRl[t] + omf * sigRl[b];
+            const double s2 = f * sigpl[t] + omf * sigpl[b];
+            const double s3 = f * sigzl[t] + omf * sigzl[b];
+            if (np == 0) { dens = .5 * d0; Sigu = .5 * s1; Sigp = .5 * s2; Sigv = .5 * s3; continue; }
+            double pe = pm1;
+            if (np >= 2) { pe = (-pm2 * L.T.lc->ea[np] + pm1 * (c2 - L.T.lc->eb[np])) * L.T.lc->eg[np]; pm2 = pm1; pm1 = pe; }
+            const double fac = .5 * (4 * np + 1);
+            dens += fac * d0 * pe; Sigu += fac * s1 * pe; Sigp += fac * s2 * pe; Sigv += fac * s3 * pe;
+        }
+    }
+    double dRv, dzv;
+    pot_phi_grad(L.T, R, z, &dRv, &dzv);
+    const double phi = pot_phi(L.T, R, z);
+    terms[0 * ncell + idx] = .5 * wi[j] * dens * phi * r * r * dr;
+    terms[1 * ncell + idx] = .5 * wi[j] * dens * ((r * Sigu) * (r * Sigu) + (r * Sigp) * (r * Sigp)) * dr;
+    terms[2 * ncell + idx] = .5 * wi[j] * dens * ((r * Sigv) * (r * Sigv)) * dr;
+    terms[3 * ncell + idx] = wi[j] * dens * R * dRv * r * r * dr;
+    terms[4 * ncell + idx] = wi[j] * dens * z * dzv * r * r * dr;
+}
+
+__global__ void k_vir_sum(int nr, int ng, const double *ar, const double *Pr, int np_, const double *terms,
+                          double *out) {
+    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    const int ncell = (nr - 1) * ng;
+    double pot = 0, KRR = 0, Kzz = 0, WRR = 0, Wzz = 0, sum = 0;
+    for (int k = 0; k < ncell; k++) {
+        pot += terms[k]; KRR += terms[ncell + k]; Kzz += terms[2 * ncell + k];
+        WRR -= terms[3 * ncell + k]; Wzz -= terms[4 * ncell + k];
+    }
+    KRR *= 4 * PI; Kzz *= 4 * PI; WRR *= 4 * PI; Wzz *= 4 * PI; pot *= 4 * PI;
+    for (int i = 0; i < nr; i++) sum += terms[5 * ncell + i];
+    out[0] = ar[nr - 1] * ar[nr - 1] * Pr[(nr - 1) * np_];
+    out[1] = sum; out[2] = KRR + Kzz; out[3] = pot; out[4] = pot / (KRR + Kzz);
+    out[5] = KRR; out[6] = WRR; out[7] = WRR / KRR; out[8] = Kzz; out[9] = Wzz; out[10] = Wzz / Kzz;
+}
+
+// ---------------------------------------------------------------------------
+// Focal distance table: delta/tabulate.cpp:13-38, delta/Delta.cpp:18-294,
+// include/Utils.h:217-263.  One lane per energy; RK8PD (Prince & Dormand
+// 1981) with the standard y-control step rule; orbit samples in global scratch.
+// ---------------------------------------------------------------------------
+__constant__ double c_rk_c[13] = {0.0, 1.0 / 18.0, 1.0 / 12.0, 1.0 / 8.0, 5.0 / 16.0, 3.0 / 8.0, 59.0 / 400.0,
+                                  93.0 / 200.0, 5490023248.0 / 9719169821.0, 13.0 / 20.0,
+                                  1201146811.0 / 1299019798.0, 1.0, 1.0};
+__constant__ double c_rk_a[13][12] = {
+    {0},
+    {1.0 / 18.0},
+    {1.0 / 48.0, 1.0 / 16.0},
+    {1.0 / 32.0, 0.0, 3.0 / 32.0},
+    {5.0 / 16.0, 0.0, -75.0 / 64.0, 75.0 / 64.0},
+    {3.0 / 80.0, 0.0, 0.0, 3.0 / 16.0, 3.0 / 20.0},
+    {29443841.0 / 614563906.0, 0.0, 0.0, 77736538.0 / 692538347.0, -28693883.0 / 1125000000.0,
+     23124283.0 / 1800000000.0},
+    {16016141.0 / 946692911.0, 0.0, 0.0, 61564180.0 / 158732637.0, 22789713.0 / 633445777.0,
+     545815736.0 / 2771057229.0, -180193667.0 / 1043307555.0},
+    {39632708.0 / 573591083.0, 0.0, 0.0, -433636366.0 / 683701615.0, -421739975.0 / 2616292301.0,
+     100302831.0 / 723423059.0, 790204164.0 / 839813087.0, 800635310.0 / 3783071287.0},
+    {246121993.0 / 1340847787.0, 0.0, 0.0, -37695042795.0 / 15268766246.0, -309121744.0 / 1061227803.0,
+     -12992083.0 / 490766935.0, 6005943493.0 / 2108947869.0, 393006217.0 / 1396673457.0,
+     123872331.0 / 1001029789.0},
+    {-1028468189.0 / 846180014.0, 0.0, 0.0, 8478235783.0 / 508512852.0, 1311729495.0 / 1432422823.0,
+     -10304129995.0 / 1701304382.0, -48777925059.0 / 3047939560.0, 15336726248.0 / 1032824649.0,
+     -45442868181.0 / 3398467696.0, 3065993473.0 / 597172653.0},
+    {185892177.0 / 718116043.0, 0.0, 0.0, -3185094517.0 / 667107341.0, -477755414.0 / 1098053517.0,
+     -703635378.0 / 230739211.0, 5731566787.0 / 1027545527.0, 5232866602.0 / 850066563.0,
+     -4093664535.0 / 808688257.0, 3962137247.0 / 1805957418.0, 65686358.0 / 487910083.0},
+    {403863854.0 / 491063109.0, 0.0, 0.0, -5068492393.0 / 434740067.0, -411421997.0 / 543043805.0,
+     652783627.0 / 914296604.0, 11173962825.0 / 925320556.0, -13158990841.0 / 6184727034.0,
+     3936647629.0 / 1978049680.0, -160528059.0 / 685178525.0, 248638103.0 / 1413531060.0, 0.0}};
+__constant__ double c_rk_b8[13] = {14005451.0 / 335480064.0, 0.0, 0.0, 0.0, 0.0, -59238493.0 / 1068277825.0,
+                                   181606767.0 / 758867731.0, 561292985.0 / 797845732.0,
+                                   -1041891430.0 / 1371343529.0, 760417239.0 / 1151165299.0,
+                                   118820643.0 / 751138087.0, -528747749.0 / 2220607170.0, 1.0 / 4.0};
+__constant__ double c_rk_b7[13] = {13451932.0 / 455176623.0, 0.0, 0.0, 0.0, 0.0, -808719846.0 / 976000145.0,
+                                   1757004468.0 / 5645159321.0, 656045339.0 / 265891186.0,
+                                   -3867574721.0 / 1518517206.0, 465885868.0 / 322736535.0,
+                                   53011238.0 / 667516719.0, 2.0 / 45.0, 0.0};
+
+struct DeltaLane {
+    const PotView *T;
+    double E, Lz, Rc, R0, D2, sqDR, Ri, zi, sv, cv, v0;
+    double *Rp, *zp;   // [200] orbit samples (scratch)
+    int N, status;
+};
+
+__device__ inline double dl_dR(const PotView &T, double R, double z) { double a, b; pot_phi_grad(T, R, z, &a, &b); return a; }
+
+__device__ inline double dl_vsq(const DeltaLane &d, double x) {   // Delta.cpp:28-30
+    const double t = d.Lz / x;
+    return 2 * (d.E - (pot_phi(*d.T, x, 0.) + .5 * (t * t)));
+}
+
+__device__ inline void dl_eqs(const DeltaLane &d, const double *y, double *f) {   // Delta.cpp:33-42
+    double dR, dz;
+    pot_phi_grad(*d.T, y[0], y[1], &dR, &dz);
+    f[0] = y[2];
+    f[1] = y[3];
+    f[2] = -dR + (d.Lz * d.Lz) / pow(y[0], 3.0);
+    f[3] = -dz;
+}
+
+// one GSL-style evolve_apply step (RK8PD, y-control abs 1e-10 rel 1e-9)
+__device__ inline void dl_evolve(const DeltaLane &d, double &t, double t1, double &h, double *y) {
+    const double t0 = t, dt = t1 - t0;
+    double h0 = h, y0[4], yerr[4], k[13][4], yt[4];
+    for (int i = 0; i < 4; i++) y0[i] = y[i];
+    dl_eqs(d, y, k[0]);
+    for (int tries = 0; tries < 200; tries++) {
+        int final_step;
+        if ((dt >= 0.0 && h0 > dt) || (dt < 0.0 && h0 < dt)) { h0 = dt; final_step = 1; }
+        else final_step = 0;
+        for (int s = 1; s < 13; s++) {
+            for (int i = 0; i < 4; i++) {
+                double acc = 0;
+                for (int j = 0; j < s; j++)
+                    if (c_rk_a[s][j] != 0.0) acc += c_rk_a[s][j] * k[j][i];
+                yt[i] = y0[i] + h0 * acc;
+            }
+            dl_eqs(d, yt, k[s]);
+        }
+        for (int i = 0; i < 4; i++) {
+            double s8 = 0, s7 = 0;
+            for (int j = 0; j < 13; j++) {
+                if (c_rk_b8[j] != 0.0) s8 += c_rk_b8[j] * k[j][i];
+                if (c_rk_b7[j] != 0.0) s7 += c_rk_b7[j] * k[j][i];
+            }
+            y[i] = y0[i] + h0 * s8;
+            yerr[i] = h0 * (s7 - s8);
+        }
+        t = final_step ? t1 : t0 + h0;
+        const double h_old = h0;
+        double rmax = DBL_MINV;
+        for (int i = 0; i < 4; i++) {
+            const double D0 = 1e-9 * fabs(y[i]) + 1e-10;
+            rmax = fmax(fabs(yerr[i]) / fabs(D0), rmax);
+        }
+        if (rmax > 1.1) {
+            double r = 0.9 / pow(rmax, 1.0 / 8.0);
+            if (r < 0.2) r = 0.2;
+            h0 = r * h_old;
+            if (fabs(h0) < fabs(h_old) && (t + h0) != t) continue;
+            h0 = h_old;
+        } else if (rmax < 0.5) {
+            double r = 0.9 / pow(rmax, 1.0 / 9.0);
+            if (r > 5.0) r = 5.0;
+            if (r < 1.0) r = 1.0;
+            h0 = r * h_old;
+        }
+        if (!final_step) h = h0;
+        return;
+    }
+}
+
+// Delta.cpp:53-90; cond 0: z >= 0, 1: pz >= 0
+__device__ inline int dl_orbit(DeltaLane &d, double x0, int cond) {
+    double steph = 2e-3, y[4], t = 0;
+    y[0] = x0; y[1] = 0.; y[2] = 0.; y[3] = sqrt(dl_vsq(d, x0));
+    int i = 0;
+    d.Rp[i] = y[0]; d.zp[i] = y[1];
+    while (i < 199 && (cond == 0 ? y[1] : y[3]) >= 0.) {
+        dl_evolve(d, t, 5 * x0 / sqrt(dl_vsq(d, x0)), steph, y);
+        i++;
+        d.Rp[i] = y[0]; d.zp[i] = y[1];
+    }
+    return i;
+}
+
+struct DLgetDR {   // Delta.cpp:93-104
+    DeltaLane *d;
+    __device__ double operator()(double x0) const {
+        const int N = dl_orbit(*d, x0, 0);
+        const double *R = d->Rp, *z = d->zp;
+        if (fabs(z[N - 1] - z[N]) > 0 && fabs(R[N - 1] - R[N]) != 0)
+            return x0 - (R[N] - z[N] * (R[N - 1] - R[N]) / (z[N - 1] - z[N]));
+        return x0 - R[N];
+    }
+};
+
+__device__ inline double dl_ensure(DeltaLane &d, double x0) {   // Delta.cpp:116-124
+    int count = 0;
+    while (dl_vsq(d, x0) < 0) {
+        const double dpe = dl_dR(*d.T, x0, 0) - (d.Lz * d.Lz) / pow(x0, 3);
+        if (dpe < 0) x0 *= 1.1; else x0 *= .9;
+        if (++count > 50) { d.status |= 1; return x0; }
+    }
+    return x0;
+}
+
+struct DLds2dv {   // Delta.cpp:170-172
+    const DeltaLane *d;
+    __device__ double operator()(double v) const {
+        return 2 * (d->zi * d->sqDR * sin(v) - cos(v) * (d->Ri * d->R0 + d->D2 * sin(v)));
+    }
+};
+
+struct DLds2dD2 {  // Delta.cpp:210-234 with getv 189-208
+    DeltaLane *d;
+    __device__ double operator()(double D2h) const {
+        DeltaLane &D = *d;
+        D.D2 = D2h; D.sqDR = sqrt(D.D2 + D.R0 * D.R0);
+        double sum = 0.;
+        DLds2dv g{d};
+        for (int i = 0; i < D.N; i++) {
+            D.Ri = D.Rp[i]; D.zi = D.zp[i];
+            const double fl = g(0), fh = g(PIH);
+            if (fabs(fl) < EPSABSROOTS) D.v0 = 0;
+            else if (fabs(fh) < EPSABSROOTS) D.v0 = PIH;
+            else D.v0 = brent_solve(g, 0, PIH, fl, fh);
+            D.sv = sin(D.v0); D.cv = cos(D.v0);
+            const double sv = D.sv, cv = D.cv, zi = D.zi, Ri = D.Ri, sqDR = D.sqDR, R0 = D.R0, D2 = D.D2;
+            const double dvdD2 = .5 * sv * (zi / sqDR - 2 * cv) / (cv * zi * sqDR + sv * Ri * R0 - D2 * (cv * cv - sv * sv));
+            sum += (cv - zi / sqDR) * cv + g(D.v0) * dvdD2;
+        }
+        return sum;
+    }
+};
+
+struct DLRcE {   // Utils.h:243-246
+    const PotView *T; double E;
+    __device__ double operator()(double R) const {
+        double dR, dz;
+        const double phi = pot_phi_grad(*T, R, 0, &dR, &dz);
+        (void)phi;
+        return .5 * R * dR + pot_phi(*T, R, 0) - E;
+    }
+};
+struct DLRc {    // Utils.h:218-221
+    const PotView *T; double Lzsq;
+    __device__ double operator()(double R) const { return Lzsq / pow(R, 3) - dl_dR(*T, R, 0); }
+};
+
+// Utils.h:222-241 GetRc
+__device__ inline double dl_getrc(const PotView &T, double Lzsq, double R) {
+    DLRc F{&T, Lzsq};
+    double Ri = R, Ro = R, fi = 0, fo = 0, f = F(R);
+    int guard = 0;
+    if (f > 0) { while (f > 0 && guard++ < 4000) { Ri = R; fi = f; R *= 1.2; f = F(R); } Ro = R; fo = f; }
+    else { while (f < 0 && guard++ < 4000) { Ro = R; fo = f; R *= .8; f = F(R); } Ri = R; fi = f; }
+    return brent_solve(F, Ri, Ro, fi, fo);
+}
+
+// tabulateDelta prologue (tabulate.cpp:14-25), entries 0 and 1 of the radius loop.
+__global__ void k_delta_prep(TabArgs ta, double *prep) {
+    extern __shared__ double smem[];
+    const LdsTabs L = stage_tables(ta, smem);
+    if (threadIdx.x != 0) return;
+    const PotView &T = L.T;
+    const int NR = T.nr;
+    const double Rmax = T.ar[NR - 1];
+    const double Lzmax = Rmax * 0.98 * sqrt(-2 * pot_phi(T, Rmax, 0)), Rmin = 1e-3 * Rmax;
+    const double Lzmin = Rmin * sqrt(Rmin * dl_dR(T, Rmin, 0));
+    double R = .01 * Rmax, Lz[2], Rg[2], Ec[2], Vm[2];
+    for (int i = 0; i < 2; i++) {
+        Lz[i] = Lzmin + i * (Lzmax - Lzmin) / (double)(NR - 1);
+        R = Rg[i] = dl_getrc(T, Lz[i] * Lz[i], R);
+        const double vc2 = R * dl_dR(T, R, 0);
+        Ec[i] = .5 * vc2 + pot_phi(T, R, 0);
+        Vm[i] = .98 * sqrt(-2 * Ec[i]);
+    }
+    prep[0] = Lz[1]; prep[1] = Rg[1]; prep[2] = Ec[1]; prep[3] = Vm[1];
+}
+
+// tabulate.cpp:27-35 + Delta::getBestDelta (Delta.cpp:248-289), one lane per energy.
+__global__ void k_delta(TabArgs ta, const double *prep, double *scratch, double *Eout, double *Dout, int *status) {
+    extern __shared__ double smem[];
+    const LdsTabs L = stage_tables(ta, smem);
+    const int nE = blockIdx.x * blockDim.x + threadIdx.x;
+    const int NG = L.ngrid;
+    if (nE >= NG) return;
+    const double Lz1 = prep[0], Rg1 = prep[1], Ec1 = prep[2], Vm1 = prep[3];
+    const double t = Vm1 * (nE + 1) / (double)NG;
+    const double E = Ec1 + .5 * (t * t);
+    DeltaLane d;
+    d.T = &L.T; d.E = E; d.Lz = .25 * Lz1; d.status = 0;
+    d.Rp = scratch + (size_t)nE * 400; d.zp = d.Rp + 200;
+    d.D2 = 0; d.sqDR = 0; d.Ri = 0; d.zi = 0; d.sv = 0; d.cv = 0; d.v0 = 0; d.N = 0;
+    // Delta ctor: Rc = RcE(E, R0) (Utils.h:247-263)
+    {
+        DLRcE F{&L.T, E};
+        double R0 = .8 * Rg1, f0 = F(R0);
+        int guard = 0;
+        while (f0 > 0 && guard++ < 4000) { R0 *= .8; f0 = F(R0); }
+        double R1 = 1.2 * Rg1, f1 = F(R1);
+        while (f1 < 0 && guard++ < 8000) { R1 *= 1.2; f1 = F(R1); }
+        d.Rc = brent_solve(F, R0, R1, f0, f1);
+    }
+    // getJr0 (Delta.cpp:129-164)
+    DLgetDR gdr{&d};
+    double x0 = d.Rc, x1, dx0, dx1, A = .8;
+    int k = 0;
+    double R0res;
+    x0 = dl_ensure(d, x0); dx0 = gdr(x0);
+    x1 = dl_ensure(d, x0 / A); dx1 = gdr(x1);
+    bool early = false;
+    while (dx0 * dx1 > 0) {
+        if (fabs(dx1) < fabs(dx0)) { x0 = x1; dx0 = dx1; }
+        else A = pow(A, -0.9);
+        x1 = dl_ensure(d, x0 * A); dx1 = gdr(x1);
+        k++;
+        if (k == 21) {
+            d.status |= 2;
+            R0res = fabs(dx0) < 1.e-7 ? -x0 : -1;
+            early = true;
+            break;
+        }
+    }
+    if (!early) {
+        const double low = fmin(x0, x1), high = fmax(x0, x1);
+        const double flow = (low == x0) ? dx0 : dx1, fhigh = (high == x1) ? dx1 : dx0;
+        R0res = brent_solve(gdr, low, high, x0 == x1 ? dx0 : flow, x0 == x1 ? dx1 : fhigh);
+    }
+    d.R0 = R0res;
+    d.N = dl_orbit(d, d.R0, 1);
+    DLds2dD2 F{&d};
+    double low = -0.001, high = 5, D2out = 0;
+    double Dmin = F(low), Dmax = F(high);
+    int guard = 0;
+    while (Dmin * Dmax > 0 && fabs(Dmax) > 1e-4 && fabs(Dmin) > 1e-4) {
+        if (Dmax < 0) { high *= 5; Dmax = F(high); }
+        else { low -= .1; Dmin = F(low); }
+        if (++guard > 400) { d.status |= 8; break; }
+    }
+    if (fabs(Dmin) < EPSABSROOTS) D2out = low;
+    else if (fabs(Dmax) < EPSABSROOTS) D2out = high;
+    else D2out = brent_solve(F, low, high, Dmin, Dmax);
+    Eout[nE] = E;
+    Dout[nE] = D2out;
+    if (d.status) atomicOr(status, d.status);
+}
+
+}  // namespace
+
+// ===========================================================================
+// Host side
+// ===========================================================================
+struct fj_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    int nr = 0, ntheta = 0, npoly = 0, ngrid = 0, quad = 0, prec = 0;
+    bool have_pot = false, have_delta = false, have_df = false;
+    std::vector<double> ar, ci, wi, si, poly;
+    fj_df_params df{};
+    double *d_ar = nullptr, *d_phil = nullptr, *d_Pr = nullptr, *d_Pr2 = nullptr, *d_E = nullptr, *d_D = nullptr;
+    double *d_ci = nullptr, *d_si = nullptr, *d_poly = nullptr;
+    LegCoef *d_lc = nullptr;
+    // node phase scratch
+    long long node_cap = 0;
+    double *d_nodef = nullptr, *d_Jr = nullptr, *d_Jz = nullptr;
+    int *d_flag = nullptr, *d_list_u = nullptr, *d_list_v = nullptr, *d_ovf = nullptr;
+    unsigned int *d_cnt = nullptr;   // [0] u, [1] v, [2] next, [3] ovf, [4] next2
+    Counters *d_ctr = nullptr;
+    double *d_ws = nullptr; long long ws_lanes = 0; int ws_cap = 0;
+    double *d_wsfull = nullptr; long long wsfull_lanes = 0;
+    // point buffers
+    int pt_cap = 0;
+    double *d_pR = nullptr, *d_pz = nullptr, *d_pPhi = nullptr, *d_pVe = nullptr, *d_out6 = nullptr, *d_mom = nullptr;
+    // grid buffers
+    double *d_rhl = nullptr, *d_gl = nullptr, *d_pw = nullptr, *d_pws = nullptr; int *d_status = nullptr;
+    int integ_lanes = 0;
+    fj_stats st{};
+    char err[512] = {0};
+};
+
+namespace {
+
+int set_err(fj_ctx *c, int code, const char *fmt, ...) {
+    if (c) {
+        va_list ap;
+        va_start(ap, fmt);
+        vsnprintf(c->err, sizeof c->err, fmt, ap);
+        va_end(ap);
+    }
+    return code;
+}
+
+#define HIPCHK(ctx, x)                                                                         \
+    do {                                                                                       \
+        hipError_t e_ = (x);                                                                   \
+        if (e_ != hipSuccess)                                                                  \
+            return set_err(ctx, FJ_E_HIP, "%s failed: %s (%s:%d)", #x, hipGetErrorString(e_), \
+                           __FILE__, __LINE__);                                                \
+    } while (0)
+
+template <class T>
+int dalloc(fj_ctx *c, T **p, size_t n) {
+    if (*p) { (void)hipFree(*p); *p = nullptr; }
+    if (n == 0) return 0;
+    HIPCHK(c, hipMalloc((void **)p, n * sizeof(T)));
+    return 0;
+}
+
+// include/UtilsLeg.h:18-39 gauleg (constant tables, built once per grid)
+void host_gauleg(int n, double x1, double x2, double *x, double *w) {
+    int m = (n + 1) / 2;
+    double xm = 0.5 * (x2 + x1), xl = 0.5 * (x2 - x1), pp = 0, z1;
+    for (int i = 0; i < m; i++) {
+        double z = cos(3.141592654 * (i + 0.75) / (n + 0.5));
+        do {
+            double p1 = 1, p2 = 0, p3;
+            for (int j = 0; j < n; j++) {
+                p3 = p2; p2 = p1;
+                p1 = ((2 * (j + 1) - 1) * z * p2 - j * p3) / (double)(j + 1);
+            }
+            pp = n * (z * p1 - p2) / (z * z - 1.0);
+            z1 = z;
+            z = z1 - p1 / pp;
+        } while (fabs(z - z1) > 3.0e-11);
+        x[i] = xm - xl * z;
+        x[n - i - 1] = xm + xl * z;
+        w[i] = 2 * xl / ((1 - z * z) * pp * pp);
+        w[n - i - 1] = w[i];
+    }
+}
+
+// evenlegend (UtilsLeg.h:68-79), host side for the projection weights
+void host_evenlegend(int npoly, double c, double *pol) {
+    double c2 = c * c;
+    pol[0] = 1;
+    if (npoly < 2) return;
+    pol[1] = 1.5 * c2 - .5;
+    for (int np = 2; np < npoly; np++) {
+        int l = 2 * (np - 1), l2 = 2 * l;
+        pol[np] = -pol[np - 2] * l * (l - 1) / (double)((l2 + 1) * (l2 - 1)) +
+                  pol[np - 1] * (c2 - (l2 * l + l2 - 1) / (double)((l2 - 1) * (l2 + 3)));
+        pol[np] *= (l2 + 1) * (l2 + 3) / (double)((l + 1) * (l + 2));
+    }
+}
+
+LegCoef make_legcoef(int npoly) {
+    LegCoef lc;
+    memset(&lc, 0, sizeof lc);
+    for (int np = 2; np < npoly && np < MAXPOLY; np++) {
+        int l = 2 * (np - 1), l2 = 2 * l;
+        lc.ea[np] = (double)(l * (l - 1)) / (double)((l2 + 1) * (l2 - 1));
+        lc.eb[np] = (l2 * l + l2 - 1) / (double)((l2 - 1) * (l2 + 3));
+        lc.eg[np] = (l2 + 1) * (l2 + 3) / (double)((l + 1) * (l + 2));
+    }
+    lc.la[1] = 1.0; lc.lb[1] = 0.0;
+    for (int i = 2; i < 2 * MAXPOLY; i++) {
+        lc.la[i] = (double)(2 * i - 1) / (double)i;
+        lc.lb[i] = (double)(i - 1) / (double)i;
+    }
+    return lc;
+}
+
+TabArgs tab_args(const fj_ctx *c) {
+    TabArgs t;
+    t.nr = c->nr; t.npoly = c->npoly; t.ngrid = c->ngrid;
+    t.ar = c->d_ar; t.phil = c->d_phil; t.Pr = c->d_Pr; t.E = c->d_E; t.D = c->d_D; t.lc = c->d_lc;
+    return t;
+}
+size_t tab_bytes(const fj_ctx *c) { return tab_lds_doubles(c->nr, c->npoly, c->ngrid) * sizeof(double); }
+
+int ensure_points(fj_ctx *c, int n) {
+    if (n <= c->pt_cap) return 0;
+    int r = 0;
+    if ((r = dalloc(c, &c->d_pR, n)) || (r = dalloc(c, &c->d_pz, n)) || (r = dalloc(c, &c->d_pPhi, n)) ||
+        (r = dalloc(c, &c->d_pVe, n)) || (r = dalloc(c, &c->d_out6, (size_t)n * 6)) ||
+        (r = dalloc(c, &c->d_mom, (size_t)n * 6)))
+        return r;
+    c->pt_cap = n;
+    return 0;
+}
+
+// Persistent-kernel lane count: fill the device at the kernel's occupancy.
+int integ_lanes(fj_ctx *c) {
+    if (c->integ_lanes) return c->integ_lanes;
+    hipDeviceProp_t prop;
+    HIPCHK(c, hipGetDeviceProperties(&prop, c->device));
+    int per_cu = 0;
+    HIPCHK(c, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_integrals, 256, tab_bytes(c)));
+    if (per_cu < 1) per_cu = 1;
+    c->integ_lanes = prop.multiProcessorCount * per_cu * 256;
+    return c->integ_lanes;
+}
+
+size_t lane_ws_doubles(int cap) { return 4 * (size_t)(cap + 1) + (cap + 1) / 2 + 1 + 56 + 4; }
+
+int ensure_nodes(fj_ctx *c, long long nodes) {
+    int r = 0;
+    if (nodes > c->node_cap) {
+        if ((r = dalloc(c, &c->d_nodef, (size_t)nodes * NF_COUNT)) || (r = dalloc(c, &c->d_Jr, (size_t)nodes)) ||
+            (r = dalloc(c, &c->d_Jz, (size_t)nodes)) || (r = dalloc(c, &c->d_flag, (size_t)nodes)) ||
+            (r = dalloc(c, &c->d_list_u, (size_t)nodes)) || (r = dalloc(c, &c->d_list_v, (size_t)nodes)) ||
+            (r = dalloc(c, &c->d_ovf, (size_t)2 * nodes)))
+            return r;
+        c->node_cap = nodes;
+    }
+    if (!c->d_cnt) {
+        if ((r = dalloc(c, &c->d_cnt, 8)) || (r = dalloc(c, &c->d_ctr, 1))) return r;
+    }
+    const int lanes = integ_lanes(c);
+    if (lanes < 0) return FJ_E_HIP;
+    if (lanes > c->ws_lanes || c->ws_cap != QCAP) {
+        if ((r = dalloc(c, &c->d_ws, (size_t)lanes * lane_ws_doubles(QCAP)))) return r;
+        c->ws_lanes = lanes; c->ws_cap = QCAP;
+    }
+    return 0;
+}
+
+long long chunk_points(const fj_ctx *c, int npts) {
+    // bound the node scratch to ~4M nodes (~0.7 GB)
+    long long per = (4ll << 20) / NQ;
+    if (per < 1) per = 1;
+    return npts < per ? npts : per;
+    (void)c;
+}
+
+float elapsed(hipEvent_t a, hipEvent_t b) {
+    float ms = 0;
+    (void)hipEventElapsedTime(&ms, a, b);
+    return ms;
+}
+
+// Node phase for npts points already in d_pR/d_pz; results in d_out6 and d_mom.
+int run_nodes(fj_ctx *c, int npts) {
+    if (!c->have_pot || !c->have_delta || !c->have_df)
+        return set_err(c, FJ_E_STATE, "potential, Delta table and DF must be set before the node phase");
+    if (c->prec != FJ_PREC_FP64) return set_err(c, FJ_E_ARG, "precision mode %d is not available", c->prec);
+    const long long chunk = chunk_points(c, npts);
+    int r = ensure_nodes(c, chunk * NQ);
+    if (r) return r;
+    const TabArgs ta = tab_args(c);
+    const size_t lds = tab_bytes(c);
+    hipEvent_t ev[8];
+    for (int i = 0; i < 8; i++) HIPCHK(c, hipEventCreate(&ev[i]));
+    HIPCHK(c, hipMemsetAsync(c->d_ctr, 0, sizeof(Counters), c->stream));
+    memset(&c->st, 0, sizeof c->st);
+    HIPCHK(c, hipEventRecord(ev[0], c->stream));
+    PointArgs pa{c->d_pR, c->d_pz, c->d_pPhi, c->d_pVe};
+    k_points<<<(npts + 255) / 256, 256, lds, c->stream>>>(ta, pa, npts);
+    HIPCHK(c, hipGetLastError());
+    HIPCHK(c, hipEventRecord(ev[1], c->stream));
+    c->st.ms_points = 0;
+    float t_setup = 0, t_int = 0, t_ovf = 0, t_mom = 0;
+    const int lanes = c->ws_lanes;
+    c->st.integral_lanes = lanes;
+    for (long long p0 = 0; p0 < npts; p0 += chunk) {
+        const int np = (int)((npts - p0) < chunk ? (npts - p0) : chunk);
+        const long long nn = (long long)np * NQ;
+        NodeBuf nb{c->d_nodef, nn, c->d_Jr, c->d_Jz, c->d_flag};
+        HIPCHK(c, hipMemsetAsync(c->d_cnt, 0, 8 * sizeof(unsigned int), c->stream));
+        SetupArgs sa{c->d_pR, c->d_pz, c->d_pPhi, c->d_pVe, (int)p0, np, c->quad, nb,
+                     c->d_cnt + 0, c->d_cnt + 1, c->d_list_u, c->d_list_v, c->d_ctr,
+                     nullptr, nullptr, nullptr, nullptr};
+        HIPCHK(c, hipEventRecord(ev[2], c->stream));
+        k_setup<<<(unsigned)((nn + 255) / 256), 256, lds, c->stream>>>(ta, sa);
+        HIPCHK(c, hipGetLastError());
+        HIPCHK(c, hipEventRecord(ev[3], c->stream));
+        IntegArgs ia{nb, c->d_list_u, c->d_list_v, c->d_cnt + 0, c->d_cnt + 1, c->d_cnt + 2,
+                     c->d_ovf, c->d_cnt + 3, c->d_ws, QCAP, -1, c->d_ctr};
+        k_integrals<<<lanes / 256, 256, lds, c->stream>>>(ta, ia);
+        HIPCHK(c, hipGetLastError());
+        HIPCHK(c, hipEventRecord(ev[4], c->stream));
+        // overflow pass: tasks whose interval list outgrew QCAP, redone with the full limit
+        unsigned int novf = 0;
+        HIPCHK(c, hipMemcpyAsync(&novf, c->d_cnt + 3, sizeof novf, hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        if (novf) {
+            const int fl = 256 * 64;
+            if (fl > c->wsfull_lanes) {
+                if ((r = dalloc(c, &c->d_wsfull, (size_t)fl * lane_ws_doubles(QCAP_FULL)))) return r;
+                c->wsfull_lanes = fl;
+            }
+            IntegArgs io = ia;
+            io.next = c->d_cnt + 4; io.ws = c->d_wsfull; io.cap = QCAP_FULL; io.ntask_override = (int)novf;
+            io.ovf_cnt = c->d_cnt + 5;
+            k_integrals<<<fl / 256, 256, lds, c->stream>>>(ta, io);
+            HIPCHK(c, hipGetLastError());
+        }
+        HIPCHK(c, hipEventRecord(ev[5], c->stream));
+        MomArgs ma{c->d_pPhi, c->d_pVe, c->d_pR, (int)p0, np, c->quad, nb, c->df, c->d_out6,
+                   c->d_mom, c->d_mom + npts, c->d_mom + 2 * npts, c->d_mom + 3 * npts, c->d_mom + 4 * npts,
+                   c->d_mom + 5 * npts};
+        k_moments<<<np, 256, 0, c->stream>>>(ma);
+        HIPCHK(c, hipGetLastError());
+        HIPCHK(c, hipEventRecord(ev[6], c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        t_setup += elapsed(ev[2], ev[3]);
+        t_int += elapsed(ev[3], ev[4]);
+        t_ovf += elapsed(ev[4], ev[5]);
+        t_mom += elapsed(ev[5], ev[6]);
+        unsigned int cnts[2];
+        HIPCHK(c, hipMemcpy(cnts, c->d_cnt, sizeof cnts, hipMemcpyDeviceToHost));
+        c->st.tasks += (long long)cnts[0] + cnts[1];
+        c->st.overflow_tasks += novf;
+    }
+    HIPCHK(c, hipEventRecord(ev[7], c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    Counters h;
+    HIPCHK(c, hipMemcpy(&h, c->d_ctr, sizeof h, hipMemcpyDeviceToHost));
+    c->st.ms_points = elapsed(ev[0], ev[1]);
+    c->st.ms_setup = t_setup; c->st.ms_integrals = t_int; c->st.ms_overflow = t_ovf; c->st.ms_moments = t_mom;
+    c->st.ms_total = elapsed(ev[0], ev[7]);
+    c->st.points = npts;
+    c->st.nodes = (long long)npts * NQ;
+    c->st.bound_nodes = (long long)h.bound;
+    c->st.qk_calls = (long long)h.qk;
+    c->st.brent_iters = (long long)h.brent;
+    for (int i = 0; i < 8; i++) (void)hipEventDestroy(ev[i]);
+    return 0;
+}
+
+int ensure_grid_bufs(fj_ctx *c) {
+    int r = 0;
+    const size_t nl = (size_t)c->nr * c->npoly;
+    if (!c->d_rhl) {
+        if ((r = dalloc(c, &c->d_rhl, nl)) || (r = dalloc(c, &c->d_gl, nl * 9)) || (r = dalloc(c, &c->d_status, 1)))
+            return r;
+        if ((r = dalloc(c, &c->d_pws, nl * lane_ws_doubles(QCAP_FULL)))) return r;
+    }
+    return 0;
+}
+
+int grid_points(fj_ctx *c) {
+    const int npts = c->nr * c->ntheta;
+    int r = ensure_points(c, npts);
+    if (r) return r;
+    std::vector<double> R(npts), z(npts);
+    for (int n = 0; n < c->nr; n++)
+        for (int g = 0; g < c->ntheta; g++) {
+            R[n * c->ntheta + g] = c->ar[n] * c->si[g];
+            z[n * c->ntheta + g] = c->ar[n] * c->ci[g];
+        }
+    HIPCHK(c, hipMemcpyAsync(c->d_pR, R.data(), npts * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(c->d_pz, z.data(), npts * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    return 0;
+}
+
+int run_poisson(fj_ctx *c, int mode, const double *d_rhl, double *d_phil, double *d_Pr, double *d_Pr2) {
+    if (mode != FJ_POISSON_QUADPACK) return set_err(c, FJ_E_ARG, "Poisson mode %d is not available", mode);
+    int r = ensure_grid_bufs(c);
+    if (r) return r;
+    const int n = c->nr * c->npoly;
+    hipEvent_t a, b;
+    HIPCHK(c, hipEventCreate(&a));
+    HIPCHK(c, hipEventCreate(&b));
+    HIPCHK(c, hipMemsetAsync(c->d_status, 0, sizeof(int), c->stream));
+    HIPCHK(c, hipEventRecord(a, c->stream));
+    k_poisson<<<(n + 63) / 64, 64, 0, c->stream>>>(c->nr, c->npoly, c->d_ar, d_rhl, c->d_pws, QCAP_FULL, d_phil,
+                                                   d_Pr, d_Pr2, c->d_status);
+    HIPCHK(c, hipGetLastError());
+    HIPCHK(c, hipEventRecord(b, c->stream));
+    int status = 0;
+    HIPCHK(c, hipMemcpyAsync(&status, c->d_status, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    c->st.ms_poisson = elapsed(a, b);
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    if (status) return set_err(c, FJ_E_NUMERIC, "NaN in the external Poisson integral (Potential.cpp:195)");
+    return 0;
+}
+
+
+// Actions at explicit phase-space points (d_pR, d_pz filled; velocities given).
+int run_actions(fj_ctx *c, int n, const double *vR, const double *vphi, const double *vz, const double *Dx) {
+    if (!c->have_pot || !c->have_delta) return set_err(c, FJ_E_STATE, "potential and Delta table must be set");
+    int r = ensure_nodes(c, n);
+    if (r) return r;
+    double *dv = nullptr;
+    HIPCHK(c, hipMalloc((void **)&dv, (size_t)n * 4 * sizeof(double)));
+    HIPCHK(c, hipMemcpy(dv, vR, n * sizeof(double), hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(dv + n, vphi, n * sizeof(double), hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(dv + 2 * n, vz, n * sizeof(double), hipMemcpyHostToDevice));
+    if (Dx) HIPCHK(c, hipMemcpy(dv + 3 * n, Dx, n * sizeof(double), hipMemcpyHostToDevice));
+    const TabArgs ta = tab_args(c);
+    const size_t lds = tab_bytes(c);
+    PointArgs pa{c->d_pR, c->d_pz, c->d_pPhi, c->d_pVe};
+    k_points<<<(n + 255) / 256, 256, lds, c->stream>>>(ta, pa, n);
+    NodeBuf nb{c->d_nodef, n, c->d_Jr, c->d_Jz, c->d_flag};
+    HIPCHK(c, hipMemsetAsync(c->d_cnt, 0, 8 * sizeof(unsigned int), c->stream));
+    HIPCHK(c, hipMemsetAsync(c->d_ctr, 0, sizeof(Counters), c->stream));
+    SetupArgs sa{c->d_pR, c->d_pz, c->d_pPhi, c->d_pVe, 0, n, c->quad, nb, c->d_cnt + 0, c->d_cnt + 1,
+                 c->d_list_u, c->d_list_v, c->d_ctr, dv, dv + n, dv + 2 * n, Dx ? dv + 3 * n : nullptr};
+    k_setup<<<(n + 255) / 256, 256, lds, c->stream>>>(ta, sa);
+    HIPCHK(c, hipGetLastError());
+    IntegArgs ia{nb, c->d_list_u, c->d_list_v, c->d_cnt + 0, c->d_cnt + 1, c->d_cnt + 2,
+                 c->d_ovf, c->d_cnt + 3, c->d_ws, QCAP, -1, c->d_ctr};
+    k_integrals<<<c->ws_lanes / 256, 256, lds, c->stream>>>(ta, ia);
+    HIPCHK(c, hipGetLastError());
+    unsigned int novf = 0;
+    HIPCHK(c, hipMemcpyAsync(&novf, c->d_cnt + 3, sizeof novf, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    if (novf) {
+        const int fl = 256 * 64;
+        if (fl > c->wsfull_lanes) {
+            if ((r = dalloc(c, &c->d_wsfull, (size_t)fl * lane_ws_doubles(QCAP_FULL)))) return r;
+            c->wsfull_lanes = fl;
+        }
+        IntegArgs io = ia;
+        io.next = c->d_cnt + 4; io.ws = c->d_wsfull; io.cap = QCAP_FULL; io.ntask_override = (int)novf;
+        io.ovf_cnt = c->d_cnt + 5;
+        k_integrals<<<fl / 256, 256, lds, c->stream>>>(ta, io);
+        HIPCHK(c, hipGetLastError());
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+    }
+    (void)hipFree(dv);
+    return 0;
+}
+}  // namespace
+
+// ===========================================================================
+// extern "C"
+// ===========================================================================
+extern "C" {
+
+int fj_ctx_create(int device, fj_ctx **out) {
+    if (!out) return FJ_E_ARG;
+    *out = nullptr;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return FJ_E_HIP;
+    if (device < 0 || device >= ndev) return FJ_E_ARG;
+    fj_ctx *c = new fj_ctx();
+    c->device = device;
+    if (hipSetDevice(device) != hipSuccess || hipStreamCreate(&c->stream) != hipSuccess) {
+        delete c;
+        return FJ_E_HIP;
+    }
+    *out = c;
+    return 0;
+}
+
+int fj_ctx_destroy(fj_ctx *c) {
+    if (!c) return 0;
+    (void)hipSetDevice(c->device);
+    void *ptrs[] = {c->d_ar, c->d_phil, c->d_Pr, c->d_Pr2, c->d_E, c->d_D, c->d_ci, c->d_si, c->d_poly, c->d_lc,
+                    c->d_nodef, c->d_Jr, c->d_Jz, c->d_flag, c->d_list_u, c->d_list_v, c->d_ovf, c->d_cnt,
+                    c->d_ctr, c->d_ws, c->d_wsfull, c->d_pR, c->d_pz, c->d_pPhi, c->d_pVe, c->d_out6,
+                    c->d_mom, c->d_rhl, c->d_gl, c->d_pw, c->d_pws, c->d_status};
+    for (void *p : ptrs)
+        if (p) (void)hipFree(p);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+    return 0;
+}
+
+const char *fj_last_error(const fj_ctx *c) { return c ? c->err : "null context"; }
+
+int fj_set_grid(fj_ctx *c, int nr, const double *ar, int ntheta, int npoly, int ngrid) {
+    if (!c) return FJ_E_ARG;
+    if (nr < 3 || !ar || ntheta < 1 || npoly < 1 || npoly > MAXPOLY || ngrid < 2)
+        return set_err(c, FJ_E_ARG, "bad grid sizes nr=%d ntheta=%d npoly=%d ngrid=%d", nr, ntheta, npoly, ngrid);
+    for (int i = 1; i < nr; i++)
+        if (!(ar[i] > ar[i - 1])) return set_err(c, FJ_E_ARG, "radii must increase");
+    HIPCHK(c, hipSetDevice(c->device));
+    c->nr = nr; c->ntheta = ntheta; c->npoly = npoly; c->ngrid = ngrid;
+    c->ar.assign(ar, ar + nr);
+    c->ci.assign(ntheta, 0); c->wi.assign(ntheta, 0); c->si.assign(ntheta, 0);
+    host_gauleg(ntheta, 0, 1, c->ci.data(), c->wi.data());
+    for (int i = 0; i < ntheta; i++) c->si[i] = sqrt(1 - c->ci[i] * c->ci[i]);
+    c->poly.assign((size_t)npoly * ntheta, 0);
+    std::vector<double> pol(npoly);
+    for (int g = 0; g < ntheta; g++) {
+        host_evenlegend(npoly, c->ci[g], pol.data());
+        for (int np = 0; np < npoly; np++) c->poly[np * ntheta + g] = 2 * pol[np] * c->wi[g];
+    }
+    const size_t nl = (size_t)nr * npoly;
+    int r = 0;
+    if ((r = dalloc(c, &c->d_ar, nr)) || (r = dalloc(c, &c->d_phil, nl)) || (r = dalloc(c, &c->d_Pr, nl)) ||
+        (r = dalloc(c, &c->d_Pr2, nl)) || (r = dalloc(c, &c->d_E, ngrid)) || (r = dalloc(c, &c->d_D, ngrid)) ||
+        (r = dalloc(c, &c->d_ci, ntheta)) || (r = dalloc(c, &c->d_si, ntheta)) ||
+        (r = dalloc(c, &c->d_poly, nl * 0 + (size_t)npoly * ntheta)) || (r = dalloc(c, &c->d_lc, 1)))
+        return r;
+    // grid-size dependent scratch is re-made on demand
+    (void)dalloc(c, &c->d_rhl, 0); (void)dalloc(c, &c->d_gl, 0); (void)dalloc(c, &c->d_pws, 0);
+    (void)dalloc(c, &c->d_status, 0);
+    c->integ_lanes = 0;
+    LegCoef lc = make_legcoef(npoly);
+    HIPCHK(c, hipMemcpy(c->d_ar, ar, nr * sizeof(double), hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(c->d_ci, c->ci.data(), ntheta * sizeof(double), hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(c->d_si, c->si.data(), ntheta * sizeof(double), hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(c->d_poly, c->poly.data(), c->poly.size() * sizeof(double), hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(c->d_lc, &lc, sizeof lc, hipMemcpyHostToDevice));
+    c->have_pot = false; c->have_delta = false;
+    return 0;
+}
+
+int fj_set_quad_rule(fj_ctx *c, int preset) {
+    if (!c) return FJ_E_ARG;
+    if (preset != FJ_QUAD_LOG22 && preset != FJ_QUAD_TANHSINH22) return set_err(c, FJ_E_ARG, "unknown rule %d", preset);
+    c->quad = preset;
+    return 0;
+}
+
+int fj_set_precision(fj_ctx *c, int mode) {
+    if (!c) return FJ_E_ARG;
+    if (mode != FJ_PREC_FP64 && mode != FJ_PREC_MIXED) return set_err(c, FJ_E_ARG, "unknown precision %d", mode);
+    c->prec = mode;
+    return 0;
+}
+
+int fj_set_potential(fj_ctx *c, const double *phil, const double *Pr, const double *Pr2) {
+    if (!c || !phil || !Pr || !Pr2) return FJ_E_ARG;
+    if (!c->nr) return set_err(c, FJ_E_STATE, "fj_set_grid first");
+    const size_t nb = (size_t)c->nr * c->npoly * sizeof(double);
+    HIPCHK(c, hipMemcpy(c->d_phil, phil, nb, hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(c->d_Pr, Pr, nb, hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(c->d_Pr2, Pr2, nb, hipMemcpyHostToDevice));
+    c->have_pot = true;
+    return 0;
+}
+
+int fj_get_potential(fj_ctx *c, double *phil, double *Pr, double *Pr2) {
+    if (!c || !phil || !Pr || !Pr2) return FJ_E_ARG;
+    if (!c->have_pot) return set_err(c, FJ_E_STATE, "no potential set");
+    const size_t nb = (size_t)c->nr * c->npoly * sizeof(double);
+    HIPCHK(c, hipMemcpy(phil, c->d_phil, nb, hipMemcpyDeviceToHost));
+    HIPCHK(c, hipMemcpy(Pr, c->d_Pr, nb, hipMemcpyDeviceToHost));
+    HIPCHK(c, hipMemcpy(Pr2, c->d_Pr2, nb, hipMemcpyDeviceToHost));
+    return 0;
+}
+
+int fj_set_delta_table(fj_ctx *c, const double *E, const double *D) {
+    if (!c || !E || !D) return FJ_E_ARG;
+    if (!c->nr) return set_err(c, FJ_E_STATE, "fj_set_grid first");
+    HIPCHK(c, hipMemcpy(c->d_E, E, c->ngrid * sizeof(double), hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(c->d_D, D, c->ngrid * sizeof(double), hipMemcpyHostToDevice));
+    c->have_delta = true;
+    return 0;
+}
+
+int fj_get_delta_table(fj_ctx *c, double *E, double *D) {
+    if (!c || !E || !D) return FJ_E_ARG;
+    if (!c->have_delta) return set_err(c, FJ_E_STATE, "no Delta table");
+    HIPCHK(c, hipMemcpy(E, c->d_E, c->ngrid * sizeof(double), hipMemcpyDeviceToHost));
+    HIPCHK(c, hipMemcpy(D, c->d_D, c->ngrid * sizeof(double), hipMemcpyDeviceToHost));
+    return 0;
+}
+
+int fj_set_df(fj_ctx *c, const fj_df_params *df) {
+    if (!c || !df) return FJ_E_ARG;
+    if (df->chi < 0) return set_err(c, FJ_E_ARG, "chi must be >= 0 (DF.cpp:120-122)");
+    if (!(df->J0 > 0)) return set_err(c, FJ_E_ARG, "J0 must be > 0");
+    c->df = *df;
+    c->have_df = true;
+    return 0;
+}
+
+int fj_guess_rhl(fj_ctx *c, int model, double mass, double r0, double q, double *rhl) {
+    if (!c || !rhl || model < 0 || model > 3) return FJ_E_ARG;
+    if (!c->nr) return set_err(c, FJ_E_STATE, "fj_set_grid first");
+    int r = ensure_grid_bufs(c);
+    if (r) return r;
+    const int n = c->nr * c->npoly;
+    k_guess<<<(n + 63) / 64, 64, 0, c->stream>>>(c->nr, c->ntheta, c->npoly, c->d_ar, c->d_ci, c->d_si, c->d_poly,
+                                                 model, mass, r0, q, c->d_rhl);
+    HIPCHK(c, hipGetLastError());
+    HIPCHK(c, hipMemcpyAsync(rhl, c->d_rhl, n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+int fj_poisson(fj_ctx *c, int mode, const double *rhl, double *philP, double *PrP, double *Pr2P) {
+    if (!c || !rhl || !philP || !PrP || !Pr2P) return FJ_E_ARG;
+    if (!c->nr) return set_err(c, FJ_E_STATE, "fj_set_grid first");
+    int r = ensure_grid_bufs(c);
+    if (r) return r;
+    const size_t n = (size_t)c->nr * c->npoly;
+    HIPCHK(c, hipMemcpyAsync(c->d_rhl, rhl, n * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    double *o = c->d_gl + 6 * n;
+    if ((r = run_poisson(c, mode, c->d_rhl, o, o + n, o + 2 * n))) return r;
+    HIPCHK(c, hipMemcpy(philP, o, n * sizeof(double), hipMemcpyDeviceToHost));
+    HIPCHK(c, hipMemcpy(PrP, o + n, n * sizeof(double), hipMemcpyDeviceToHost));
+    HIPCHK(c, hipMemcpy(Pr2P, o + 2 * n, n * sizeof(double), hipMemcpyDeviceToHost));
+    return 0;
+}
+
+int fj_moments(fj_ctx *c, double *rho, double *vrot, double *sigR, double *sigp, double *sigz, double *sigRz) {
+    if (!c || !rho || !vrot || !sigR || !sigp || !sigz || !sigRz) return FJ_E_ARG;
+    if (!c->nr) return set_err(c, FJ_E_STATE, "fj_set_grid first");
+    int r = grid_points(c);
+    if (r) return r;
+    const int npts = c->nr * c->ntheta;
+    if ((r = run_nodes(c, npts))) return r;
+    double *outs[6] = {rho, vrot, sigR, sigp, sigz, sigRz};
+    for (int m = 0; m < 6; m++)
+        HIPCHK(c, hipMemcpy(outs[m], c->d_mom + (size_t)m * npts, npts * sizeof(double), hipMemcpyDeviceToHost));
+    return 0;
+}
+
+int fj_rhofDF(fj_ctx *c, int n, const double *R, const double *z, double *out6, double *rho, double *vrot,
+              double *sigR, double *sigp, double *sigz, double *sigRz) {
+    if (!c || n < 1 || !R || !z) return FJ_E_ARG;
+    int r = ensure_points(c, n);
+    if (r) return r;
+    HIPCHK(c, hipMemcpy(c->d_pR, R, n * sizeof(double), hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(c->d_pz, z, n * sizeof(double), hipMemcpyHostToDevice));
+    if ((r = run_nodes(c, n))) return r;
+    if (out6) HIPCHK(c, hipMemcpy(out6, c->d_out6, (size_t)n * 6 * sizeof(double), hipMemcpyDeviceToHost));
+    double *outs[6] = {rho, vrot, sigR, sigp, sigz, sigRz};
+    for (int m = 0; m < 6; m++)
+        if (outs[m]) HIPCHK(c, hipMemcpy(outs[m], c->d_mom + (size_t)m * n, n * sizeof(double), hipMemcpyDeviceToHost));
+    return 0;
+}
+
+int fj_project_legendre(fj_ctx *c, const double *rho, const double *vrot, const double *sigR, const double *sigp,
+                        const double *sigz, const double *sigRz, double *rhl, double *vrotl, double *sigRl,
+                        double *sigpl, double *sigzl, double *sigRzl) {
+    if (!c || !rho || !vrot || !sigR || !sigp || !sigz || !sigRz || !rhl || !vrotl || !sigRl || !sigpl || !sigzl ||
+        !sigRzl)
+        return FJ_E_ARG;
+    if (!c->nr) return set_err(c, FJ_E_STATE, "fj_set_grid first");
+    const int npts = c->nr * c->ntheta;
+    int r = ensure_points(c, npts);
+    if (r || (r = ensure_grid_bufs(c))) return r;
+    const double *in[6] = {rho, vrot, sigR, sigp, sigz, sigRz};
+    for (int m = 0; m < 6; m++)
+        HIPCHK(c, hipMemcpy(c->d_mom + (size_t)m * npts, in[m], npts * sizeof(double), hipMemcpyHostToDevice));
+    const size_t nl = (size_t)c->nr * c->npoly;
+    double *g = c->d_gl;
+    k_project<<<(unsigned)((nl + 63) / 64), 64, 0, c->stream>>>(
+        c->nr, c->ntheta, c->npoly, c->d_ar, c->d_poly, c->d_mom, c->d_mom + npts, c->d_mom + 2 * npts,
+        c->d_mom + 3 * npts, c->d_mom + 4 * npts, c->d_mom + 5 * npts, g, g + nl, g + 2 * nl, g + 3 * nl, g + 4 * nl,
+        g + 5 * nl);
+    HIPCHK(c, hipGetLastError());
+    double *outs[6] = {rhl, vrotl, sigRl, sigpl, sigzl, sigRzl};
+    for (int m = 0; m < 6; m++) HIPCHK(c, hipMemcpy(outs[m], g + m * nl, nl * sizeof(double), hipMemcpyDeviceToHost));
+    return 0;
+}
+
+int fj_compute_new_phi(fj_ctx *c, int mode, double *rhl, double *sigRl, double *sigpl, double *sigzl,
+                       double *sigRzl, double *vrotl, double *philP, double *PrP, double *Pr2P) {
+    if (!c || !rhl || !sigRl || !sigpl || !sigzl || !sigRzl || !vrotl || !philP || !PrP || !Pr2P) return FJ_E_ARG;
+    if (!c->nr) return set_err(c, FJ_E_STATE, "fj_set_grid first");
+    int r = grid_points(c);
+    if (r || (r = ensure_grid_bufs(c))) return r;
+    const int npts = c->nr * c->ntheta;
+    if ((r = run_nodes(c, npts))) return r;
+    const size_t nl = (size_t)c->nr * c->npoly;
+    double *g = c->d_gl;   // rhl, vrotl, sigRl, sigpl, sigzl, sigRzl, phil, Pr, Pr2
+    hipEvent_t a, b;
+    HIPCHK(c, hipEventCreate(&a));
+    HIPCHK(c, hipEventCreate(&b));
+    HIPCHK(c, hipEventRecord(a, c->stream));
+    k_project<<<(unsigned)((nl + 63) / 64), 64, 0, c->stream>>>(
+        c->nr, c->ntheta, c->npoly, c->d_ar, c->d_poly, c->d_mom, c->d_mom + npts, c->d_mom + 2 * npts,
+        c->d_mom + 3 * npts, c->d_mom + 4 * npts, c->d_mom + 5 * npts, g, g + nl, g + 2 * nl, g + 3 * nl, g + 4 * nl,
+        g + 5 * nl);
+    HIPCHK(c, hipGetLastError());
+    HIPCHK(c, hipEventRecord(b, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    const float tproj = elapsed(a, b);
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    if ((r = run_poisson(c, mode, g, g + 6 * nl, g + 7 * nl, g + 8 * nl))) return r;
+    c->st.ms_project = tproj;
+    double *outs[9] = {rhl, vrotl, sigRl, sigpl, sigzl, sigRzl, philP, PrP, Pr2P};
+    for (int m = 0; m < 9; m++) HIPCHK(c, hipMemcpy(outs[m], g + m * nl, nl * sizeof(double), hipMemcpyDeviceToHost));
+    return 0;
+}
+
+int fj_eval_potential(fj_ctx *c, int n, const double *R, const double *z, double *phi, double *dR, double *dz) {
+    if (!c || n < 1 || !R || !z || !phi || (dR && !dz) || (!dR && dz)) return FJ_E_ARG;
+    if (!c->have_pot) return set_err(c, FJ_E_STATE, "no potential set");
+    double *d = nullptr;
+    HIPCHK(c, hipMalloc((void **)&d, (size_t)n * 5 * sizeof(double)));
+    HIPCHK(c, hipMemcpy(d, R, n * sizeof(double), hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(d + n, z, n * sizeof(double), hipMemcpyHostToDevice));
+    const TabArgs ta = tab_args(c);
+    k_eval<<<(n + 255) / 256, 256, tab_bytes(c), c->stream>>>(ta, n, d, d + n, d + 2 * n, dR ? d + 3 * n : nullptr,
+                                                              dR ? d + 4 * n : nullptr);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    if (e == hipSuccess) e = hipMemcpy(phi, d + 2 * n, n * sizeof(double), hipMemcpyDeviceToHost);
+    if (e == hipSuccess && dR) e = hipMemcpy(dR, d + 3 * n, n * sizeof(double), hipMemcpyDeviceToHost);
+    if (e == hipSuccess && dR) e = hipMemcpy(dz, d + 4 * n, n * sizeof(double), hipMemcpyDeviceToHost);
+    (void)hipFree(d);
+    if (e != hipSuccess) return set_err(c, FJ_E_HIP, "fj_eval_potential: %s", hipGetErrorString(e));
+    return 0;
+}
+
+
+int fj_actions(fj_ctx *c, int n, const double *R, const double *z, const double *vR, const double *vphi,
+               const double *vz, const double *Delta, double *Jr, double *Jz) {
+    if (!c || n < 1 || !R || !z || !vR || !vphi || !vz || !Jr || !Jz) return FJ_E_ARG;
+    int r = ensure_points(c, n);
+    if (r) return r;
+    HIPCHK(c, hipMemcpy(c->d_pR, R, n * sizeof(double), hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(c->d_pz, z, n * sizeof(double), hipMemcpyHostToDevice));
+    if ((r = run_actions(c, n, vR, vphi, vz, Delta))) return r;
+    HIPCHK(c, hipMemcpy(Jr, c->d_Jr, n * sizeof(double), hipMemcpyDeviceToHost));
+    HIPCHK(c, hipMemcpy(Jz, c->d_Jz, n * sizeof(double), hipMemcpyDeviceToHost));
+    return 0;
+}
+
+int fj_virial(fj_ctx *c, const double *rhl, const double *vrotl, const double *sigRl, const double *sigpl,
+              const double *sigzl, double *out11) {
+    if (!c || !rhl || !vrotl || !sigRl || !sigpl || !sigzl || !out11) return FJ_E_ARG;
+    if (!c->have_pot) return set_err(c, FJ_E_STATE, "no potential set");
+    const size_t nl = (size_t)c->nr * c->npoly;
+    const int ncell = (c->nr - 1) * c->ntheta;
+    double *d = nullptr;
+    const size_t tot = 5 * nl + 6 * (size_t)(ncell + c->nr) + 16 + c->ntheta;
+    HIPCHK(c, hipMalloc((void **)&d, tot * sizeof(double)));
+    const double *in[5] = {rhl, vrotl, sigRl, sigpl, sigzl};
+    for (int m = 0; m < 5; m++) HIPCHK(c, hipMemcpy(d + m * nl, in[m], nl * sizeof(double), hipMemcpyHostToDevice));
+    double *terms = d + 5 * nl, *out = terms + 6 * (size_t)(ncell + c->nr), *wi = out + 16;
+    HIPCHK(c, hipMemcpy(wi, c->wi.data(), c->ntheta * sizeof(double), hipMemcpyHostToDevice));
+    const TabArgs ta = tab_args(c);
+    const int nth = ncell + c->nr;
+    k_vir_terms<<<(nth + 255) / 256, 256, tab_bytes(c), c->stream>>>(ta, c->ntheta, c->d_ci, wi, d, d + 2 * nl,
+                                                                    d + 3 * nl, d + 4 * nl, d + nl, terms);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) {
+        k_vir_sum<<<1, 1, 0, c->stream>>>(c->nr, c->ntheta, c->d_ar, c->d_Pr, c->npoly, terms, out);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    if (e == hipSuccess) e = hipMemcpy(out11, out, 11 * sizeof(double), hipMemcpyDeviceToHost);
+    (void)hipFree(d);
+    if (e != hipSuccess) return set_err(c, FJ_E_HIP, "fj_virial: %s", hipGetErrorString(e));
+    return 0;
+}
+
+int fj_tabulate_delta(fj_ctx *c, double *Egrid, double *Dgrid) {
+    if (!c) return FJ_E_ARG;
+    if (!c->have_pot) return set_err(c, FJ_E_STATE, "no potential set");
+    double *d = nullptr;
+    int *dst = nullptr;
+    const int NG = c->ngrid;
+    HIPCHK(c, hipMalloc((void **)&d, (8 + (size_t)NG * 402) * sizeof(double)));
+    HIPCHK(c, hipMalloc((void **)&dst, sizeof(int)));
+    HIPCHK(c, hipMemsetAsync(dst, 0, sizeof(int), c->stream));
+    const TabArgs ta = tab_args(c);
+    double *prep = d, *E = d + 8, *D = E + NG, *scr = D + NG;
+    k_delta_prep<<<1, 64, tab_bytes(c), c->stream>>>(ta, prep);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) {
+        k_delta<<<(NG + 63) / 64, 64, tab_bytes(c), c->stream>>>(ta, prep, scr, E, D, dst);
+        e = hipGetLastError();
+    }
+    int status = 0;
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    if (e == hipSuccess) e = hipMemcpy(&status, dst, sizeof(int), hipMemcpyDeviceToHost);
+    if (e == hipSuccess) e = hipMemcpy(c->d_E, E, NG * sizeof(double), hipMemcpyDeviceToDevice);
+    if (e == hipSuccess) e = hipMemcpy(c->d_D, D, NG * sizeof(double), hipMemcpyDeviceToDevice);
+    if (e == hipSuccess && Egrid) e = hipMemcpy(Egrid, E, NG * sizeof(double), hipMemcpyDeviceToHost);
+    if (e == hipSuccess && Dgrid) e = hipMemcpy(Dgrid, D, NG * sizeof(double), hipMemcpyDeviceToHost);
+    (void)hipFree(d);
+    (void)hipFree(dst);
+    if (e != hipSuccess) return set_err(c, FJ_E_HIP, "fj_tabulate_delta: %s", hipGetErrorString(e));
+    c->have_delta = true;
+    if (status & 1) return set_err(c, FJ_E_DELTA, "ensureVsq gave up after 50 tries (Delta.cpp:121)");
+    if (status & 2) return set_err(c, FJ_E_DELTA, "no Jr=0 orbit bracket after 21 tries (Delta.cpp:145-151)");
+    if (status & 8) return set_err(c, FJ_E_DELTA, "no Delta^2 bracket");
+    return 0;
+}
+
+int fj_get_stats(const fj_ctx *c, fj_stats *out) {
+    if (!c || !out) return FJ_E_ARG;
+    *out = c->st;
+    return 0;
+}
+
+}  // extern "C"

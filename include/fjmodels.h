@@ -1,0 +1,159 @@
+/*
+ * fjmodels.h -- C ABI of the MI355X (gfx950) implementation of the fJmodels
+ * self-consistent-field iteration.
+ *
+ * The reference has no plugin or FFI layer: its boundary is the C++
+ * free-function / class surface called by the driver (fJmodels.cpp) and the
+ * per-iteration models/<name>_<iter>.out files.  Each entry point below names the
+ * reference interface it replaces.  All arrays are caller-owned, contiguous,
+ * row-major doubles in host memory ([nr][npoly] for Legendre coefficient
+ * tables, [nr][ntheta] for grid values); the library keeps its own device
+ * copies.  Every function returns 0 on success and a nonzero FJ_E* code
+ * otherwise (the reference prints and exit()s); fj_last_error() gives the text.
+ * One context per host thread; calls on a context are stream-ordered.
+ */
+#ifndef FJMODELS_H
+#define FJMODELS_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct fj_ctx fj_ctx;
+
+enum {
+    FJ_OK = 0,
+    FJ_E_ARG = 1,        /* bad argument / sizes                          */
+    FJ_E_HIP = 2,        /* HIP runtime error                             */
+    FJ_E_STATE = 3,      /* call made before the data it needs was set    */
+    FJ_E_NUMERIC = 4,    /* NaN in the Poisson external integral
+                            (pot/Potential.cpp:195-197 exits with 1)      */
+    FJ_E_DELTA = 5       /* Delta tabulation hit a reference exit/failure
+                            path (delta/Delta.cpp:121,145-151)            */
+};
+
+/* Quadrature presets for the velocity integral (include/GaussPts.h:12-136). */
+enum { FJ_QUAD_LOG22 = 0,        /* xpt11/xw11 on all axes: Int3D_111111_vec */
+       FJ_QUAD_TANHSINH22 = 1 }; /* xpt01 x xpt11 x xpt01: Int3D_011101_vec  */
+
+/* Guess densities (pot/models.cpp:85-110, Potential::selectGuessRho). */
+enum { FJ_GUESS_HERNQUIST = 0, FJ_GUESS_ISOCHRONE = 1, FJ_GUESS_NFW = 2, FJ_GUESS_NFWEXT = 3 };
+
+/* Poisson radial integrals: QUADPACK QAGS/QAGIU as the reference
+ * (Potential.cpp:161-206), or exact piecewise-linear integration (block scan). */
+enum { FJ_POISSON_QUADPACK = 0, FJ_POISSON_BLOCKSCAN = 1 };
+
+/* Arithmetic of the node phase (DF evaluation and velocity quadrature). */
+enum { FJ_PREC_FP64 = 0, FJ_PREC_MIXED = 1 };
+
+/* DF parameters: df/DF.cpp:18-20,36-63 (alpha[2], dphi_h, dz_h, dphi_g, dz_g)
+ * and pot/models.cpp:13 (mass, J0, chi).  rot_form 0 is the code's
+ * f(1 + k tanh(chi Jphi/J0)) (DF.cpp:117, k = 1); 1 is the documented
+ * (1-k) f + k tanh(chi Jphi/J0) f (DF.cpp:116, README.md:80-83, k = 0.5). */
+typedef struct {
+    double A, B;
+    double dphi_h, dz_h, dphi_g, dz_g;
+    double mass, J0, chi;
+    double k;
+    int rot_form;
+} fj_df_params;
+
+/* Per-call statistics of the last node-phase call (fj_moments / fj_rhofDF /
+ * fj_compute_new_phi) and of the last Poisson solve. */
+typedef struct {
+    double ms_points, ms_setup, ms_integrals, ms_overflow, ms_moments, ms_project, ms_poisson, ms_total;
+    long long points, nodes, bound_nodes, tasks, qk_calls, overflow_tasks, brent_iters;
+    int integral_lanes, setup_lanes;
+} fj_stats;
+
+/* Context on HIP device `device` (no reference counterpart: the reference
+ * keeps this state in globals, include/Grid.h:19-24). */
+int fj_ctx_create(int device, fj_ctx **out);
+int fj_ctx_destroy(fj_ctx *ctx);
+const char *fj_last_error(const fj_ctx *ctx);
+
+/* Grid sizes and radii: replaces the compile-time NR/NGAUSS/NPOLY/NGRID of
+ * include/Grid.h:11-14 and the ar[] fill of SetGrid (include/Utils.h:96-108).
+ * The theta nodes are gauleg(0,1) with ntheta points (UtilsLeg.h:18-39). */
+int fj_set_grid(fj_ctx *ctx, int nr, const double *ar, int ntheta, int npoly, int ngrid);
+
+/* Velocity quadrature rule (include/GaussPts.h:12; Integ.cpp:157-158). */
+int fj_set_quad_rule(fj_ctx *ctx, int preset);
+
+/* Node-phase arithmetic (FJ_PREC_FP64 default). */
+int fj_set_precision(fj_ctx *ctx, int mode);
+
+/* Total potential multipole tables phil, Pr, Pr2 [nr][npoly] that every
+ * potential evaluation reads (Grid.h:20 globals; Potential::computePhil
+ * copies at Potential.cpp:182-188, updatePhil potLeg.cpp:21-29). */
+int fj_set_potential(fj_ctx *ctx, const double *phil, const double *Pr, const double *Pr2);
+int fj_get_potential(fj_ctx *ctx, double *phil, double *Pr, double *Pr2);
+
+/* Delta^2(E) table Egrid/Dgrid [ngrid] read by Deltafn (delta/tabulate.cpp:40-44). */
+int fj_set_delta_table(fj_ctx *ctx, const double *Egrid, const double *Dgrid);
+int fj_get_delta_table(fj_ctx *ctx, double *Egrid, double *Dgrid);
+
+/* setDF (df/DF.cpp:22-63): DF parameters of the component being iterated. */
+int fj_set_df(fj_ctx *ctx, const fj_df_params *df);
+
+/* Potential::computeGuessRhl (Potential.cpp:63-80) for the guess density
+ * `model` with (mass, r0, q) (SetModel, models.cpp:16-25) -> rhl [nr][npoly]. */
+int fj_guess_rhl(fj_ctx *ctx, int model, double mass, double r0, double q, double *rhl);
+
+/* Potential::computePhil (Potential.cpp:110-200): rhl [nr][npoly] ->
+ * philP, PrP, Pr2P [nr][npoly].  Does not change the context's potential. */
+int fj_poisson(fj_ctx *ctx, int mode, const double *rhl, double *philP, double *PrP, double *Pr2P);
+
+/* The grid loop of computeRhl (potLeg.cpp:54-58): rhofDF (8-argument,
+ * Integ.cpp:150-163) at every (ar[n] sin(theta_g), ar[n] cos(theta_g)).
+ * Outputs [nr][ntheta]; rho before the rho[0][0] extrapolation. */
+int fj_moments(fj_ctx *ctx, double *rho, double *vrot, double *sigR, double *sigp,
+               double *sigz, double *sigRz);
+
+/* rhofDF (Integ.cpp:150-163) at n arbitrary points; out6 [n][6] are the six
+ * raw quadrature sums (GaussQuad.cpp:160-180), rho/vrot/... [n]. */
+int fj_rhofDF(fj_ctx *ctx, int n, const double *R, const double *z, double *out6,
+              double *rho, double *vrot, double *sigR, double *sigp, double *sigz, double *sigRz);
+
+/* The rho[0][0] fix (potLeg.cpp:68) and get_Ylm (UtilsLeg.h:200-253):
+ * grid moments [nr][ntheta] -> Legendre coefficients [nr][npoly]. */
+int fj_project_legendre(fj_ctx *ctx, const double *rho, const double *vrot, const double *sigR,
+                        const double *sigp, const double *sigz, const double *sigRz,
+                        double *rhl, double *vrotl, double *sigRl, double *sigpl,
+                        double *sigzl, double *sigRzl);
+
+/* computeNewPhi (potLeg.cpp:76-82): moments + projection + Poisson, device
+ * resident.  Output tables [nr][npoly]; the context's potential is not
+ * changed (the caller installs philP/PrP/Pr2P, or their sum over
+ * components, with fj_set_potential as the reference driver does). */
+int fj_compute_new_phi(fj_ctx *ctx, int poisson_mode, double *rhl, double *sigRl, double *sigpl,
+                       double *sigzl, double *sigRzl, double *vrotl,
+                       double *philP, double *PrP, double *Pr2P);
+
+/* tabulateDelta (delta/tabulate.cpp:13-38) on the context's potential;
+ * fills the context's table and copies it to Egrid/Dgrid [ngrid]. */
+int fj_tabulate_delta(fj_ctx *ctx, double *Egrid, double *Dgrid);
+
+/* vir2 (libs/Stats.cpp:12-41) with ev_dens (UtilsLeg.h:300-354):
+ * out11 = {r_max^2 Pr[nr-1][0], mass sum, KE, PE, W/K, KRR, WRR, WRR/KRR,
+ *          Kzz, Wzz, Wzz/Kzz}. */
+int fj_virial(fj_ctx *ctx, const double *rhl, const double *vrotl, const double *sigRl,
+              const double *sigpl, const double *sigzl, double *out11);
+
+/* Potential::operator(), dR, dz (Potential.cpp:206-276) at n points;
+ * dR/dz may be NULL. */
+int fj_eval_potential(fj_ctx *ctx, int n, const double *R, const double *z,
+                      double *phi, double *dR, double *dz);
+
+/* uvOrb(Delta, Lz = R vphi, Phi(R,z), x, p, pot).Ju()/Jv() (uvOrb.cpp:15-262)
+ * at n phase-space points; Delta[i] <= 0 selects Deltafn(H) as df() does
+ * (DF.cpp:83-89).  Unbound points give 1e31 (uvOrb.cpp:38-40). */
+int fj_actions(fj_ctx *ctx, int n, const double *R, const double *z, const double *vR,
+               const double *vphi, const double *vz, const double *Delta, double *Jr, double *Jz);
+
+int fj_get_stats(const fj_ctx *ctx, fj_stats *out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,117 @@
+/*
+ * fjoracle.h -- CPU restatement of the fJmodels self-consistent-field path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the checker for the HIP
+ * product path: only tests/, __graft_entry__.smoke() and bench.py's
+ * cpu_baseline leg load it.  Nothing in fjmodels_amd/ links or calls it.
+ *
+ * It follows the reference algorithm step for step (same tolerances,
+ * brackets, loop orders and quirks), with the grid sizes that the reference
+ * fixes at compile time (include/Grid.h:11-14) turned into run-time values
+ * and the double-power-law f(J) (df/DF.cpp:108-123) made live.
+ * The third-party arithmetic the reference takes from GSL (Brent roots,
+ * QAGS/QAGIU, RK8PD with the standard step control) is restated from the
+ * published algorithms: Brent (1973, ch. 4) in GSL's iterate form,
+ * QUADPACK dqagse/dqelg/dqk21/dqk15 (Piessens et al. 1983), the GSL QAGIU
+ * map x = a + (1-t)/t, and Prince & Dormand (1981) RK8(7)13M.
+ */
+#ifndef FJORACLE_H
+#define FJORACLE_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Potential tables and grid (the reference's globals in include/Grid.h). */
+typedef struct fjo_pot fjo_pot;
+
+/* DF parameters (df/DF.cpp:18-20, models.cpp:13). */
+typedef struct {
+    double A, B;                        /* alpha[0], alpha[1]             */
+    double dphi_h, dz_h, dphi_g, dz_g;  /* h(J), g(J) coefficients        */
+    double mass, J0, chi;
+    double k;                           /* rotation amplitude             */
+    int rot_form;                       /* 0: f(1+k tanh), 1: (1-k)f+k tanh f */
+} fjo_df;
+
+/* ---- grid / tables ---------------------------------------------------- */
+fjo_pot *fjo_pot_new(int nr, const double *ar, int npoly, int ngauss, int ngrid);
+void fjo_pot_free(fjo_pot *p);
+void fjo_pot_set_tables(fjo_pot *p, const double *phil, const double *Pr, const double *Pr2);
+void fjo_pot_get_tables(const fjo_pot *p, double *phil, double *Pr, double *Pr2);
+void fjo_pot_set_delta(fjo_pot *p, const double *Egrid, const double *Dgrid);
+void fjo_pot_get_delta(const fjo_pot *p, double *Egrid, double *Dgrid);
+void fjo_pot_set_quad(fjo_pot *p, int preset);   /* 0 = log22 (xpt11), 1 = tanh-sinh22 */
+void fjo_log_grid(int n, double rmin, double rmax, double *ar);
+
+/* ---- numerics ----------------------------------------------------------- */
+typedef double (*fjo_fn)(double x, void *params);
+void fjo_gauleg(int n, double x1, double x2, double *x, double *w);
+void fjo_evenlegend(int npoly, double c, double *pol);
+void fjo_legend(double *allpol, double c, int npoly);
+double fjo_dlegend(double c, int n);
+double fjo_linterp(const double *xm, const double *ym, int np, double x);
+double fjo_roots(fjo_fn F, void *params, double low, double high, int *iters);
+void fjo_qk21(fjo_fn f, void *params, double a, double b,
+              double *result, double *abserr, double *resabs, double *resasc);
+void fjo_qk15(fjo_fn f, void *params, double a, double b,
+              double *result, double *abserr, double *resabs, double *resasc);
+int fjo_qags(fjo_fn f, void *params, double a, double b, double epsabs, double epsrel,
+             int limit, double *result, double *abserr, int *neval, int *last);
+int fjo_qagiu(fjo_fn f, void *params, double a, double epsabs, double epsrel,
+              int limit, double *result, double *abserr, int *neval, int *last);
+typedef void (*fjo_ode_fn)(double t, const double *y, double *f, void *params);
+/* One GSL-style evolve step of RK8PD with y-control (eps_abs, eps_rel).  */
+int fjo_evolve_apply(fjo_ode_fn fn, void *params, int dim, double eps_abs, double eps_rel,
+                     double *t, double t1, double *h, double *y);
+void fjo_rk8pd_step(fjo_ode_fn fn, void *params, int dim, double t, double h,
+                    double *y, double *yerr, const double *dydt_in);
+
+/* ---- potential ---------------------------------------------------------- */
+double fjo_phi(const fjo_pot *p, double R, double z);
+double fjo_dR(const fjo_pot *p, double R, double z);
+double fjo_dz(const fjo_pot *p, double R, double z);
+void fjo_guess_rhl(const fjo_pot *p, int model, double mass, double r0, double q, double *rhl);
+int fjo_compute_phil(const fjo_pot *p, const double *rhl, double *philP, double *PrP, double *Pr2P);
+void fjo_get_ylm(const fjo_pot *p, double *rho, const double *vrot, const double *sigR,
+                 const double *sigp, const double *sigz, const double *sigRz,
+                 double *rhl, double *vrotl, double *sigRl, double *sigpl,
+                 double *sigzl, double *sigRzl);
+
+/* ---- focal distance ------------------------------------------------------ */
+double fjo_deltafn(const fjo_pot *p, double E);
+int fjo_tabulate_delta(fjo_pot *p);
+double fjo_best_delta(const fjo_pot *p, double E, double Lz, double R0, int *status);
+
+/* ---- actions / DF / moments ---------------------------------------------- */
+/* Stackel-fudge actions at one phase-space point; returns 0 if bound. */
+int fjo_actions(const fjo_pot *p, double Delta, double R, double z, double Phi0,
+                double vR, double vphi, double vz, double *Jr, double *Jz);
+double fjo_df_eval(const fjo_pot *p, const fjo_df *df, const double *x, const double *v);
+double fjo_df_hg(const fjo_df *df, double Jr, double Jphi, double Jz);
+double fjo_rhofDF(const fjo_pot *p, const fjo_df *df, double R, double z, double *out6,
+                  double *vrot, double *sigR, double *sigp, double *sigz, double *sigRz);
+/* Grid moments at every (ar[n]*si, ar[n]*ci) point; arrays are [nr][ngauss]. */
+void fjo_compute_rhl_grid(const fjo_pot *p, const fjo_df *df, int nthreads,
+                          double *rho, double *vrot, double *sigR, double *sigp,
+                          double *sigz, double *sigRz);
+int fjo_compute_new_phi(fjo_pot *p, const fjo_df *df, int nthreads,
+                        double *rhl, double *vrotl, double *sigRl, double *sigpl,
+                        double *sigzl, double *sigRzl,
+                        double *philP, double *PrP, double *Pr2P);
+/* Virial diagnostic (libs/Stats.cpp:12-41); out = {mass_r2Pr, mass_sum, KE, PE,
+ * W/K, KRR, WRR, WRR/KRR, Kzz, Wzz, Wzz/Kzz}. */
+void fjo_vir2(const fjo_pot *p, const double *rhl, const double *vrotl, const double *sigRl,
+              const double *sigpl, const double *sigzl, double *out11);
+
+/* Node-work counters of the last fjo_rhofDF/fjo_actions calls on this thread. */
+typedef struct {
+    long long nodes, bound_nodes, phi_evals, grad_evals, qk21_calls, brent_iters;
+} fjo_counters;
+void fjo_counters_get(fjo_counters *c);
+void fjo_counters_reset(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
