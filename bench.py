@@ -1,0 +1,129 @@
+"""Benchmark: grid-point DF moments per second per SCF iteration (plain FP64).
+
+Workload (one GPU): single-component Hernquist double-power-law f(J)
+(A = 1.667, B = 5; h: dphi = dz = 0.55; g: dphi = dz = 1.0; chi = 0; M = r0 = 1),
+40 log-spaced radii in [0.01, 100] x 20 Gauss-Legendre cos(theta) nodes,
+l <= 8 (npoly = 5), 22^3 velocity nodes per point, guess Hernquist potential,
+Delta table from the guess potential.  A step is one SCF iteration
+(computeNewPhi: node phase + projection + Poisson) followed by installing the
+new potential.  Inputs are device resident; the timed loop runs K iterations.
+
+Prints one JSON line.  The roofline entry is for the dominant kernel
+(k_integrals, the persistent QAGS over the action integrals); its algorithmic
+flops per launch = (number of QK21 rules applied) x F_QK21(npoly), with
+F_QK21 = 21 * (11 npoly + 25) + 190 (operation count of one integrand
+evaluation and of the rule's sums; transcendentals count 1).
+The CPU baseline times the oracle (the C restatement of the reference
+algorithm) with OpenMP on a bounded sample of the same grid points.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+PEAK_FP64 = 78.6   # TFLOP/s, MI355X
+
+
+def f_qk21(npoly):
+    return 21 * (11 * npoly + 25) + 190
+
+
+def cpu_baseline(npoly, ntheta, nr, ar, tables, delta, sample_every=25, threads=None):
+    import numpy as np
+    from oracle import oracle as O
+    threads = threads or min(16, os.cpu_count() or 1)
+    p = O.Pot(ar, npoly, ntheta)
+    p.set_tables(*tables)
+    p.set_delta(*delta)
+    df = O.df_params("Hernquist")
+    ci, _ = O.gauleg(ntheta)
+    si = np.sqrt(1 - ci * ci)
+    idx = list(range(0, nr * ntheta, sample_every))
+    # rhofDF at the sampled grid points, parallel over points
+    sub = O.Pot(ar, npoly, ntheta)   # same tables; points evaluated one by one in threads
+    sub.set_tables(*tables)
+    sub.set_delta(*delta)
+    import concurrent.futures as cf
+    pts = [(ar[i // ntheta] * si[i % ntheta], ar[i // ntheta] * ci[i % ntheta]) for i in idx]
+    t0 = time.perf_counter()
+    with cf.ThreadPoolExecutor(max_workers=threads) as ex:
+        list(ex.map(lambda rz: p.rhofDF(df, rz[0], rz[1]), pts))
+    dt = time.perf_counter() - t0
+    return {"value": len(pts) / dt, "unit": "grid-pt DF moments/s", "cores": threads, "kind": "port",
+            "sample": f"{len(pts)} of the {nr * ntheta} grid points (every {sample_every}th), one rhofDF each, "
+                      f"oracle C restatement, {threads} threads, {dt:.1f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 "
+                                   "(multi-GPU sharding is described in DESIGN.md, not built)"}))
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+    import fjmodels_amd as fj
+
+    nr, ntheta, npoly = 40, 20, 5
+    ar = fj.log_grid(nr, 0.01, 100.0)
+    ctx = fj.Context(0)
+    comp = {"model": "Hernquist", "mass": 1.0, "r0": 1.0, "q": 1.0, "df": fj.df_params("Hernquist")}
+    model = fj.Model(ctx, ar, ntheta, npoly, [comp])
+    model.setup()
+    guess_tables = [x.copy() for x in model.total]
+    delta = (model.Egrid.copy(), model.Dgrid.copy())
+
+    for _ in range(args.warmup):
+        model.iterate()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    stats = []
+    for _ in range(args.steps):
+        model.iterate()
+        stats.append(ctx.stats())
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    ms = 1e3 * dt / args.steps
+    pts = nr * ntheta * len(model.components)
+    value = pts / (dt / args.steps)
+
+    qk = np.mean([s["qk_calls"] for s in stats])
+    t_int = np.mean([s["ms_integrals"] for s in stats]) * 1e-3
+    flops = qk * f_qk21(npoly)
+    achieved = flops / t_int / 1e12
+    line = {
+        "metric": "grid-pt DF moments/s per SCF iter, plain FP64",
+        "value": value, "unit": "grid-pt DF moments/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64", "data": "synthetic (Hernquist guess potential, cfg grid)",
+        "config": {"workload": "single-component Hernquist DPL f(J), 40 log radii [0.01,100] x 20 GL cos(theta), "
+                               "l<=8, 22^3 velocity nodes/point, guess potential + Delta table, SCF iterations",
+                   "nr": nr, "ntheta": ntheta, "npoly": npoly, "nodes_per_point": 10648},
+        "roofline": {"bound": "mfma", "pipe": "VALU FP64 (MI355X FP64 vector and matrix peaks are equal)",
+                     "kernel": "k_integrals", "achieved": achieved, "peak": PEAK_FP64, "unit": "TFLOP/s",
+                     "frac": achieved / PEAK_FP64, "traffic": None,
+                     "flops_per_launch": flops, "ms_per_launch": t_int * 1e3},
+        "phases_ms": {k: float(np.mean([s[k] for s in stats])) for k in
+                      ("ms_points", "ms_setup", "ms_integrals", "ms_overflow", "ms_moments", "ms_project",
+                       "ms_poisson")},
+        "counters": {k: int(np.mean([s[k] for s in stats])) for k in
+                     ("bound_nodes", "tasks", "qk_calls", "overflow_tasks", "brent_iters", "integral_lanes")},
+    }
+    if not args.no_cpu:
+        line["cpu_baseline"] = cpu_baseline(npoly, ntheta, nr, ar, guess_tables, delta)
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()

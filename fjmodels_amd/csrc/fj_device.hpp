@@ -8,7 +8,18 @@
 // (include/Utils.h:200-212, pot/Potential.cpp:161-206) are written as
 // steppers so that a kernel can drive many of them in lock step.
 #pragma once
+#if defined(__HIPCC__)
 #include <hip/hip_runtime.h>
+#define FJ_HD __host__ __device__
+#else
+// Host build of the same code (test harness only): plain C++.
+#include <math.h>
+#define __device__
+#define __host__
+#define __forceinline__ inline
+#define __constant__
+#define FJ_HD
+#endif
 #include <stdint.h>
 
 namespace fj {
@@ -57,7 +68,7 @@ struct Interp {
     double q;   // ar[nr-1]/r on the Keplerian branch
 };
 
-__device__ __forceinline__ Interp radial_interp(const PotView &T, double r) {
+FJ_HD __forceinline__ Interp radial_interp(const PotView &T, double r) {
     Interp I;
     const int nr = T.nr;
     const double *ar = T.ar;
@@ -88,7 +99,7 @@ __device__ __forceinline__ Interp radial_interp(const PotView &T, double r) {
 }
 
 // Interpolated coefficient np of table tab (interior branch).
-__device__ __forceinline__ double interp_coef(const Interp &I, const double *tab, int npoly, int np) {
+FJ_HD __forceinline__ double interp_coef(const Interp &I, const double *tab, int npoly, int np) {
     const double t = tab[I.top * npoly + np], b = tab[I.bot * npoly + np];
     double v = I.f1 * t + (1 - I.f1) * b;
     if (I.quad) {
@@ -99,7 +110,7 @@ __device__ __forceinline__ double interp_coef(const Interp &I, const double *tab
 }
 
 // Phi(R,z): Potential::operator() (Potential.cpp:206-219).
-__device__ __forceinline__ double pot_phi(const PotView &T, double R, double z) {
+FJ_HD __forceinline__ double pot_phi(const PotView &T, double R, double z) {
     const double r = sqrt(R * R + z * z);
     const double c = z / r;
     const double rc = fmax(r, T.ar[0]);
@@ -134,7 +145,7 @@ __device__ __forceinline__ double pot_phi(const PotView &T, double R, double z) 
 // Phi, dPhi/dR, dPhi/dz at one point: Potential::operator(), dR, dz
 // (Potential.cpp:206-219, 224-244, 261-276).  dr() evaluates P_l at
 // cos(acos(c)); c is used directly here (the two agree to rounding).
-__device__ __forceinline__ double pot_phi_grad(const PotView &T, double R, double z, double *dR, double *dz) {
+FJ_HD __forceinline__ double pot_phi_grad(const PotView &T, double R, double z, double *dR, double *dz) {
     const double rr = sqrt(R * R + z * z);
     const double cphi = z / rr;
     const double r = fmax(rr, T.ar[0]);
@@ -222,7 +233,7 @@ struct Brent {
 
 // Returns true if the wrapper answers without iterating (|F| < 5e-4 at an end,
 // or a bracket already narrower than the tolerance, which returns 0 as there).
-__device__ __forceinline__ bool brent_start(Brent &s, double low, double high, double flow, double fhigh) {
+FJ_HD __forceinline__ bool brent_start(Brent &s, double low, double high, double flow, double fhigh) {
     s.iters = 0;
     if (fabs(flow) < 5e-4) { s.root = low; s.cont = 0; return true; }
     if (fabs(fhigh) < 5e-4) { s.root = high; s.cont = 0; return true; }
@@ -235,7 +246,7 @@ __device__ __forceinline__ bool brent_start(Brent &s, double low, double high, d
 
 // Computes the next abscissa to evaluate; returns false if this iterate
 // terminates without a new evaluation (fb == 0 or |m| <= tol).
-__device__ __forceinline__ bool brent_propose(Brent &s, double &xnew) {
+FJ_HD __forceinline__ bool brent_propose(Brent &s, double &xnew) {
     double a = s.a, b = s.b, c = s.c, fa = s.fa, fb = s.fb, fc = s.fc, d = s.d, e = s.e;
     int ac_equal = 0;
     if ((fb < 0 && fc < 0) || (fb > 0 && fc > 0)) { ac_equal = 1; c = a; fc = fa; d = b - a; e = b - a; }
@@ -273,7 +284,7 @@ __device__ __forceinline__ bool brent_propose(Brent &s, double &xnew) {
     return true;
 }
 
-__device__ __forceinline__ void brent_accept(Brent &s, double fb) {
+FJ_HD __forceinline__ void brent_accept(Brent &s, double fb) {
     s.fb = fb;
     s.root = s.b;
     double c = s.c;
@@ -283,7 +294,7 @@ __device__ __forceinline__ void brent_accept(Brent &s, double fb) {
 
 // gsl_root_test_interval + the wrapper's width test: decides whether the
 // wrapper loop runs again.
-__device__ __forceinline__ void brent_after(Brent &s) {
+FJ_HD __forceinline__ void brent_after(Brent &s) {
     s.iters++;
     double lo = s.lo, hi = s.hi, min_abs;
     if ((lo > 0.0 && hi > 0.0) || (lo < 0.0 && hi < 0.0)) min_abs = fmin(fabs(lo), fabs(hi));
@@ -296,7 +307,7 @@ __device__ __forceinline__ void brent_after(Brent &s) {
 
 // Full root find for a functor F (one lane).
 template <class F>
-__device__ __forceinline__ double brent_solve(F &f, double low, double high, double flow, double fhigh, int *nit = nullptr) {
+FJ_HD __forceinline__ double brent_solve(F &f, double low, double high, double flow, double fhigh, int *nit = nullptr) {
     Brent s;
     if (brent_start(s, low, high, flow, fhigh)) { if (nit) *nit = 0; return s.root; }
     while (s.cont) {
@@ -344,7 +355,7 @@ __constant__ static const double kWG7[4] = {
 
 struct QKOut { double result, abserr, resabs, resasc; };
 
-__device__ __forceinline__ double qk_rescale(double err, double resabs, double resasc) {
+FJ_HD __forceinline__ double qk_rescale(double err, double resabs, double resasc) {
     err = fabs(err);
     if (resasc != 0 && err != 0) {
         const double t = 200 * err / resasc;
@@ -360,7 +371,7 @@ __device__ __forceinline__ double qk_rescale(double err, double resabs, double r
 
 // Kronrod rule with NK abscissae >= 0 (11: QK21, 8: QK15).
 template <int NK, class F>
-__device__ __forceinline__ QKOut qk_rule(F &f, double a, double b) {
+FJ_HD __forceinline__ QKOut qk_rule(F &f, double a, double b) {
     const double *xgk = NK == 11 ? kXGK21 : kXGK15;
     const double *wgk = NK == 11 ? kWGK21 : kWGK15;
     const double *wg = NK == 11 ? kWG10 : kWG7;
@@ -427,11 +438,11 @@ struct Qags {
     int overflow;  // workspace capacity reached before limit
 };
 
-__device__ void qags_begin(Qags &q, double a, double b, double epsabs, double epsrel, int limit);
-__device__ void qags_consume(Qags &q, const QagsWork &w, int cap, const QKOut &r);
+FJ_HD void qags_begin(Qags &q, double a, double b, double epsabs, double epsrel, int limit);
+FJ_HD void qags_consume(Qags &q, const QagsWork &w, int cap, const QKOut &r);
 
 template <int NK, class F>
-__device__ __forceinline__ double qags_solve(F &f, double a, double b, double epsabs, double epsrel, int limit,
+FJ_HD __forceinline__ double qags_solve(F &f, double a, double b, double epsabs, double epsrel, int limit,
                                              const QagsWork &w, int cap, int *ier = nullptr, int *nqk = nullptr) {
     Qags q;
     qags_begin(q, a, b, epsabs, epsrel, limit);
@@ -451,7 +462,7 @@ __device__ __forceinline__ double qags_solve(F &f, double a, double b, double ep
 namespace fj {
 
 // QUADPACK dqpsrt (1-based lists).
-__device__ __forceinline__ void qpsrt(int limit, int last, int &maxerr, double &ermax, const double *elist,
+FJ_HD __forceinline__ void qpsrt(int limit, int last, int &maxerr, double &ermax, const double *elist,
                                       int *iord, int &nrmax) {
     int i, j, k, ido, ibeg, jbnd, isucc, jupbn;
     double errmax, errmin;
@@ -497,7 +508,7 @@ L90:
 }
 
 // QUADPACK dqelg (epsilon algorithm), epstab and res3la 1-based.
-__device__ __forceinline__ void qelg(int &n, double *epstab, double &result, double &abserr, double *res3la, int &nres) {
+FJ_HD __forceinline__ void qelg(int &n, double *epstab, double &result, double &abserr, double *res3la, int &nres) {
     const double epmach = DBL_EPS, oflow = DBL_MAXV;
     nres++;
     abserr = oflow;
@@ -559,7 +570,7 @@ __device__ __forceinline__ void qelg(int &n, double *epstab, double &result, dou
     abserr = fmax(abserr, 0.5e+01 * epmach * fabs(result));
 }
 
-__device__ __forceinline__ void qags_begin(Qags &q, double a, double b, double epsabs, double epsrel, int limit) {
+FJ_HD __forceinline__ void qags_begin(Qags &q, double a, double b, double epsabs, double epsrel, int limit) {
     q.a0 = a; q.b0 = b; q.epsabs = epsabs; q.epsrel = epsrel; q.limit = limit;
     q.state = 0; q.next_a = a; q.next_b = b;
     q.ier = 0; q.last = 0; q.result = 0; q.abserr = 0; q.overflow = 0;
@@ -567,7 +578,7 @@ __device__ __forceinline__ void qags_begin(Qags &q, double a, double b, double e
 }
 
 // Sets up the bisection of interval maxerr for loop index q.last.
-__device__ __forceinline__ void qags_start_iter(Qags &q, const QagsWork &w, int cap) {
+FJ_HD __forceinline__ void qags_start_iter(Qags &q, const QagsWork &w, int cap) {
     if (q.last > cap) { q.overflow = 1; q.state = 3; return; }
     q.a1 = w.alist[q.maxerr];
     q.b1 = 0.5 * (w.alist[q.maxerr] + w.blist[q.maxerr]);
@@ -579,7 +590,7 @@ __device__ __forceinline__ void qags_start_iter(Qags &q, const QagsWork &w, int 
 }
 
 // dqagse labels 100..140
-__device__ __forceinline__ void qags_finish(Qags &q, const QagsWork &w, int from115) {
+FJ_HD __forceinline__ void qags_finish(Qags &q, const QagsWork &w, int from115) {
     const double oflow = DBL_MAXV;
     int go115 = from115, go130 = 0;
     if (!go115) {
@@ -608,7 +619,7 @@ __device__ __forceinline__ void qags_finish(Qags &q, const QagsWork &w, int from
     q.state = 3;
 }
 
-__device__ __forceinline__ void qags_consume(Qags &q, const QagsWork &w, int cap, const QKOut &r) {
+FJ_HD __forceinline__ void qags_consume(Qags &q, const QagsWork &w, int cap, const QKOut &r) {
     const double epmach = DBL_EPS, uflow = DBL_MINV, oflow = DBL_MAXV;
     if (q.state == 0) {
         q.result = r.result; q.abserr = r.abserr; q.defabs = r.resabs;
@@ -736,6 +747,46 @@ __device__ __forceinline__ void qags_consume(Qags &q, const QagsWork &w, int cap
     q.last = last + 1;
     if (q.last > limit) { q.last = limit; qags_finish(q, w, 0); return; }
     qags_start_iter(q, w, cap);
+}
+
+}  // namespace fj
+
+namespace fj {
+
+// Per-lane QAGS workspace: alist, blist, rlist, elist [cap+1], iord [cap+1]
+// ints, epsilon table [56], res3la [4].
+inline size_t lane_ws_doubles(int cap) { return 4 * (size_t)(cap + 1) + (cap + 1) / 2 + 1 + 56 + 4; }
+
+FJ_HD inline QagsWork lane_work(double *ws, int cap, long long lane) {
+    const long long per = 4 * (long long)(cap + 1) + (cap + 1) / 2 + 1 + 56 + 4;
+    double *base = ws + lane * per;
+    QagsWork w;
+    w.alist = base;
+    w.blist = base + (cap + 1);
+    w.rlist = base + 2 * (cap + 1);
+    w.elist = base + 3 * (cap + 1);
+    w.iord = reinterpret_cast<int *>(base + 4 * (cap + 1));
+    w.rlist2 = base + 4 * (cap + 1) + (cap + 1) / 2 + 1;
+    w.res3la = w.rlist2 + 56;
+    return w;
+}
+
+// Host-side fill of the recurrence coefficients.
+inline LegCoef make_legcoef(int npoly) {
+    LegCoef lc;
+    lc = LegCoef{};
+    for (int np = 2; np < npoly && np < MAXPOLY; np++) {
+        int l = 2 * (np - 1), l2 = 2 * l;
+        lc.ea[np] = (double)(l * (l - 1)) / (double)((l2 + 1) * (l2 - 1));
+        lc.eb[np] = (l2 * l + l2 - 1) / (double)((l2 - 1) * (l2 + 3));
+        lc.eg[np] = (l2 + 1) * (l2 + 3) / (double)((l + 1) * (l + 2));
+    }
+    lc.la[1] = 1.0; lc.lb[1] = 0.0;
+    for (int i = 2; i < 2 * MAXPOLY; i++) {
+        lc.la[i] = (double)(2 * i - 1) / (double)i;
+        lc.lb[i] = (double)(i - 1) / (double)i;
+    }
+    return lc;
 }
 
 }  // namespace fj

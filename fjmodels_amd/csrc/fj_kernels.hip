@@ -25,6 +25,7 @@
 
 #include "../../include/fjmodels.h"
 #include "fj_device.hpp"
+#include "fj_node.hpp"
 
 using namespace fj;
 
@@ -96,106 +97,6 @@ __device__ inline LdsTabs stage_tables(const TabArgs &a, double *smem) {
     t.E = E; t.D = D; t.ngrid = ng;
     return t;
 }
-
-// include/Utils.h:149-155 linterp (x strictly inside uses the bisection)
-__device__ inline double d_linterp(const double *xm, const double *ym, int np, double x) {
-    if (x <= xm[0]) return ym[0];
-    if (x >= xm[np - 1]) return ym[np - 1];
-    int top = np - 1, bot = 0;
-    while (top - bot > 1) {
-        int m = (top + bot) >> 1;
-        if (x >= xm[m]) bot = m; else top = m;
-    }
-    return ym[bot] + (x - xm[bot]) / (xm[top] - xm[bot]) * (ym[top] - ym[bot]);
-}
-
-// delta/tabulate.cpp:40-44
-__device__ inline double d_deltafn(const double *Eg, const double *Dg, int n, double E) {
-    if (E < Eg[0]) return sqrt(fmax(.000001, Dg[0]));
-    else if (E < Eg[n - 1]) return sqrt(fmax(.000001, d_linterp(Eg, Dg, n, E)));
-    return sqrt(fmax(.000001, Dg[n - 1]));
-}
-
-// ---------------------------------------------------------------------------
-// Orbit functions of uvOrb (uvOrb.cpp:42-83) with per-node constants
-// ---------------------------------------------------------------------------
-struct NodeConst {
-    double Delta, E, Lzsq, sv, cv, sv2, sinv, cosv;  // u-side
-    double shuX, chuX, shu2;                         // v-side: sinh/cosh(X.u), X.shu2
-    double sh1sq, K1, K2, K3, invsh1sq;              // uturnfn: Er*ch1sq, (sh1sq+sv2)*Phiu1, .5Lzsq/D^2, 1/sh1sq
-    double Iv, KV;                                   // vturnfn: Iv, (1+shu2)*Phiv(PIH)
-};
-
-struct DUturn {   // duturnfn (uvOrb.cpp:74-79)
-    const PotView *T;
-    const NodeConst *c;
-    long long *nev;
-    __device__ double operator()(double u) const {
-        const double shu = sinh(u), chu = cosh(u), shu2 = shu * shu;
-        const double D = c->Delta;
-        double dR, dz;
-        const double Phi0 = pot_phi_grad(*T, D * shu * c->sinv, D * chu * c->cosv, &dR, &dz);
-        (*nev)++;
-        const double dPhidu = D * (dR * chu * c->sv + dz * shu * c->cv);
-        return 2 * (c->E - Phi0) * shu * chu - (shu2 + c->sv2) * dPhidu + c->Lzsq * chu / (D * D * shu2 * shu);
-    }
-};
-
-struct Uturn {    // uturnfn (uvOrb.cpp:70-73) with dU (61-63)
-    const PotView *T;
-    const NodeConst *c;
-    __device__ double operator()(double u) const {
-        const double shu = sinh(u), chu = cosh(u), shu2 = shu * shu;
-        const double D = c->Delta;
-        const double Phi = pot_phi(*T, D * shu * c->sinv, D * chu * c->cosv);
-        const double dU = (shu2 + c->sv2) * Phi - c->K2;
-        return c->E * (shu2 - c->sh1sq) + c->K1 - dU - c->K3 * (1 / shu2 - c->invsh1sq);
-    }
-};
-
-struct Vturn {    // vturnfn (uvOrb.cpp:49-52) with dV (46-48)
-    const PotView *T;
-    const NodeConst *c;
-    __device__ double operator()(double v) const {
-        double s, cs;
-        sincos(v, &s, &cs);
-        const double sv2 = s * s;
-        const double D = c->Delta;
-        const double Phi = pot_phi(*T, D * c->shuX * s, D * c->chuX * cs);
-        const double dV = c->KV - (c->shu2 + sv2) * Phi;
-        return c->E * sv2 + c->Iv + dV - .5 * c->Lzsq / (D * D * sv2);
-    }
-};
-
-// Action integrands Juint / Jvint (uvOrb.cpp:53-56, 80-83)
-struct JInt {
-    const PotView *T;
-    const NodeConst *c;
-    double ubar, Deltau, vturn;
-    int type;   // 0: Ju, 1: Jv
-    __device__ double operator()(double psi) const {
-        double sp, cp;
-        sincos(psi, &sp, &cp);
-        double F;
-        const double D = c->Delta;
-        if (type == 0) {
-            const double u = ubar - Deltau * cp;
-            const double shu = sinh(u), chu = cosh(u), shu2 = shu * shu;
-            const double Phi = pot_phi(*T, D * shu * c->sinv, D * chu * c->cosv);
-            const double dU = (shu2 + c->sv2) * Phi - c->K2;
-            F = c->E * (shu2 - c->sh1sq) + c->K1 - dU - c->K3 * (1 / shu2 - c->invsh1sq);
-        } else {
-            const double v = PIH - (PIH - vturn) * cp;
-            double s, cs;
-            sincos(v, &s, &cs);
-            const double sv2 = s * s;
-            const double Phi = pot_phi(*T, D * c->shuX * s, D * c->chuX * cs);
-            const double dV = c->KV - (c->shu2 + sv2) * Phi;
-            F = c->E * sv2 + c->Iv + dV - .5 * c->Lzsq / (D * D * sv2);
-        }
-        return sqrt(fmax(0., F)) * sp;
-    }
-};
 
 // Node record between the kernels (structure of arrays).
 enum { NF_DELTA, NF_E, NF_LZSQ, NF_SV, NF_CV, NF_SV2, NF_SINV, NF_COSV, NF_SHUX, NF_CHUX, NF_SHU2,
@@ -311,172 +212,16 @@ __global__ void __launch_bounds__(256) k_setup(TabArgs ta, SetupArgs sa) {
         double V[3], w;
         if (explicit_v) { V[0] = sa.vR[p]; V[1] = sa.vphi[p]; V[2] = sa.vz[p]; }
         else node_velocity(sa.quad, node, Ve, V, w);
-        const double H = .5 * (V[0] * V[0] + V[1] * V[1] + V[2] * V[2]) + Phi_h;   // DF.cpp:83
         int flag = 0;
         double Jr = 0, Jz = 0;
-        if (H < 0 || explicit_v) {
-            NodeConst c;
-            const double Delta = (explicit_v && sa.Dx && sa.Dx[p] > 0) ? sa.Dx[p] : d_deltafn(L.E, L.D, L.ngrid, H);
-            // pt(Delta, x, p): Uvpt.cpp:11-24
-            const double Delta2 = Delta * Delta, R2 = R * R, z2 = z * z, r2 = R2 + z2;
-            const double shu2 = .5 * (r2 - Delta2 + sqrt((Delta2 - r2) * (Delta2 - r2) + 4 * Delta2 * R2)) / Delta2;
-            const double shu = sqrt(shu2), u = asinh(shu);
-            const double chu = sqrt(1 + shu2);
-            const double cv2 = .5 * (r2 + Delta2 - sqrt((Delta2 + r2) * (Delta2 + r2) - 4 * Delta2 * z2)) / Delta2;
-            const double cv = z < 0 ? -sqrt(cv2) : sqrt(cv2);
-            const double v = acos(cv);
-            const double sv = sin(v), sv2 = 1 - cv2;
-            const double pR = V[0], pz = V[2], p2 = pR * pR + pz * pz;
-            const double pu = Delta * (pR * chu * sv + pz * shu * cv);
-            const double pv = Delta * (pR * shu * cv - pz * chu * sv);
-            const double Lz = R * V[1];
-            c.Delta = Delta; c.Lzsq = Lz * Lz;
-            c.E = .5 * p2 + .5 * c.Lzsq / (R * R) + Phi_h;   // uvOrb.cpp:22
-            if (c.E >= 0 || isnan(c.E)) {
-                flag = 2; Jr = 1e31; Jz = 1e31;   // uvOrb.cpp:38-40
-            } else {
-                flag = 1;
-                c.sv = sv; c.cv = cv; c.sv2 = sv2; c.sinv = sv; c.cosv = cos(v); c.shu2 = shu2;
-                // Getu0(us = 58): uvOrb.cpp:109-129
-                DUturn du{&L.T, &c, &nev};
-                double utry = 58, ui = 58, uo = 58, dfui = 0, dfuo = 0;
-                double dut = du(utry);
-                int guard = 0;
-                if (dut > 0) {
-                    while (dut > 0 && guard++ < 4000) { ui = utry; dfui = dut; utry *= 1.2; dut = du(utry); }
-                    uo = utry; dfuo = dut;
-                } else {
-                    while (dut < 0 && guard++ < 4000) { uo = utry; dfuo = dut; utry *= .8; dut = du(utry); }
-                    ui = utry; dfui = dut;
-                }
-                int it = 0;
-                const double umid = brent_solve(du, ui, uo, dfui, dfuo, &it);
-                brent_it += it;
-                // second pass at umid: uvOrb.cpp:30-33
-                const double shm = sinh(umid), chm = cosh(umid);
-                c.sh1sq = shm * shm;
-                const double ch1sq = chm * chm;
-                const double Phiu1 = pot_phi(L.T, Delta * shm * c.sinv, Delta * chm * c.cosv);
-                c.shuX = sinh(u); c.chuX = cosh(u);
-                const double PhiX = pot_phi(L.T, Delta * c.shuX * c.sinv, Delta * c.chuX * c.cosv);
-                c.invsh1sq = 1 / c.sh1sq;
-                c.K2 = (c.sh1sq + sv2) * Phiu1;
-                const double dUX = (c.shuX * c.shuX + sv2) * PhiX - c.K2;
-                const double Er = -c.E * (shu2 - c.sh1sq) / ch1sq + .5 * (pu * pu) / (Delta2 * ch1sq) +
-                                  .5 * c.Lzsq / (Delta2 * ch1sq) * (1 / shu2 - c.invsh1sq) + dUX / ch1sq;
-                c.K1 = Er * ch1sq;
-                c.K3 = .5 * c.Lzsq / (Delta * Delta);
-                double sP, cP;
-                sincos(PIH, &sP, &cP);
-                const double PhivPIH = pot_phi(L.T, Delta * c.shuX * sP, Delta * c.chuX * cP);
-                c.KV = (1 + shu2) * PhivPIH;
-                const double dVX = c.KV - (shu2 + c.sinv * c.sinv) * PhiX;
-                c.Iv = .5 * (pv * pv) / Delta2 - c.E * sv2 + .5 * c.Lzsq / (Delta2 * sv2) - dVX;
-
-                // GetTurnu(1): uvOrb.cpp:131-201
-                Uturn U{&L.T, &c};
-                double ut_ = 1, ut = U(ut_), dutt = du(ut_), eps = .1;
-                double uinner = ut_, uouter = ut_, u1 = ut_, u2 = ut_, fu1 = ut, fu2 = ut, fuinner = ut, fuouter = ut;
-                bool noroot = false;
-                guard = 0;
-                if (ut < 0) {
-                    if (dutt > 0) {
-                        uinner = ut_; fuinner = ut;
-                        while (guard++ < 4000) {
-                            while (ut < 0 && dutt > 0 && guard++ < 4000) {
-                                uinner = ut_; fuinner = ut; ut_ *= (1 + eps);
-                                ut = U(ut_); dutt = du(ut_);
-                            }
-                            if (ut >= 0) { u1 = ut_; fu1 = ut; break; }
-                            ut_ /= (1 + eps); eps *= .2;
-                            if (eps < TINY) { uinner = uouter = ut_; noroot = true; break; }
-                            ut_ *= (1 + eps);
-                            ut = U(ut_); dutt = du(ut_);
-                        }
-                        if (!noroot) {
-                            while (ut > 0 && guard++ < 8000) { u2 = ut_; fu2 = ut; ut_ *= 1.1; ut = U(ut_); }
-                            uouter = ut_; fuouter = ut;
-                        }
-                    } else {
-                        while (guard++ < 4000) {
-                            while (ut < 0 && dutt < 0 && guard++ < 4000) {
-                                uouter = ut_; fuouter = ut; ut_ /= (1 + eps);
-                                ut = U(ut_); dutt = du(ut_);
-                            }
-                            if (ut >= 0) { u2 = ut_; fu2 = ut; break; }
-                            ut_ *= (1 + eps); eps *= .2;
-                            if (eps < TINY) { uinner = uouter = ut_; noroot = true; break; }
-                            ut_ /= (1 + eps);
-                            ut = U(ut_); dutt = du(ut_);
-                        }
-                        if (!noroot) {
-                            while (ut > 0 && guard++ < 8000) { u1 = ut_; fu1 = ut; ut_ *= .9; ut = U(ut_); }
-                            uinner = ut_; fuinner = ut;
-                        }
-                    }
-                } else {
-                    u1 = u2 = ut_; fu1 = fu2 = ut;
-                    while (ut > 0 && guard++ < 4000) { u2 = ut_; fu2 = ut; ut_ *= 1.1; ut = U(ut_); }
-                    uouter = ut_; fuouter = ut;
-                    // the inward walk starts where the outward one stopped; its first
-                    // pass pairs u1 = uouter with the stale fu1, whose true F is fuouter
-                    ut = fu1;
-                    bool first = true;
-                    while (ut > 0 && guard++ < 8000) {
-                        u1 = ut_; fu1 = first ? fuouter : ut; first = false;
-                        ut_ *= .9; ut = U(ut_);
-                    }
-                    uinner = ut_; fuinner = ut;
-                }
-                if (!noroot) {
-                    int i1 = 0, i2 = 0;
-                    const double a = brent_solve(U, uinner, u1, fuinner, fu1, &i1);
-                    const double b = brent_solve(U, u2, uouter, fu2, fuouter, &i2);
-                    uinner = a; uouter = b;
-                    brent_it += i1 + i2;
-                }
-                double ubar = 0, Deltau = 0, vturn = 0;
-                if (fabs(uouter - uinner) < TINY) Jr = 0;
-                else {
-                    ubar = .5 * (uinner + uouter);
-                    Deltau = .5 * (uouter - uinner);
-                    want_u = true;
-                }
-                // GetTurnv(.5*PIH): uvOrb.cpp:203-224
-                Vturn Vf{&L.T, &c};
-                {
-                    const double vtry = .5 * PIH;
-                    double vtop = vtry, vbot = vtry, ft, fb;
-                    const double f0 = Vf(vtry);
-                    bool done = false;
-                    guard = 0;
-                    if (f0 < 0) {
-                        fb = f0; ft = f0;
-                        while (fb < 0 && guard++ < 4000) {
-                            vbot = .5 * (PIH + vbot);
-                            if (PIH - vbot < TINY) { vturn = PIH; done = true; break; }
-                            fb = Vf(vbot);
-                        }
-                    } else {
-                        ft = f0; fb = f0;
-                        while (ft > 0 && guard++ < 4000) {
-                            vtop *= .5;
-                            if (vtop < TINY) { vturn = 0; done = true; break; }
-                            ft = Vf(vtop);
-                        }
-                    }
-                    if (!done) {
-                        int i3 = 0;
-                        if (!(vbot < vtop)) vturn = brent_solve(Vf, vtop, vbot, ft, fb, &i3);
-                        else vturn = brent_solve(Vf, vbot, vtop, fb, ft, &i3);
-                        brent_it += i3;
-                    }
-                }
-                if (fabs(vturn - PIH) < TINY) Jz = 0;
-                else want_v = true;
-                store_const(sa.nb, gid, c, ubar, Deltau, vturn);
-            }
-        }
+        NodeConst c;
+        double ubar = 0, Deltau = 0, vturn = 0;
+        int wu = 0, wv = 0;
+        const double Dx = (explicit_v && sa.Dx) ? sa.Dx[p] : 0.0;
+        flag = node_setup(L.T, L.E, L.D, L.ngrid, R, z, Phi_h, V, Dx, !explicit_v, c, ubar, Deltau, vturn,
+                          wu, wv, Jr, Jz, nev, brent_it);
+        want_u = wu; want_v = wv;
+        if (flag == 1) store_const(sa.nb, gid, c, ubar, Deltau, vturn);
         sa.nb.flag[gid] = flag;
         sa.nb.Jr[gid] = Jr;
         sa.nb.Jz[gid] = Jz;
@@ -503,19 +248,6 @@ struct IntegArgs {
     Counters *ctr;
 };
 
-__device__ inline QagsWork lane_work(double *ws, int cap, long long lane) {
-    const long long per = 4 * (long long)(cap + 1) + (cap + 1) / 2 + 1 + 56 + 4;
-    double *base = ws + lane * per;
-    QagsWork w;
-    w.alist = base;
-    w.blist = base + (cap + 1);
-    w.rlist = base + 2 * (cap + 1);
-    w.elist = base + 3 * (cap + 1);
-    w.iord = reinterpret_cast<int *>(base + 4 * (cap + 1));
-    w.rlist2 = base + 4 * (cap + 1) + (cap + 1) / 2 + 1;
-    w.res3la = w.rlist2 + 56;
-    return w;
-}
 
 // k_integrals: persistent QAGS over the Ju and Jv task lists (Utils.h:200-212,
 // uvOrb.cpp:226-262).  Each loop step every busy lane applies one QK21 rule to
@@ -1155,6 +887,26 @@ __global__ void k_delta(TabArgs ta, const double *prep, double *scratch, double 
     if (d.status) atomicOr(status, d.status);
 }
 
+
+// Diagnostic kernel: integrand samples and one QAGS run for one stored node.
+__global__ void k_debug_integral(TabArgs ta, NodeBuf nb, long long node, int type, const double *psi, int npsi,
+                                 double *ws, double *out) {
+    extern __shared__ double smem[];
+    const LdsTabs L = stage_tables(ta, smem);
+    if (threadIdx.x != 0) return;
+    NodeConst c;
+    double ubar, Deltau, vturn;
+    load_const(nb, node, c, ubar, Deltau, vturn);
+    JInt f{&L.T, &c, ubar, Deltau, vturn, type};
+    for (int i = 0; i < npsi; i++) out[i] = f(psi[i]);
+    const QagsWork w = lane_work(ws, QCAP_FULL, 0);
+    int ier = 0, nqk = 0;
+    out[npsi] = qags_solve<11>(f, 0, type == 0 ? PI : PIH, EPSABSINTEG, EPSRELINTEG, 1000, w, QCAP_FULL, &ier, &nqk);
+    out[npsi + 1] = ier;
+    out[npsi + 2] = nqk;
+    const QKOut r = qk_rule<11>(f, 0, type == 0 ? PI : PIH);
+    out[npsi + 3] = r.result; out[npsi + 4] = r.abserr; out[npsi + 5] = r.resabs; out[npsi + 6] = r.resasc;
+}
 }  // namespace
 
 // ===========================================================================
@@ -1184,6 +936,7 @@ struct fj_ctx {
     // grid buffers
     double *d_rhl = nullptr, *d_gl = nullptr, *d_pw = nullptr, *d_pws = nullptr; int *d_status = nullptr;
     int integ_lanes = 0;
+    long long last_nodes = 0;
     fj_stats st{};
     char err[512] = {0};
 };
@@ -1253,22 +1006,6 @@ void host_evenlegend(int npoly, double c, double *pol) {
     }
 }
 
-LegCoef make_legcoef(int npoly) {
-    LegCoef lc;
-    memset(&lc, 0, sizeof lc);
-    for (int np = 2; np < npoly && np < MAXPOLY; np++) {
-        int l = 2 * (np - 1), l2 = 2 * l;
-        lc.ea[np] = (double)(l * (l - 1)) / (double)((l2 + 1) * (l2 - 1));
-        lc.eb[np] = (l2 * l + l2 - 1) / (double)((l2 - 1) * (l2 + 3));
-        lc.eg[np] = (l2 + 1) * (l2 + 3) / (double)((l + 1) * (l + 2));
-    }
-    lc.la[1] = 1.0; lc.lb[1] = 0.0;
-    for (int i = 2; i < 2 * MAXPOLY; i++) {
-        lc.la[i] = (double)(2 * i - 1) / (double)i;
-        lc.lb[i] = (double)(i - 1) / (double)i;
-    }
-    return lc;
-}
 
 TabArgs tab_args(const fj_ctx *c) {
     TabArgs t;
@@ -1301,7 +1038,6 @@ int integ_lanes(fj_ctx *c) {
     return c->integ_lanes;
 }
 
-size_t lane_ws_doubles(int cap) { return 4 * (size_t)(cap + 1) + (cap + 1) / 2 + 1 + 56 + 4; }
 
 int ensure_nodes(fj_ctx *c, long long nodes) {
     int r = 0;
@@ -1449,8 +1185,9 @@ int grid_points(fj_ctx *c) {
             R[n * c->ntheta + g] = c->ar[n] * c->si[g];
             z[n * c->ntheta + g] = c->ar[n] * c->ci[g];
         }
-    HIPCHK(c, hipMemcpyAsync(c->d_pR, R.data(), npts * sizeof(double), hipMemcpyHostToDevice, c->stream));
-    HIPCHK(c, hipMemcpyAsync(c->d_pz, z.data(), npts * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    // synchronous: R and z are released when this function returns
+    HIPCHK(c, hipMemcpy(c->d_pR, R.data(), npts * sizeof(double), hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(c->d_pz, z.data(), npts * sizeof(double), hipMemcpyHostToDevice));
     return 0;
 }
 
@@ -1495,6 +1232,7 @@ int run_actions(fj_ctx *c, int n, const double *vR, const double *vphi, const do
     PointArgs pa{c->d_pR, c->d_pz, c->d_pPhi, c->d_pVe};
     k_points<<<(n + 255) / 256, 256, lds, c->stream>>>(ta, pa, n);
     NodeBuf nb{c->d_nodef, n, c->d_Jr, c->d_Jz, c->d_flag};
+    c->last_nodes = n;
     HIPCHK(c, hipMemsetAsync(c->d_cnt, 0, 8 * sizeof(unsigned int), c->stream));
     HIPCHK(c, hipMemsetAsync(c->d_ctr, 0, sizeof(Counters), c->stream));
     SetupArgs sa{c->d_pR, c->d_pz, c->d_pPhi, c->d_pVe, 0, n, c->quad, nb, c->d_cnt + 0, c->d_cnt + 1,
@@ -1565,8 +1303,12 @@ const char *fj_last_error(const fj_ctx *c) { return c ? c->err : "null context";
 
 int fj_set_grid(fj_ctx *c, int nr, const double *ar, int ntheta, int npoly, int ngrid) {
     if (!c) return FJ_E_ARG;
-    if (nr < 3 || !ar || ntheta < 1 || npoly < 1 || npoly > MAXPOLY || ngrid < 2)
-        return set_err(c, FJ_E_ARG, "bad grid sizes nr=%d ntheta=%d npoly=%d ngrid=%d", nr, ntheta, npoly, ngrid);
+    // nr >= 11: the quadratic interpolation of the first 10 cells uses the
+    // point above the cell (UtilsLeg.h:113-115), which lies past the table
+    // when the last cell is one of them.
+    if (nr < 11 || !ar || ntheta < 1 || npoly < 1 || npoly > MAXPOLY || ngrid < 2)
+        return set_err(c, FJ_E_ARG, "bad grid sizes nr=%d (>= 11) ntheta=%d npoly=%d (<= %d) ngrid=%d", nr, ntheta,
+                       npoly, MAXPOLY, ngrid);
     for (int i = 1; i < nr; i++)
         if (!(ar[i] > ar[i - 1])) return set_err(c, FJ_E_ARG, "radii must increase");
     HIPCHK(c, hipSetDevice(c->device));
@@ -1870,6 +1612,38 @@ int fj_tabulate_delta(fj_ctx *c, double *Egrid, double *Dgrid) {
     if (status & 1) return set_err(c, FJ_E_DELTA, "ensureVsq gave up after 50 tries (Delta.cpp:121)");
     if (status & 2) return set_err(c, FJ_E_DELTA, "no Jr=0 orbit bracket after 21 tries (Delta.cpp:145-151)");
     if (status & 8) return set_err(c, FJ_E_DELTA, "no Delta^2 bracket");
+    return 0;
+}
+
+
+// Diagnostic (not part of include/fjmodels.h): the per-node setup record of
+// the last fj_actions call, NF_COUNT fields x n nodes, plus the flags.
+int fj_debug_nodes(fj_ctx *c, int n, double *out, int *flags) {
+    if (!c || n < 1 || n > c->node_cap || !out) return FJ_E_ARG;
+    for (int f = 0; f < NF_COUNT; f++)
+        HIPCHK(c, hipMemcpy(out + (size_t)f * n, c->d_nodef + (size_t)f * n, n * sizeof(double), hipMemcpyDeviceToHost));
+    if (flags) HIPCHK(c, hipMemcpy(flags, c->d_flag, n * sizeof(int), hipMemcpyDeviceToHost));
+    return 0;
+}
+
+
+// Diagnostic (not part of include/fjmodels.h): integrand samples at psi[],
+// then {QAGS result, ier, QK21 calls, first-rule result, abserr, resabs, resasc}
+// for stored node `node` of the last fj_actions call.
+int fj_debug_integral(fj_ctx *c, long long node, int type, int npsi, const double *psi, double *out) {
+    if (!c || node < 0 || node >= c->last_nodes || npsi < 0 || !out) return FJ_E_ARG;
+    double *d = nullptr;
+    const size_t nws = lane_ws_doubles(QCAP_FULL);
+    const size_t nout = (size_t)npsi + 8;
+    HIPCHK(c, hipMalloc((void **)&d, (npsi + nout + nws) * sizeof(double)));
+    if (npsi) HIPCHK(c, hipMemcpy(d, psi, npsi * sizeof(double), hipMemcpyHostToDevice));
+    NodeBuf nb{c->d_nodef, c->last_nodes, c->d_Jr, c->d_Jz, c->d_flag};
+    k_debug_integral<<<1, 64, tab_bytes(c), c->stream>>>(tab_args(c), nb, node, type, d, npsi, d + npsi + nout,
+                                                        d + npsi);
+    HIPCHK(c, hipGetLastError());
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    HIPCHK(c, hipMemcpy(out, d + npsi, (npsi + 7) * sizeof(double), hipMemcpyDeviceToHost));
+    (void)hipFree(d);
     return 0;
 }
 

@@ -81,6 +81,9 @@ void fjo_counters_reset(void) { memset(&g_cnt, 0, sizeof g_cnt); }
 /* grid                                                                      */
 /* ------------------------------------------------------------------------ */
 fjo_pot *fjo_pot_new(int nr, const double *ar, int npoly, int ngauss, int ngrid) {
+    /* nr >= 11 keeps the quadratic-interpolation neighbour (UtilsLeg.h:113-115,
+     * used for the first 10 cells) inside the table. */
+    if (nr < 11 || npoly < 1 || npoly > 64 || ngauss < 1 || ngrid < 2) return NULL;
     fjo_pot *p = (fjo_pot *)calloc(1, sizeof *p);
     p->nr = nr; p->npoly = npoly; p->ngauss = ngauss; p->ngrid = ngrid; p->quad = 0;
     p->ar = (double *)malloc(sizeof(double) * nr);
@@ -811,7 +814,7 @@ L115:
 L130:
     if (ier > 2) ier = ier - 1;
 L140:
-    *neval = 42 * last - 21;
+    *neval = (rule == fjo_qk15) ? 30 * last - 15 : 42 * last - 21;
 L999:
     if (last_out) *last_out = last;
     free(alist); free(blist); free(rlist); free(elist); free(iord);

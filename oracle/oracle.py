@@ -124,6 +124,8 @@ class Pot:
         self.ar = arr(ar)
         self.nr, self.npoly, self.ngauss, self.ngrid = len(self.ar), npoly, ngauss, ngrid
         self.h = lib.fjo_pot_new(self.nr, ptr(self.ar), npoly, ngauss, ngrid)
+        if not self.h:
+            raise ValueError("fjo_pot_new: need nr >= 11, 1 <= npoly <= 64")
 
     def __del__(self):
         try:
