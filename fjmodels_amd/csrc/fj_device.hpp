@@ -22,6 +22,8 @@
 #endif
 #include <stdint.h>
 
+#include "fj_math.h"
+
 namespace fj {
 
 // include/Utils.h:49-60
@@ -39,14 +41,25 @@ constexpr double DBL_MINV = 2.2250738585072014e-308;
 constexpr double DBL_MAXV = 1.7976931348623157e+308;
 constexpr int MAXPOLY = 32;
 
-// Recurrence coefficients, filled by the host once per npoly.
-//   evenlegend: P2k from P2k-2, P2k-4 (UtilsLeg.h:68-79):
-//     pol[np] = (-pol[np-2]*ea[np] + pol[np-1]*(c2 - eb[np])) * eg[np]
-//   legend: P_i = la[i]*c*P_{i-1} - lb[i]*P_{i-2} (UtilsLeg.h:43-50)
+// MAX/MIN of include/Utils.h:47-48 (a NaN first argument compares false).
+FJ_HD __forceinline__ double fj_max(double a, double b) { return a > b ? a : b; }
+FJ_HD __forceinline__ double fj_min(double a, double b) { return a < b ? a : b; }
+
+// Integer and quotient constants of the Legendre recurrences, filled by the
+// host with the same integer-to-double conversions and divisions as the
+// reference expressions (UtilsLeg.h:43-79), so every step rounds as there:
+//   evenlegend step np:  p = -pm2*el[np]*elm1[np]/ed[np] + pm1*(c2 - eb[np]);  p *= eg[np]
+//   legend step i:       P_i = (l2m1[i]*c*P_{i-1} - lm1[i]*P_{i-2}) / li[i]
 struct LegCoef {
-    double ea[MAXPOLY], eb[MAXPOLY], eg[MAXPOLY];
-    double la[2 * MAXPOLY], lb[2 * MAXPOLY];
+    double el[MAXPOLY], elm1[MAXPOLY], ed[MAXPOLY], eb[MAXPOLY], eg[MAXPOLY];
+    double l2m1[2 * MAXPOLY], lm1[2 * MAXPOLY], li[2 * MAXPOLY];
 };
+
+FJ_HD __forceinline__ double evenleg_step(const LegCoef *lc, int np, double pm2, double pm1, double c2) {
+    double p = -pm2 * lc->el[np] * lc->elm1[np] / lc->ed[np] + pm1 * (c2 - lc->eb[np]);
+    p *= lc->eg[np];
+    return p;
+}
 
 // Potential tables as the node kernels see them (LDS or global).
 struct PotView {
@@ -109,114 +122,91 @@ FJ_HD __forceinline__ double interp_coef(const Interp &I, const double *tab, int
     return v;
 }
 
-// Phi(R,z): Potential::operator() (Potential.cpp:206-219).
+// intpo2's radius: unchanged beyond the grid, else MAX(r, ar[0]).
+FJ_HD __forceinline__ double I_clamp(const PotView &T, double r) {
+    return r >= T.ar[T.nr - 1] ? r : fj_max(r, T.ar[0]);
+}
+
+// Interpolated coefficient np on either branch (intpo2, UtilsLeg.h:96-124):
+// beyond r_max phil[nr-1][np] (ar[nr-1]/r)^(2np+1).
+FJ_HD __forceinline__ double coef_at(const PotView &T, const Interp &I, const double *tab, int np) {
+    if (I.kep) return tab[(T.nr - 1) * T.npoly + np] * fjm_powi(I.q, 2 * np + 1);
+    return interp_coef(I, tab, T.npoly, np);
+}
+
+// Phi(R,z): Potential::operator() (Potential.cpp:206-219) with evenlegend.
 FJ_HD __forceinline__ double pot_phi(const PotView &T, double R, double z) {
-    const double r = sqrt(R * R + z * z);
-    const double c = z / r;
-    const double rc = fmax(r, T.ar[0]);
-    const Interp I = radial_interp(T, rc);
+    const double c = z / sqrt(R * R + z * z), r = sqrt(R * R + z * z);
+    const Interp I = radial_interp(T, I_clamp(T, r));
     const int np_ = T.npoly;
     const double c2 = c * c;
-    double phi, pm2 = 1.0, pm1 = 1.5 * c2 - .5;
-    if (I.kep) {
-        const double q2 = I.q * I.q;
-        double qk = I.q;
-        const double *last = T.phil + (T.nr - 1) * np_;
-        phi = last[0] * qk;
-        if (np_ > 1) { qk *= q2; phi += last[1] * qk * pm1; }
-        for (int np = 2; np < np_; np++) {
-            const double p = (-pm2 * T.lc->ea[np] + pm1 * (c2 - T.lc->eb[np])) * T.lc->eg[np];
-            qk *= q2;
-            phi += last[np] * qk * p;
-            pm2 = pm1; pm1 = p;
-        }
-    } else {
-        phi = interp_coef(I, T.phil, np_, 0);
-        if (np_ > 1) phi += interp_coef(I, T.phil, np_, 1) * pm1;
-        for (int np = 2; np < np_; np++) {
-            const double p = (-pm2 * T.lc->ea[np] + pm1 * (c2 - T.lc->eb[np])) * T.lc->eg[np];
-            phi += interp_coef(I, T.phil, np_, np) * p;
-            pm2 = pm1; pm1 = p;
-        }
+    double phi = coef_at(T, I, T.phil, 0);
+    double pm2 = 1.0, pm1 = 1.5 * c2 - .5;
+    for (int np = 1; np < np_; np++) {
+        double p;
+        if (np == 1) p = pm1;
+        else { p = evenleg_step(T.lc, np, pm2, pm1, c2); pm2 = pm1; pm1 = p; }
+        phi += coef_at(T, I, T.phil, np) * p;
     }
     return phi;
 }
 
-// Phi, dPhi/dR, dPhi/dz at one point: Potential::operator(), dR, dz
-// (Potential.cpp:206-219, 224-244, 261-276).  dr() evaluates P_l at
-// cos(acos(c)); c is used directly here (the two agree to rounding).
+// Phi, dPhi/dR, dPhi/dz at one point: Potential::operator(), dR, dz, dtheta,
+// dr (Potential.cpp:206-244, 261-276) with dlegend (UtilsLeg.h:51-57) and
+// the three-table intpo2 (UtilsLeg.h:127-158).  One radial search serves
+// all three; every value rounds as in the separate reference routines.
 FJ_HD __forceinline__ double pot_phi_grad(const PotView &T, double R, double z, double *dR, double *dz) {
-    const double rr = sqrt(R * R + z * z);
-    const double cphi = z / rr;
-    const double r = fmax(rr, T.ar[0]);
+    const double cphi = z / sqrt(R * R + z * z);
+    double r = sqrt(z * z + R * R);
+    if (r < T.ar[0]) r = T.ar[0];
     const double s = R / r;
     double c = z / r;
     if (c == 1) c -= 1e-8;
     const Interp I = radial_interp(T, r);
     const int np_ = T.npoly;
-    const double c2 = c * c;
-    const double inv1mc2 = 1.0 / (1 - c2);
-    // evenlegend(c) for dr(); legend recurrence for dlegend (dtheta)
-    double pm2 = 1.0, pm1 = 1.5 * c2 - .5;
-    // (P_{2np-2}, P_{2np-3}) of the full recurrence; la[1] = 1, lb[1] = 0 gives P_1 = c
-    double lm1 = 1.0, lm2 = 0.0;
-    double dth = 0, g1 = 0, phi = 0;
-    const double *last = T.phil + (T.nr - 1) * np_;
-    const double q2 = I.q * I.q;
-    double qk = I.q;
-    for (int np = 0; np < np_; np++) {
-        double ph, dph;
-        if (I.kep) {
-            ph = last[np] * qk;
-            dph = -(2 * np + 1) * ph / r;
-            qk *= q2;
-        } else {
-            ph = interp_coef(I, T.phil, np_, np);
-            dph = interp_coef(I, T.Pr, np_, np);
-        }
-        double pe;
-        if (np == 0) pe = 1.0;
-        else if (np == 1) pe = pm1;
-        else {
-            pe = (-pm2 * T.lc->ea[np] + pm1 * (c2 - T.lc->eb[np])) * T.lc->eg[np];
-            pm2 = pm1; pm1 = pe;
-        }
-        // dP_{2np}/dc = n (P_{n-1} - c P_n)/(1-c^2), n = 2np
+    // dtheta: sum_j phip[j] dlegend(c, 2j), times -sqrt(1-c^2)
+    double dth = 0, Pm2 = 0, Pm1 = 1.0;   // P_{i-2}, P_{i-1}
+    for (int j = 0; j < np_; j++) {
         double dpl = 0;
-        if (np > 0) {
-            const int i1 = 2 * np - 1, i2 = 2 * np;
-            const double p1 = T.lc->la[i1] * c * lm1 - T.lc->lb[i1] * lm2;
-            const double p2 = T.lc->la[i2] * c * p1 - T.lc->lb[i2] * lm1;
-            lm2 = p1; lm1 = p2;
-            const double n = (double)i2;
-            dpl = (n * p1 - n * c * p2) * inv1mc2;
-        }
-        dth += ph * dpl;
-        g1 += dph * pe;
-        phi = (np == 0) ? ph : phi + ph * pe;
-    }
-    if (cphi != c) {
-        // Phi uses c = z/|x| with the unclamped radius (only differs near r < ar[0] or c = 1)
-        const double cc2 = cphi * cphi;
-        double a2 = 1.0, a1 = 1.5 * cc2 - .5;
-        qk = I.q;
-        for (int np = 0; np < np_; np++) {
-            double ph;
-            if (I.kep) { ph = last[np] * qk; qk *= q2; }
-            else ph = interp_coef(I, T.phil, np_, np);
-            double pe;
-            if (np == 0) pe = 1.0;
-            else if (np == 1) pe = a1;
-            else {
-                pe = (-a2 * T.lc->ea[np] + a1 * (cc2 - T.lc->eb[np])) * T.lc->eg[np];
-                a2 = a1; a1 = pe;
+        if (j > 0) {
+            const int n = 2 * j;
+            double P1 = Pm1, P0 = Pm2;
+            for (int i = n - 1; i <= n; i++) {
+                double Pi;
+                if (i == 1) Pi = c;
+                else Pi = (T.lc->l2m1[i] * c * P1 - T.lc->lm1[i] * P0) / T.lc->li[i];
+                P0 = P1; P1 = Pi;
             }
-            phi = (np == 0) ? ph : phi + ph * pe;
+            Pm2 = P0; Pm1 = P1;   // P_{n-1}, P_n
+            dpl = (T.lc->li[n] * Pm2 - T.lc->li[n] * c * Pm1) / (1 - c * c);
         }
+        dth += coef_at(T, I, T.phil, j) * dpl;
     }
-    dth *= -sqrt(1 - c2);
+    dth *= -sqrt(1 - c * c);
+    // dr: sum dphip[np] P_2np(cos(acos(c)))
+    const double ct = fjm_cos(fjm_acos(c)), ct2 = ct * ct;
+    double g1 = 0, pm2 = 1.0, pm1 = 1.5 * ct2 - .5;
+    for (int np = 0; np < np_; np++) {
+        double p;
+        if (np == 0) p = 1.0;
+        else if (np == 1) p = pm1;
+        else { p = evenleg_step(T.lc, np, pm2, pm1, ct2); pm2 = pm1; pm1 = p; }
+        double dph;
+        if (I.kep) dph = -(2 * np + 1) * T.phil[(T.nr - 1) * np_ + np] * fjm_powi(I.q, 2 * np + 1) / r;
+        else dph = interp_coef(I, T.Pr, np_, np);
+        g1 += dph * p;
+    }
     *dR = dth * (c / r) + g1 * s;
     *dz = dth * (-s / r) + g1 * c;
+    // Phi at (R, z): c from the unclamped radius
+    const double cc2 = cphi * cphi;
+    double phi = coef_at(T, I, T.phil, 0), a2 = 1.0, a1 = 1.5 * cc2 - .5;
+    for (int np = 1; np < np_; np++) {
+        double p;
+        if (np == 1) p = a1;
+        else { p = evenleg_step(T.lc, np, a2, a1, cc2); a2 = a1; a1 = p; }
+        phi += coef_at(T, I, T.phil, np) * p;
+    }
     return phi;
 }
 
@@ -302,7 +292,7 @@ FJ_HD __forceinline__ void brent_after(Brent &s) {
     const double tol = EPSABSROOTS + EPSRELROOTS * min_abs;
     const int cont = fabs(hi - lo) < tol ? 0 : 1;
     s.cont = (cont || hi - lo > EPSABSROOTS) ? 1 : 0;
-    if (s.iters > 400) s.cont = 0;
+    if (s.iters > 1000) s.cont = 0;
 }
 
 // Full root find for a functor F (one lane).
@@ -773,18 +763,19 @@ FJ_HD inline QagsWork lane_work(double *ws, int cap, long long lane) {
 
 // Host-side fill of the recurrence coefficients.
 inline LegCoef make_legcoef(int npoly) {
-    LegCoef lc;
-    lc = LegCoef{};
+    LegCoef lc{};
     for (int np = 2; np < npoly && np < MAXPOLY; np++) {
-        int l = 2 * (np - 1), l2 = 2 * l;
-        lc.ea[np] = (double)(l * (l - 1)) / (double)((l2 + 1) * (l2 - 1));
+        const int l = 2 * (np - 1), l2 = 2 * l;
+        lc.el[np] = (double)l;
+        lc.elm1[np] = (double)(l - 1);
+        lc.ed[np] = (double)((l2 + 1) * (l2 - 1));
         lc.eb[np] = (l2 * l + l2 - 1) / (double)((l2 - 1) * (l2 + 3));
         lc.eg[np] = (l2 + 1) * (l2 + 3) / (double)((l + 1) * (l + 2));
     }
-    lc.la[1] = 1.0; lc.lb[1] = 0.0;
-    for (int i = 2; i < 2 * MAXPOLY; i++) {
-        lc.la[i] = (double)(2 * i - 1) / (double)i;
-        lc.lb[i] = (double)(i - 1) / (double)i;
+    for (int i = 0; i < 2 * MAXPOLY; i++) {
+        lc.l2m1[i] = (double)(2 * i - 1);
+        lc.lm1[i] = (double)(i - 1);
+        lc.li[i] = (double)i;
     }
     return lc;
 }

@@ -330,11 +330,10 @@ struct MomArgs {
 __device__ inline double d_df(const fj_df_params &d, double Jr, double Jphi, double Jz) {
     const double h = Jr + d.dphi_h * fabs(Jphi) + d.dz_h * Jz;
     const double g = Jr + d.dphi_g * fabs(Jphi) + d.dz_g * Jz;
-    const double J03 = d.J0 * d.J0 * d.J0;
-    const double TPI3 = TPI * TPI * TPI;
-    const double even = d.mass / J03 * fmax(0., pow(1. + d.J0 / h, d.A) / (pow(1. + g / d.J0, d.B))) / TPI3;
+    const double even = d.mass / fjm_powi(d.J0, 3) *
+                        fj_max(0., fjm_pow(1. + d.J0 / h, d.A) / (fjm_pow(1. + g / d.J0, d.B))) / fjm_powi(TPI, 3);
     if (d.chi == 0.) return even;
-    const double t = tanh(d.chi * Jphi / d.J0);
+    const double t = fjm_tanh(d.chi * Jphi / d.J0);
     if (d.rot_form == 1) return (1. - d.k) * even + d.k * t * even;
     return even + d.k * t * even;
 }
@@ -418,13 +417,13 @@ __global__ void k_project(int nr, int ng, int np_, const double *ar, const doubl
 __device__ inline double d_rho_guess(int model, double mass, double r0, double q, double q2, double R, double z) {
     const double m = sqrt(R * R + z * z / q2);
     switch (model) {
-    case 0: return mass / (m / r0 * pow(r0 + m, 3)) / TPI;
+    case 0: return mass / (m / r0 * fjm_powi(r0 + m, 3)) / TPI;
     case 1: {
         const double a = sqrt(r0 * r0 + m * m);
-        return mass / FPI / (q * pow((a + r0) * a, 2) * a) * r0 * (r0 + 2 * a);
+        return mass / FPI / (q * fjm_powi((a + r0) * a, 2) * a) * r0 * (r0 + 2 * a);
     }
-    case 2: return mass / (m / r0 * pow(r0 + m, 2)) / FPI;
-    default: return .01 / (m / 25. * pow(25. + m, 1)) / FPI;
+    case 2: return mass / (m / r0 * fjm_powi(r0 + m, 2)) / FPI;
+    default: return .01 / (m / 25. * fjm_powi(25. + m, 1)) / FPI;
     }
 }
 
@@ -452,7 +451,7 @@ struct IInt {
         const double yb = rhl[bot * np_ + col], yt = rhl[top * np_ + col];
         return yb + (x - ar[bot]) / (ar[top] - ar[bot]) * (yt - yb);
     }
-    __device__ double operator()(double x) const { return pow(x, l + 2) * val(x); }
+    __device__ double operator()(double x) const { return fjm_powi(x, l + 2) * val(x); }
 };
 struct IExtT {   // QAGIU transform of I_external: x = a + (1-t)/t, (f(x)/t)/t
     IInt in; double a;
@@ -460,9 +459,9 @@ struct IExtT {   // QAGIU transform of I_external: x = a + (1-t)/t, (f(x)/t)/t
         const double x = a + (1 - t) / t;
         const int nr = in.nr;
         double rhol;
-        if (x > in.ar[nr - 1]) rhol = fmax(0., in.rhl[(nr - 1) * in.np_ + in.col] * pow(in.ar[nr - 1] / x, 4));
+        if (x > in.ar[nr - 1]) rhol = fj_max(0., in.rhl[(nr - 1) * in.np_ + in.col] * fjm_powi(in.ar[nr - 1] / x, 4));
         else rhol = in.val(x);
-        const double y = rhol / pow(x, in.l - 1);
+        const double y = rhol / fjm_powi(x, in.l - 1);
         return (y / t) / t;
     }
 };
@@ -486,9 +485,9 @@ __global__ void k_poisson(int nr, int np_, const double *ar, const double *rhl, 
         if (isnan(Iext) || ++guard > 200) { atomicOr(status, 1); break; }
     }
     const double G = 1, rn = ar[n];
-    philP[idx] = -TPI * G * (Iint / pow(rn, l + 1) + Iext * pow(rn, l));
-    PrP[idx] = -TPI * G * (-(l + 1) * Iint / pow(rn, l + 2) + l * Iext * pow(rn, l - 1));
-    Pr2P[idx] = -TPI * G * ((l + 2) * (l + 1) * Iint / pow(rn, l + 3) + (l - 1) * l * Iext * pow(rn, l - 2) -
+    philP[idx] = -TPI * G * (Iint / fjm_powi(rn, l + 1) + Iext * fjm_powi(rn, l));
+    PrP[idx] = -TPI * G * (-(l + 1) * Iint / fjm_powi(rn, l + 2) + l * Iext * fjm_powi(rn, l - 1));
+    Pr2P[idx] = -TPI * G * ((l + 2) * (l + 1) * Iint / fjm_powi(rn, l + 3) + (l - 1) * l * Iext * fjm_powi(rn, l - 2) -
                             (2 * l + 1) * rhl[idx]);
 }
 
@@ -534,7 +533,7 @@ __global__ void __launch_bounds__(256) k_vir_terms(TabArgs ta, int ng, const dou
         double pm2 = 1.0, pm1 = 1.5 * c2 - .5;
         for (int np = 1; np < np_; np++) {
             double pe = pm1;
-            if (np >= 2) { pe = (-pm2 * L.T.lc->ea[np] + pm1 * (c2 - L.T.lc->eb[np])) * L.T.lc->eg[np]; pm2 = pm1; pm1 = pe; }
+            if (np >= 2) { pe = evenleg_step(L.T.lc, np, pm2, pm1, c2); pm2 = pm1; pm1 = pe; }
             const double dp = f * rhl[top * np_ + np] + (1 - f) * rhl[bot * np_ + np];
             dens += .5 * (4 * np + 1) * dp * pe;
         }
@@ -562,7 +561,7 @@ __global__ void __launch_bounds__(256) k_vir_terms(TabArgs ta, int ng, const dou
             const double s3 = f * sigzl[t] + omf * sigzl[b];
             if (np == 0) { dens = .5 * d0; Sigu = .5 * s1; Sigp = .5 * s2; Sigv = .5 * s3; continue; }
             double pe = pm1;
-            if (np >= 2) { pe = (-pm2 * L.T.lc->ea[np] + pm1 * (c2 - L.T.lc->eb[np])) * L.T.lc->eg[np]; pm2 = pm1; pm1 = pe; }
+            if (np >= 2) { pe = evenleg_step(L.T.lc, np, pm2, pm1, c2); pm2 = pm1; pm1 = pe; }
             const double fac = .5 * (4 * np + 1);
             dens += fac * d0 * pe; Sigu += fac * s1 * pe; Sigp += fac * s2 * pe; Sigv += fac * s3 * pe;
         }
@@ -646,7 +645,7 @@ __device__ inline double dl_dR(const PotView &T, double R, double z) { double a,
 
 __device__ inline double dl_vsq(const DeltaLane &d, double x) {   // Delta.cpp:28-30
     const double t = d.Lz / x;
-    return 2 * (d.E - (pot_phi(*d.T, x, 0.) + .5 * (t * t)));
+    return 2 * (d.E - (pot_phi(*d.T, x, 0.) + .5 * fjm_pow(t, 2)));
 }
 
 __device__ inline void dl_eqs(const DeltaLane &d, const double *y, double *f) {   // Delta.cpp:33-42
@@ -654,7 +653,7 @@ __device__ inline void dl_eqs(const DeltaLane &d, const double *y, double *f) { 
     pot_phi_grad(*d.T, y[0], y[1], &dR, &dz);
     f[0] = y[2];
     f[1] = y[3];
-    f[2] = -dR + (d.Lz * d.Lz) / pow(y[0], 3.0);
+    f[2] = -dR + fjm_pow(d.Lz, 2.0) / fjm_pow(y[0], 3.0);
     f[3] = -dz;
 }
 
@@ -691,16 +690,16 @@ __device__ inline void dl_evolve(const DeltaLane &d, double &t, double t1, doubl
         double rmax = DBL_MINV;
         for (int i = 0; i < 4; i++) {
             const double D0 = 1e-9 * fabs(y[i]) + 1e-10;
-            rmax = fmax(fabs(yerr[i]) / fabs(D0), rmax);
+            rmax = fj_max(fabs(yerr[i]) / fabs(D0), rmax);
         }
         if (rmax > 1.1) {
-            double r = 0.9 / pow(rmax, 1.0 / 8.0);
+            double r = 0.9 / fjm_pow(rmax, 1.0 / 8.0);
             if (r < 0.2) r = 0.2;
             h0 = r * h_old;
             if (fabs(h0) < fabs(h_old) && (t + h0) != t) continue;
             h0 = h_old;
         } else if (rmax < 0.5) {
-            double r = 0.9 / pow(rmax, 1.0 / 9.0);
+            double r = 0.9 / fjm_pow(rmax, 1.0 / 9.0);
             if (r > 5.0) r = 5.0;
             if (r < 1.0) r = 1.0;
             h0 = r * h_old;
@@ -738,7 +737,7 @@ struct DLgetDR {   // Delta.cpp:93-104
 __device__ inline double dl_ensure(DeltaLane &d, double x0) {   // Delta.cpp:116-124
     int count = 0;
     while (dl_vsq(d, x0) < 0) {
-        const double dpe = dl_dR(*d.T, x0, 0) - (d.Lz * d.Lz) / pow(x0, 3);
+        const double dpe = dl_dR(*d.T, x0, 0) - fjm_pow(d.Lz, 2) / fjm_powi(x0, 3);
         if (dpe < 0) x0 *= 1.1; else x0 *= .9;
         if (++count > 50) { d.status |= 1; return x0; }
     }
@@ -748,7 +747,7 @@ __device__ inline double dl_ensure(DeltaLane &d, double x0) {   // Delta.cpp:116
 struct DLds2dv {   // Delta.cpp:170-172
     const DeltaLane *d;
     __device__ double operator()(double v) const {
-        return 2 * (d->zi * d->sqDR * sin(v) - cos(v) * (d->Ri * d->R0 + d->D2 * sin(v)));
+        return 2 * (d->zi * d->sqDR * fjm_sin(v) - fjm_cos(v) * (d->Ri * d->R0 + d->D2 * fjm_sin(v)));
     }
 };
 
@@ -765,7 +764,7 @@ struct DLds2dD2 {  // Delta.cpp:210-234 with getv 189-208
             if (fabs(fl) < EPSABSROOTS) D.v0 = 0;
             else if (fabs(fh) < EPSABSROOTS) D.v0 = PIH;
             else D.v0 = brent_solve(g, 0, PIH, fl, fh);
-            D.sv = sin(D.v0); D.cv = cos(D.v0);
+            D.sv = fjm_sin(D.v0); D.cv = fjm_cos(D.v0);
             const double sv = D.sv, cv = D.cv, zi = D.zi, Ri = D.Ri, sqDR = D.sqDR, R0 = D.R0, D2 = D.D2;
             const double dvdD2 = .5 * sv * (zi / sqDR - 2 * cv) / (cv * zi * sqDR + sv * Ri * R0 - D2 * (cv * cv - sv * sv));
             sum += (cv - zi / sqDR) * cv + g(D.v0) * dvdD2;
@@ -785,7 +784,7 @@ struct DLRcE {   // Utils.h:243-246
 };
 struct DLRc {    // Utils.h:218-221
     const PotView *T; double Lzsq;
-    __device__ double operator()(double R) const { return Lzsq / pow(R, 3) - dl_dR(*T, R, 0); }
+    __device__ double operator()(double R) const { return Lzsq / fjm_powi(R, 3) - dl_dR(*T, R, 0); }
 };
 
 // Utils.h:222-241 GetRc
@@ -853,7 +852,7 @@ __global__ void k_delta(TabArgs ta, const double *prep, double *scratch, double 
     bool early = false;
     while (dx0 * dx1 > 0) {
         if (fabs(dx1) < fabs(dx0)) { x0 = x1; dx0 = dx1; }
-        else A = pow(A, -0.9);
+        else A = fjm_pow(A, -0.9);
         x1 = dl_ensure(d, x0 * A); dx1 = gdr(x1);
         k++;
         if (k == 21) {
@@ -864,7 +863,7 @@ __global__ void k_delta(TabArgs ta, const double *prep, double *scratch, double 
         }
     }
     if (!early) {
-        const double low = fmin(x0, x1), high = fmax(x0, x1);
+        const double low = fj_min(x0, x1), high = fj_max(x0, x1);
         const double flow = (low == x0) ? dx0 : dx1, fhigh = (high == x1) ? dx1 : dx0;
         R0res = brent_solve(gdr, low, high, x0 == x1 ? dx0 : flow, x0 == x1 ? dx1 : fhigh);
     }

@@ -21,9 +21,9 @@ FJ_HD inline double d_linterp(const double *xm, const double *ym, int np, double
 
 // delta/tabulate.cpp:40-44
 FJ_HD inline double d_deltafn(const double *Eg, const double *Dg, int n, double E) {
-    if (E < Eg[0]) return sqrt(fmax(.000001, Dg[0]));
-    else if (E < Eg[n - 1]) return sqrt(fmax(.000001, d_linterp(Eg, Dg, n, E)));
-    return sqrt(fmax(.000001, Dg[n - 1]));
+    if (E < Eg[0]) return sqrt(fj_max(.000001, Dg[0]));
+    else if (E < Eg[n - 1]) return sqrt(fj_max(.000001, d_linterp(Eg, Dg, n, E)));
+    return sqrt(fj_max(.000001, Dg[n - 1]));
 }
 
 // ---------------------------------------------------------------------------
@@ -41,7 +41,9 @@ struct DUturn {   // duturnfn (uvOrb.cpp:74-79)
     const NodeConst *c;
     long long *nev;
     FJ_HD double operator()(double u) const {
-        const double shu = sinh(u), chu = cosh(u), shu2 = shu * shu;
+        double shu, chu;
+        fjm_sinhcosh(u, &shu, &chu);
+        const double shu2 = shu * shu;
         const double D = c->Delta;
         double dR, dz;
         const double Phi0 = pot_phi_grad(*T, D * shu * c->sinv, D * chu * c->cosv, &dR, &dz);
@@ -55,7 +57,9 @@ struct Uturn {    // uturnfn (uvOrb.cpp:70-73) with dU (61-63)
     const PotView *T;
     const NodeConst *c;
     FJ_HD double operator()(double u) const {
-        const double shu = sinh(u), chu = cosh(u), shu2 = shu * shu;
+        double shu, chu;
+        fjm_sinhcosh(u, &shu, &chu);
+        const double shu2 = shu * shu;
         const double D = c->Delta;
         const double Phi = pot_phi(*T, D * shu * c->sinv, D * chu * c->cosv);
         const double dU = (shu2 + c->sv2) * Phi - c->K2;
@@ -68,7 +72,7 @@ struct Vturn {    // vturnfn (uvOrb.cpp:49-52) with dV (46-48)
     const NodeConst *c;
     FJ_HD double operator()(double v) const {
         double s, cs;
-        sincos(v, &s, &cs);
+        fjm_sincos(v, &s, &cs);
         const double sv2 = s * s;
         const double D = c->Delta;
         const double Phi = pot_phi(*T, D * c->shuX * s, D * c->chuX * cs);
@@ -85,25 +89,27 @@ struct JInt {
     int type;   // 0: Ju, 1: Jv
     FJ_HD double operator()(double psi) const {
         double sp, cp;
-        sincos(psi, &sp, &cp);
+        fjm_sincos(psi, &sp, &cp);
         double F;
         const double D = c->Delta;
         if (type == 0) {
             const double u = ubar - Deltau * cp;
-            const double shu = sinh(u), chu = cosh(u), shu2 = shu * shu;
+            double shu, chu;
+        fjm_sinhcosh(u, &shu, &chu);
+        const double shu2 = shu * shu;
             const double Phi = pot_phi(*T, D * shu * c->sinv, D * chu * c->cosv);
             const double dU = (shu2 + c->sv2) * Phi - c->K2;
             F = c->E * (shu2 - c->sh1sq) + c->K1 - dU - c->K3 * (1 / shu2 - c->invsh1sq);
         } else {
             const double v = PIH - (PIH - vturn) * cp;
             double s, cs;
-            sincos(v, &s, &cs);
+            fjm_sincos(v, &s, &cs);
             const double sv2 = s * s;
             const double Phi = pot_phi(*T, D * c->shuX * s, D * c->chuX * cs);
             const double dV = c->KV - (c->shu2 + sv2) * Phi;
             F = c->E * sv2 + c->Iv + dV - .5 * c->Lzsq / (D * D * sv2);
         }
-        return sqrt(fmax(0., F)) * sp;
+        return sqrt(fj_max(0., F)) * sp;
     }
 };
 
@@ -124,12 +130,12 @@ FJ_HD inline int node_setup(const PotView &T, const double *Eg, const double *Dg
     // pt(Delta, x, p): Uvpt.cpp:11-24
     const double Delta2 = Delta * Delta, R2 = R * R, z2 = z * z, r2 = R2 + z2;
     const double shu2 = .5 * (r2 - Delta2 + sqrt((Delta2 - r2) * (Delta2 - r2) + 4 * Delta2 * R2)) / Delta2;
-    const double shu = sqrt(shu2), u = asinh(shu);
+    const double shu = sqrt(shu2), u = fjm_asinh(shu);
     const double chu = sqrt(1 + shu2);
     const double cv2 = .5 * (r2 + Delta2 - sqrt((Delta2 + r2) * (Delta2 + r2) - 4 * Delta2 * z2)) / Delta2;
     const double cv = z < 0 ? -sqrt(cv2) : sqrt(cv2);
-    const double v = acos(cv);
-    const double sv = sin(v), sv2 = 1 - cv2;
+    const double v = fjm_acos(cv);
+    const double sv = fjm_sin(v), sv2 = 1 - cv2;
     const double pR = V[0], pz = V[2], p2 = pR * pR + pz * pz;
     const double pu = Delta * (pR * chu * sv + pz * shu * cv);
     const double pv = Delta * (pR * shu * cv - pz * chu * sv);
@@ -140,7 +146,7 @@ FJ_HD inline int node_setup(const PotView &T, const double *Eg, const double *Dg
         flag = 2; Jr = 1e31; Jz = 1e31;   // uvOrb.cpp:38-40
     } else {
         flag = 1;
-        c.sv = sv; c.cv = cv; c.sv2 = sv2; c.sinv = sv; c.cosv = cos(v); c.shu2 = shu2;
+        c.sv = sv; c.cv = cv; c.sv2 = sv2; c.sinv = sv; c.cosv = fjm_cos(v); c.shu2 = shu2;
         // Getu0(us = 58): uvOrb.cpp:109-129
         DUturn du{&T, &c, &nev};
         double utry = 58, ui = 58, uo = 58, dfui = 0, dfuo = 0;
@@ -157,11 +163,12 @@ FJ_HD inline int node_setup(const PotView &T, const double *Eg, const double *Dg
         const double umid = brent_solve(du, ui, uo, dfui, dfuo, &it);
         brent_it += it;
         // second pass at umid: uvOrb.cpp:30-33
-        const double shm = sinh(umid), chm = cosh(umid);
+        double shm, chm;
+    fjm_sinhcosh(umid, &shm, &chm);
         c.sh1sq = shm * shm;
         const double ch1sq = chm * chm;
         const double Phiu1 = pot_phi(T, Delta * shm * c.sinv, Delta * chm * c.cosv);
-        c.shuX = sinh(u); c.chuX = cosh(u);
+        fjm_sinhcosh(u, &c.shuX, &c.chuX);
         const double PhiX = pot_phi(T, Delta * c.shuX * c.sinv, Delta * c.chuX * c.cosv);
         c.invsh1sq = 1 / c.sh1sq;
         c.K2 = (c.sh1sq + sv2) * Phiu1;
@@ -171,7 +178,7 @@ FJ_HD inline int node_setup(const PotView &T, const double *Eg, const double *Dg
         c.K1 = Er * ch1sq;
         c.K3 = .5 * c.Lzsq / (Delta * Delta);
         double sP, cP;
-        sincos(PIH, &sP, &cP);
+        fjm_sincos(PIH, &sP, &cP);
         const double PhivPIH = pot_phi(T, Delta * c.shuX * sP, Delta * c.chuX * cP);
         c.KV = (1 + shu2) * PhivPIH;
         const double dVX = c.KV - (shu2 + c.sinv * c.sinv) * PhiX;

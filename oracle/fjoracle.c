@@ -11,6 +11,11 @@
  */
 #include "fjoracle.h"
 
+/* Elementary functions shared with the GPU path (see the header of that file:
+ * one implementation on both sides makes the reference's discrete decisions
+ * reproducible bit for bit).  Only tests and the checker use this library. */
+#include "../fjmodels_amd/csrc/fj_math.h"
+
 #include <float.h>
 #include <math.h>
 #include <stdio.h>
@@ -219,7 +224,7 @@ static void intpo2_1(const fjo_pot *p, double r, const double *philH, double *ph
     const double *ar = p->ar;
     if (r >= ar[NR - 1]) {
         for (int k = 0; k < npoly; k++)
-            phip[k] = philH[(NR - 1) * npoly + k] * pow(ar[NR - 1] / r, 2 * k + 1);
+            phip[k] = philH[(NR - 1) * npoly + k] * fjm_powi(ar[NR - 1] / r, 2 * k + 1);
     } else {
         int top = 0, bot = 0;
         r = MAXD(r, ar[0]);
@@ -246,10 +251,10 @@ static void intpo2_3(const fjo_pot *p, double r, double *phip, double *dphip, do
     const double *ar = p->ar, *philH = p->phil, *PrH = p->Pr, *Pr2H = p->Pr2;
     if (r >= ar[nr - 1]) {
         for (int k = 0; k < npoly; k++) {
-            phip[k] = philH[(nr - 1) * npoly + k] * pow(ar[nr - 1] / r, 2 * k + 1);
-            dphip[k] = -(2 * k + 1) * philH[(nr - 1) * npoly + k] * pow(ar[nr - 1] / r, 2 * k + 1) / r;
+            phip[k] = philH[(nr - 1) * npoly + k] * fjm_powi(ar[nr - 1] / r, 2 * k + 1);
+            dphip[k] = -(2 * k + 1) * philH[(nr - 1) * npoly + k] * fjm_powi(ar[nr - 1] / r, 2 * k + 1) / r;
             d2phip[k] = (2 * k + 2) * (2 * k + 1) * philH[(nr - 1) * npoly + k] *
-                        pow(ar[nr - 1] / r, 2 * k + 1) / r / r;
+                        fjm_powi(ar[nr - 1] / r, 2 * k + 1) / r / r;
         }
     } else {
         int top = 0, bot = 0;
@@ -293,7 +298,7 @@ double fjo_phi(const fjo_pot *p, double R, double z) {
 static double pot_dr(const fjo_pot *p, double r, double theta, double *d2Phidr2) {
     double pol[64], phip[64], dphip[64], d2phip[64];
     double grad1 = 0, grad2 = 0;
-    fjo_evenlegend(p->npoly, cos(theta), pol);
+    fjo_evenlegend(p->npoly, fjm_cos(theta), pol);
     intpo2_3(p, r, phip, dphip, d2phip);
     for (int np = 0; np < p->npoly; np++) {
         grad1 += dphip[np] * pol[np];
@@ -318,7 +323,7 @@ double fjo_dR(const fjo_pot *p, double R, double z) {
     if (r < p->ar[0]) r = p->ar[0];
     double s = R / r, c = z / r;
     if (c == 1) c -= 1e-8;
-    double theta = acos(c), d2;
+    double theta = fjm_acos(c), d2;
     return pot_dtheta(p, r, c) * (c / r) + pot_dr(p, r, theta, &d2) * s;
 }
 
@@ -327,7 +332,7 @@ double fjo_dz(const fjo_pot *p, double R, double z) {
     if (r < p->ar[0]) r = p->ar[0];
     double s = R / r, c = z / r;
     if (c == 1) c -= 1e-8;
-    double theta = acos(c), d2;
+    double theta = fjm_acos(c), d2;
     return pot_dtheta(p, r, c) * (-s / r) + pot_dr(p, r, theta, &d2) * c;
 }
 
@@ -467,7 +472,9 @@ static const double wg7[4] = {
 static double rescale_error(double err, double resabs, double resasc) {
     err = fabs(err);
     if (resasc != 0 && err != 0) {
-        double scale = pow(200 * err / resasc, 1.5);
+        /* (200 err / resasc)^1.5, evaluated as t sqrt(t) */
+        double t = 200 * err / resasc;
+        double scale = t * sqrt(t);
         if (scale < 1) err = resasc * scale; else err = resasc;
     }
     if (resabs > DBL_MIN / (50 * DBL_EPSILON)) {
@@ -938,7 +945,7 @@ int fjo_evolve_apply(fjo_ode_fn fn, void *prm, int dim, double eps_abs, double e
             rmax = MAXD(r, rmax);
         }
         if (rmax > 1.1) {
-            double r = 0.9 / pow(rmax, 1.0 / 8.0);
+            double r = 0.9 / fjm_pow(rmax, 1.0 / 8.0);
             if (r < 0.2) r = 0.2;
             h0 = r * h_old;
             double t_curr = *t, t_next = *t + h0;
@@ -948,7 +955,7 @@ int fjo_evolve_apply(fjo_ode_fn fn, void *prm, int dim, double eps_abs, double e
             }
             h0 = h_old;
         } else if (rmax < 0.5) {
-            double r = 0.9 / pow(rmax, 1.0 / 9.0);
+            double r = 0.9 / fjm_pow(rmax, 1.0 / 9.0);
             if (r > 5.0) r = 5.0;
             if (r < 1.0) r = 1.0;
             h0 = r * h_old;
@@ -967,15 +974,15 @@ static double rho_guess(int model, const guess_par *g, double R, double z) {
     double m = sqrt(R * R + z * z / g->q2);
     switch (model) {
     case 0: /* Hernquist */
-        return g->mass / (m / g->r0 * pow(g->r0 + m, 3)) / TPI;
+        return g->mass / (m / g->r0 * fjm_powi(g->r0 + m, 3)) / TPI;
     case 1: { /* Isochrone */
         double a = sqrt(g->r0 * g->r0 + m * m);
-        return g->mass / FPI / (g->q * pow((a + g->r0) * a, 2) * a) * g->r0 * (g->r0 + 2 * a);
+        return g->mass / FPI / (g->q * fjm_powi((a + g->r0) * a, 2) * a) * g->r0 * (g->r0 + 2 * a);
     }
     case 2: /* NFW */
-        return g->mass / (m / g->r0 * pow(g->r0 + m, 2)) / FPI;
+        return g->mass / (m / g->r0 * fjm_powi(g->r0 + m, 2)) / FPI;
     default: /* NFWext */
-        return .01 / (m / 25. * pow(25. + m, 1)) / FPI;
+        return .01 / (m / 25. * fjm_powi(25. + m, 1)) / FPI;
     }
 }
 
@@ -1006,16 +1013,16 @@ void fjo_guess_rhl(const fjo_pot *p, int model, double mass, double r0, double q
 typedef struct { int l; const double *rhl_h; const double *ar; int nr; } I_par;
 static double I_internal(double x, void *params) {
     I_par *p = (I_par *)params;
-    return pow(x, p->l + 2) * fjo_linterp(p->ar, p->rhl_h, p->nr, x);
+    return fjm_powi(x, p->l + 2) * fjo_linterp(p->ar, p->rhl_h, p->nr, x);
 }
 static double I_external(double x, void *params) {
     I_par *p = (I_par *)params;
     const double *ar = p->ar;
     int NR = p->nr;
     double rhol;
-    if (x > ar[NR - 1]) rhol = MAXD(0., p->rhl_h[NR - 1] * pow(ar[NR - 1] / x, 4));
+    if (x > ar[NR - 1]) rhol = MAXD(0., p->rhl_h[NR - 1] * fjm_powi(ar[NR - 1] / x, 4));
     else rhol = fjo_linterp(ar, p->rhl_h, NR, x);
-    return rhol / pow(x, p->l - 1);
+    return rhol / fjm_powi(x, p->l - 1);
 }
 
 /* pot/Potential.cpp:110-155 computeInts + 163-200 computePhil.  Returns 0, or 1
@@ -1049,10 +1056,10 @@ int fjo_compute_phil(const fjo_pot *p, const double *rhl, double *philP, double 
         double G = 1;
         for (int n = 0; n < NR; n++) {
             double Ii = Iint[n * NP + np], Ie = Iext[n * NP + np];
-            philP[n * NP + np] = -TPI * G * (Ii / pow(ar[n], l + 1) + Ie * pow(ar[n], l));
-            PrP[n * NP + np] = -TPI * G * (-(l + 1) * Ii / pow(ar[n], l + 2) + l * Ie * pow(ar[n], l - 1));
-            Pr2P[n * NP + np] = -TPI * G * ((l + 2) * (l + 1) * Ii / pow(ar[n], l + 3) +
-                                            (l - 1) * l * Ie * pow(ar[n], l - 2) - (2 * l + 1) * rhl[n * NP + np]);
+            philP[n * NP + np] = -TPI * G * (Ii / fjm_powi(ar[n], l + 1) + Ie * fjm_powi(ar[n], l));
+            PrP[n * NP + np] = -TPI * G * (-(l + 1) * Ii / fjm_powi(ar[n], l + 2) + l * Ie * fjm_powi(ar[n], l - 1));
+            Pr2P[n * NP + np] = -TPI * G * ((l + 2) * (l + 1) * Ii / fjm_powi(ar[n], l + 3) +
+                                            (l - 1) * l * Ie * fjm_powi(ar[n], l - 2) - (2 * l + 1) * rhl[n * NP + np]);
         }
     }
     free(rhl_h); free(Iint); free(Iext);
@@ -1104,7 +1111,7 @@ void fjo_get_ylm(const fjo_pot *p, double *rho, const double *vrot, const double
 typedef struct { const fjo_pot *p; double E, Lzsq; } pELz;
 static double Rcfn(double R, void *params) {   /* Utils.h:218-221 */
     pELz *par = (pELz *)params;
-    return par->Lzsq / pow(R, 3) - fjo_dR(par->p, R, 0);
+    return par->Lzsq / fjm_powi(R, 3) - fjo_dR(par->p, R, 0);
 }
 static double GetRc(double Lzsq, double R, const fjo_pot *p) {   /* Utils.h:222-241 */
     pELz par = {p, 0., Lzsq};
@@ -1141,14 +1148,14 @@ typedef struct {
 } delta_t;
 
 static double vsq(const delta_t *d, double x) {  /* Delta.cpp:28-30 */
-    return 2 * (d->E - (fjo_phi(d->p, x, 0.) + .5 * pow(d->Lz / x, 2)));
+    return 2 * (d->E - (fjo_phi(d->p, x, 0.) + .5 * fjm_pow(d->Lz / x, 2)));
 }
 static void eqsMotion(double t, const double *y, double *f, void *params) {  /* Delta.cpp:33-42 */
     (void)t;
     delta_t *d = (delta_t *)params;
     f[0] = y[2];
     f[1] = y[3];
-    f[2] = -fjo_dR(d->p, y[0], y[1]) + pow(d->Lz, 2.0) / pow(y[0], 3.0);
+    f[2] = -fjo_dR(d->p, y[0], y[1]) + fjm_pow(d->Lz, 2.0) / fjm_pow(y[0], 3.0);
     f[3] = -fjo_dz(d->p, y[0], y[1]);
 }
 /* Delta.cpp:53-90; cond 0: z>=0 (Cond_zgt0), 1: pz>=0 (Cond_pzgt0) */
@@ -1174,7 +1181,7 @@ static double getDR(delta_t *d, double x0) {   /* Delta.cpp:93-104 */
         return x0 - R[N];
 }
 static double wrp_getDR(double x0, void *params) { return getDR((delta_t *)params, x0); }
-static double dPhiEff(const delta_t *d, double x) { return fjo_dR(d->p, x, 0) - pow(d->Lz, 2) / pow(x, 3); }
+static double dPhiEff(const delta_t *d, double x) { return fjo_dR(d->p, x, 0) - fjm_pow(d->Lz, 2) / fjm_powi(x, 3); }
 static double ensureVsq(delta_t *d, double x0) {   /* Delta.cpp:116-124 */
     int count = 0;
     while (vsq(d, x0) < 0) {
@@ -1192,7 +1199,7 @@ static double getJr0(delta_t *d) {   /* Delta.cpp:129-164 */
     x1 = ensureVsq(d, x0 / A); dx1 = getDR(d, x1);
     while (dx0 * dx1 > 0) {
         if (fabs(dx1) < fabs(dx0)) { x0 = x1; dx0 = dx1; }
-        else A = pow(A, -0.9);
+        else A = fjm_pow(A, -0.9);
         x1 = ensureVsq(d, x0 * A); dx1 = getDR(d, x1);
         k++;
         if (k == 21) {
@@ -1207,7 +1214,7 @@ static double getJr0(delta_t *d) {   /* Delta.cpp:129-164 */
     return roots_known(wrp_getDR, d, low, high, flow, fhigh, NULL);
 }
 static double ds2dv(const delta_t *d, double v) {   /* Delta.cpp:170-172 */
-    return 2 * (d->zi * d->sqDR * sin(v) - cos(v) * (d->Ri * d->R0 + d->D2 * sin(v)));
+    return 2 * (d->zi * d->sqDR * fjm_sin(v) - fjm_cos(v) * (d->Ri * d->R0 + d->D2 * fjm_sin(v)));
 }
 static double wrp_ds2dv(double v, void *params) { return ds2dv((delta_t *)params, v); }
 static double getv(delta_t *d) {   /* Delta.cpp:189-208 */
@@ -1216,7 +1223,7 @@ static double getv(delta_t *d) {   /* Delta.cpp:189-208 */
     if (fabs(fl) < EPSABSROOTS) d->v0 = low;
     else if (fabs(fh) < EPSABSROOTS) d->v0 = high;
     else d->v0 = roots_known(wrp_ds2dv, d, low, high, fl, fh, NULL);
-    d->sv = sin(d->v0); d->cv = cos(d->v0);
+    d->sv = fjm_sin(d->v0); d->cv = fjm_cos(d->v0);
     return d->v0;
 }
 static double ds2dD2TOT(delta_t *d, double D2h) {   /* Delta.cpp:210-234 */
@@ -1269,7 +1276,7 @@ int fjo_tabulate_delta(fjo_pot *p) {
     double Lzmin = Rmin * sqrt(Rmin * fjo_dR(p, Rmin, 0)), R = .01 * Rmax;
     for (int i = 0; i < 2; i++) {
         Lzgrid[i] = Lzmin + i * (Lzmax - Lzmin) / (double)(NR - 1);
-        double Lzsq = pow(Lzgrid[i], 2);
+        double Lzsq = Lzgrid[i] * Lzgrid[i];
         R = Rgrid[i] = GetRc(Lzsq, R, p);
         double vc2 = R * fjo_dR(p, R, 0);
         Ecgrid[i] = .5 * vc2 + fjo_phi(p, R, 0);
@@ -1277,7 +1284,8 @@ int fjo_tabulate_delta(fjo_pot *p) {
     }
     int st = 0;
     for (int nE = 0; nE < p->ngrid; nE++) {
-        p->Egrid[nE] = Ecgrid[1] + .5 * pow(Vmax[1] * (nE + 1) / (double)p->ngrid, 2);
+        const double tv = Vmax[1] * (nE + 1) / (double)p->ngrid;
+        p->Egrid[nE] = Ecgrid[1] + .5 * (tv * tv);
         int s = 0;
         p->Dgrid[nE] = fjo_best_delta(p, p->Egrid[nE], .25 * Lzgrid[1], Rgrid[1], &s);
         st |= s;
@@ -1305,43 +1313,43 @@ typedef struct {
 } uvorb_t;
 
 static double Phiu(uvorb_t *o, double u) {
-    double R = o->Delta * sinh(u) * sin(o->v), z = o->Delta * cosh(u) * cos(o->v);
+    double R = o->Delta * fjm_sinh(u) * fjm_sin(o->v), z = o->Delta * fjm_cosh(u) * fjm_cos(o->v);
     return fjo_phi(o->pot, R, z);
 }
 static double Phiv(uvorb_t *o, double v) {
-    double R = o->Delta * sinh(o->u) * sin(v), z = o->Delta * cosh(o->u) * cos(v);
+    double R = o->Delta * fjm_sinh(o->u) * fjm_sin(v), z = o->Delta * fjm_cosh(o->u) * fjm_cos(v);
     return fjo_phi(o->pot, R, z);
 }
 static double dV(uvorb_t *o, double v) {   /* uvOrb.cpp:46-48 */
-    return (1 + o->shu2) * Phiv(o, PIH) - (o->shu2 + pow(sin(v), 2)) * Phiv(o, v);
+    return (1 + o->shu2) * Phiv(o, PIH) - (o->shu2 + fjm_sin(v) * fjm_sin(v)) * Phiv(o, v);
 }
 static double vturnfn(uvorb_t *o, double v) {   /* uvOrb.cpp:49-52 */
-    double sv2 = pow(sin(v), 2);
+    double sv2 = fjm_sin(v) * fjm_sin(v);
     return o->E * sv2 + o->Iv + dV(o, v) - .5 * o->Lzsq / (o->Delta * o->Delta * sv2);
 }
 static double Jvint(double psi, void *prm) {   /* uvOrb.cpp:53-56 */
     uvorb_t *o = (uvorb_t *)prm;
-    double v = PIH - (PIH - o->vturn) * cos(psi);
-    return sqrt(MAXD(0., vturnfn(o, v))) * sin(psi);
+    double v = PIH - (PIH - o->vturn) * fjm_cos(psi);
+    return sqrt(MAXD(0., vturnfn(o, v))) * fjm_sin(psi);
 }
 static double dU(uvorb_t *o, double u) {   /* uvOrb.cpp:61-63 */
-    return (pow(sinh(u), 2) + o->sv2) * Phiu(o, u) - (o->sh1sq + o->sv2) * o->Phiu1;
+    return (fjm_sinh(u) * fjm_sinh(u) + o->sv2) * Phiu(o, u) - (o->sh1sq + o->sv2) * o->Phiu1;
 }
 static double dPhiu(uvorb_t *o, double u, double *f) {   /* uvOrb.cpp:64-69 */
-    double R = o->Delta * sinh(u) * sin(o->v), z = o->Delta * cosh(u) * cos(o->v);
+    double R = o->Delta * fjm_sinh(u) * fjm_sin(o->v), z = o->Delta * fjm_cosh(u) * fjm_cos(o->v);
     f[0] = fjo_dR(o->pot, R, z);
     f[1] = fjo_dz(o->pot, R, z);
     return fjo_phi(o->pot, R, z);
 }
 static double uturnfn(double u, void *prm) {   /* uvOrb.cpp:70-73 */
     uvorb_t *o = (uvorb_t *)prm;
-    double shu2 = pow(sinh(u), 2);
+    double shu2 = fjm_sinh(u) * fjm_sinh(u);
     return o->E * (shu2 - o->sh1sq) + o->Er * o->ch1sq - dU(o, u) -
            .5 * o->Lzsq / (o->Delta * o->Delta) * (1 / shu2 - 1 / o->sh1sq);
 }
 static double duturnfn(double u, void *prm) {   /* uvOrb.cpp:74-79 */
     uvorb_t *o = (uvorb_t *)prm;
-    double shu = sinh(u), chu = cosh(u), shu2 = shu * shu, f[2];
+    double shu = fjm_sinh(u), chu = fjm_cosh(u), shu2 = shu * shu, f[2];
     double Phi0 = dPhiu(o, u, f);
     double dPhidu = o->Delta * (f[0] * chu * o->sv + f[1] * shu * o->cv);
     return 2 * (o->E - Phi0) * shu * chu - (shu2 + o->sv2) * dPhidu +
@@ -1349,8 +1357,8 @@ static double duturnfn(double u, void *prm) {   /* uvOrb.cpp:74-79 */
 }
 static double Juint(double psi, void *prm) {   /* uvOrb.cpp:80-83 */
     uvorb_t *o = (uvorb_t *)prm;
-    double u = o->ubar - o->Deltau * cos(psi);
-    return sqrt(MAXD(0., uturnfn(u, o))) * sin(psi);
+    double u = o->ubar - o->Deltau * fjm_cos(psi);
+    return sqrt(MAXD(0., uturnfn(u, o))) * fjm_sin(psi);
 }
 static double wrp_vturnfn(double v, void *prm) { return vturnfn((uvorb_t *)prm, v); }
 
@@ -1453,25 +1461,26 @@ int fjo_actions(const fjo_pot *p, double Delta, double R, double z, double Phi0,
     o.Lz = Lz; o.Lzsq = Lz * Lz; o.Delta = Delta;
     /* pt(Delta, x, p): Uvpt.cpp:11-24 */
     double Delta2 = Delta * Delta, R2 = R * R, z2 = z * z, r2 = R2 + z2;
-    double shu2 = .5 * (r2 - Delta2 + sqrt(pow(Delta2 - r2, 2) + 4 * Delta2 * R2)) / Delta2;
-    double shu = sqrt(shu2), u = asinh(shu);
+    double shu2 = .5 * (r2 - Delta2 + sqrt((Delta2 - r2) * (Delta2 - r2) + 4 * Delta2 * R2)) / Delta2;
+    double shu = sqrt(shu2), u = fjm_asinh(shu);
     double chu2 = 1 + shu2, chu = sqrt(chu2);
-    double cv2 = .5 * (r2 + Delta2 - sqrt(pow(Delta2 + r2, 2) - 4 * Delta2 * z2)) / Delta2;
+    double cv2 = .5 * (r2 + Delta2 - sqrt((Delta2 + r2) * (Delta2 + r2) - 4 * Delta2 * z2)) / Delta2;
     double cv = z < 0 ? -sqrt(cv2) : sqrt(cv2);
-    double v = acos(cv), sv = sin(v), sv2 = 1 - cv2;
+    double v = fjm_acos(cv), sv = fjm_sin(v), sv2 = 1 - cv2;
     double pR = vR, pz = vz, p2 = pR * pR + pz * pz;
     double pu = Delta * (pR * chu * sv + pz * shu * cv);
     double pv = Delta * (pR * shu * cv - pz * chu * sv);
     (void)chu2;
     o.u = u; o.v = v; o.shu2 = shu2; o.sv = sv; o.cv = cv; o.sv2 = sv2;
-    o.E = .5 * p2 + .5 * o.Lzsq / pow(R, 2) + Phi0;
+    o.E = .5 * p2 + .5 * o.Lzsq / (R * R) + Phi0;
     if (o.E >= 0 || isnan(o.E)) { *Jr = 1e31; *Jz = 1e31; return 1; }
     g_cnt.bound_nodes++;
     Getu0(&o, 58);
-    o.sh1sq = pow(sinh(o.umid), 2); o.ch1sq = pow(cosh(o.umid), 2); o.Phiu1 = Phiu(&o, o.umid);
-    o.Er = -o.E * (shu2 - o.sh1sq) / o.ch1sq + .5 * pow(pu, 2) / (Delta2 * o.ch1sq) +
+    o.sh1sq = fjm_sinh(o.umid) * fjm_sinh(o.umid); o.ch1sq = fjm_cosh(o.umid) * fjm_cosh(o.umid);
+    o.Phiu1 = Phiu(&o, o.umid);
+    o.Er = -o.E * (shu2 - o.sh1sq) / o.ch1sq + .5 * (pu * pu) / (Delta2 * o.ch1sq) +
            .5 * o.Lzsq / (Delta2 * o.ch1sq) * (1 / shu2 - 1 / o.sh1sq) + dU(&o, u) / o.ch1sq;
-    o.Iv = .5 * pow(pv, 2) / Delta2 - o.E * sv2 + .5 * o.Lzsq / (Delta2 * sv2) - dV(&o, v);
+    o.Iv = .5 * (pv * pv) / Delta2 - o.E * sv2 + .5 * o.Lzsq / (Delta2 * sv2) - dV(&o, v);
     const double rtwo = sqrt(2);
     /* GetJu */
     GetTurnu(&o, 1);
@@ -1498,21 +1507,21 @@ int fjo_actions(const fjo_pot *p, double Delta, double R, double z, double Phi0,
 double fjo_df_hg(const fjo_df *d, double Jr, double Jphi, double Jz) {
     double h = Jr + d->dphi_h * fabs(Jphi) + d->dz_h * Jz;
     double g = Jr + d->dphi_g * fabs(Jphi) + d->dz_g * Jz;
-    return d->mass / pow(d->J0, 3) * MAXD(0., pow(1. + d->J0 / h, d->A) / (pow(1. + g / d->J0, d->B))) /
-           (pow(TPI, 3.));
+    return d->mass / fjm_powi(d->J0, 3) * MAXD(0., fjm_pow(1. + d->J0 / h, d->A) / (fjm_pow(1. + g / d->J0, d->B))) /
+           (fjm_powi(TPI, 3));
 }
 
 /* df/DF.cpp:78-124 with the double-power-law branch (108-123) live. */
 double fjo_df_eval(const fjo_pot *p, const fjo_df *d, const double *x, const double *v) {
     double R = x[0], Lz = R * v[1], Phi_h = x[2];
-    double H = .5 * (pow(v[0], 2) + pow(v[1], 2) + pow(v[2], 2)) + Phi_h;
+    double H = .5 * (v[0] * v[0] + v[1] * v[1] + v[2] * v[2]) + Phi_h;
     g_cnt.nodes++;
     if (H >= 0) return 0.;
     double Jr, Jz, Jphi = Lz;
     fjo_actions(p, fjo_deltafn(p, H), x[0], x[1], Phi_h, v[0], v[1], v[2], &Jr, &Jz);
     double DFeven = fjo_df_hg(d, Jr, Jphi, Jz);
     if (d->chi == 0.) return DFeven;
-    double t = tanh(d->chi * Jphi / d->J0);
+    double t = fjm_tanh(d->chi * Jphi / d->J0);
     if (d->rot_form == 1) return (1. - d->k) * DFeven + d->k * t * DFeven;
     return DFeven + d->k * t * DFeven;
 }
@@ -1527,10 +1536,16 @@ double fjo_rhofDF(const fjo_pot *p, const fjo_df *d, double R, double z, double 
     double Ve = sqrt(-2 * (Phi_h - fjo_phi(p, far, far)));
     double x[3] = {R, z, Phi_h};
     const double *xa = p->quad ? xpt01 : xpt11, *wa = p->quad ? xw01 : xw11;
-    double s[6] = {0, 0, 0, 0, 0, 0};
+    /* Summation order of the GPU reduction: node n goes to partial sum n mod
+     * 256 in increasing n, then the 256 partials are added pairwise (stride
+     * 128, 64, ..., 1).  The reference leaves the order to the OpenMP
+     * reduction (GaussQuad.cpp:164). */
+    double part[256][6];
+    memset(part, 0, sizeof part);
     for (int i = 0; i < 22; i++)
         for (int j = 0; j < 22; j++)
             for (int k = 0; k < 22; k++) {
+                const int node = (i * 22 + j) * 22 + k;
                 double V[3] = {1e-6 + Ve * xa[i], 1e-6 + Ve * xpt11[j], 1e-6 + Ve * xa[k]};
                 double jacob = Ve * Ve * Ve;
                 double DF = fjo_df_eval(p, d, x, V);
@@ -1538,9 +1553,14 @@ double fjo_rhofDF(const fjo_pot *p, const fjo_df *d, double R, double z, double 
                 double r0 = jacob * DF, r1 = jacob * DF * V[1], r2 = jacob * DF * V[0] * V[0],
                        r3 = jacob * DF * V[1] * V[1], r4 = jacob * DF * V[2] * V[2],
                        r5 = jacob * DF * V[0] * V[2];
+                double *s = part[node & 255];
                 s[0] += w * r0; s[1] += w * r1; s[2] += w * r2;
                 s[3] += w * r3; s[4] += w * r4; s[5] += w * r5;
             }
+    for (int st = 128; st > 0; st >>= 1)
+        for (int t = 0; t < st; t++)
+            for (int m = 0; m < 6; m++) part[t][m] += part[t + st][m];
+    double *s = part[0];
     for (int m = 0; m < 6; m++) out[m] = s[m];
     *vrot = out[1] / out[0];
     *sigR = sqrt(out[2] / out[0]); *sigp = sqrt(out[3] / out[0]); *sigz = sqrt(out[4] / out[0]);
@@ -1650,8 +1670,8 @@ void fjo_vir2(const fjo_pot *p, const double *rhl, const double *vrotl, const do
             }
             (void)Vrot;
             pot += .5 * wi[j] * dens * fjo_phi(p, R, z) * r * r * dr;
-            KRR += .5 * wi[j] * dens * (pow(r * Sigu, 2) + pow(r * Sigp, 2)) * dr;
-            Kzz += .5 * wi[j] * dens * pow(r * Sigv, 2) * dr;
+            KRR += .5 * wi[j] * dens * ((r * Sigu) * (r * Sigu) + (r * Sigp) * (r * Sigp)) * dr;
+            Kzz += .5 * wi[j] * dens * ((r * Sigv) * (r * Sigv)) * dr;
             WRR -= wi[j] * dens * R * fjo_dR(p, R, z) * r * r * dr;
             Wzz -= wi[j] * dens * z * fjo_dz(p, R, z) * r * r * dr;
         }
@@ -1668,7 +1688,7 @@ void fjo_vir2(const fjo_pot *p, const double *rhl, const double *vrotl, const do
         for (int np = 1; np < NP; np++) dens += .5 * (4 * np + 1) * densp[np] * pol[np];
         sum += 2 * TPI * ar[i] * ar[i] * dens;
     }
-    out[0] = pow(ar[NR - 1], 2) * p->Pr[(NR - 1) * NP + 0];
+    out[0] = ar[NR - 1] * ar[NR - 1] * p->Pr[(NR - 1) * NP + 0];
     out[1] = sum;
     out[2] = KRR + Kzz; out[3] = pot; out[4] = pot / (KRR + Kzz);
     out[5] = KRR; out[6] = WRR; out[7] = WRR / KRR;

@@ -4,6 +4,7 @@
 #include <vector>
 
 #include "fj_node.hpp"
+#include "fj_math.h"
 
 using namespace fj;
 
@@ -16,6 +17,30 @@ struct CFn {
 }  // namespace
 
 extern "C" {
+
+// Elementary functions of fj_math.h on n arguments (which: 0 exp, 1 expm1,
+// 2 sinh, 3 cosh, 4 sin, 5 cos, 6 acos, 7 asinh, 8 log, 9 tanh, 10 pow(x, y[i])).
+int fjh_math(int which, int n, const double *x, double *y) {
+    for (int i = 0; i < n; i++) {
+        const double a = x[i];
+        double r = 0;
+        switch (which) {
+        case 0: r = fjm_exp(a); break;
+        case 1: r = fjm_expm1(a); break;
+        case 2: r = fjm_sinh(a); break;
+        case 3: r = fjm_cosh(a); break;
+        case 4: r = fjm_sin(a); break;
+        case 5: r = fjm_cos(a); break;
+        case 6: r = fjm_acos(a); break;
+        case 7: r = fjm_asinh(a); break;
+        case 8: r = fjm_log(a); break;
+        case 9: r = fjm_tanh(a); break;
+        case 10: r = fjm_pow(a, y[i]); break;
+        }
+        y[i] = r;
+    }
+    return 0;
+}
 
 int fjh_qags(double (*f)(double, void *), void *p, double a, double b, double epsabs, double epsrel, int limit,
              int cap, int rule15, double *result, int *ier, int *nqk) {
@@ -82,6 +107,61 @@ int fjh_integral(int nr, const double *ar, int npoly, const double *phil, const 
     out[npsi + 2] = nqk;
     const QKOut r = qk_rule<11>(f, 0, type == 0 ? PI : PIH);
     out[npsi + 3] = r.result; out[npsi + 4] = r.abserr; out[npsi + 5] = r.resabs; out[npsi + 6] = r.resasc;
+    return 0;
+}
+
+// rhofDF sums at one point as k_points + k_setup + k_integrals + k_moments
+// compute them (same per-node code, same reduction order).
+int fjh_rhofDF(int nr, const double *ar, int npoly, const double *phil, const double *Pr, int ng, const double *Eg,
+               const double *Dg, const double *xa, const double *wa, const double *xp, const double *wp, double R,
+               double z, double A, double B, double dphi_h, double dz_h, double dphi_g, double dz_g, double mass,
+               double J0, double chi, double k, int rot_form, double *out6) {
+    LegCoef lc = make_legcoef(npoly);
+    PotView T{nr, npoly, ar, phil, Pr, &lc};
+    std::vector<double> ws(lane_ws_doubles(1000));
+    QagsWork w = lane_work(ws.data(), 1000, 0);
+    const double rtwo = 1.4142135623730951;
+    const double far = 10 * ar[nr - 1];
+    const double Phi_h = pot_phi(T, R, z);
+    const double Ve = sqrt(-2 * (Phi_h - pot_phi(T, far, far)));
+    const double jacob = Ve * Ve * Ve;
+    double part[256][6] = {};
+    for (int node = 0; node < 22 * 22 * 22; node++) {
+        const int i = node / 484, j = (node / 22) % 22, kk = node % 22;
+        const double V[3] = {1e-6 + Ve * xa[i], 1e-6 + Ve * xp[j], 1e-6 + Ve * xa[kk]};
+        const double wt = wa[i] * wp[j] * wa[kk];
+        NodeConst c;
+        double ubar, Deltau, vturn, jr, jz;
+        int wu, wv, bit = 0;
+        long long nev = 0;
+        const int flag = node_setup(T, Eg, Dg, ng, R, z, Phi_h, V, 0.0, true, c, ubar, Deltau, vturn, wu, wv, jr,
+                                    jz, nev, bit);
+        if (flag == 0) continue;
+        if (wu) {
+            JInt f{&T, &c, ubar, Deltau, vturn, 0};
+            jr = rtwo * c.Delta * Deltau * qags_solve<11>(f, 0, PI, EPSABSINTEG, EPSRELINTEG, 1000, w, 1000) / PI;
+        }
+        if (wv) {
+            JInt f{&T, &c, ubar, Deltau, vturn, 1};
+            jz = 2 * rtwo * c.Delta * (PIH - vturn) * qags_solve<11>(f, 0, PIH, EPSABSINTEG, EPSRELINTEG, 1000, w, 1000) / PI;
+        }
+        const double Jphi = R * V[1];
+        const double hh = jr + dphi_h * fabs(Jphi) + dz_h * jz, gg = jr + dphi_g * fabs(Jphi) + dz_g * jz;
+        double DF = mass / fjm_powi(J0, 3) * fj_max(0., fjm_pow(1. + J0 / hh, A) / (fjm_pow(1. + gg / J0, B))) /
+                    fjm_powi(TPI, 3);
+        if (chi != 0.) {
+            const double t = fjm_tanh(chi * Jphi / J0);
+            DF = rot_form == 1 ? (1. - k) * DF + k * t * DF : DF + k * t * DF;
+        }
+        const double jd = jacob * DF;
+        double *s = part[node & 255];
+        s[0] += wt * jd; s[1] += wt * (jd * V[1]); s[2] += wt * (jd * V[0] * V[0]);
+        s[3] += wt * (jd * V[1] * V[1]); s[4] += wt * (jd * V[2] * V[2]); s[5] += wt * (jd * V[0] * V[2]);
+    }
+    for (int st = 128; st > 0; st >>= 1)
+        for (int t = 0; t < st; t++)
+            for (int m = 0; m < 6; m++) part[t][m] += part[t + st][m];
+    for (int m = 0; m < 6; m++) out6[m] = part[0][m];
     return 0;
 }
 

@@ -1,0 +1,146 @@
+"""The gfx950 node code (fj_device.hpp / fj_node.hpp), compiled for the host,
+reproduces the oracle bit for bit (CPU only).
+
+Both sides take every discrete decision of the reference algorithm on the
+same bits: the potential and its gradient, the spheroidal coordinates,
+Getu0 / GetTurnu / GetTurnv with their Brent calls, QAGS on the action
+integrands, the DF and the six sums in one reduction order.  The GPU tests
+then check that the kernels give the same bits as this host build.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+import pytest
+
+from conftest import ROOT
+from oracle import oracle as O
+
+HB = os.path.join(ROOT, "tests", "hostbuild")
+dp = C.POINTER(C.c_double)
+
+X11 = np.array([-1., -0.50118723, -0.25118864, -0.12589254, -0.06309573, -0.03162278, -0.01584893,
+                -0.00794328, -0.00398107, -0.00199526, -0.001, 0.001, 0.00199526, 0.00398107,
+                0.00794328, 0.01584893, 0.03162278, 0.06309573, 0.12589254, 0.25118864, 0.50118723, 1.])
+W11 = np.array([0.49881277, 0.37440568, 0.18764735, 0.09404645, 0.04713488, 0.0236234, 0.01183975,
+                0.00593393, 0.00297401, 0.00149054, 0.00149763, 0.00149763, 0.00149054, 0.00297401,
+                0.00593393, 0.01183975, 0.0236234, 0.04713488, 0.09404645, 0.18764735, 0.37440568, 0.49881277])
+
+
+def P(a):
+    return np.ascontiguousarray(a, dtype=np.float64).ctypes.data_as(dp)
+
+
+@pytest.fixture(scope="module")
+def H():
+    import subprocess
+    subprocess.run(["make", "-s", "-C", HB], check=True)
+    lib = C.CDLL(os.path.join(HB, "libfjhost.so"))
+    lib.fjh_phi_grad.restype = C.c_double
+    lib.fjh_phi_grad.argtypes = [C.c_int, dp, C.c_int, dp, dp, C.c_double, C.c_double, dp, dp]
+    lib.fjh_phi.restype = C.c_double
+    lib.fjh_phi.argtypes = [C.c_int, dp, C.c_int, dp, dp, C.c_double, C.c_double]
+    lib.fjh_actions.argtypes = [C.c_int, dp, C.c_int, dp, dp, C.c_int, dp, dp, C.c_int] + [dp] * 7 + \
+        [C.c_int, dp, dp, C.POINTER(C.c_int)]
+    lib.fjh_rhofDF.argtypes = [C.c_int, dp, C.c_int, dp, dp, C.c_int, dp, dp, dp, dp, dp, dp, C.c_double,
+                               C.c_double] + [C.c_double] * 9 + [C.c_double, C.c_int, dp]
+    lib.fjh_qags.argtypes = [O.FN, C.c_void_p, C.c_double, C.c_double, C.c_double, C.c_double, C.c_int, C.c_int,
+                             C.c_int, dp, C.POINTER(C.c_int), C.POINTER(C.c_int)]
+    return lib
+
+
+@pytest.fixture(scope="module")
+def models():
+    out = {}
+    out["cfg1"] = O.guess_model(40, 0.01, 100.0, 5, 20)[0]
+    out["small"] = O.guess_model(12, 0.05, 50.0, 3, 4)[0]
+    return out
+
+
+def tabs(p):
+    phil, Pr, Pr2 = p.tables()
+    E, D = p.delta()
+    return phil, Pr, E, D
+
+
+@pytest.mark.parametrize("name", ["cfg1", "small"])
+def test_potential_bitwise(H, models, name):
+    p = models[name]
+    phil, Pr, _, _ = tabs(p)
+    rng = np.random.default_rng(11)
+    r = 10 ** rng.uniform(-3.5, 3.5, 3000)
+    th = rng.uniform(0, np.pi, 3000)
+    for R, z in zip(r * np.sin(th), r * np.cos(th)):
+        a, b = C.c_double(), C.c_double()
+        phi = H.fjh_phi_grad(p.nr, P(p.ar), p.npoly, P(phil), P(Pr), R, z, C.byref(a), C.byref(b))
+        assert phi == p.phi(R, z) and a.value == p.dR(R, z) and b.value == p.dz(R, z), (R, z)
+        assert H.fjh_phi(p.nr, P(p.ar), p.npoly, P(phil), P(Pr), R, z) == phi
+
+
+def point_nodes(p, R, z):
+    Phi = p.phi(R, z)
+    far = 10 * p.ar[-1]
+    Ve = np.sqrt(-2 * (Phi - p.phi(far, far)))
+    V = 1e-6 + Ve * np.stack(np.meshgrid(X11, X11, X11, indexing="ij"), -1).reshape(-1, 3)
+    H_ = 0.5 * np.sum(V ** 2, axis=1) + Phi
+    b = H_ < 0
+    return Phi, V[b].copy(), np.array([p.deltafn(h) for h in H_[b]])
+
+
+@pytest.mark.parametrize("name,pts", [("cfg1", [(0.866, 0.5), (60.0, 40.0), (0.011, 0.004)]),
+                                      ("small", [(47.198880926701165, 16.50047391037859), (0.9, 2.0)])])
+def test_actions_bitwise(H, models, name, pts):
+    p = models[name]
+    phil, Pr, E, D = tabs(p)
+    for R, z in pts:
+        Phi, V, Dl = point_nodes(p, R, z)
+        if name == "cfg1":   # a sample keeps the CPU time down
+            V, Dl = V[::7].copy(), Dl[::7].copy()
+        n = len(V)
+        Jr, Jz = np.zeros(n), np.zeros(n)
+        H.fjh_actions(p.nr, P(p.ar), p.npoly, P(phil), P(Pr), p.ngrid, P(E), P(D), n, P(np.full(n, R)),
+                      P(np.full(n, z)), P(np.full(n, Phi)), P(V[:, 0]), P(V[:, 1]), P(V[:, 2]), P(Dl), 48,
+                      P(Jr), P(Jz), None)
+        ref = np.array([p.actions(Dl[i], R, z, V[i, 0], V[i, 1], V[i, 2], Phi) for i in range(n)])
+        bad = np.where((Jr != ref[:, 0]) | (Jz != ref[:, 1]))[0]
+        assert len(bad) == 0, (name, R, z, len(bad), n, bad[:5])
+
+
+@pytest.mark.parametrize("name,R,z", [("small", 47.198880926701165, 16.50047391037859), ("small", 2.0, 1.0),
+                                      ("cfg1", 0.866, 0.5)])
+def test_rhofDF_sums_bitwise(H, models, name, R, z):
+    p = models[name]
+    phil, Pr, E, D = tabs(p)
+    d = O.df_params("Hernquist")
+    out = np.zeros(6)
+    H.fjh_rhofDF(p.nr, P(p.ar), p.npoly, P(phil), P(Pr), p.ngrid, P(E), P(D), P(X11), P(W11), P(X11), P(W11),
+                 R, z, d.A, d.B, d.dphi_h, d.dz_h, d.dphi_g, d.dz_g, d.mass, d.J0, d.chi, d.k, d.rot_form, P(out))
+    _, ref, _ = p.rhofDF(d, R, z)
+    assert np.array_equal(out, ref), (out, ref)
+
+
+def test_qags_stepper_bitwise(H):
+    """The device QAGS stepper against the oracle's dqagse on integrands that
+    subdivide and extrapolate (piecewise-linear Poisson-type, endpoint
+    singularities, kinks), including a 48-interval workspace that overflows."""
+    rng = np.random.default_rng(5)
+    xs = np.sort(np.concatenate([[0.0], 10 ** rng.uniform(-2, 1, 30)]))
+    ys = np.abs(rng.normal(size=31)) + 0.1
+    fs = [lambda x: x ** 2 * np.interp(x, xs, ys),
+          lambda x: np.log(x) / np.sqrt(x) if x > 0 else 0.0,
+          lambda x: np.sqrt(abs(x - 0.3137)),
+          lambda x: np.sqrt(max(0.0, np.cos(3 * x) + 0.2)) * np.sin(x)]
+    for f in fs:
+        cb = O.FN(lambda x, prm: f(x))
+        for a, b in ((0.0, 1.0), (0.0, 3.0)):
+            for cap in (1000, 48):
+                r, ier, nqk = C.c_double(), C.c_int(), C.c_int()
+                H.fjh_qags(cb, None, a, b, 1e-6, 1e-6, 1000, cap, 0, C.byref(r), C.byref(ier), C.byref(nqk))
+                rr, ee = C.c_double(), C.c_double()
+                ne, last = C.c_int(), C.c_int()
+                ier_o = O.lib.fjo_qags(cb, None, a, b, 1e-6, 1e-6, 1000, C.byref(rr), C.byref(ee), C.byref(ne),
+                                       C.byref(last))
+                if ier.value == -1:        # workspace overflow: the kernel reruns the task at full capacity
+                    assert cap == 48 and last.value > 48
+                    continue
+                assert r.value == rr.value and ier.value == ier_o, (f, a, b, cap)

@@ -1,9 +1,12 @@
 """HIP path vs the CPU oracle, through the C ABI, on seeded inputs.
 
-Tolerances: the node phase is FP64 on both sides; the GPU fuses and
-reassociates the potential evaluation (ulp-level differences), so adaptive
-decisions (Brent steps, QAGS bisections) can in rare cases differ; the bar on
-every output is 1e-9 relative, as the reference's FP64 run is matched.
+The FP64 path is bit-reproducible: both sides evaluate every expression in
+the same order without contraction, take transcendentals from the shared
+fj_math.h, and add the six moment sums in one fixed order.  The reference
+algorithm takes discrete decisions on computed values, so anything short of
+bitwise agreement lets a few nodes in 10^5 branch differently; the tests
+therefore require equality and report the largest relative deviation (the
+bar of the specification, 1e-9, is the fallback message).
 """
 import numpy as np
 import pytest
@@ -51,9 +54,7 @@ def test_potential_eval(gctx, cfg1):
     scale = np.abs(ref).max(axis=0)
     print("phi", relerr(phi, ref[:, 0]), "dR", np.max(np.abs(dR - ref[:, 1])) / scale[1],
           "dz", np.max(np.abs(dz - ref[:, 2])) / scale[2])
-    assert relerr(phi, ref[:, 0]) < 1e-13
-    assert np.max(np.abs(dR - ref[:, 1]) / np.maximum(np.abs(ref[:, 1]), 1e-6 * scale[1])) < 1e-11
-    assert np.max(np.abs(dz - ref[:, 2]) / np.maximum(np.abs(ref[:, 2]), 1e-6 * scale[2])) < 1e-11
+    assert np.array_equal(phi, ref[:, 0]) and np.array_equal(dR, ref[:, 1]) and np.array_equal(dz, ref[:, 2])
 
 
 def test_guess_and_poisson(gctx, cfg1):
@@ -61,13 +62,13 @@ def test_guess_and_poisson(gctx, cfg1):
     g = gctx.guess_rhl("Hernquist")
     # l > 0 coefficients of a spherical model are rounding noise: scale by the monopole
     scale = np.abs(rhl[:, :1])
-    assert np.max(np.abs(g - rhl) / scale) < 1e-13
+    assert np.array_equal(g, rhl), np.max(np.abs(g - rhl) / scale)
     phil, Pr, Pr2 = gctx.poisson(rhl)
     rphil, rPr, rPr2 = p.compute_phil(rhl)
     for a, b in ((phil, rphil), (Pr, rPr), (Pr2, rPr2)):
         e = np.max(np.abs(a - b) / np.maximum(np.abs(b), 1e-12 * np.abs(b).max()))
         print("poisson", e)
-        assert e < RTOL
+        assert np.array_equal(a, b), e
 
 
 def test_actions_sample(gctx, cfg1):
@@ -87,7 +88,7 @@ def test_actions_sample(gctx, cfg1):
     er = np.abs(Jr - ref[:, 0]) / np.maximum(np.abs(ref[:, 0]), 1e-12)
     ez = np.abs(Jz - ref[:, 1]) / np.maximum(np.abs(ref[:, 1]), 1e-12)
     print("Jr", np.max(er), np.median(er), "Jz", np.max(ez), np.median(ez))
-    assert np.max(er) < RTOL and np.max(ez) < RTOL
+    assert np.array_equal(Jr, ref[:, 0]) and np.array_equal(Jz, ref[:, 1]), (er.max(), ez.max())
 
 
 def moment_errors(got, ref):
@@ -114,6 +115,7 @@ def test_rhofDF_points(gctx, cfg1):
         e = moment_errors(out6[i], ref)
         print("point", i, e)
         worst = max(worst, e.max())
+        assert np.array_equal(out6[i], ref), (i, e)
     assert worst < RTOL
 
 
@@ -138,7 +140,7 @@ def test_all_nodes_of_one_point(gctx, cfg1):
     ref = np.array([p.actions(D[i], R, z, V[i, 0], V[i, 1], V[i, 2], Phi) for i in range(n)])
     er = np.abs(Jr - ref[:, 0]) / np.maximum(np.abs(ref[:, 0]), 1e-300)
     ez = np.abs(Jz - ref[:, 1]) / np.maximum(np.abs(ref[:, 1]), 1e-300)
-    bad = np.where((er > RTOL) | (ez > RTOL))[0]
+    bad = np.where((Jr != ref[:, 0]) | (Jz != ref[:, 1]))[0]
     print("nodes", n, "bad", len(bad), "max Jr", er.max(), "max Jz", ez.max())
     for i in bad[:10]:
         print("  node", i, V[i], D[i], "gpu", Jr[i], Jz[i], "ref", ref[i])
@@ -148,17 +150,14 @@ def test_all_nodes_of_one_point(gctx, cfg1):
 def test_delta_table(gctx, cfg1):
     """tabulateDelta is ill-conditioned: 1-ulp perturbations of the potential
     tables move the oracle's own Delta^2 entries by up to ~30% (the shell-orbit
-    ellipse fit depends on where the adaptive RK8PD samples the orbit).  The
-    energy grid must agree to rounding; Delta^2 must agree with the oracle as
-    closely as the oracle agrees with itself under such perturbations."""
+    ellipse fit depends on where the adaptive RK8PD samples the orbit); the
+    bit-reproducible arithmetic makes the GPU table equal to the oracle's."""
     p, _ = cfg1
     E, D = gctx.tabulate_delta()
     rE, rD = p.delta()
     d = np.abs(D / rD - 1)
     print("E", relerr(E, rE), "D max", d.max(), "median", np.median(d), np.round(d, 6))
-    assert relerr(E, rE) < 1e-12
-    assert np.median(d) < 1e-3 and d.max() < 0.6
-    assert np.all(D > 0) and np.all(np.isfinite(D))
+    assert np.array_equal(E, rE) and np.array_equal(D, rD), d.max()
     gctx.set_delta(rE, rD)
 
 
@@ -196,7 +195,7 @@ def test_grid_moments_and_projection_small(ctx, small):
     for k, (a, b) in enumerate(zip(g, r)):
         e = np.max(np.abs(a - b) / np.abs(b))
         print("grid moment", k, e)
-        assert e < RTOL
+        assert np.array_equal(a, b), (k, e)
     pr = ctx.project(*r)
     rr = {}
     rho = r[0].copy()
@@ -207,7 +206,7 @@ def test_grid_moments_and_projection_small(ctx, small):
     for name, o in zip(["rhl", "vrotl", "sigRl", "sigpl", "sigzl", "sigRzl"], outs):
         e = table_err(pr[name], o)
         print("project", name, e)
-        assert e < 1e-13
+        assert np.array_equal(pr[name], o), (name, e)
 
 
 def test_scf_iterations_small_grid(ctx, small):
@@ -224,10 +223,10 @@ def test_scf_iterations_small_grid(ctx, small):
         r = p.compute_new_phi(dfo, nthreads=16)
         errs = {k: table_err(g[k], r[k]) for k in ("rhl", "vrotl", "sigRl", "sigpl", "sigzl", "phil", "Pr", "Pr2")}
         print("iter", it, {k: f"{v:.2e}" for k, v in errs.items()})
-        assert max(errs.values()) < RTOL, errs
+        assert all(np.array_equal(g[k], r[k]) for k in errs), errs
         p.set_tables(r["phil"], r["Pr"], r["Pr2"])
         ctx.set_potential(r["phil"], r["Pr"], r["Pr2"])
     vg = ctx.virial(r["rhl"], r["vrotl"], r["sigRl"], r["sigpl"], r["sigzl"])
     vo = p.vir2(r["rhl"], r["vrotl"], r["sigRl"], r["sigpl"], r["sigzl"])
     print("virial gpu", vg, "oracle", vo)
-    assert np.max(np.abs(vg - vo) / np.abs(vo)) < 1e-9
+    assert np.array_equal(vg, vo), np.max(np.abs(vg - vo) / np.abs(vo))
