@@ -30,6 +30,8 @@
 #include <stdint.h>
 #include <string.h>
 
+/* Translation units that include this file are compiled without contraction
+ * (-ffp-contract=off); the pragma keeps these functions so regardless. */
 #if defined(__HIPCC__)
 #pragma clang fp contract(off)
 #endif
@@ -295,9 +297,5 @@ FJM_FN double fjm_pow(double x, double y) {
     if (x == 0) return y > 0 ? 0.0 : fjm_bits2d(0x7ff0000000000000ull);
     return fjm_exp(y * fjm_log(x));
 }
-
-#if defined(__HIPCC__)
-#pragma clang fp contract(on)
-#endif
 
 #endif
