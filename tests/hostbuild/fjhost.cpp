@@ -165,7 +165,8 @@ int fjh_rhofDF(int nr, const double *ar, int npoly, const double *phil, const do
     return 0;
 }
 
-// Actions at n phase-space points exactly as k_setup + k_integrals compute them.
+// Actions at n phase-space points exactly as k_setup + k_integrals compute them;
+// nqk (optional, 2n) receives the number of QK21 rules of the Ju and Jv integrals.
 int fjh_actions(int nr, const double *ar, int npoly, const double *phil, const double *Pr, int ng,
                 const double *Eg, const double *Dg, int n, const double *R, const double *z, const double *Phi,
                 const double *vR, const double *vphi, const double *vz, const double *Dx, int cap, double *Jr,
@@ -196,7 +197,7 @@ int fjh_actions(int nr, const double *ar, int npoly, const double *phil, const d
         }
         Jr[i] = jr;
         Jz[i] = jz;
-        if (nqk) nqk[i] = q1 + q2;
+        if (nqk) { nqk[2 * i] = q1; nqk[2 * i + 1] = q2; }
     }
     return 0;
 }
