@@ -359,40 +359,59 @@ FJ_HD __forceinline__ double qk_rescale(double err, double resabs, double resasc
     return err;
 }
 
-// Kronrod rule with NK abscissae >= 0 (11: QK21, 8: QK15).
-template <int NK, class F>
-FJ_HD __forceinline__ QKOut qk_rule(F &f, double a, double b) {
+// Storage for the 2(NK-1) off-centre function values of one rule.  FvLocal
+// keeps them in the caller's frame; FvStrided points into memory shared by a
+// group of lanes (LDS on the device), element k at p[k * ld].
+template <int N>
+struct FvLocal {
+    double v[N];
+    FJ_HD double &operator[](int k) { return v[k]; }
+};
+struct FvStrided {
+    double *p;
+    int ld;
+    FJ_HD double &operator[](int k) const { return p[k * ld]; }
+};
+
+// Kronrod rule with NK abscissae >= 0 (11: QK21, 8: QK15).  The integrand is
+// evaluated in one rolled loop (one inlined copy of f); the sums are taken in
+// QUADPACK's order: the pairs at the Gauss abscissae xgk[1], xgk[3], ...,
+// then the Kronrod-only pairs xgk[0], xgk[2], ...  fv[2m], fv[2m+1] hold the
+// values left and right of the centre for pair m of that visiting order.
+template <int NK, class F, class FV>
+FJ_HD __forceinline__ QKOut qk_rule(F &f, double a, double b, FV &&fv) {
     const double *xgk = NK == 11 ? kXGK21 : kXGK15;
     const double *wgk = NK == 11 ? kWGK21 : kWGK15;
     const double *wg = NK == 11 ? kWG10 : kWG7;
-    double fv1[NK], fv2[NK];
+    constexpr int NG = (NK - 1) / 2;     // Gauss pairs
     const double center = 0.5 * (a + b), half = 0.5 * (b - a), ahalf = fabs(half);
     const double fc = f(center);
     double rg = 0, rk = fc * wgk[NK - 1], rabs = fabs(rk);
     if (NK % 2 == 0) rg = fc * wg[NK / 2 - 1];
-#pragma unroll
-    for (int j = 0; j < (NK - 1) / 2; j++) {
-        const int jtw = j * 2 + 1;
-        const double absc = half * xgk[jtw];
-        const double f1 = f(center - absc), f2 = f(center + absc), fs = f1 + f2;
-        fv1[jtw] = f1; fv2[jtw] = f2;
-        rg += wg[j] * fs;
-        rk += wgk[jtw] * fs;
-        rabs += wgk[jtw] * (fabs(f1) + fabs(f2));
-    }
-#pragma unroll
-    for (int j = 0; j < NK / 2; j++) {
-        const int jtwm1 = j * 2;
-        const double absc = half * xgk[jtwm1];
-        const double f1 = f(center - absc), f2 = f(center + absc);
-        fv1[jtwm1] = f1; fv2[jtwm1] = f2;
-        rk += wgk[jtwm1] * (f1 + f2);
-        rabs += wgk[jtwm1] * (fabs(f1) + fabs(f2));
+    double f1 = 0;
+#pragma unroll 1
+    for (int s = 0; s < 2 * (NK - 1); s++) {
+        const int m = s >> 1;
+        const int jj = m < NG ? 2 * m + 1 : 2 * (m - NG);
+        const double absc = half * xgk[jj];
+        const double fx = f((s & 1) ? center + absc : center - absc);
+        fv[s] = fx;
+        if (s & 1) {
+            const double fs = f1 + fx;
+            if (m < NG) rg += wg[m] * fs;
+            rk += wgk[jj] * fs;
+            rabs += wgk[jj] * (fabs(f1) + fabs(fx));
+        } else {
+            f1 = fx;
+        }
     }
     const double mean = rk * 0.5;
     double rasc = wgk[NK - 1] * fabs(fc - mean);
 #pragma unroll
-    for (int j = 0; j < NK - 1; j++) rasc += wgk[j] * (fabs(fv1[j] - mean) + fabs(fv2[j] - mean));
+    for (int j = 0; j < NK - 1; j++) {
+        const int m = (j & 1) ? (j >> 1) : NG + (j >> 1);
+        rasc += wgk[j] * (fabs(fv[2 * m] - mean) + fabs(fv[2 * m + 1] - mean));
+    }
     const double err = (rk - rg) * half;
     QKOut o;
     o.result = rk * half;
@@ -407,11 +426,34 @@ FJ_HD __forceinline__ QKOut qk_rule(F &f, double a, double b) {
 // [next_a, next_b] and hands the result to qags_consume().  Lists are
 // 1-based arrays of length cap+1 in a per-lane workspace.
 // ---------------------------------------------------------------------------
+// One lane's view of a list stored interleaved across lanes: element i at
+// p[i * ld], so a wave touching the same index reads one contiguous run.
+template <class T>
+struct Lanes {
+    T *p;
+    long long ld;
+    FJ_HD T &operator[](int i) const { return p[(long long)i * ld]; }
+};
+
+// A lane's QAGS workspace inside a buffer shared by nlanes lanes (layout at
+// lane_work()).  Addresses are formed at each access from the uniform base
+// and stride and the lane index, so they do not occupy registers across the
+// integrand evaluations.
 struct QagsWork {
-    double *alist, *blist, *rlist, *elist;  // [cap+1]
-    int *iord;                              // [cap+1]
-    double *rlist2;                         // [56]
-    double *res3la;                         // [4]
+    double *ws;      // buffer base (uniform)
+    long long ld;    // lanes sharing the buffer (uniform)
+    int n1;          // cap + 1 (uniform)
+    int lane;
+    FJ_HD Lanes<double> list(int k) const { return {ws + (long long)k * n1 * ld + lane, ld}; }
+    FJ_HD Lanes<double> alist() const { return list(0); }
+    FJ_HD Lanes<double> blist() const { return list(1); }
+    FJ_HD Lanes<double> rlist() const { return list(2); }
+    FJ_HD Lanes<double> elist() const { return list(3); }
+    FJ_HD Lanes<double> rlist2() const { return {ws + 4ll * n1 * ld + lane, ld}; }
+    FJ_HD Lanes<double> res3la() const { return {ws + (4ll * n1 + 56) * ld + lane, ld}; }
+    FJ_HD Lanes<int> iord() const {
+        return {reinterpret_cast<int *>(ws + (4ll * n1 + 60) * ld) + lane, ld};
+    }
 };
 
 struct Qags {
@@ -422,7 +464,7 @@ struct Qags {
     int state;
     double next_a, next_b;
     // dqagse locals
-    double area, area1, error1, defab1, a1, b1, a2, b2, defabs, dres, erlarg, erlast, errbnd, errmax,
+    double area, area1, error1, defab1, defabs, dres, erlarg, erlast, errbnd, errmax,
         errsum, ertest, small, correc, result, abserr;
     int ier, ierro, iroff1, iroff2, iroff3, ksgn, ktmin, last, maxerr, nres, nrmax, numrl2, extrap, noext;
     int overflow;  // workspace capacity reached before limit
@@ -437,8 +479,9 @@ FJ_HD __forceinline__ double qags_solve(F &f, double a, double b, double epsabs,
     Qags q;
     qags_begin(q, a, b, epsabs, epsrel, limit);
     int n = 0;
+    FvLocal<2 * (NK - 1)> fv;
     while (q.state != 3) {
-        QKOut r = qk_rule<NK>(f, q.next_a, q.next_b);
+        QKOut r = qk_rule<NK>(f, q.next_a, q.next_b, fv);
         n++;
         qags_consume(q, w, cap, r);
     }
@@ -452,8 +495,8 @@ FJ_HD __forceinline__ double qags_solve(F &f, double a, double b, double epsabs,
 namespace fj {
 
 // QUADPACK dqpsrt (1-based lists).
-FJ_HD __forceinline__ void qpsrt(int limit, int last, int &maxerr, double &ermax, const double *elist,
-                                      int *iord, int &nrmax) {
+FJ_HD __forceinline__ void qpsrt(int limit, int last, int &maxerr, double &ermax, const Lanes<double> &elist,
+                                      const Lanes<int> &iord, int &nrmax) {
     int i, j, k, ido, ibeg, jbnd, isucc, jupbn;
     double errmax, errmin;
     if (last <= 2) { iord[1] = 1; iord[2] = 2; goto L90; }
@@ -498,7 +541,8 @@ L90:
 }
 
 // QUADPACK dqelg (epsilon algorithm), epstab and res3la 1-based.
-FJ_HD __forceinline__ void qelg(int &n, double *epstab, double &result, double &abserr, double *res3la, int &nres) {
+FJ_HD __forceinline__ void qelg(int &n, const Lanes<double> &epstab, double &result, double &abserr,
+                                     const Lanes<double> &res3la, int &nres) {
     const double epmach = DBL_EPS, oflow = DBL_MAXV;
     nres++;
     abserr = oflow;
@@ -567,15 +611,14 @@ FJ_HD __forceinline__ void qags_begin(Qags &q, double a, double b, double epsabs
     if (epsabs <= 0 && epsrel < fmax(50 * DBL_EPS, 0.5e-28)) { q.ier = 6; q.state = 3; }
 }
 
-// Sets up the bisection of interval maxerr for loop index q.last.
+// Sets up the bisection of interval maxerr for loop index q.last: the rule
+// runs on the left half [a1, b1] first.  The interval stays in the workspace
+// until both halves are done, so its ends are read back there.
 FJ_HD __forceinline__ void qags_start_iter(Qags &q, const QagsWork &w, int cap) {
     if (q.last > cap) { q.overflow = 1; q.state = 3; return; }
-    q.a1 = w.alist[q.maxerr];
-    q.b1 = 0.5 * (w.alist[q.maxerr] + w.blist[q.maxerr]);
-    q.a2 = q.b1;
-    q.b2 = w.blist[q.maxerr];
+    const double a = w.alist()[q.maxerr], b = w.blist()[q.maxerr];
     q.erlast = q.errmax;
-    q.next_a = q.a1; q.next_b = q.b1;
+    q.next_a = a; q.next_b = 0.5 * (a + b);
     q.state = 1;
 }
 
@@ -601,7 +644,7 @@ FJ_HD __forceinline__ void qags_finish(Qags &q, const QagsWork &w, int from115) 
     }
     if (go115) {
         double s = 0.0;
-        for (int k = 1; k <= q.last; k++) s = s + w.rlist[k];
+        for (int k = 1; k <= q.last; k++) s = s + w.rlist()[k];
         q.result = s;
         q.abserr = q.errsum;
     }
@@ -617,12 +660,12 @@ FJ_HD __forceinline__ void qags_consume(Qags &q, const QagsWork &w, int cap, con
         q.dres = fabs(q.result);
         q.errbnd = fmax(q.epsabs, q.epsrel * q.dres);
         q.last = 1;
-        w.alist[1] = q.a0; w.blist[1] = q.b0;
-        w.rlist[1] = q.result; w.elist[1] = q.abserr; w.iord[1] = 1;
+        w.alist()[1] = q.a0; w.blist()[1] = q.b0;
+        w.rlist()[1] = q.result; w.elist()[1] = q.abserr; w.iord()[1] = 1;
         if (q.abserr <= 1.0e+02 * epmach * q.defabs && q.abserr > q.errbnd) q.ier = 2;
         if (q.limit == 1) q.ier = 1;
         if (q.ier != 0 || (q.abserr <= q.errbnd && q.abserr != resabs) || q.abserr == 0.0) { q.state = 3; return; }
-        w.rlist2[1] = q.result;
+        w.rlist2()[1] = q.result;
         q.errmax = q.abserr; q.maxerr = 1; q.area = q.result; q.errsum = q.abserr; q.abserr = oflow;
         q.nrmax = 1; q.nres = 0; q.numrl2 = 2; q.ktmin = 0; q.extrap = 0; q.noext = 0;
         q.ierro = 0; q.iroff1 = 0; q.iroff2 = 0; q.iroff3 = 0;
@@ -635,48 +678,49 @@ FJ_HD __forceinline__ void qags_consume(Qags &q, const QagsWork &w, int cap, con
     }
     if (q.state == 1) {
         q.area1 = r.result; q.error1 = r.abserr; q.defab1 = r.resasc;
-        q.next_a = q.a2; q.next_b = q.b2;
+        q.next_a = q.next_b; q.next_b = w.blist()[q.maxerr];   // right half [a2, b2], a2 = b1
         q.state = 2;
         return;
     }
     // state 2: both halves known -- body of the dqagse loop for index last
     const int last = q.last, limit = q.limit;
     const double area2 = r.result, error2 = r.abserr, defab2 = r.resasc;
-    const double area1 = q.area1, error1 = q.error1, a1 = q.a1, b1 = q.b1, a2 = q.a2, b2 = q.b2;
+    const double area1 = q.area1, error1 = q.error1;
+    const double a1 = w.alist()[q.maxerr], b2 = w.blist()[q.maxerr], b1 = 0.5 * (a1 + b2), a2 = b1;
     const double area12 = area1 + area2;
     const double erro12 = error1 + error2;
     q.errsum = q.errsum + erro12 - q.errmax;
-    q.area = q.area + area12 - w.rlist[q.maxerr];
+    q.area = q.area + area12 - w.rlist()[q.maxerr];
     if (!(q.defab1 == error1 || defab2 == error2)) {
-        if (!(fabs(w.rlist[q.maxerr] - area12) > 0.1e-04 * fabs(area12) || erro12 < 0.99 * q.errmax)) {
+        if (!(fabs(w.rlist()[q.maxerr] - area12) > 0.1e-04 * fabs(area12) || erro12 < 0.99 * q.errmax)) {
             if (q.extrap) q.iroff2++;
             else q.iroff1++;
         }
         if (last > 10 && erro12 > q.errmax) q.iroff3++;
     }
-    w.rlist[q.maxerr] = area1;
-    w.rlist[last] = area2;
+    w.rlist()[q.maxerr] = area1;
+    w.rlist()[last] = area2;
     q.errbnd = fmax(q.epsabs, q.epsrel * fabs(q.area));
     if (q.iroff1 + q.iroff2 >= 10 || q.iroff3 >= 20) q.ier = 2;
     if (q.iroff2 >= 5) q.ierro = 3;
     if (last == limit) q.ier = 1;
     if (fmax(fabs(a1), fabs(b2)) <= (1.0 + 0.1e+03 * epmach) * (fabs(a2) + 0.1e+04 * uflow)) q.ier = 4;
     if (error2 > error1) {
-        w.alist[q.maxerr] = a2;
-        w.alist[last] = a1;
-        w.blist[last] = b1;
-        w.rlist[q.maxerr] = area2;
-        w.rlist[last] = area1;
-        w.elist[q.maxerr] = error2;
-        w.elist[last] = error1;
+        w.alist()[q.maxerr] = a2;
+        w.alist()[last] = a1;
+        w.blist()[last] = b1;
+        w.rlist()[q.maxerr] = area2;
+        w.rlist()[last] = area1;
+        w.elist()[q.maxerr] = error2;
+        w.elist()[last] = error1;
     } else {
-        w.alist[last] = a2;
-        w.blist[q.maxerr] = b1;
-        w.blist[last] = b2;
-        w.elist[q.maxerr] = error1;
-        w.elist[last] = error2;
+        w.alist()[last] = a2;
+        w.blist()[q.maxerr] = b1;
+        w.blist()[last] = b2;
+        w.elist()[q.maxerr] = error1;
+        w.elist()[last] = error2;
     }
-    qpsrt(limit, last, q.maxerr, q.errmax, w.elist, w.iord, q.nrmax);
+    qpsrt(limit, last, q.maxerr, q.errmax, w.elist(), w.iord(), q.nrmax);
     if (q.errsum <= q.errbnd) { qags_finish(q, w, 1); return; }
     if (q.ier != 0) { qags_finish(q, w, 0); return; }
     bool next = false;
@@ -684,7 +728,7 @@ FJ_HD __forceinline__ void qags_consume(Qags &q, const QagsWork &w, int cap, con
         q.small = fabs(q.b0 - q.a0) * 0.375;
         q.erlarg = q.errsum;
         q.ertest = q.errbnd;
-        w.rlist2[2] = q.area;
+        w.rlist2()[2] = q.area;
         next = true;
     } else if (q.noext) {
         next = true;
@@ -692,7 +736,7 @@ FJ_HD __forceinline__ void qags_consume(Qags &q, const QagsWork &w, int cap, con
         q.erlarg = q.erlarg - q.erlast;
         if (fabs(b1 - a1) > q.small) q.erlarg = q.erlarg + erro12;
         if (!q.extrap) {
-            if (fabs(w.blist[q.maxerr] - w.alist[q.maxerr]) > q.small) next = true;
+            if (fabs(w.blist()[q.maxerr] - w.alist()[q.maxerr]) > q.small) next = true;
             else { q.extrap = 1; q.nrmax = 2; }
         }
         if (!next) {
@@ -701,18 +745,18 @@ FJ_HD __forceinline__ void qags_consume(Qags &q, const QagsWork &w, int cap, con
                 int jupbnd = last;
                 if (last > (2 + limit / 2)) jupbnd = limit + 3 - last;
                 for (int k = id; k <= jupbnd; k++) {
-                    q.maxerr = w.iord[q.nrmax];
-                    q.errmax = w.elist[q.maxerr];
-                    if (fabs(w.blist[q.maxerr] - w.alist[q.maxerr]) > q.small) { next = true; break; }
+                    q.maxerr = w.iord()[q.nrmax];
+                    q.errmax = w.elist()[q.maxerr];
+                    if (fabs(w.blist()[q.maxerr] - w.alist()[q.maxerr]) > q.small) { next = true; break; }
                     q.nrmax++;
                 }
             }
         }
         if (!next) {
             q.numrl2++;
-            w.rlist2[q.numrl2] = q.area;
+            w.rlist2()[q.numrl2] = q.area;
             double reseps, abseps;
-            qelg(q.numrl2, w.rlist2, reseps, abseps, w.res3la, q.nres);
+            qelg(q.numrl2, w.rlist2(), reseps, abseps, w.res3la(), q.nres);
             q.ktmin++;
             if (q.ktmin > 5 && q.abserr < 0.1e-02 * q.errsum) q.ier = 5;
             if (abseps < q.abserr) {
@@ -725,8 +769,8 @@ FJ_HD __forceinline__ void qags_consume(Qags &q, const QagsWork &w, int cap, con
             }
             if (q.numrl2 == 1) q.noext = 1;
             if (q.ier == 5) { qags_finish(q, w, 0); return; }
-            q.maxerr = w.iord[1];
-            q.errmax = w.elist[q.maxerr];
+            q.maxerr = w.iord()[1];
+            q.errmax = w.elist()[q.maxerr];
             q.nrmax = 1;
             q.extrap = 0;
             q.small = q.small * 0.5;
@@ -743,21 +787,17 @@ FJ_HD __forceinline__ void qags_consume(Qags &q, const QagsWork &w, int cap, con
 
 namespace fj {
 
-// Per-lane QAGS workspace: alist, blist, rlist, elist [cap+1], iord [cap+1]
-// ints, epsilon table [56], res3la [4].
-inline size_t lane_ws_doubles(int cap) { return 4 * (size_t)(cap + 1) + (cap + 1) / 2 + 1 + 56 + 4; }
+// QAGS workspace of nlanes lanes, interleaved by lane: per lane the doubles
+// alist, blist, rlist, elist [cap+1], epsilon table [56], res3la [4], then
+// the ints iord [cap+1]; element i of a list of lane L sits at i * nlanes + L.
+FJ_HD inline long long ws_lane_doubles(int cap) { return 4 * (long long)(cap + 1) + 56 + 4; }
+inline size_t ws_doubles(int cap, long long nlanes) {
+    return (size_t)(ws_lane_doubles(cap) * nlanes + ((long long)(cap + 1) * nlanes + 1) / 2);
+}
 
-FJ_HD inline QagsWork lane_work(double *ws, int cap, long long lane) {
-    const long long per = 4 * (long long)(cap + 1) + (cap + 1) / 2 + 1 + 56 + 4;
-    double *base = ws + lane * per;
+FJ_HD inline QagsWork lane_work(double *ws, int cap, long long lane, long long nlanes) {
     QagsWork w;
-    w.alist = base;
-    w.blist = base + (cap + 1);
-    w.rlist = base + 2 * (cap + 1);
-    w.elist = base + 3 * (cap + 1);
-    w.iord = reinterpret_cast<int *>(base + 4 * (cap + 1));
-    w.rlist2 = base + 4 * (cap + 1) + (cap + 1) / 2 + 1;
-    w.res3la = w.rlist2 + 56;
+    w.ws = ws; w.ld = nlanes; w.n1 = cap + 1; w.lane = (int)lane;
     return w;
 }
 

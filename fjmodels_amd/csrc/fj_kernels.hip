@@ -33,7 +33,7 @@ namespace {
 
 constexpr int NQ1 = 22;                 // nodes per velocity axis (QUADORD)
 constexpr int NQ = NQ1 * NQ1 * NQ1;     // 10,648 nodes per point
-constexpr int QCAP = 48;                // QAGS intervals kept per lane before overflow
+constexpr int QCAP = 64;                // QAGS intervals kept per lane before overflow
 constexpr int QCAP_FULL = 1000;         // the reference's workspace limit (Utils.h:55)
 
 // include/GaussPts.h:76-136
@@ -122,17 +122,19 @@ __device__ inline void store_const(const NodeBuf &nb, long long i, const NodeCon
     f[NF_UBAR * s + i] = ubar; f[NF_DELTAU * s + i] = Deltau; f[NF_VTURN * s + i] = vturn;
 }
 
-__device__ inline void load_const(const NodeBuf &nb, long long i, NodeConst &c, double &ubar, double &Deltau,
-                                  double &vturn) {
-    const double *f = nb.f;
-    const long long s = nb.cap;
-    c.Delta = f[NF_DELTA * s + i]; c.E = f[NF_E * s + i]; c.Lzsq = f[NF_LZSQ * s + i];
-    c.sv = f[NF_SV * s + i]; c.cv = f[NF_CV * s + i]; c.sv2 = f[NF_SV2 * s + i];
-    c.sinv = f[NF_SINV * s + i]; c.cosv = f[NF_COSV * s + i]; c.shuX = f[NF_SHUX * s + i];
-    c.chuX = f[NF_CHUX * s + i]; c.shu2 = f[NF_SHU2 * s + i]; c.sh1sq = f[NF_SH1SQ * s + i];
-    c.K1 = f[NF_K1 * s + i]; c.K2 = f[NF_K2 * s + i]; c.K3 = f[NF_K3 * s + i];
-    c.invsh1sq = f[NF_INVSH1 * s + i]; c.Iv = f[NF_IV * s + i]; c.KV = f[NF_KV * s + i];
-    ubar = f[NF_UBAR * s + i]; Deltau = f[NF_DELTAU * s + i]; vturn = f[NF_VTURN * s + i];
+// Node-record fields of the packed integrand constants (JInt::k) per kind;
+// -1: unused slot.
+__constant__ int c_jfield[2][JK] = {
+    {NF_DELTA, NF_E, NF_UBAR, NF_DELTAU, NF_SINV, NF_COSV, NF_SV2, NF_SH1SQ, NF_K1, NF_K2, NF_K3, NF_INVSH1},
+    {NF_DELTA, NF_E, NF_VTURN, NF_SHUX, NF_CHUX, NF_SHU2, NF_IV, NF_KV, NF_LZSQ, -1, -1, -1}};
+
+__device__ inline void load_jint(const NodeBuf &nb, long long i, int type, JInt &f) {
+    f.type = type;
+#pragma unroll
+    for (int m = 0; m < JK; m++) {
+        const int fld = c_jfield[type][m];
+        f.k[m] = fld >= 0 ? nb.f[fld * nb.cap + i] : 0.0;
+    }
 }
 
 // Device counters
@@ -252,17 +254,22 @@ struct IntegArgs {
 // k_integrals: persistent QAGS over the Ju and Jv task lists (Utils.h:200-212,
 // uvOrb.cpp:226-262).  Each loop step every busy lane applies one QK21 rule to
 // its own current interval; lanes that finish fetch the next task.
-__global__ void __launch_bounds__(256) k_integrals(TabArgs ta, IntegArgs ia) {
+#ifndef FJ_INTEG_WAVES
+#define FJ_INTEG_WAVES 3   // waves per SIMD the register allocation of k_integrals aims at
+#endif
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(FJ_INTEG_WAVES)))
+k_integrals(TabArgs ta, IntegArgs ia) {
     extern __shared__ double smem[];
     const LdsTabs L = stage_tables(ta, smem);
     const long long lane_g = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    const QagsWork w = lane_work(ia.ws, ia.cap, lane_g);
+    const QagsWork w = lane_work(ia.ws, ia.cap, lane_g, (long long)gridDim.x * blockDim.x);
+    const FvStrided fv{smem + tab_lds_doubles(ta.nr, ta.npoly, ta.ngrid) + threadIdx.x, (int)blockDim.x};
     const unsigned int nu = *ia.cnt_u, nv = *ia.cnt_v;
     const unsigned int ntask = ia.ntask_override >= 0 ? (unsigned int)ia.ntask_override : nu + nv;
     int task = -1;    // -1: needs a task, -2: queue drained
     long long node = 0;
-    NodeConst c;
-    JInt f{&L.T, &c, 0, 0, 0, 0};
+    JInt f;
+    f.T = &L.T;
     Qags q;
     unsigned long long nqk = 0, novf = 0;
     while (true) {
@@ -281,12 +288,10 @@ __global__ void __launch_bounds__(256) k_integrals(TabArgs ta, IntegArgs ia) {
                 if (t < ntask) {
                     const unsigned int tt = ia.ntask_override >= 0 ? (unsigned int)ia.ovf_list[t] : t;
                     task = (int)tt;
-                    f.type = tt < nu ? 0 : 1;
-                    node = f.type == 0 ? ia.list_u[tt] : ia.list_v[tt - nu];
-                    double ubar, Deltau, vturn;
-                    load_const(ia.nb, node, c, ubar, Deltau, vturn);
-                    f.ubar = ubar; f.Deltau = Deltau; f.vturn = vturn;
-                    qags_begin(q, 0, f.type == 0 ? PI : PIH, EPSABSINTEG, EPSRELINTEG, 1000);
+                    const int type = tt < nu ? 0 : 1;
+                    node = type == 0 ? ia.list_u[tt] : ia.list_v[tt - nu];
+                    load_jint(ia.nb, node, type, f);
+                    qags_begin(q, 0, type == 0 ? PI : PIH, EPSABSINTEG, EPSRELINTEG, 1000);
                 } else {
                     task = -2;
                 }
@@ -294,7 +299,7 @@ __global__ void __launch_bounds__(256) k_integrals(TabArgs ta, IntegArgs ia) {
         }
         if (__ballot(task >= 0) == 0) break;
         if (task >= 0) {
-            const QKOut r = qk_rule<11>(f, q.next_a, q.next_b);
+            const QKOut r = qk_rule<11>(f, q.next_a, q.next_b, fv);
             nqk++;
             qags_consume(q, w, ia.cap, r);
             if (q.state == 3) {
@@ -303,11 +308,7 @@ __global__ void __launch_bounds__(256) k_integrals(TabArgs ta, IntegArgs ia) {
                     ia.ovf_list[slot] = task;
                     novf++;
                 } else {
-                    const double rtwo = 1.4142135623730951;   // sqrt(2) (uvOrb.cpp:13)
-                    if (f.type == 0)
-                        ia.nb.Jr[node] = rtwo * c.Delta * f.Deltau * q.result / PI;          // uvOrb.cpp:236
-                    else
-                        ia.nb.Jz[node] = 2 * rtwo * c.Delta * (PIH - f.vturn) * q.result / PI;  // uvOrb.cpp:254
+                    (f.type == 0 ? ia.nb.Jr : ia.nb.Jz)[node] = f.action(q.result);
                 }
                 task = -1;
             }
@@ -473,7 +474,7 @@ __global__ void k_poisson(int nr, int np_, const double *ar, const double *rhl, 
     if (idx >= nr * np_) return;
     const int n = idx / np_, np = idx % np_;
     const int l = 2 * np;
-    const QagsWork w = lane_work(ws, cap, idx);
+    const QagsWork w = lane_work(ws, cap, idx, (long long)gridDim.x * blockDim.x);
     IInt fi{ar, rhl, nr, np_, np, l};
     const double Iint = qags_solve<11>(fi, 0., ar[n], 1e-4, 1e-3, 1000, w, cap);
     IExtT fe{fi, ar[n]};
@@ -893,17 +894,16 @@ __global__ void k_debug_integral(TabArgs ta, NodeBuf nb, long long node, int typ
     extern __shared__ double smem[];
     const LdsTabs L = stage_tables(ta, smem);
     if (threadIdx.x != 0) return;
-    NodeConst c;
-    double ubar, Deltau, vturn;
-    load_const(nb, node, c, ubar, Deltau, vturn);
-    JInt f{&L.T, &c, ubar, Deltau, vturn, type};
+    JInt f;
+    f.T = &L.T;
+    load_jint(nb, node, type, f);
     for (int i = 0; i < npsi; i++) out[i] = f(psi[i]);
-    const QagsWork w = lane_work(ws, QCAP_FULL, 0);
+    const QagsWork w = lane_work(ws, QCAP_FULL, 0, 1);
     int ier = 0, nqk = 0;
     out[npsi] = qags_solve<11>(f, 0, type == 0 ? PI : PIH, EPSABSINTEG, EPSRELINTEG, 1000, w, QCAP_FULL, &ier, &nqk);
     out[npsi + 1] = ier;
     out[npsi + 2] = nqk;
-    const QKOut r = qk_rule<11>(f, 0, type == 0 ? PI : PIH);
+    const QKOut r = qk_rule<11>(f, 0, type == 0 ? PI : PIH, FvLocal<20>{});
     out[npsi + 3] = r.result; out[npsi + 4] = r.abserr; out[npsi + 5] = r.resabs; out[npsi + 6] = r.resasc;
 }
 }  // namespace
@@ -964,7 +964,11 @@ template <class T>
 int dalloc(fj_ctx *c, T **p, size_t n) {
     if (*p) { (void)hipFree(*p); *p = nullptr; }
     if (n == 0) return 0;
-    HIPCHK(c, hipMalloc((void **)p, n * sizeof(T)));
+    const hipError_t e = hipMalloc((void **)p, n * sizeof(T));
+    if (e != hipSuccess) {
+        *p = nullptr;
+        return set_err(c, FJ_E_HIP, "hipMalloc of %zu bytes failed: %s", n * sizeof(T), hipGetErrorString(e));
+    }
     return 0;
 }
 
@@ -1013,6 +1017,8 @@ TabArgs tab_args(const fj_ctx *c) {
     return t;
 }
 size_t tab_bytes(const fj_ctx *c) { return tab_lds_doubles(c->nr, c->npoly, c->ngrid) * sizeof(double); }
+// k_integrals: tables + the 20 off-centre QK21 values of each of its 256 lanes
+size_t integ_bytes(const fj_ctx *c) { return tab_bytes(c) + 20 * 256 * sizeof(double); }
 
 int ensure_points(fj_ctx *c, int n) {
     if (n <= c->pt_cap) return 0;
@@ -1031,7 +1037,7 @@ int integ_lanes(fj_ctx *c) {
     hipDeviceProp_t prop;
     HIPCHK(c, hipGetDeviceProperties(&prop, c->device));
     int per_cu = 0;
-    HIPCHK(c, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_integrals, 256, tab_bytes(c)));
+    HIPCHK(c, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_integrals, 256, integ_bytes(c)));
     if (per_cu < 1) per_cu = 1;
     c->integ_lanes = prop.multiProcessorCount * per_cu * 256;
     return c->integ_lanes;
@@ -1054,15 +1060,15 @@ int ensure_nodes(fj_ctx *c, long long nodes) {
     const int lanes = integ_lanes(c);
     if (lanes < 0) return FJ_E_HIP;
     if (lanes > c->ws_lanes || c->ws_cap != QCAP) {
-        if ((r = dalloc(c, &c->d_ws, (size_t)lanes * lane_ws_doubles(QCAP)))) return r;
+        if ((r = dalloc(c, &c->d_ws, ws_doubles(QCAP, lanes)))) return r;
         c->ws_lanes = lanes; c->ws_cap = QCAP;
     }
     return 0;
 }
 
 long long chunk_points(const fj_ctx *c, int npts) {
-    // bound the node scratch to ~4M nodes (~0.7 GB)
-    long long per = (4ll << 20) / NQ;
+    // one launch per phase up to 32M nodes (node records ~6 GB of HBM)
+    long long per = (32ll << 20) / NQ;
     if (per < 1) per = 1;
     return npts < per ? npts : per;
     (void)c;
@@ -1111,7 +1117,7 @@ int run_nodes(fj_ctx *c, int npts) {
         HIPCHK(c, hipEventRecord(ev[3], c->stream));
         IntegArgs ia{nb, c->d_list_u, c->d_list_v, c->d_cnt + 0, c->d_cnt + 1, c->d_cnt + 2,
                      c->d_ovf, c->d_cnt + 3, c->d_ws, QCAP, -1, c->d_ctr};
-        k_integrals<<<lanes / 256, 256, lds, c->stream>>>(ta, ia);
+        k_integrals<<<lanes / 256, 256, integ_bytes(c), c->stream>>>(ta, ia);
         HIPCHK(c, hipGetLastError());
         HIPCHK(c, hipEventRecord(ev[4], c->stream));
         // overflow pass: tasks whose interval list outgrew QCAP, redone with the full limit
@@ -1121,13 +1127,13 @@ int run_nodes(fj_ctx *c, int npts) {
         if (novf) {
             const int fl = 256 * 64;
             if (fl > c->wsfull_lanes) {
-                if ((r = dalloc(c, &c->d_wsfull, (size_t)fl * lane_ws_doubles(QCAP_FULL)))) return r;
+                if ((r = dalloc(c, &c->d_wsfull, ws_doubles(QCAP_FULL, fl)))) return r;
                 c->wsfull_lanes = fl;
             }
             IntegArgs io = ia;
             io.next = c->d_cnt + 4; io.ws = c->d_wsfull; io.cap = QCAP_FULL; io.ntask_override = (int)novf;
             io.ovf_cnt = c->d_cnt + 5;
-            k_integrals<<<fl / 256, 256, lds, c->stream>>>(ta, io);
+            k_integrals<<<fl / 256, 256, integ_bytes(c), c->stream>>>(ta, io);
             HIPCHK(c, hipGetLastError());
         }
         HIPCHK(c, hipEventRecord(ev[5], c->stream));
@@ -1169,7 +1175,8 @@ int ensure_grid_bufs(fj_ctx *c) {
     if (!c->d_rhl) {
         if ((r = dalloc(c, &c->d_rhl, nl)) || (r = dalloc(c, &c->d_gl, nl * 9)) || (r = dalloc(c, &c->d_status, 1)))
             return r;
-        if ((r = dalloc(c, &c->d_pws, nl * lane_ws_doubles(QCAP_FULL)))) return r;
+        // one lane per (radius, order), launched in blocks of 64
+        if ((r = dalloc(c, &c->d_pws, ws_doubles(QCAP_FULL, (long long)((nl + 63) / 64 * 64))))) return r;
     }
     return 0;
 }
@@ -1240,7 +1247,7 @@ int run_actions(fj_ctx *c, int n, const double *vR, const double *vphi, const do
     HIPCHK(c, hipGetLastError());
     IntegArgs ia{nb, c->d_list_u, c->d_list_v, c->d_cnt + 0, c->d_cnt + 1, c->d_cnt + 2,
                  c->d_ovf, c->d_cnt + 3, c->d_ws, QCAP, -1, c->d_ctr};
-    k_integrals<<<c->ws_lanes / 256, 256, lds, c->stream>>>(ta, ia);
+    k_integrals<<<c->ws_lanes / 256, 256, integ_bytes(c), c->stream>>>(ta, ia);
     HIPCHK(c, hipGetLastError());
     unsigned int novf = 0;
     HIPCHK(c, hipMemcpyAsync(&novf, c->d_cnt + 3, sizeof novf, hipMemcpyDeviceToHost, c->stream));
@@ -1248,13 +1255,13 @@ int run_actions(fj_ctx *c, int n, const double *vR, const double *vphi, const do
     if (novf) {
         const int fl = 256 * 64;
         if (fl > c->wsfull_lanes) {
-            if ((r = dalloc(c, &c->d_wsfull, (size_t)fl * lane_ws_doubles(QCAP_FULL)))) return r;
+            if ((r = dalloc(c, &c->d_wsfull, ws_doubles(QCAP_FULL, fl)))) return r;
             c->wsfull_lanes = fl;
         }
         IntegArgs io = ia;
         io.next = c->d_cnt + 4; io.ws = c->d_wsfull; io.cap = QCAP_FULL; io.ntask_override = (int)novf;
         io.ovf_cnt = c->d_cnt + 5;
-        k_integrals<<<fl / 256, 256, lds, c->stream>>>(ta, io);
+        k_integrals<<<fl / 256, 256, integ_bytes(c), c->stream>>>(ta, io);
         HIPCHK(c, hipGetLastError());
         HIPCHK(c, hipStreamSynchronize(c->stream));
     }
@@ -1632,7 +1639,7 @@ int fj_debug_nodes(fj_ctx *c, int n, double *out, int *flags) {
 int fj_debug_integral(fj_ctx *c, long long node, int type, int npsi, const double *psi, double *out) {
     if (!c || node < 0 || node >= c->last_nodes || npsi < 0 || !out) return FJ_E_ARG;
     double *d = nullptr;
-    const size_t nws = lane_ws_doubles(QCAP_FULL);
+    const size_t nws = ws_doubles(QCAP_FULL, 1);
     const size_t nout = (size_t)npsi + 8;
     HIPCHK(c, hipMalloc((void **)&d, (npsi + nout + nws) * sizeof(double)));
     if (npsi) HIPCHK(c, hipMemcpy(d, psi, npsi * sizeof(double), hipMemcpyHostToDevice));

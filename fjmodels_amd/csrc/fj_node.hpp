@@ -82,36 +82,66 @@ struct Vturn {    // vturnfn (uvOrb.cpp:49-52) with dV (46-48)
 };
 
 // Action integrands Juint / Jvint (uvOrb.cpp:53-56, 80-83)
+// The node constants one action integrand reads (uvOrb.cpp:61-67, 70-73,
+// 84-90, 226-262), packed so that a lane holds only those of its own kind:
+//   k[0] Delta, k[1] E, then
+//   Ju (type 0): ubar, Deltau, sinv, cosv, sv2, sh1sq, K1, K2, K3, invsh1sq
+//   Jv (type 1): vturn, shuX, chuX, shu2, Iv, KV, Lzsq
+constexpr int JK = 12;
+
 struct JInt {
     const PotView *T;
-    const NodeConst *c;
-    double ubar, Deltau, vturn;
+    double k[JK];
     int type;   // 0: Ju, 1: Jv
     FJ_HD double operator()(double psi) const {
         double sp, cp;
         fjm_sincos(psi, &sp, &cp);
         double F;
-        const double D = c->Delta;
+        const double D = k[0], E = k[1];
         if (type == 0) {
-            const double u = ubar - Deltau * cp;
+            // Juint (uvOrb.cpp:226-231): u = ubar - Deltau cos(psi)
+            const double u = k[2] - k[3] * cp;
             double shu, chu;
-        fjm_sinhcosh(u, &shu, &chu);
-        const double shu2 = shu * shu;
-            const double Phi = pot_phi(*T, D * shu * c->sinv, D * chu * c->cosv);
-            const double dU = (shu2 + c->sv2) * Phi - c->K2;
-            F = c->E * (shu2 - c->sh1sq) + c->K1 - dU - c->K3 * (1 / shu2 - c->invsh1sq);
+            fjm_sinhcosh(u, &shu, &chu);
+            const double shu2 = shu * shu;
+            const double Phi = pot_phi(*T, D * shu * k[4], D * chu * k[5]);
+            const double dU = (shu2 + k[6]) * Phi - k[9];
+            F = E * (shu2 - k[7]) + k[8] - dU - k[10] * (1 / shu2 - k[11]);
         } else {
-            const double v = PIH - (PIH - vturn) * cp;
+            // Jvint (uvOrb.cpp:245-250): v = pi/2 - (pi/2 - vturn) cos(psi)
+            const double v = PIH - (PIH - k[2]) * cp;
             double s, cs;
             fjm_sincos(v, &s, &cs);
             const double sv2 = s * s;
-            const double Phi = pot_phi(*T, D * c->shuX * s, D * c->chuX * cs);
-            const double dV = c->KV - (c->shu2 + sv2) * Phi;
-            F = c->E * sv2 + c->Iv + dV - .5 * c->Lzsq / (D * D * sv2);
+            const double Phi = pot_phi(*T, D * k[3] * s, D * k[4] * cs);
+            const double dV = k[7] - (k[5] + sv2) * Phi;
+            F = E * sv2 + k[6] + dV - .5 * k[8] / (D * D * sv2);
         }
         return sqrt(fj_max(0., F)) * sp;
     }
+    // Jr (uvOrb.cpp:236) or Jz (uvOrb.cpp:254) from the integral
+    FJ_HD double action(double integral) const {
+        const double rtwo = 1.4142135623730951;   // sqrt(2) (uvOrb.cpp:13)
+        if (type == 0) return rtwo * k[0] * k[3] * integral / PI;
+        return 2 * rtwo * k[0] * (PIH - k[2]) * integral / PI;
+    }
 };
+
+FJ_HD inline JInt make_jint(const PotView *T, const NodeConst &c, double ubar, double Deltau, double vturn,
+                            int type) {
+    JInt f;
+    f.T = T;
+    f.type = type;
+    f.k[0] = c.Delta; f.k[1] = c.E;
+    if (type == 0) {
+        f.k[2] = ubar; f.k[3] = Deltau; f.k[4] = c.sinv; f.k[5] = c.cosv; f.k[6] = c.sv2; f.k[7] = c.sh1sq;
+        f.k[8] = c.K1; f.k[9] = c.K2; f.k[10] = c.K3; f.k[11] = c.invsh1sq;
+    } else {
+        f.k[2] = vturn; f.k[3] = c.shuX; f.k[4] = c.chuX; f.k[5] = c.shu2; f.k[6] = c.Iv; f.k[7] = c.KV;
+        f.k[8] = c.Lzsq; f.k[9] = 0; f.k[10] = 0; f.k[11] = 0;
+    }
+    return f;
+}
 
 
 // Everything of df()/uvOrb before the two QAGS integrals for one node.

@@ -44,8 +44,9 @@ int fjh_math(int which, int n, const double *x, double *y) {
 
 int fjh_qags(double (*f)(double, void *), void *p, double a, double b, double epsabs, double epsrel, int limit,
              int cap, int rule15, double *result, int *ier, int *nqk) {
-    std::vector<double> ws(lane_ws_doubles(cap));
-    QagsWork w = lane_work(ws.data(), cap, 0);
+    // lane 2 of 3 lanes sharing an interleaved workspace
+    std::vector<double> ws(ws_doubles(cap, 3));
+    QagsWork w = lane_work(ws.data(), cap, 2, 3);
     CFn fn{f, p};
     if (rule15) *result = qags_solve<8>(fn, a, b, epsabs, epsrel, limit, w, cap, ier, nqk);
     else *result = qags_solve<11>(fn, a, b, epsabs, epsrel, limit, w, cap, ier, nqk);
@@ -97,15 +98,15 @@ int fjh_integral(int nr, const double *ar, int npoly, const double *phil, const 
     int wu, wv, bit = 0;
     long long nev = 0;
     node_setup(T, Eg, Dg, ng, R, z, Phi, V, Dx, false, c, ubar, Deltau, vturn, wu, wv, jr, jz, nev, bit);
-    JInt f{&T, &c, ubar, Deltau, vturn, type};
+    const JInt f = make_jint(&T, c, ubar, Deltau, vturn, type);
     for (int i = 0; i < npsi; i++) out[i] = f(psi[i]);
-    std::vector<double> ws(lane_ws_doubles(1000));
-    QagsWork w = lane_work(ws.data(), 1000, 0);
+    std::vector<double> ws(ws_doubles(1000, 1));
+    QagsWork w = lane_work(ws.data(), 1000, 0, 1);
     int ier = 0, nqk = 0;
     out[npsi] = qags_solve<11>(f, 0, type == 0 ? PI : PIH, EPSABSINTEG, EPSRELINTEG, 1000, w, 1000, &ier, &nqk);
     out[npsi + 1] = ier;
     out[npsi + 2] = nqk;
-    const QKOut r = qk_rule<11>(f, 0, type == 0 ? PI : PIH);
+    const QKOut r = qk_rule<11>(f, 0, type == 0 ? PI : PIH, FvLocal<20>{});
     out[npsi + 3] = r.result; out[npsi + 4] = r.abserr; out[npsi + 5] = r.resabs; out[npsi + 6] = r.resasc;
     return 0;
 }
@@ -118,9 +119,8 @@ int fjh_rhofDF(int nr, const double *ar, int npoly, const double *phil, const do
                double J0, double chi, double k, int rot_form, double *out6) {
     LegCoef lc = make_legcoef(npoly);
     PotView T{nr, npoly, ar, phil, Pr, &lc};
-    std::vector<double> ws(lane_ws_doubles(1000));
-    QagsWork w = lane_work(ws.data(), 1000, 0);
-    const double rtwo = 1.4142135623730951;
+    std::vector<double> ws(ws_doubles(1000, 1));
+    QagsWork w = lane_work(ws.data(), 1000, 0, 1);
     const double far = 10 * ar[nr - 1];
     const double Phi_h = pot_phi(T, R, z);
     const double Ve = sqrt(-2 * (Phi_h - pot_phi(T, far, far)));
@@ -138,12 +138,12 @@ int fjh_rhofDF(int nr, const double *ar, int npoly, const double *phil, const do
                                     jz, nev, bit);
         if (flag == 0) continue;
         if (wu) {
-            JInt f{&T, &c, ubar, Deltau, vturn, 0};
-            jr = rtwo * c.Delta * Deltau * qags_solve<11>(f, 0, PI, EPSABSINTEG, EPSRELINTEG, 1000, w, 1000) / PI;
+            const JInt f = make_jint(&T, c, ubar, Deltau, vturn, 0);
+            jr = f.action(qags_solve<11>(f, 0, PI, EPSABSINTEG, EPSRELINTEG, 1000, w, 1000));
         }
         if (wv) {
-            JInt f{&T, &c, ubar, Deltau, vturn, 1};
-            jz = 2 * rtwo * c.Delta * (PIH - vturn) * qags_solve<11>(f, 0, PIH, EPSABSINTEG, EPSRELINTEG, 1000, w, 1000) / PI;
+            const JInt f = make_jint(&T, c, ubar, Deltau, vturn, 1);
+            jz = f.action(qags_solve<11>(f, 0, PIH, EPSABSINTEG, EPSRELINTEG, 1000, w, 1000));
         }
         const double Jphi = R * V[1];
         const double hh = jr + dphi_h * fabs(Jphi) + dz_h * jz, gg = jr + dphi_g * fabs(Jphi) + dz_g * jz;
@@ -173,9 +173,8 @@ int fjh_actions(int nr, const double *ar, int npoly, const double *phil, const d
                 double *Jz, int *nqk) {
     LegCoef lc = make_legcoef(npoly);
     PotView T{nr, npoly, ar, phil, Pr, &lc};
-    std::vector<double> ws(lane_ws_doubles(cap));
-    QagsWork w = lane_work(ws.data(), cap, 0);
-    const double rtwo = 1.4142135623730951;
+    std::vector<double> ws(ws_doubles(cap, 1));
+    QagsWork w = lane_work(ws.data(), cap, 0, 1);
     for (int i = 0; i < n; i++) {
         NodeConst c;
         double ubar, Deltau, vturn, jr, jz;
@@ -186,14 +185,14 @@ int fjh_actions(int nr, const double *ar, int npoly, const double *phil, const d
                    jr, jz, nev, bit);
         int q1 = 0, q2 = 0, ier = 0;
         if (wu) {
-            JInt f{&T, &c, ubar, Deltau, vturn, 0};
+            const JInt f = make_jint(&T, c, ubar, Deltau, vturn, 0);
             const double res = qags_solve<11>(f, 0, PI, EPSABSINTEG, EPSRELINTEG, 1000, w, cap, &ier, &q1);
-            jr = rtwo * c.Delta * Deltau * res / PI;
+            jr = f.action(res);
         }
         if (wv) {
-            JInt f{&T, &c, ubar, Deltau, vturn, 1};
+            const JInt f = make_jint(&T, c, ubar, Deltau, vturn, 1);
             const double res = qags_solve<11>(f, 0, PIH, EPSABSINTEG, EPSRELINTEG, 1000, w, cap, &ier, &q2);
-            jz = 2 * rtwo * c.Delta * (PIH - vturn) * res / PI;
+            jz = f.action(res);
         }
         Jr[i] = jr;
         Jz[i] = jz;
