@@ -1,0 +1,134 @@
+// fjmodels -- self-consistent f(J) models on one MI355X.
+//
+// The driver of fJmodels.cpp:33-186 over the facade: read the parameter
+// file, set the grid, build the guess potential, tabulate Delta, iterate
+// computeNewPhi, write models/*.out and print the virial diagnostic.
+//
+//   fjmodels [param.txt] [--outdir DIR] [--device N]
+//
+// Besides the reference's keys the parameter file may set nr, rmin, rmax,
+// ntheta, npoly, ngrid (grid), quad (log22 | tanhsinh22), rot_form / rot_k
+// (rotation term) and tol (stop when the potential changes by less).
+#include <math.h>
+#include <stdio.h>
+#include <string.h>
+#include <sys/stat.h>
+
+#include <chrono>
+#include <string>
+
+#include "fjm.h"
+
+using namespace fjm;
+
+namespace {
+
+double now() {
+    using C = std::chrono::steady_clock;
+    return std::chrono::duration<double>(C::now().time_since_epoch()).count();
+}
+
+void copy_mat(double **dst, double **src) { memcpy(dst[0], src[0], sizeof(double) * NR * NPOLY); }
+
+// twoComp (fJmodels.cpp:38-81); the virial diagnostic runs on the total
+// density, which the reference does not print for two components.
+void twoComp(fJParams P, const std::string &dir) {
+    printf("\nSetting up two-component f(J) model...\n");
+    double **rhl2 = mat(NR, NPOLY), **vrotl2 = mat(NR, NPOLY), **sigRl2 = mat(NR, NPOLY),
+           **sigpl2 = mat(NR, NPOLY), **sigzl2 = mat(NR, NPOLY), **sigRzl2 = mat(NR, NPOLY);
+    SetModel(P, 1);
+    Potential p(1);
+    p.selectGuessRho(P.modName != "null" ? P.modName : "Hernquist");
+    p.computeGuessRhl();
+    p.computePhil(p.rhlP);
+    SetModel(P, 2);
+    Potential p2(2);
+    p2.selectGuessRho(P.modName2 != "null" ? P.modName2 : "Hernquist");
+    p2.computeGuessRhl();
+    p2.computePhil(p2.rhlP);
+    updatePhil(&p, &p2);
+    tabulateDelta(&p);
+    double **old = mat(NR, NPOLY);
+    for (int k = 0; k < P.itermax; k++) {
+        printf("\n Iter:%d\n", k);
+        const double t0 = now();
+        copy_mat(old, phil);
+        setDF(&P, &p, 1);
+        computeNewPhi(&p, rhl, sigRl, sigpl, sigzl, sigRzl, vrotl);
+        setDF(&P, &p, 2);
+        computeNewPhi(&p2, rhl2, sigRl2, sigpl2, sigzl2, sigRzl2, vrotl2);
+        updatePhil(&p, &p2);
+        writeOut(P, k, 1, rhl, sigRl, sigpl, sigzl, sigRzl, vrotl, p.philP, p.PrP, p.Pr2P, dir);
+        writeOut(P, k, 2, rhl2, sigRl2, sigpl2, sigzl2, sigRzl2, vrotl2, p2.philP, p2.PrP, p2.Pr2P, dir);
+        const double d = potentialDistance(old, phil);
+        printf("-- iteration %d: %.3f s, potential change %.3e\n", k, now() - t0, d);
+        if (P.tol > 0 && d < P.tol) {
+            printf("-- converged (tolerance %g)\n", P.tol);
+            break;
+        }
+    }
+    double **M[] = {rhl2, vrotl2, sigRl2, sigpl2, sigzl2, sigRzl2, old};
+    for (double **m : M) free_mat(m);
+}
+
+// oneComp (fJmodels.cpp:86-144)
+void oneComp(fJParams P, const std::string &dir) {
+    printf("\nSetting up one-component f(J) model...\n");
+    SetModel(P);
+    Potential p;
+    p.selectGuessRho(P.modName != "null" ? P.modName : "Hernquist");
+    p.computeGuessRhl();
+    p.computePhil();
+    tabulateDelta(&p);
+    setDF(&P, &p, 1);
+    double **old = mat(NR, NPOLY);
+    for (int k = 0; k < P.itermax; k++) {
+        printf("\n Iter:%d\n", k);
+        const double t0 = now();
+        copy_mat(old, phil);
+        computeNewPhi(&p, rhl, sigRl, sigpl, sigzl, sigRzl, vrotl);
+        writeOut(P, k, 1, rhl, sigRl, sigpl, sigzl, sigRzl, vrotl, phil, Pr, Pr2, dir);
+        vir2(&p);
+        const double d = potentialDistance(old, phil);
+        printf("-- iteration %d: %.3f s, potential change %.3e\n", k, now() - t0, d);
+        if (P.tol > 0 && d < P.tol) {
+            printf("-- converged (tolerance %g)\n", P.tol);
+            break;
+        }
+    }
+    free_mat(old);
+}
+
+}  // namespace
+
+int main(int argc, char **argv) {
+    const double start = now();
+    const char *param = "param.txt";
+    std::string dir = "models";
+    int device = 0;
+    for (int i = 1; i < argc; i++) {
+        if (!strcmp(argv[i], "--outdir") && i + 1 < argc) dir = argv[++i];
+        else if (!strcmp(argv[i], "--device") && i + 1 < argc) device = atoi(argv[++i]);
+        else if (argv[i][0] != '-') param = argv[i];
+        else {
+            fprintf(stderr, "usage: %s [param.txt] [--outdir DIR] [--device N]\n", argv[0]);
+            return 2;
+        }
+    }
+    try {
+        fJParams P = readParam(param);
+        printParam(&P);
+        SetGrid(P.rmax, P.nr, P.ntheta, P.npoly, P.ngrid, P.rmin > 0 ? P.rmin : 0, device);
+        set_quad_rule(P.quad);
+        set_poisson_mode(P.poisson);
+        mkdir(dir.c_str(), 0755);
+        printf("=\n======================================================================\n");
+        if (components == 1) oneComp(P, dir);
+        else twoComp(P, dir);
+    } catch (const Error &e) {
+        fprintf(stderr, "%s\n", e.what());
+        return e.code ? e.code : 1;
+    }
+    printf("\n----> Elapsed time of computation: %7.5f s\n", now() - start);
+    return 0;
+}

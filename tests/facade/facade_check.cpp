@@ -1,0 +1,59 @@
+// Host-only checks of the C++ facade (no GPU): parameter file parsing, the
+// models/*.out file name and layout, and the .out / .f64 round trip.
+// Prints "key value" lines that tests/test_facade.py compares.
+#include <stdio.h>
+#include <string.h>
+
+#include "../../fjmodels_amd/host/fjm.h"
+
+using namespace fjm;
+
+int main(int argc, char **argv) {
+    if (argc < 3) return 2;
+    try {
+        fJParams P = readParam(argv[1]);
+        printf("components %u\n", components);
+        printf("itermax %d\n", P.itermax);
+        printf("model %s\n", P.modName.c_str());
+        printf("outfile %s\n", P.outName.c_str());
+        printf("dphi_h %.17g dz_h %.17g dphi_g %.17g dz_g %.17g\n", P.dphi_h_in, P.dz_h_in, P.dphi_g_in, P.dz_g_in);
+        printf("chi %.17g mass %.17g r0 %.17g q %.17g\n", P.chi, P.mass, P.r0, P.q);
+        printf("A2 %.17g B2 %.17g mass2 %.17g\n", P.alpha_2, P.beta_2, P.mass_2);
+        printf("grid %d %d %d %d %.17g %.17g\n", P.nr, P.ntheta, P.npoly, P.ngrid, P.rmin, P.rmax);
+        printf("rot %d %.17g tol %.17g quad %d\n", P.rot_form, P.rot_k, P.tol, P.quad);
+        printf("name1 %s\n", composeName(P, 3, 1).c_str());
+        printf("name2 %s\n", composeName(P, 3, 2).c_str());
+        SetGrid(P.rmax, P.nr, P.ntheta, P.npoly, P.ngrid, P.rmin > 0 ? P.rmin : 0);
+        printf("ar");
+        for (int i = 0; i < NR; i++) printf(" %.17g", ar[i]);
+        printf("\n");
+        double **M[9] = {rhl, vrotl, sigRl, sigpl, sigzl, sigRzl, phil, Pr, Pr2};
+        for (int m = 0; m < 9; m++)
+            for (int i = 0; i < NR; i++)
+                for (int j = 0; j < NPOLY; j++) M[m][i][j] = (m + 1) * 1.0 / 3.0 + i * 1e-3 - j * 1e5 / 7.0;
+        P.outName = "null";
+        const std::string path = writeOut(P, 7, 1, rhl, sigRl, sigpl, sigzl, sigRzl, vrotl, phil, Pr, Pr2, argv[2]);
+        printf("path %s\n", path.c_str());
+        OutFile full = readOut(path + ".f64"), shrt = readOut(path);
+        int exact = 1;
+        for (int m = 0; m < 9; m++)
+            for (int i = 0; i < NR; i++)
+                for (int j = 0; j < NPOLY; j++) exact &= full.m[m][(size_t)i * NPOLY + j] == M[m][i][j];
+        for (int i = 0; i < NR; i++) exact &= full.ar[i] == ar[i];
+        printf("f64_roundtrip %d\n", exact);
+        double maxrel = 0;
+        for (int m = 0; m < 9; m++)
+            for (int i = 0; i < NR; i++)
+                for (int j = 0; j < NPOLY; j++) {
+                    const double a = shrt.m[m][(size_t)i * NPOLY + j], b = M[m][i][j];
+                    const double r = b != 0 ? (a - b) / b : a;
+                    maxrel = r > maxrel ? r : (-r > maxrel ? -r : maxrel);
+                }
+        printf("out_maxrel %.3e\n", maxrel);
+        printf("header %d %d %d %g %g\n", shrt.nr, shrt.npoly, shrt.ngauss, shrt.alpha, shrt.beta);
+    } catch (const Error &e) {
+        printf("error %d %s\n", e.code, e.what());
+        return 1;
+    }
+    return 0;
+}
