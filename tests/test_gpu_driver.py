@@ -36,7 +36,7 @@ NAMES = ["rhl", "vrotl", "sigRl", "sigpl", "sigzl", "sigRzl", "phil", "Pr", "Pr2
 
 def read_f64(path, nr, npoly):
     tok = open(path).read().split()
-    v = np.array([float(x) for x in tok[13:]])
+    v = np.array([float(x) for x in tok[12:]])   # header: 3 + 4 + 3 + 2 values
     ar, rest = v[:nr], v[nr:]
     return ar, {n: rest[i * nr * npoly:(i + 1) * nr * npoly].reshape(nr, npoly) for i, n in enumerate(NAMES)}
 
