@@ -58,12 +58,46 @@ def cpu_baseline(npoly, ntheta, nr, ar, tables, delta, sample_every=25, threads=
                       f"oracle C restatement, {threads} threads, {dt:.1f} s"}
 
 
+def mixed_leg(ctx, model, prev, last, steps, pts):
+    """Mixed precision (FP32 action integrands and DF, FP64 roots, Delta and
+    Poisson) with the exact block-scan Poisson solve.  Accuracy: one iteration
+    from the same potential as the last FP64 iteration, every coefficient's
+    difference scaled by the monopole at its radius; speed: `steps`
+    iterations continuing from there."""
+    import numpy as np
+    import torch
+    import fjmodels_amd as fj
+    ref = last[0]
+    ctx.set_potential(*prev)
+    model.total = prev
+    model.mode = fj.FJ_POISSON_BLOCKSCAN
+    ctx.set_precision(fj.FJ_PREC_MIXED)
+    got = model.iterate()[0]
+    err = {k: float(np.max(np.abs(got[k] - ref[k]) / np.abs(ref[k][:, :1]))) for k in ("rhl", "phil", "sigRl", "sigzl")}
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    st = []
+    for _ in range(steps):
+        model.iterate()
+        st.append(ctx.stats())
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / steps
+    ctx.set_precision(fj.FJ_PREC_FP64)
+    model.mode = fj.FJ_POISSON_QUADPACK
+    return {"value": pts / dt, "ms_per_step": 1e3 * dt, "poisson": "blockscan",
+            "err_vs_fp64": err,
+            "phases_ms": {k: float(np.mean([x[k] for x in st])) for k in
+                          ("ms_setup", "ms_integrals", "ms_overflow", "ms_moments", "ms_poisson")},
+            "counters": {k: int(np.mean([x[k] for x in st])) for k in ("qk_calls", "overflow_tasks", "integral_lanes")}}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision / block-scan leg")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run a 1 GiB device-to-device copy (reference for FETCH_SIZE/WRITE_SIZE counters)")
     args = ap.parse_args()
@@ -90,8 +124,10 @@ def main():
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     stats = []
+    last = None
     for _ in range(args.steps):
-        model.iterate()
+        prev = [x.copy() for x in model.total]
+        last = model.iterate()
         stats.append(ctx.stats())
     torch.cuda.synchronize()
     dt = time.perf_counter() - t0
@@ -122,6 +158,8 @@ def main():
         "counters": {k: int(np.mean([s[k] for s in stats])) for k in
                      ("bound_nodes", "tasks", "qk_calls", "overflow_tasks", "brent_iters", "integral_lanes")},
     }
+    if not args.no_mixed:
+        line["mixed"] = mixed_leg(ctx, model, prev, last, args.steps, pts)
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(npoly, ntheta, nr, ar, guess_tables, delta)
     if args.copy_calibration:

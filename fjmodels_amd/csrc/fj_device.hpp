@@ -62,13 +62,43 @@ FJ_HD __forceinline__ double evenleg_step(const LegCoef *lc, int np, double pm2,
 }
 
 // Potential tables as the node kernels see them (LDS or global).
+// Radial bracket table: cell k of NLUT equal cells in log2(r) over
+// [ar[0], ar[nr-1]] holds the bracket index of the cell's left edge; a lookup
+// plus a short exact walk gives the same bracket as the bisection of
+// topbottom (Utils.h:103-128) for strictly increasing radii.
+constexpr int NLUT = 256;
+
 struct PotView {
     int nr, npoly;
     const double *ar;    // [nr]
     const double *phil;  // [nr][npoly]
     const double *Pr;    // [nr][npoly]
     const LegCoef *lc;
+    const int *lut;      // [NLUT]
+    float lg0, lsc;      // log2(ar[0]), NLUT / (log2(ar[nr-1]) - log2(ar[0]))
 };
+
+FJ_HD __forceinline__ float fj_log2f(float x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __log2f(x);
+#else
+    return log2f(x);
+#endif
+}
+
+// Host fill of the bracket table (any float log2 is fine: the walk in
+// radial_interp corrects a cell off by one).
+inline void make_lut(int nr, const double *ar, int *lut, float *lg0, float *lsc) {
+    const double a = log2((double)(float)ar[0]), b = log2((double)(float)ar[nr - 1]);
+    *lg0 = (float)a;
+    *lsc = b > a ? (float)(NLUT / (b - a)) : 0.f;
+    int m = 0;
+    for (int k = 0; k < NLUT; k++) {
+        const double x = b > a ? exp2(a + k * (b - a) / NLUT) : ar[0];
+        while (m < nr - 2 && ar[m + 1] <= x) m++;
+        lut[k] = m;
+    }
+}
 
 // ---------------------------------------------------------------------------
 // Radial bracket + interpolation weights (UtilsLeg.h:96-158, Utils.h:130-146).
@@ -92,12 +122,13 @@ FJ_HD __forceinline__ Interp radial_interp(const PotView &T, double r) {
         I.q = ar[nr - 1] / r;
         return I;
     }
-    int top = nr - 1, bot = 0;
-    // (xm[top]-x)*(x-xm[m]) >= 0 is x >= xm[m] for x inside [xm[0], xm[top]]
-    while (top - bot > 1) {
-        int m = (top + bot) >> 1;
-        if (r >= ar[m]) bot = m; else top = m;
-    }
+    // (xm[top]-x)*(x-xm[m]) >= 0 is x >= xm[m] for x inside [xm[0], xm[top]]:
+    // the bracket is the largest bot <= nr-2 with r >= ar[bot] (0 if none)
+    const float t = (fj_log2f((float)r) - T.lg0) * T.lsc;
+    int bot = T.lut[t > 0.f ? (t < (float)(NLUT - 1) ? (int)t : NLUT - 1) : 0];
+    while (bot < nr - 2 && r >= ar[bot + 1]) bot++;
+    while (bot > 0 && !(r >= ar[bot])) bot--;
+    const int top = bot + 1;
     I.bot = bot; I.top = top;
     const double db = r - ar[bot];
     I.f1 = db / (ar[top] - ar[bot]);

@@ -26,6 +26,7 @@
 #include "../../include/fjmodels.h"
 #include "fj_device.hpp"
 #include "fj_node.hpp"
+#include "fj_mixed.hpp"
 
 using namespace fj;
 
@@ -34,6 +35,10 @@ namespace {
 constexpr int NQ1 = 22;                 // nodes per velocity axis (QUADORD)
 constexpr int NQ = NQ1 * NQ1 * NQ1;     // 10,648 nodes per point
 constexpr int QCAP = 64;                // QAGS intervals kept per lane before overflow
+// Mixed mode: FP32 integrands whose QAGS passes this many intervals are
+// limited by their own rounding (near-circular or near-planar orbits, where
+// F(u) is a small difference of large terms); they are redone in FP64.
+constexpr int QCAP_MIXED = 24;
 constexpr int QCAP_FULL = 1000;         // the reference's workspace limit (Utils.h:55)
 
 // include/GaussPts.h:76-136
@@ -67,11 +72,16 @@ __constant__ double c_xw01[22] = {
 struct TabArgs {
     int nr, npoly, ngrid;
     const double *ar, *phil, *Pr, *E, *D;
-    const LegCoef *lc;
+    const int *lut;
+    float lg0, lsc;
 };
 
+// Legendre recurrence constants (index-only values, the same for every grid)
+// in constant memory: wave-uniform reads go through the scalar cache.
+__constant__ LegCoef c_legcoef;
+
 __host__ __device__ inline size_t tab_lds_doubles(int nr, int npoly, int ngrid) {
-    return (size_t)nr + 2 * (size_t)nr * npoly + 2 * (size_t)ngrid + sizeof(LegCoef) / sizeof(double);
+    return (size_t)nr + 2 * (size_t)nr * npoly + 2 * (size_t)ngrid + NLUT / 2;
 }
 
 struct LdsTabs {
@@ -83,19 +93,42 @@ struct LdsTabs {
 __device__ inline LdsTabs stage_tables(const TabArgs &a, double *smem) {
     const int nr = a.nr, np = a.npoly, ng = a.ngrid;
     double *ar = smem, *phil = ar + nr, *Pr = phil + nr * np, *E = Pr + nr * np, *D = E + ng;
-    double *lc = D + ng;
-    const int nlc = sizeof(LegCoef) / sizeof(double);
+    int *lut = reinterpret_cast<int *>(D + ng);
     for (int i = threadIdx.x; i < nr; i += blockDim.x) ar[i] = a.ar[i];
     for (int i = threadIdx.x; i < nr * np; i += blockDim.x) { phil[i] = a.phil[i]; Pr[i] = a.Pr[i]; }
     for (int i = threadIdx.x; i < ng; i += blockDim.x) { E[i] = a.E[i]; D[i] = a.D[i]; }
-    const double *lsrc = reinterpret_cast<const double *>(a.lc);
-    for (int i = threadIdx.x; i < nlc; i += blockDim.x) lc[i] = lsrc[i];
+    for (int i = threadIdx.x; i < NLUT; i += blockDim.x) lut[i] = a.lut[i];
     __syncthreads();
     LdsTabs t;
     t.T.nr = nr; t.T.npoly = np; t.T.ar = ar; t.T.phil = phil; t.T.Pr = Pr;
-    t.T.lc = reinterpret_cast<const LegCoef *>(lc);
+    t.T.lc = &c_legcoef;
+    t.T.lut = lut; t.T.lg0 = a.lg0; t.T.lsc = a.lsc;
     t.E = E; t.D = D; t.ngrid = ng;
     return t;
+}
+
+// FP32 copies of the radial grid, Phi_l table and even-Legendre step
+// constants for the mixed-precision integrand, after the FP64 tables.
+__host__ __device__ inline size_t tab32_lds_floats(int nr, int npoly) {
+    return (size_t)nr + (size_t)nr * npoly + 3 * MAXPOLY;
+}
+
+__device__ inline PotView32 stage_tables32(const LdsTabs &L, float *f) {
+    const int nr = L.T.nr, np = L.T.npoly;
+    float *ar = f, *phil = ar + nr, *la = phil + nr * np, *lb = la + MAXPOLY, *lg = lb + MAXPOLY;
+    for (int i = threadIdx.x; i < nr; i += blockDim.x) ar[i] = (float)L.T.ar[i];
+    for (int i = threadIdx.x; i < nr * np; i += blockDim.x) phil[i] = (float)L.T.phil[i];
+    for (int i = threadIdx.x; i < MAXPOLY; i += blockDim.x) {
+        const LegCoef *lc = L.T.lc;
+        la[i] = i >= 2 ? (float)(-lc->el[i] * lc->elm1[i] / lc->ed[i]) : 0.f;
+        lb[i] = (float)lc->eb[i];
+        lg[i] = (float)lc->eg[i];
+    }
+    __syncthreads();
+    PotView32 T;
+    T.nr = nr; T.npoly = np; T.ar = ar; T.phil = phil; T.la = la; T.lb = lb; T.lg = lg;
+    T.lut = L.T.lut; T.lg0 = L.T.lg0; T.lsc = L.T.lsc;
+    return T;
 }
 
 // Node record between the kernels (structure of arrays).
@@ -136,6 +169,29 @@ __device__ inline void load_jint(const NodeBuf &nb, long long i, int type, JInt 
         f.k[m] = fld >= 0 ? nb.f[fld * nb.cap + i] : 0.0;
     }
 }
+
+__device__ inline void load_jint(const NodeBuf &nb, long long i, int type, JInt32 &f) {
+    JInt g;
+    load_jint(nb, i, type, g);
+    f.type = type;
+#pragma unroll
+    for (int m = 0; m < JK; m++) f.k[m] = (float)g.k[m];
+    const double rtwo = 1.4142135623730951;
+    f.pre = type == 0 ? rtwo * g.k[0] * g.k[3] : 2 * rtwo * g.k[0] * (PIH - g.k[2]);
+}
+
+// The 20 off-centre values of one QK21 rule kept as floats (mixed mode: the
+// integrand is an FP32 value, so the float holds it exactly).
+struct FvRefF {
+    float *p;
+    __device__ FvRefF &operator=(double x) { *p = (float)x; return *this; }
+    __device__ operator double() const { return (double)*p; }
+};
+struct FvStridedF {
+    float *p;
+    int ld;
+    __device__ FvRefF operator[](int k) const { return FvRefF{p + k * ld}; }
+};
 
 // Device counters
 struct Counters {
@@ -257,19 +313,13 @@ struct IntegArgs {
 #ifndef FJ_INTEG_WAVES
 #define FJ_INTEG_WAVES 3   // waves per SIMD the register allocation of k_integrals aims at
 #endif
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(FJ_INTEG_WAVES)))
-k_integrals(TabArgs ta, IntegArgs ia) {
-    extern __shared__ double smem[];
-    const LdsTabs L = stage_tables(ta, smem);
-    const long long lane_g = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    const QagsWork w = lane_work(ia.ws, ia.cap, lane_g, (long long)gridDim.x * blockDim.x);
-    const FvStrided fv{smem + tab_lds_doubles(ta.nr, ta.npoly, ta.ngrid) + threadIdx.x, (int)blockDim.x};
+
+template <class FI, class FV>
+__device__ __forceinline__ void integrals_loop(const IntegArgs &ia, FI &f, const QagsWork &w, const FV &fv) {
     const unsigned int nu = *ia.cnt_u, nv = *ia.cnt_v;
     const unsigned int ntask = ia.ntask_override >= 0 ? (unsigned int)ia.ntask_override : nu + nv;
     int task = -1;    // -1: needs a task, -2: queue drained
     long long node = 0;
-    JInt f;
-    f.T = &L.T;
     Qags q;
     unsigned long long nqk = 0, novf = 0;
     while (true) {
@@ -316,6 +366,36 @@ k_integrals(TabArgs ta, IntegArgs ia) {
     }
     if (nqk) atomicAdd(&ia.ctr->qk, nqk);
     if (novf) atomicAdd(&ia.ctr->overflow, novf);
+}
+
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(FJ_INTEG_WAVES)))
+k_integrals(TabArgs ta, IntegArgs ia) {
+    extern __shared__ double smem[];
+    const LdsTabs L = stage_tables(ta, smem);
+    const long long lane_g = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const QagsWork w = lane_work(ia.ws, ia.cap, lane_g, (long long)gridDim.x * blockDim.x);
+    const FvStrided fv{smem + tab_lds_doubles(ta.nr, ta.npoly, ta.ngrid) + threadIdx.x, (int)blockDim.x};
+    JInt f;
+    f.T = &L.T;
+    integrals_loop(ia, f, w, fv);
+}
+
+// Mixed precision: FP32 integrand, FP64 QAGS.
+#ifndef FJ_INTEG32_WAVES
+#define FJ_INTEG32_WAVES 3
+#endif
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(FJ_INTEG32_WAVES)))
+k_integrals_f32(TabArgs ta, IntegArgs ia) {
+    extern __shared__ double smem[];
+    const LdsTabs L = stage_tables(ta, smem);
+    float *f32 = reinterpret_cast<float *>(smem + tab_lds_doubles(ta.nr, ta.npoly, ta.ngrid));
+    const PotView32 T32 = stage_tables32(L, f32);
+    const long long lane_g = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const QagsWork w = lane_work(ia.ws, ia.cap, lane_g, (long long)gridDim.x * blockDim.x);
+    const FvStridedF fv{f32 + tab32_lds_floats(ta.nr, ta.npoly) + threadIdx.x, (int)blockDim.x};
+    JInt32 f;
+    f.T = &T32;
+    integrals_loop(ia, f, w, fv);
 }
 
 struct MomArgs {
@@ -374,6 +454,64 @@ __global__ void __launch_bounds__(256) k_moments(MomArgs ma) {
         double o[6];
         for (int m = 0; m < 6; m++) { o[m] = red[m][0]; ma.out6[(long long)p * 6 + m] = o[m]; }
         // Integ.cpp:160-161
+        ma.vrot[p] = o[1] / o[0];
+        ma.sigR[p] = sqrt(o[2] / o[0]);
+        ma.sigp[p] = sqrt(o[3] / o[0]);
+        ma.sigz[p] = sqrt(o[4] / o[0]);
+        ma.sigRz[p] = sqrt(fabs(o[5]) / o[0]);
+        ma.rho[p] = ma.quad ? 4 * o[0] : o[0];
+    }
+}
+
+// Mixed precision: f(J) in FP32 with compensated FP32 sums; the pairwise
+// tree adds the (sum, compensation) pairs with an exact two-sum.
+__global__ void __launch_bounds__(256) k_moments_f32(MomArgs ma) {
+    __shared__ float red[6][2][256];
+    const int pl = blockIdx.x;
+    const int p = ma.p0 + pl;
+    const double R = ma.R[p], Ve = ma.Ve[p];
+    const float jacob = (float)(Ve * Ve * Ve);
+    DF32 d;
+    d.A = (float)ma.df.A; d.B = (float)ma.df.B;
+    d.dphi_h = (float)ma.df.dphi_h; d.dz_h = (float)ma.df.dz_h;
+    d.dphi_g = (float)ma.df.dphi_g; d.dz_g = (float)ma.df.dz_g;
+    d.norm = (float)(ma.df.mass / (ma.df.J0 * ma.df.J0 * ma.df.J0) / (TPI * TPI * TPI));
+    d.J0 = (float)ma.df.J0; d.invJ0 = (float)(1.0 / ma.df.J0);
+    d.chi = (float)ma.df.chi; d.k = (float)ma.df.k; d.rot_form = ma.df.rot_form;
+    KSum s[6];
+    for (int node = threadIdx.x; node < NQ; node += blockDim.x) {
+        const long long gid = (long long)pl * NQ + node;
+        if (ma.nb.flag[gid] == 0) continue;
+        double Vd[3], wd;
+        node_velocity(ma.quad, node, Ve, Vd, wd);
+        const float V0 = (float)Vd[0], V1 = (float)Vd[1], V2 = (float)Vd[2], w = (float)wd;
+        const float DF = df32(d, (float)ma.nb.Jr[gid], (float)(R * Vd[1]), (float)ma.nb.Jz[gid]);
+        const float jd = jacob * DF;
+        s[0].add(w * jd);
+        s[1].add(w * (jd * V1));
+        s[2].add(w * (jd * V0 * V0));
+        s[3].add(w * (jd * V1 * V1));
+        s[4].add(w * (jd * V2 * V2));
+        s[5].add(w * (jd * V0 * V2));
+    }
+    for (int m = 0; m < 6; m++) { red[m][0][threadIdx.x] = s[m].s; red[m][1][threadIdx.x] = s[m].c; }
+    __syncthreads();
+    for (int st = blockDim.x / 2; st > 0; st >>= 1) {
+        if ((int)threadIdx.x < st)
+            for (int m = 0; m < 6; m++) {
+                const float a = red[m][0][threadIdx.x], b = red[m][0][threadIdx.x + st];
+                const float t = a + b, bb = t - a, e = (a - (t - bb)) + (b - bb);   // a + b = t + e exactly
+                red[m][0][threadIdx.x] = t;
+                red[m][1][threadIdx.x] = red[m][1][threadIdx.x] + red[m][1][threadIdx.x + st] - e;
+            }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        double o[6];
+        for (int m = 0; m < 6; m++) {
+            o[m] = (double)red[m][0][0] - (double)red[m][1][0];
+            ma.out6[(long long)p * 6 + m] = o[m];
+        }
         ma.vrot[p] = o[1] / o[0];
         ma.sigR[p] = sqrt(o[2] / o[0]);
         ma.sigp[p] = sqrt(o[3] / o[0]);
@@ -490,6 +628,94 @@ __global__ void k_poisson(int nr, int np_, const double *ar, const double *rhl, 
     PrP[idx] = -TPI * G * (-(l + 1) * Iint / fjm_powi(rn, l + 2) + l * Iext * fjm_powi(rn, l - 1));
     Pr2P[idx] = -TPI * G * ((l + 2) * (l + 1) * Iint / fjm_powi(rn, l + 3) + (l - 1) * l * Iext * fjm_powi(rn, l - 2) -
                             (2 * l + 1) * rhl[idx]);
+}
+
+// computePhil (Potential.cpp:110-200) with the radial integrals taken
+// exactly for the piecewise-linear rho_l that I_internal / I_external
+// integrate (linterp, constant below ar[0], max(0, rho_l(r_max)) (r_max/x)^4
+// beyond r_max): closed-form segment integrals, then a block-wide inclusive
+// scan (I_int, from the centre out) and a reverse scan (I_ext, from the
+// outside in).  One block per Legendre order; FP64 throughout.  The result
+// differs from the QUADPACK path by that path's quadrature error.
+constexpr int PSCAN_THREADS = 256;
+constexpr int PSCAN_MAXNR = 4 * PSCAN_THREADS;
+
+__device__ inline double ipow_signed(double x, int k) { return fjm_powi(x, k); }
+
+// int_{x0}^{x1} x^k dx
+__device__ inline double seg_pow_int(double x0, double x1, int k) {
+    if (k == -1) return fjm_log(x1 / x0);
+    return (ipow_signed(x1, k + 1) - ipow_signed(x0, k + 1)) / (k + 1);
+}
+
+// inclusive scan of v[0..n) (n <= PSCAN_MAXNR) in place; reverse = suffix sums
+__device__ void block_scan(double *v, int n, bool reverse, double *part) {
+    const int t = threadIdx.x;
+    const int per = (n + PSCAN_THREADS - 1) / PSCAN_THREADS;
+    const int i0 = t * per, i1 = min(n, i0 + per);
+    // serial scan of this thread's run
+    double acc = 0;
+    if (!reverse) for (int i = i0; i < i1; i++) { acc += v[i]; v[i] = acc; }
+    else for (int i = i1 - 1; i >= i0; i--) { acc += v[i]; v[i] = acc; }
+    part[t] = acc;
+    __syncthreads();
+    // Hillis-Steele over the run totals
+    for (int off = 1; off < PSCAN_THREADS; off <<= 1) {
+        double add = 0;
+        if (!reverse) { if (t >= off) add = part[t - off]; }
+        else { if (t + off < PSCAN_THREADS) add = part[t + off]; }
+        __syncthreads();
+        part[t] += add;
+        __syncthreads();
+    }
+    // offset of this run: total of the runs before it (after it for reverse)
+    double off = 0;
+    if (!reverse) { if (t > 0) off = part[t - 1]; }
+    else { if (t + 1 < PSCAN_THREADS) off = part[t + 1]; }
+    for (int i = i0; i < i1; i++) v[i] += off;
+    __syncthreads();
+}
+
+__global__ void __launch_bounds__(PSCAN_THREADS) k_poisson_scan(int nr, int np_, const double *ar, const double *rhl,
+                                                              double *philP, double *PrP, double *Pr2P) {
+    __shared__ double Iint[PSCAN_MAXNR], Iext[PSCAN_MAXNR], part[PSCAN_THREADS];
+    const int np = blockIdx.x;
+    const int l = 2 * np;
+    // segment i = [ar[i], ar[i+1]] stored at i+1 (internal) and i (external)
+    for (int i = threadIdx.x; i < nr; i += blockDim.x) {
+        double si = 0, se = 0;
+        if (i == 0) {
+            si = rhl[np] * ipow_signed(ar[0], l + 3) / (l + 3);   // constant below ar[0]
+        } else {
+            const double x0 = ar[i - 1], x1 = ar[i];
+            const double y0 = rhl[(i - 1) * np_ + np], y1 = rhl[i * np_ + np];
+            const double s = (y1 - y0) / (x1 - x0), c0 = y0 - s * x0;
+            si = c0 * seg_pow_int(x0, x1, l + 2) + s * seg_pow_int(x0, x1, l + 3);
+        }
+        if (i < nr - 1) {
+            const double x0 = ar[i], x1 = ar[i + 1];
+            const double y0 = rhl[i * np_ + np], y1 = rhl[(i + 1) * np_ + np];
+            const double s = (y1 - y0) / (x1 - x0), c0 = y0 - s * x0;
+            se = c0 * seg_pow_int(x0, x1, 1 - l) + s * seg_pow_int(x0, x1, 2 - l);
+        } else {
+            const double rm = ar[nr - 1], C = fj_max(0., rhl[(nr - 1) * np_ + np]);
+            se = C * ipow_signed(rm, 2 - l) / (l + 2);            // r^-4 tail beyond r_max
+        }
+        Iint[i] = si;
+        Iext[i] = se;
+    }
+    __syncthreads();
+    block_scan(Iint, nr, false, part);
+    block_scan(Iext, nr, true, part);
+    const double G = 1;
+    for (int n = threadIdx.x; n < nr; n += blockDim.x) {
+        const double rn = ar[n], Ii = Iint[n], Ie = Iext[n];
+        const int idx = n * np_ + np;
+        philP[idx] = -TPI * G * (Ii / fjm_powi(rn, l + 1) + Ie * fjm_powi(rn, l));
+        PrP[idx] = -TPI * G * (-(l + 1) * Ii / fjm_powi(rn, l + 2) + l * Ie * fjm_powi(rn, l - 1));
+        Pr2P[idx] = -TPI * G * ((l + 2) * (l + 1) * Ii / fjm_powi(rn, l + 3) + (l - 1) * l * Ie * fjm_powi(rn, l - 2) -
+                                (2 * l + 1) * rhl[idx]);
+    }
 }
 
 // Potential evaluation at a batch of points (facade, tests).
@@ -920,7 +1146,8 @@ struct fj_ctx {
     fj_df_params df{};
     double *d_ar = nullptr, *d_phil = nullptr, *d_Pr = nullptr, *d_Pr2 = nullptr, *d_E = nullptr, *d_D = nullptr;
     double *d_ci = nullptr, *d_si = nullptr, *d_poly = nullptr;
-    LegCoef *d_lc = nullptr;
+    int *d_lut = nullptr;
+    float lut_lg0 = 0, lut_lsc = 0;
     // node phase scratch
     long long node_cap = 0;
     double *d_nodef = nullptr, *d_Jr = nullptr, *d_Jz = nullptr;
@@ -934,7 +1161,7 @@ struct fj_ctx {
     double *d_pR = nullptr, *d_pz = nullptr, *d_pPhi = nullptr, *d_pVe = nullptr, *d_out6 = nullptr, *d_mom = nullptr;
     // grid buffers
     double *d_rhl = nullptr, *d_gl = nullptr, *d_pw = nullptr, *d_pws = nullptr; int *d_status = nullptr;
-    int integ_lanes = 0;
+    int integ_lanes = 0, integ_lanes32 = 0;
     long long last_nodes = 0;
     fj_stats st{};
     char err[512] = {0};
@@ -1013,12 +1240,17 @@ void host_evenlegend(int npoly, double c, double *pol) {
 TabArgs tab_args(const fj_ctx *c) {
     TabArgs t;
     t.nr = c->nr; t.npoly = c->npoly; t.ngrid = c->ngrid;
-    t.ar = c->d_ar; t.phil = c->d_phil; t.Pr = c->d_Pr; t.E = c->d_E; t.D = c->d_D; t.lc = c->d_lc;
+    t.ar = c->d_ar; t.phil = c->d_phil; t.Pr = c->d_Pr; t.E = c->d_E; t.D = c->d_D;
+    t.lut = c->d_lut; t.lg0 = c->lut_lg0; t.lsc = c->lut_lsc;
     return t;
 }
 size_t tab_bytes(const fj_ctx *c) { return tab_lds_doubles(c->nr, c->npoly, c->ngrid) * sizeof(double); }
-// k_integrals: tables + the 20 off-centre QK21 values of each of its 256 lanes
-size_t integ_bytes(const fj_ctx *c) { return tab_bytes(c) + 20 * 256 * sizeof(double); }
+// k_integrals: tables + the 20 off-centre QK21 values of each of its 256 lanes;
+// k_integrals_f32 adds the FP32 tables and keeps the 20 values as floats.
+size_t integ_bytes(const fj_ctx *c, bool mixed) {
+    if (mixed) return tab_bytes(c) + (tab32_lds_floats(c->nr, c->npoly) + 20 * 256) * sizeof(float);
+    return tab_bytes(c) + 20 * 256 * sizeof(double);
+}
 
 int ensure_points(fj_ctx *c, int n) {
     if (n <= c->pt_cap) return 0;
@@ -1032,15 +1264,19 @@ int ensure_points(fj_ctx *c, int n) {
 }
 
 // Persistent-kernel lane count: fill the device at the kernel's occupancy.
-int integ_lanes(fj_ctx *c) {
-    if (c->integ_lanes) return c->integ_lanes;
+int integ_lanes(fj_ctx *c, bool mixed) {
+    int &cached = mixed ? c->integ_lanes32 : c->integ_lanes;
+    if (cached) return cached;
     hipDeviceProp_t prop;
     HIPCHK(c, hipGetDeviceProperties(&prop, c->device));
     int per_cu = 0;
-    HIPCHK(c, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_integrals, 256, integ_bytes(c)));
+    if (mixed)
+        HIPCHK(c, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_integrals_f32, 256, integ_bytes(c, true)));
+    else
+        HIPCHK(c, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_integrals, 256, integ_bytes(c, false)));
     if (per_cu < 1) per_cu = 1;
-    c->integ_lanes = prop.multiProcessorCount * per_cu * 256;
-    return c->integ_lanes;
+    cached = prop.multiProcessorCount * per_cu * 256;
+    return cached;
 }
 
 
@@ -1057,8 +1293,9 @@ int ensure_nodes(fj_ctx *c, long long nodes) {
     if (!c->d_cnt) {
         if ((r = dalloc(c, &c->d_cnt, 8)) || (r = dalloc(c, &c->d_ctr, 1))) return r;
     }
-    const int lanes = integ_lanes(c);
-    if (lanes < 0) return FJ_E_HIP;
+    const int l64 = integ_lanes(c, false), l32 = integ_lanes(c, true);
+    if (l64 == FJ_E_HIP || l32 == FJ_E_HIP) return FJ_E_HIP;
+    const int lanes = l64 > l32 ? l64 : l32;
     if (lanes > c->ws_lanes || c->ws_cap != QCAP) {
         if ((r = dalloc(c, &c->d_ws, ws_doubles(QCAP, lanes)))) return r;
         c->ws_lanes = lanes; c->ws_cap = QCAP;
@@ -1080,11 +1317,16 @@ float elapsed(hipEvent_t a, hipEvent_t b) {
     return ms;
 }
 
+void launch_integrals(fj_ctx *c, const TabArgs &ta, const IntegArgs &ia, int lanes, bool mixed) {
+    if (mixed) k_integrals_f32<<<lanes / 256, 256, integ_bytes(c, true), c->stream>>>(ta, ia);
+    else k_integrals<<<lanes / 256, 256, integ_bytes(c, false), c->stream>>>(ta, ia);
+}
+
 // Node phase for npts points already in d_pR/d_pz; results in d_out6 and d_mom.
 int run_nodes(fj_ctx *c, int npts) {
     if (!c->have_pot || !c->have_delta || !c->have_df)
         return set_err(c, FJ_E_STATE, "potential, Delta table and DF must be set before the node phase");
-    if (c->prec != FJ_PREC_FP64) return set_err(c, FJ_E_ARG, "precision mode %d is not available", c->prec);
+    const bool mixed = c->prec == FJ_PREC_MIXED;
     const long long chunk = chunk_points(c, npts);
     int r = ensure_nodes(c, chunk * NQ);
     if (r) return r;
@@ -1101,7 +1343,7 @@ int run_nodes(fj_ctx *c, int npts) {
     HIPCHK(c, hipEventRecord(ev[1], c->stream));
     c->st.ms_points = 0;
     float t_setup = 0, t_int = 0, t_ovf = 0, t_mom = 0;
-    const int lanes = c->ws_lanes;
+    const int lanes = integ_lanes(c, mixed);
     c->st.integral_lanes = lanes;
     for (long long p0 = 0; p0 < npts; p0 += chunk) {
         const int np = (int)((npts - p0) < chunk ? (npts - p0) : chunk);
@@ -1116,8 +1358,8 @@ int run_nodes(fj_ctx *c, int npts) {
         HIPCHK(c, hipGetLastError());
         HIPCHK(c, hipEventRecord(ev[3], c->stream));
         IntegArgs ia{nb, c->d_list_u, c->d_list_v, c->d_cnt + 0, c->d_cnt + 1, c->d_cnt + 2,
-                     c->d_ovf, c->d_cnt + 3, c->d_ws, QCAP, -1, c->d_ctr};
-        k_integrals<<<lanes / 256, 256, integ_bytes(c), c->stream>>>(ta, ia);
+                     c->d_ovf, c->d_cnt + 3, c->d_ws, mixed ? QCAP_MIXED : QCAP, -1, c->d_ctr};
+        launch_integrals(c, ta, ia, lanes, mixed);
         HIPCHK(c, hipGetLastError());
         HIPCHK(c, hipEventRecord(ev[4], c->stream));
         // overflow pass: tasks whose interval list outgrew QCAP, redone with the full limit
@@ -1125,7 +1367,11 @@ int run_nodes(fj_ctx *c, int npts) {
         HIPCHK(c, hipMemcpyAsync(&novf, c->d_cnt + 3, sizeof novf, hipMemcpyDeviceToHost, c->stream));
         HIPCHK(c, hipStreamSynchronize(c->stream));
         if (novf) {
-            const int fl = 256 * 64;
+            // FP64 integrand, full 1000-interval limit; as many lanes as tasks
+            // (in blocks of 256), at most one full device
+            const int l64 = integ_lanes(c, false);
+            int fl = (int)((novf + 255) / 256 * 256);
+            if (fl > l64) fl = l64;
             if (fl > c->wsfull_lanes) {
                 if ((r = dalloc(c, &c->d_wsfull, ws_doubles(QCAP_FULL, fl)))) return r;
                 c->wsfull_lanes = fl;
@@ -1133,14 +1379,15 @@ int run_nodes(fj_ctx *c, int npts) {
             IntegArgs io = ia;
             io.next = c->d_cnt + 4; io.ws = c->d_wsfull; io.cap = QCAP_FULL; io.ntask_override = (int)novf;
             io.ovf_cnt = c->d_cnt + 5;
-            k_integrals<<<fl / 256, 256, integ_bytes(c), c->stream>>>(ta, io);
+            launch_integrals(c, ta, io, fl, false);
             HIPCHK(c, hipGetLastError());
         }
         HIPCHK(c, hipEventRecord(ev[5], c->stream));
         MomArgs ma{c->d_pPhi, c->d_pVe, c->d_pR, (int)p0, np, c->quad, nb, c->df, c->d_out6,
                    c->d_mom, c->d_mom + npts, c->d_mom + 2 * npts, c->d_mom + 3 * npts, c->d_mom + 4 * npts,
                    c->d_mom + 5 * npts};
-        k_moments<<<np, 256, 0, c->stream>>>(ma);
+        if (mixed) k_moments_f32<<<np, 256, 0, c->stream>>>(ma);
+        else k_moments<<<np, 256, 0, c->stream>>>(ma);
         HIPCHK(c, hipGetLastError());
         HIPCHK(c, hipEventRecord(ev[6], c->stream));
         HIPCHK(c, hipStreamSynchronize(c->stream));
@@ -1198,13 +1445,26 @@ int grid_points(fj_ctx *c) {
 }
 
 int run_poisson(fj_ctx *c, int mode, const double *d_rhl, double *d_phil, double *d_Pr, double *d_Pr2) {
-    if (mode != FJ_POISSON_QUADPACK) return set_err(c, FJ_E_ARG, "Poisson mode %d is not available", mode);
+    if (mode != FJ_POISSON_QUADPACK && mode != FJ_POISSON_BLOCKSCAN)
+        return set_err(c, FJ_E_ARG, "unknown Poisson mode %d", mode);
     int r = ensure_grid_bufs(c);
     if (r) return r;
     const int n = c->nr * c->npoly;
     hipEvent_t a, b;
     HIPCHK(c, hipEventCreate(&a));
     HIPCHK(c, hipEventCreate(&b));
+    if (mode == FJ_POISSON_BLOCKSCAN) {
+        if (c->nr > PSCAN_MAXNR) return set_err(c, FJ_E_ARG, "block-scan Poisson supports nr <= %d", PSCAN_MAXNR);
+        HIPCHK(c, hipEventRecord(a, c->stream));
+        k_poisson_scan<<<c->npoly, PSCAN_THREADS, 0, c->stream>>>(c->nr, c->npoly, c->d_ar, d_rhl, d_phil, d_Pr, d_Pr2);
+        HIPCHK(c, hipGetLastError());
+        HIPCHK(c, hipEventRecord(b, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        c->st.ms_poisson = elapsed(a, b);
+        (void)hipEventDestroy(a);
+        (void)hipEventDestroy(b);
+        return 0;
+    }
     HIPCHK(c, hipMemsetAsync(c->d_status, 0, sizeof(int), c->stream));
     HIPCHK(c, hipEventRecord(a, c->stream));
     k_poisson<<<(n + 63) / 64, 64, 0, c->stream>>>(c->nr, c->npoly, c->d_ar, d_rhl, c->d_pws, QCAP_FULL, d_phil,
@@ -1247,7 +1507,7 @@ int run_actions(fj_ctx *c, int n, const double *vR, const double *vphi, const do
     HIPCHK(c, hipGetLastError());
     IntegArgs ia{nb, c->d_list_u, c->d_list_v, c->d_cnt + 0, c->d_cnt + 1, c->d_cnt + 2,
                  c->d_ovf, c->d_cnt + 3, c->d_ws, QCAP, -1, c->d_ctr};
-    k_integrals<<<c->ws_lanes / 256, 256, integ_bytes(c), c->stream>>>(ta, ia);
+    launch_integrals(c, ta, ia, integ_lanes(c, false), false);
     HIPCHK(c, hipGetLastError());
     unsigned int novf = 0;
     HIPCHK(c, hipMemcpyAsync(&novf, c->d_cnt + 3, sizeof novf, hipMemcpyDeviceToHost, c->stream));
@@ -1261,7 +1521,7 @@ int run_actions(fj_ctx *c, int n, const double *vR, const double *vphi, const do
         IntegArgs io = ia;
         io.next = c->d_cnt + 4; io.ws = c->d_wsfull; io.cap = QCAP_FULL; io.ntask_override = (int)novf;
         io.ovf_cnt = c->d_cnt + 5;
-        k_integrals<<<fl / 256, 256, integ_bytes(c), c->stream>>>(ta, io);
+        launch_integrals(c, ta, io, fl, false);
         HIPCHK(c, hipGetLastError());
         HIPCHK(c, hipStreamSynchronize(c->stream));
     }
@@ -1294,7 +1554,7 @@ int fj_ctx_create(int device, fj_ctx **out) {
 int fj_ctx_destroy(fj_ctx *c) {
     if (!c) return 0;
     (void)hipSetDevice(c->device);
-    void *ptrs[] = {c->d_ar, c->d_phil, c->d_Pr, c->d_Pr2, c->d_E, c->d_D, c->d_ci, c->d_si, c->d_poly, c->d_lc,
+    void *ptrs[] = {c->d_ar, c->d_phil, c->d_Pr, c->d_Pr2, c->d_E, c->d_D, c->d_ci, c->d_si, c->d_poly, c->d_lut,
                     c->d_nodef, c->d_Jr, c->d_Jz, c->d_flag, c->d_list_u, c->d_list_v, c->d_ovf, c->d_cnt,
                     c->d_ctr, c->d_ws, c->d_wsfull, c->d_pR, c->d_pz, c->d_pPhi, c->d_pVe, c->d_out6,
                     c->d_mom, c->d_rhl, c->d_gl, c->d_pw, c->d_pws, c->d_status};
@@ -1315,6 +1575,7 @@ int fj_set_grid(fj_ctx *c, int nr, const double *ar, int ntheta, int npoly, int 
     if (nr < 11 || !ar || ntheta < 1 || npoly < 1 || npoly > MAXPOLY || ngrid < 2)
         return set_err(c, FJ_E_ARG, "bad grid sizes nr=%d (>= 11) ntheta=%d npoly=%d (<= %d) ngrid=%d", nr, ntheta,
                        npoly, MAXPOLY, ngrid);
+    if (!(ar[0] > 0)) return set_err(c, FJ_E_ARG, "radii must be positive");
     for (int i = 1; i < nr; i++)
         if (!(ar[i] > ar[i - 1])) return set_err(c, FJ_E_ARG, "radii must increase");
     HIPCHK(c, hipSetDevice(c->device));
@@ -1334,18 +1595,22 @@ int fj_set_grid(fj_ctx *c, int nr, const double *ar, int ntheta, int npoly, int 
     if ((r = dalloc(c, &c->d_ar, nr)) || (r = dalloc(c, &c->d_phil, nl)) || (r = dalloc(c, &c->d_Pr, nl)) ||
         (r = dalloc(c, &c->d_Pr2, nl)) || (r = dalloc(c, &c->d_E, ngrid)) || (r = dalloc(c, &c->d_D, ngrid)) ||
         (r = dalloc(c, &c->d_ci, ntheta)) || (r = dalloc(c, &c->d_si, ntheta)) ||
-        (r = dalloc(c, &c->d_poly, nl * 0 + (size_t)npoly * ntheta)) || (r = dalloc(c, &c->d_lc, 1)))
+        (r = dalloc(c, &c->d_poly, (size_t)npoly * ntheta)) || (r = dalloc(c, &c->d_lut, NLUT)))
         return r;
     // grid-size dependent scratch is re-made on demand
     (void)dalloc(c, &c->d_rhl, 0); (void)dalloc(c, &c->d_gl, 0); (void)dalloc(c, &c->d_pws, 0);
     (void)dalloc(c, &c->d_status, 0);
     c->integ_lanes = 0;
-    LegCoef lc = make_legcoef(npoly);
+    c->integ_lanes32 = 0;
+    const LegCoef lc = make_legcoef(MAXPOLY);
+    HIPCHK(c, hipMemcpyToSymbol(HIP_SYMBOL(c_legcoef), &lc, sizeof lc));
+    int lut[NLUT];
+    make_lut(nr, ar, lut, &c->lut_lg0, &c->lut_lsc);
+    HIPCHK(c, hipMemcpy(c->d_lut, lut, sizeof lut, hipMemcpyHostToDevice));
     HIPCHK(c, hipMemcpy(c->d_ar, ar, nr * sizeof(double), hipMemcpyHostToDevice));
     HIPCHK(c, hipMemcpy(c->d_ci, c->ci.data(), ntheta * sizeof(double), hipMemcpyHostToDevice));
     HIPCHK(c, hipMemcpy(c->d_si, c->si.data(), ntheta * sizeof(double), hipMemcpyHostToDevice));
     HIPCHK(c, hipMemcpy(c->d_poly, c->poly.data(), c->poly.size() * sizeof(double), hipMemcpyHostToDevice));
-    HIPCHK(c, hipMemcpy(c->d_lc, &lc, sizeof lc, hipMemcpyHostToDevice));
     c->have_pot = false; c->have_delta = false;
     return 0;
 }

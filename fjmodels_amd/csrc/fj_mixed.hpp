@@ -1,0 +1,151 @@
+// fj_mixed.hpp -- FP32 pieces of the mixed-precision node phase.
+//
+// In the mixed mode the action integrands (uvOrb.cpp:226-262: Juint, Jvint
+// with the potential of Potential.cpp:206-219) and the DF (DF.cpp:65-119)
+// are evaluated in FP32 with the GPU's single-precision elementary
+// functions; the QAGS bookkeeping, the turning points, Delta and everything
+// of the grid phase stay FP64.  These functions are device-only: the FP64
+// path (fj_device.hpp, fj_node.hpp) is the one the oracle reproduces.
+#pragma once
+
+#include "fj_device.hpp"
+#include "fj_node.hpp"
+
+namespace fj {
+
+#pragma clang fp contract(fast)
+
+// Potential tables in FP32 (LDS).  Even Legendre step as
+// P_2np = (la[np] P_2np-4 + P_2np-2 (c^2 - lb[np])) lg[np]  (UtilsLeg.h:68-79).
+struct PotView32 {
+    int nr, npoly;
+    const float *ar, *phil;
+    const float *la, *lb, *lg;
+    const int *lut;    // bracket table of PotView (fj_device.hpp)
+    float lg0, lsc;
+};
+
+__device__ __forceinline__ float pot_phi32(const PotView32 &T, float R, float z) {
+    const float r2 = R * R + z * z;
+    const float rs = sqrtf(r2);
+    const float c = z / rs;
+    const int nr = T.nr, np_ = T.npoly;
+    const float *ar = T.ar;
+    float r = rs;
+    float coef[MAXPOLY];
+    if (r >= ar[nr - 1]) {
+        // Keplerian tail: phil[nr-1][l] (ar[nr-1]/r)^(l+1), l = 2np
+        const float q = ar[nr - 1] / r, q2 = q * q;
+        float qp = q;
+        for (int np = 0; np < np_; np++) {
+            coef[np] = T.phil[(nr - 1) * np_ + np] * qp;
+            qp *= q2;
+        }
+    } else {
+        if (r < ar[0]) r = ar[0];
+        const float t = (__log2f(r) - T.lg0) * T.lsc;
+        int bot = T.lut[t > 0.f ? (t < (float)(NLUT - 1) ? (int)t : NLUT - 1) : 0];
+        while (bot < nr - 2 && r >= ar[bot + 1]) bot++;
+        while (bot > 0 && !(r >= ar[bot])) bot--;
+        const int top = bot + 1;
+        const float db = r - ar[bot];
+        const float f1 = db / (ar[top] - ar[bot]);
+        const bool quad = top < 10;
+        float f2 = 0, f3 = 0;
+        int thr = 0;
+        if (quad) {
+            thr = (f1 < 0.5f && bot > 0) ? bot - 1 : top + 1;
+            f2 = (r - ar[top]) * db / ((ar[thr] - ar[top]) * (ar[thr] - ar[bot]));
+            f3 = (ar[thr] - ar[bot]) / (ar[top] - ar[bot]);
+        }
+        for (int np = 0; np < np_; np++) {
+            const float t = T.phil[top * np_ + np], b = T.phil[bot * np_ + np];
+            float v = b + f1 * (t - b);
+            if (quad) v += f2 * (T.phil[thr * np_ + np] - b - f3 * (t - b));
+            coef[np] = v;
+        }
+    }
+    const float c2 = c * c;
+    float phi = coef[0];
+    float pm2 = 1.f, pm1 = 1.5f * c2 - .5f;
+    for (int np = 1; np < np_; np++) {
+        float p;
+        if (np == 1) p = pm1;
+        else { p = (T.la[np] * pm2 + pm1 * (c2 - T.lb[np])) * T.lg[np]; pm2 = pm1; pm1 = p; }
+        phi += coef[np] * p;
+    }
+    return phi;
+}
+
+// sinh and cosh from one expm1 (accurate for small |u| too).
+__device__ __forceinline__ void sinhcosh32(float u, float *s, float *c) {
+    const float em = expm1f(u), ep = em + 1.f;
+    *s = 0.5f * em * (em + 2.f) / ep;
+    *c = 0.5f * (ep + 1.f / ep);
+}
+
+// Juint / Jvint in FP32 with JInt's packing of the node constants.
+struct JInt32 {
+    const PotView32 *T;
+    float k[JK];
+    double pre;   // FP64 prefactor of the action: sqrt(2) Delta Deltau, or 2 sqrt(2) Delta (pi/2 - vturn)
+    int type;
+    __device__ double operator()(double psi_d) const {
+        const float psi = (float)psi_d;
+        float sp, cp;
+        sincosf(psi, &sp, &cp);
+        float F;
+        const float D = k[0], E = k[1];
+        if (type == 0) {
+            const float u = k[2] - k[3] * cp;
+            float shu, chu;
+            sinhcosh32(u, &shu, &chu);
+            const float shu2 = shu * shu;
+            const float Phi = pot_phi32(*T, D * shu * k[4], D * chu * k[5]);
+            const float dU = (shu2 + k[6]) * Phi - k[9];
+            F = E * (shu2 - k[7]) + k[8] - dU - k[10] * (1.f / shu2 - k[11]);
+        } else {
+            const float v = 1.5707963f - (1.5707963f - k[2]) * cp;
+            float s, cs;
+            sincosf(v, &s, &cs);
+            const float sv2 = s * s;
+            const float Phi = pot_phi32(*T, D * k[3] * s, D * k[4] * cs);
+            const float dV = k[7] - (k[5] + sv2) * Phi;
+            F = E * sv2 + k[6] + dV - .5f * k[8] / (D * D * sv2);
+        }
+        return (double)(sqrtf(fmaxf(0.f, F)) * sp);
+    }
+    __device__ double action(double integral) const { return pre * integral / PI; }
+};
+
+// DF (df_hg, DF.cpp:65-76, and the rotation term DF.cpp:114-119) in FP32.
+struct DF32 {
+    float A, B, dphi_h, dz_h, dphi_g, dz_g, norm, J0, invJ0, chi, k;
+    int rot_form;
+};
+
+__device__ __forceinline__ float df32(const DF32 &d, float Jr, float Jphi, float Jz) {
+    const float aJ = fabsf(Jphi);
+    const float h = Jr + d.dphi_h * aJ + d.dz_h * Jz;
+    const float g = Jr + d.dphi_g * aJ + d.dz_g * Jz;
+    const float even = d.norm * fmaxf(0.f, powf(1.f + d.J0 / h, d.A) / powf(1.f + g * d.invJ0, d.B));
+    if (d.chi == 0.f) return even;
+    const float t = tanhf(d.chi * Jphi * d.invJ0);
+    if (d.rot_form == 1) return (1.f - d.k) * even + d.k * t * even;
+    return even + d.k * t * even;
+}
+
+// Compensated (Kahan) FP32 accumulator.
+struct KSum {
+    float s = 0.f, c = 0.f;
+    __device__ void add(float x) {
+        const float y = x - c;
+        const float t = s + y;
+        c = (t - s) - y;
+        s = t;
+    }
+};
+
+#pragma clang fp contract(off)
+
+}  // namespace fj

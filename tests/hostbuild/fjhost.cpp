@@ -53,16 +53,33 @@ int fjh_qags(double (*f)(double, void *), void *p, double a, double b, double ep
     return 0;
 }
 
+// Radial bracket of radial_interp (bracket table + walk) at n radii.
+int fjh_bracket(int nr, const double *ar, int n, const double *r, int *bot) {
+    LegCoef lc = make_legcoef(1);
+    int lut[NLUT];
+    PotView T{nr, 1, ar, ar, ar, &lc, lut, 0.f, 0.f};
+    make_lut(nr, ar, lut, &T.lg0, &T.lsc);
+    for (int i = 0; i < n; i++) {
+        const Interp I = radial_interp(T, r[i]);
+        bot[i] = I.kep ? -1 : I.bot;
+    }
+    return 0;
+}
+
 double fjh_phi_grad(int nr, const double *ar, int npoly, const double *phil, const double *Pr, double R, double z,
                     double *dR, double *dz) {
     LegCoef lc = make_legcoef(npoly);
-    PotView T{nr, npoly, ar, phil, Pr, &lc};
+    int lut[NLUT];
+    PotView T{nr, npoly, ar, phil, Pr, &lc, lut, 0.f, 0.f};
+    make_lut(nr, ar, lut, &T.lg0, &T.lsc);
     return pot_phi_grad(T, R, z, dR, dz);
 }
 
 double fjh_phi(int nr, const double *ar, int npoly, const double *phil, const double *Pr, double R, double z) {
     LegCoef lc = make_legcoef(npoly);
-    PotView T{nr, npoly, ar, phil, Pr, &lc};
+    int lut[NLUT];
+    PotView T{nr, npoly, ar, phil, Pr, &lc, lut, 0.f, 0.f};
+    make_lut(nr, ar, lut, &T.lg0, &T.lsc);
     return pot_phi(T, R, z);
 }
 
@@ -71,7 +88,9 @@ int fjh_nodes(int nr, const double *ar, int npoly, const double *phil, const dou
               const double *Dg, int n, const double *R, const double *z, const double *Phi, const double *vR,
               const double *vphi, const double *vz, const double *Dx, double *out) {
     LegCoef lc = make_legcoef(npoly);
-    PotView T{nr, npoly, ar, phil, Pr, &lc};
+    int lut[NLUT];
+    PotView T{nr, npoly, ar, phil, Pr, &lc, lut, 0.f, 0.f};
+    make_lut(nr, ar, lut, &T.lg0, &T.lsc);
     for (int i = 0; i < n; i++) {
         NodeConst c{};
         double ubar, Deltau, vturn, jr, jz;
@@ -92,7 +111,9 @@ int fjh_integral(int nr, const double *ar, int npoly, const double *phil, const 
                  const double *Dg, double R, double z, double Phi, const double *V, double Dx, int type, int npsi,
                  const double *psi, double *out) {
     LegCoef lc = make_legcoef(npoly);
-    PotView T{nr, npoly, ar, phil, Pr, &lc};
+    int lut[NLUT];
+    PotView T{nr, npoly, ar, phil, Pr, &lc, lut, 0.f, 0.f};
+    make_lut(nr, ar, lut, &T.lg0, &T.lsc);
     NodeConst c{};
     double ubar, Deltau, vturn, jr, jz;
     int wu, wv, bit = 0;
@@ -118,7 +139,9 @@ int fjh_rhofDF(int nr, const double *ar, int npoly, const double *phil, const do
                double z, double A, double B, double dphi_h, double dz_h, double dphi_g, double dz_g, double mass,
                double J0, double chi, double k, int rot_form, double *out6) {
     LegCoef lc = make_legcoef(npoly);
-    PotView T{nr, npoly, ar, phil, Pr, &lc};
+    int lut[NLUT];
+    PotView T{nr, npoly, ar, phil, Pr, &lc, lut, 0.f, 0.f};
+    make_lut(nr, ar, lut, &T.lg0, &T.lsc);
     std::vector<double> ws(ws_doubles(1000, 1));
     QagsWork w = lane_work(ws.data(), 1000, 0, 1);
     const double far = 10 * ar[nr - 1];
@@ -172,7 +195,9 @@ int fjh_actions(int nr, const double *ar, int npoly, const double *phil, const d
                 const double *vR, const double *vphi, const double *vz, const double *Dx, int cap, double *Jr,
                 double *Jz, int *nqk) {
     LegCoef lc = make_legcoef(npoly);
-    PotView T{nr, npoly, ar, phil, Pr, &lc};
+    int lut[NLUT];
+    PotView T{nr, npoly, ar, phil, Pr, &lc, lut, 0.f, 0.f};
+    make_lut(nr, ar, lut, &T.lg0, &T.lsc);
     std::vector<double> ws(ws_doubles(cap, 1));
     QagsWork w = lane_work(ws.data(), cap, 0, 1);
     for (int i = 0; i < n; i++) {

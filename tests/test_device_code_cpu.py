@@ -44,6 +44,7 @@ def H():
         [C.c_int, dp, dp, C.POINTER(C.c_int)]
     lib.fjh_rhofDF.argtypes = [C.c_int, dp, C.c_int, dp, dp, C.c_int, dp, dp, dp, dp, dp, dp, C.c_double,
                                C.c_double] + [C.c_double] * 9 + [C.c_double, C.c_int, dp]
+    lib.fjh_bracket.argtypes = [C.c_int, dp, C.c_int, dp, C.POINTER(C.c_int)]
     lib.fjh_qags.argtypes = [O.FN, C.c_void_p, C.c_double, C.c_double, C.c_double, C.c_double, C.c_int, C.c_int,
                              C.c_int, dp, C.POINTER(C.c_int), C.POINTER(C.c_int)]
     return lib
@@ -144,3 +145,34 @@ def test_qags_stepper_bitwise(H):
                     assert cap == 48 and last.value > 48
                     continue
                 assert r.value == rr.value and ier.value == ier_o, (f, a, b, cap)
+
+
+@pytest.mark.parametrize("kind", ["log", "uneven", "dense_inner"])
+def test_radial_bracket_matches_bisection(H, kind):
+    """The bracket table + walk of radial_interp gives topbottom's bisection
+    bracket (Utils.h:103-128): the largest m <= nr-2 with r >= ar[m], else 0;
+    radii on, just off and between the nodes, below ar[0], NaN."""
+    rng = np.random.default_rng(1)
+    if kind == "log":
+        ar = O.log_grid(40, 0.01, 100.0)
+    elif kind == "uneven":
+        ar = np.sort(10 ** rng.uniform(-3, 3, 97))
+    else:
+        ar = np.concatenate([np.linspace(1e-3, 1e-2, 60), O.log_grid(30, 0.02, 50.0)])
+    nr = len(ar)
+    r = np.concatenate([ar[:-1], np.nextafter(ar[:-1], 0), np.nextafter(ar[:-1], np.inf),
+                        10 ** rng.uniform(np.log10(ar[0]) - 1, np.log10(ar[-1]), 5000), [np.nan, 1e-300]])
+    r = r[~(r >= ar[-1])]
+    bot = np.zeros(len(r), dtype=np.int32)
+    H.fjh_bracket(nr, P(ar), len(r), P(r), bot.ctypes.data_as(C.POINTER(C.c_int)))
+    ref = np.zeros(len(r), dtype=np.int32)
+    for i, x in enumerate(r):
+        top, b = nr - 1, 0
+        while top - b > 1:
+            m = (top + b) >> 1
+            if x >= ar[m]:
+                b = m
+            else:
+                top = m
+        ref[i] = b
+    assert np.array_equal(bot, ref), np.where(bot != ref)[0][:5]
