@@ -26,7 +26,11 @@ EXPORTS = [
     "fj_get_delta_table", "fj_set_df", "fj_guess_rhl", "fj_poisson", "fj_moments", "fj_rhofDF",
     "fj_project_legendre", "fj_compute_new_phi", "fj_tabulate_delta", "fj_virial",
     "fj_eval_potential", "fj_actions", "fj_get_stats",
+    "fj_surrogate_init", "fj_surrogate_collect", "fj_surrogate_train", "fj_surrogate_mode",
+    "fj_surrogate_get_stats", "fj_surrogate_params", "fj_surrogate_forward", "fj_surrogate_gradient",
 ]
+
+MLP_NPARAM = 5 * 128 + 128 + 2 * (128 * 128 + 128) + 128 * 2 + 2   # 34,050
 
 
 class DFParams(C.Structure):
@@ -43,6 +47,12 @@ class Stats(C.Structure):
                 [(n, C.c_longlong) for n in ("points", "nodes", "bound_nodes", "tasks", "qk_calls",
                                              "overflow_tasks", "brent_iters")] +
                 [("integral_lanes", C.c_int), ("setup_lanes", C.c_int)])
+
+
+class SurrogateStats(C.Structure):
+    _fields_ = ([(n, C.c_longlong) for n in ("n_train", "nodes_sur", "fallback", "judged")] +
+                [(n, C.c_double) for n in ("judge_err_r_med", "judge_err_r_p95", "judge_err_z_med",
+                                           "judge_err_z_p95", "ms_surrogate", "ms_train", "last_loss")])
 
 
 def df_params(model="Hernquist", dphi_h=0.55, dz_h=0.55, dphi_g=1.0, dz_g=1.0,
@@ -85,6 +95,15 @@ def load():
         "fj_eval_potential": [v, i, _dp, _dp, _dp, _dp, _dp],
         "fj_actions": [v, i] + [_dp] * 8,
         "fj_get_stats": [v, C.POINTER(Stats)],
+        "fj_surrogate_init": [v, C.c_ulonglong, C.c_longlong],
+        "fj_surrogate_collect": [v, d],
+        "fj_surrogate_train": [v, i, i, d, C.POINTER(d)],
+        "fj_surrogate_mode": [v, i, d, d],
+        "fj_surrogate_get_stats": [v, C.POINTER(SurrogateStats)],
+        "fj_surrogate_params": [v, C.POINTER(C.c_float), C.POINTER(C.c_float)],
+        "fj_surrogate_forward": [v, C.c_longlong, C.POINTER(C.c_float), C.POINTER(C.c_float)],
+        "fj_surrogate_gradient": [v, C.c_longlong, C.POINTER(C.c_float), C.POINTER(C.c_float),
+                                  C.POINTER(C.c_float), C.POINTER(d)],
     }
     for name, args in sig.items():
         f = getattr(lib, name)
@@ -252,3 +271,50 @@ class Context:
         s = Stats()
         self._chk(self.lib.fj_get_stats(self.h, C.byref(s)), "fj_get_stats")
         return {n: getattr(s, n) for n, _ in Stats._fields_}
+
+    # ---- action surrogate -------------------------------------------------
+    def surrogate_init(self, seed=1, capacity=4 << 20):
+        self._chk(self.lib.fj_surrogate_init(self.h, seed, capacity), "fj_surrogate_init")
+
+    def surrogate_collect(self, keep):
+        self._chk(self.lib.fj_surrogate_collect(self.h, keep), "fj_surrogate_collect")
+
+    def surrogate_train(self, epochs, batch=65536, lr=2e-3):
+        loss = C.c_double()
+        self._chk(self.lib.fj_surrogate_train(self.h, epochs, batch, lr, C.byref(loss)), "fj_surrogate_train")
+        return loss.value
+
+    def surrogate_mode(self, on, judge=0.01, jz_min=1e-3):
+        self._chk(self.lib.fj_surrogate_mode(self.h, int(on), judge, jz_min), "fj_surrogate_mode")
+
+    def surrogate_stats(self):
+        s = SurrogateStats()
+        self._chk(self.lib.fj_surrogate_get_stats(self.h, C.byref(s)), "fj_surrogate_get_stats")
+        return {n: getattr(s, n) for n, _ in SurrogateStats._fields_}
+
+    def surrogate_params(self, set_=None):
+        fp = C.POINTER(C.c_float)
+        out = np.zeros(MLP_NPARAM, dtype=np.float32)
+        src = None if set_ is None else np.ascontiguousarray(set_, dtype=np.float32)
+        self._chk(self.lib.fj_surrogate_params(self.h, out.ctypes.data_as(fp),
+                                               None if src is None else src.ctypes.data_as(fp)),
+                  "fj_surrogate_params")
+        return out
+
+    def surrogate_forward(self, x):
+        fp = C.POINTER(C.c_float)
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        y = np.zeros((len(x), 2), dtype=np.float32)
+        self._chk(self.lib.fj_surrogate_forward(self.h, len(x), x.ctypes.data_as(fp), y.ctypes.data_as(fp)),
+                  "fj_surrogate_forward")
+        return y
+
+    def surrogate_gradient(self, x, t):
+        fp = C.POINTER(C.c_float)
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        t = np.ascontiguousarray(t, dtype=np.float32)
+        g = np.zeros(MLP_NPARAM, dtype=np.float32)
+        loss = C.c_double()
+        self._chk(self.lib.fj_surrogate_gradient(self.h, len(x), x.ctypes.data_as(fp), t.ctypes.data_as(fp),
+                                                 g.ctypes.data_as(fp), C.byref(loss)), "fj_surrogate_gradient")
+        return g, loss.value

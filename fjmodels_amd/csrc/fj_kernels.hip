@@ -21,12 +21,14 @@
 #include <stdio.h>
 #include <string.h>
 
+#include <algorithm>
 #include <vector>
 
 #include "../../include/fjmodels.h"
 #include "fj_device.hpp"
 #include "fj_node.hpp"
 #include "fj_mixed.hpp"
+#include "fj_mlp.hpp"
 
 using namespace fj;
 
@@ -216,6 +218,7 @@ __device__ inline void wave_append(bool pred, unsigned int *counter, int *list, 
 struct PointArgs {
     const double *R, *z;   // [npts]
     double *Phi, *Ve;      // [npts]
+    double *Vc;            // [npts] or null: circular-speed scale sqrt(R dPhi/dR + z dPhi/dz)
 };
 
 // k_points: Phi_h = Phi(R,z); Ve = sqrt(-2(Phi_h - Phi(10 r_max, 10 r_max))) (Integ.cpp:152-153)
@@ -228,6 +231,11 @@ __global__ void __launch_bounds__(256) k_points(TabArgs ta, PointArgs pa, int np
     const double Phi_h = pot_phi(L.T, pa.R[i], pa.z[i]);
     pa.Phi[i] = Phi_h;
     pa.Ve[i] = sqrt(-2 * (Phi_h - pot_phi(L.T, far, far)));
+    if (pa.Vc) {
+        double dR, dz;
+        pot_phi_grad(L.T, pa.R[i], pa.z[i], &dR, &dz);
+        pa.Vc[i] = sqrt(fj_max(1e-30, pa.R[i] * dR + pa.z[i] * dz));
+    }
 }
 
 struct SetupArgs {
@@ -241,6 +249,9 @@ struct SetupArgs {
     Counters *ctr;
     // explicit phase-space points (fj_actions): node gid is point gid
     const double *vR, *vphi, *vz, *Dx;
+    // if set: only the nodes sel[0 .. *nsel) of the chunk
+    const int *sel;
+    const unsigned int *nsel;
 };
 
 __device__ inline void node_velocity(int quad, int node, double Ve, double V[3], double &w) {
@@ -256,10 +267,11 @@ __device__ inline void node_velocity(int quad, int node, double Ve, double V[3],
 __global__ void __launch_bounds__(256) k_setup(TabArgs ta, SetupArgs sa) {
     extern __shared__ double smem[];
     const LdsTabs L = stage_tables(ta, smem);
-    const long long gid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long long tid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     const bool explicit_v = sa.vR != nullptr;
-    const long long nnodes = explicit_v ? (long long)sa.npts : (long long)sa.npts * NQ;
-    const bool valid = gid < nnodes;
+    const long long nnodes = sa.sel ? (long long)*sa.nsel : explicit_v ? (long long)sa.npts : (long long)sa.npts * NQ;
+    const bool valid = tid < nnodes;
+    const long long gid = sa.sel ? (valid ? (long long)sa.sel[tid] : 0) : tid;
     bool want_u = false, want_v = false;
     long long nev = 0;
     int brent_it = 0;
@@ -291,6 +303,358 @@ __global__ void __launch_bounds__(256) k_setup(TabArgs ta, SetupArgs sa) {
     if (__lane_id() == 0 && nb_) atomicAdd(&sa.ctr->bound, nb_);
     if (valid && brent_it) atomicAdd(&sa.ctr->brent, (unsigned long long)brent_it);
     if (valid && nev) atomicAdd(&sa.ctr->grad, (unsigned long long)nev);
+}
+
+// df_hg (DF.cpp:65-76) and the rotation term (DF.cpp:114-119)
+__device__ inline double d_df(const fj_df_params &d, double Jr, double Jphi, double Jz) {
+    const double h = Jr + d.dphi_h * fabs(Jphi) + d.dz_h * Jz;
+    const double g = Jr + d.dphi_g * fabs(Jphi) + d.dz_g * Jz;
+    const double even = d.mass / fjm_powi(d.J0, 3) *
+                        fj_max(0., fjm_pow(1. + d.J0 / h, d.A) / (fjm_pow(1. + g / d.J0, d.B))) / fjm_powi(TPI, 3);
+    if (d.chi == 0.) return even;
+    const double t = fjm_tanh(d.chi * Jphi / d.J0);
+    if (d.rot_form == 1) return (1. - d.k) * even + d.k * t * even;
+    return even + d.k * t * even;
+}
+
+// ---------------------------------------------------------------------------
+// Action surrogate in the node phase.  Inputs of node (R, z, V):
+// (log R, z/R, vR/vc, vz/vc, |vphi|/vc); outputs log(J / s + tfloor) for
+// J = Jr, Jz with the action scale s = r vc of the grid point.
+// ---------------------------------------------------------------------------
+struct SurNorm {
+    float mu[MLP_IN], sig[MLP_IN], tmu[MLP_OUT], tsig[MLP_OUT];
+    float lo[MLP_IN], hi[MLP_IN];   // training-set box of the normalised inputs
+    float tfloor;                   // targets are log(J / s + tfloor), s = r vc
+};
+
+__device__ inline void sur_features(double R, double z, double vc, const double V[3], float f[MLP_IN]) {
+    f[0] = (float)log(R);
+    f[1] = (float)(z / R);
+    f[2] = (float)(V[0] / vc);
+    f[3] = (float)(V[2] / vc);
+    f[4] = (float)(fabs(V[1]) / vc);
+}
+
+struct SurArgs {
+    const double *R, *z, *Phi, *Ve, *Vc;
+    int p0, npts, quad;
+    NodeBuf nb;
+    SurNorm nm;
+    bf16_t *x16;            // [nodes][5] normalised inputs
+    const float *y;         // [nodes][2] normalised outputs
+    double judge_frac, jz_min;
+    unsigned int seed;
+    unsigned int *nsel;     // fallback + judged nodes -> sel
+    int *sel;
+    unsigned int *njudge;   // judged nodes -> judge (node, Jr_sur, Jz_sur)
+    int *judge_node;
+    float *judge_J;         // [2 * judged]
+    unsigned int *nfall;
+};
+
+__host__ __device__ inline unsigned int hash32(unsigned int x) {
+    x ^= x >> 16; x *= 0x7feb352du; x ^= x >> 15; x *= 0x846ca68bu; x ^= x >> 16;
+    return x;
+}
+
+// Inputs for every node; flag 0 (H >= 0), 1 (bound, surrogate), 3 (bound,
+// Staeckel: outside the training box or |H| < 0.02 |Phi|).
+__global__ void __launch_bounds__(256) k_sur_prep(SurArgs a) {
+    const long long gid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (gid >= (long long)a.npts * NQ) return;
+    const int p = a.p0 + (int)(gid / NQ), node = (int)(gid % NQ);
+    const double R = a.R[p], z = a.z[p], Phi_h = a.Phi[p], Ve = a.Ve[p];
+    double V[3], w;
+    node_velocity(a.quad, node, Ve, V, w);
+    const double H = .5 * (V[0] * V[0] + V[1] * V[1] + V[2] * V[2]) + Phi_h;   // DF.cpp:83
+    int flag = H < 0 ? 1 : 0;
+    float f[MLP_IN];
+    sur_features(R, z, a.Vc[p], V, f);
+    for (int k = 0; k < MLP_IN; k++) {
+        const float x = (f[k] - a.nm.mu[k]) / a.nm.sig[k];
+        if (flag == 1 && !(x >= a.nm.lo[k] && x <= a.nm.hi[k])) flag = 3;
+        a.x16[gid * MLP_IN + k] = f2bf(x);
+    }
+    if (flag == 1 && fabs(H) < 0.02 * fabs(Phi_h)) flag = 3;
+    a.nb.flag[gid] = flag;
+    a.nb.Jr[gid] = 0;
+    a.nb.Jz[gid] = 0;
+}
+
+// Predicted actions; nodes that fall back (flag 3, predicted Jz < jz_min) or
+// are judged (hash below judge_frac) go to the Staeckel list.
+__global__ void __launch_bounds__(256) k_sur_post(SurArgs a) {
+    const long long gid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const bool valid = gid < (long long)a.npts * NQ;
+    bool fall = false, judge = false;
+    float Jr = 0.f, Jz = 0.f;
+    if (valid) {
+        const int flag = a.nb.flag[gid];
+        if (flag == 1) {
+            const int p = a.p0 + (int)(gid / NQ), node = (int)(gid % NQ);
+            const double R = a.R[p], z = a.z[p];
+            const float sc = (float)(sqrt(R * R + z * z) * a.Vc[p]);
+            const float lr = a.y[gid * MLP_OUT] * a.nm.tsig[0] + a.nm.tmu[0];
+            const float lz = a.y[gid * MLP_OUT + 1] * a.nm.tsig[1] + a.nm.tmu[1];
+            Jr = fmaxf(0.f, sc * (expf(lr) - a.nm.tfloor));
+            Jz = fmaxf(0.f, sc * (expf(lz) - a.nm.tfloor));
+            a.nb.Jr[gid] = Jr;
+            a.nb.Jz[gid] = Jz;
+            double V[3], w;
+            node_velocity(a.quad, node, a.Ve[p], V, w);
+            const float aJphi = (float)fabs(R * V[1]);
+            if (Jz < (float)a.jz_min * (Jr + Jz + aJphi)) fall = true;
+            else if ((double)hash32((unsigned int)gid ^ a.seed) < a.judge_frac * 4294967296.0) judge = true;
+        } else if (flag == 3) {
+            fall = true;
+        }
+    }
+    wave_append(fall || judge, a.nsel, a.sel, (int)gid);
+    const unsigned long long jm = __ballot(judge);
+    if (jm) {
+        const int lane = __lane_id(), leader = __ffsll((long long)jm) - 1;
+        unsigned int base = 0;
+        if (lane == leader) base = atomicAdd(a.njudge, (unsigned int)__popcll(jm));
+        base = __shfl(base, leader);
+        if (judge) {
+            const unsigned long long lower = lane == 0 ? 0ull : (jm & ((~0ull) >> (64 - lane)));
+            const unsigned int q = base + (unsigned int)__popcll(lower);
+            a.judge_node[q] = (int)gid;
+            a.judge_J[2 * q] = Jr;
+            a.judge_J[2 * q + 1] = Jz;
+        }
+    }
+    const unsigned long long fm = __ballot(fall);
+    if (__lane_id() == 0 && fm) atomicAdd(a.nfall, (unsigned int)__popcll(fm));
+    const unsigned long long sm = __ballot(valid && !fall && a.nb.flag[valid ? gid : 0] == 1);
+    if (__lane_id() == 0 && sm) atomicAdd(a.nfall + 1, (unsigned int)__popcll(sm));
+}
+
+// Fused surrogate node phase: per 32-node tile and wave, the inputs, the
+// network (mlp_tile) and the decisions of k_sur_prep / k_sur_post.
+// Dynamic LDS: the packed weights of the two 128 x 128 layers (64 KB).
+constexpr int SUR_LDS = 2 * 16 * 64 * 4 * 4 * (int)sizeof(bf16_t);
+
+__global__ void __launch_bounds__(256) k_sur_infer(SurArgs a, const bf16_t *__restrict__ wg,
+                                                   const float *__restrict__ bias) {
+    extern __shared__ double smem_sur[];
+    bf16_t *wl = reinterpret_cast<bf16_t *>(smem_sur);
+    {
+        const uint2 *src = reinterpret_cast<const uint2 *>(wg + MlpPacked::L2);
+        uint2 *dst = reinterpret_cast<uint2 *>(wl);
+        for (int i = threadIdx.x; i < SUR_LDS / 8; i += blockDim.x) dst[i] = src[i];
+    }
+    __syncthreads();
+    const long long nn = (long long)a.npts * NQ, ntiles = (nn + 31) / 32;
+    const int lane = __lane_id(), wave = threadIdx.x >> 6;
+    for (long long tile = (long long)blockIdx.x * 4 + wave; tile < ntiles; tile += (long long)gridDim.x * 4) {
+        const long long gid = tile * 32 + (lane & 31);
+        const bool valid = gid < nn;
+        int flag = 0;
+        float x[MLP_IN] = {0.f, 0.f, 0.f, 0.f, 0.f};
+        int p = 0, node = 0;
+        double V[3] = {0, 0, 0};
+        if (valid) {
+            p = a.p0 + (int)(gid / NQ);
+            node = (int)(gid % NQ);
+            double w;
+            node_velocity(a.quad, node, a.Ve[p], V, w);
+            const double H = .5 * (V[0] * V[0] + V[1] * V[1] + V[2] * V[2]) + a.Phi[p];   // DF.cpp:83
+            flag = H < 0 ? 1 : 0;
+            float f[MLP_IN];
+            sur_features(a.R[p], a.z[p], a.Vc[p], V, f);
+            for (int k = 0; k < MLP_IN; k++) {
+                x[k] = (f[k] - a.nm.mu[k]) / a.nm.sig[k];
+                if (flag == 1 && !(x[k] >= a.nm.lo[k] && x[k] <= a.nm.hi[k])) flag = 3;
+            }
+            if (flag == 1 && fabs(H) < 0.02 * fabs(a.Phi[p])) flag = 3;
+        }
+        const v4s x4 = lane < 32 ? pack4(x[0], x[1], x[2], x[3]) : pack4(x[4], 0.f, 0.f, 0.f);
+        float y0, y1;
+        mlp_tile(wg, wl, bias, x4, y0, y1);
+        bool fall = false, judge = false, served = false;
+        float Jr = 0.f, Jz = 0.f;
+        if (valid && lane < 32) {
+            if (flag == 1) {
+                const double R = a.R[p], z = a.z[p];
+                const float sc = (float)(sqrt(R * R + z * z) * a.Vc[p]);
+                Jr = fmaxf(0.f, sc * (expf(y0 * a.nm.tsig[0] + a.nm.tmu[0]) - a.nm.tfloor));
+                Jz = fmaxf(0.f, sc * (expf(y1 * a.nm.tsig[1] + a.nm.tmu[1]) - a.nm.tfloor));
+                const float aJphi = (float)fabs(R * V[1]);
+                if (Jz < (float)a.jz_min * (Jr + Jz + aJphi)) fall = true;
+                else {
+                    served = true;
+                    if ((double)hash32((unsigned int)gid ^ a.seed) < a.judge_frac * 4294967296.0) judge = true;
+                }
+            } else if (flag == 3) {
+                fall = true;
+            }
+            a.nb.flag[gid] = flag;
+            a.nb.Jr[gid] = Jr;
+            a.nb.Jz[gid] = Jz;
+        }
+        wave_append(fall || judge, a.nsel, a.sel, (int)gid);
+        const unsigned long long jm = __ballot(judge);
+        if (jm) {
+            const int leader = __ffsll((long long)jm) - 1;
+            unsigned int base = 0;
+            if (lane == leader) base = atomicAdd(a.njudge, (unsigned int)__popcll(jm));
+            base = __shfl(base, leader);
+            if (judge) {
+                const unsigned long long lower = lane == 0 ? 0ull : (jm & ((~0ull) >> (64 - lane)));
+                const unsigned int q = base + (unsigned int)__popcll(lower);
+                a.judge_node[q] = (int)gid;
+                a.judge_J[2 * q] = Jr;
+                a.judge_J[2 * q + 1] = Jz;
+            }
+        }
+        const unsigned long long fm = __ballot(fall), sm = __ballot(served);
+        if (lane == 0 && fm) atomicAdd(a.nfall, (unsigned int)__popcll(fm));
+        if (lane == 0 && sm) atomicAdd(a.nfall + 1, (unsigned int)__popcll(sm));
+    }
+}
+
+// Network on given normalised inputs x [n][5] through the fused tile -> y [n][2].
+__global__ void __launch_bounds__(256) k_mlp_infer_x(long long n, const float *__restrict__ x,
+                                                     const bf16_t *__restrict__ wg, const float *__restrict__ bias,
+                                                     float *__restrict__ y) {
+    extern __shared__ double smem_sur[];
+    bf16_t *wl = reinterpret_cast<bf16_t *>(smem_sur);
+    {
+        const uint2 *src = reinterpret_cast<const uint2 *>(wg + MlpPacked::L2);
+        uint2 *dst = reinterpret_cast<uint2 *>(wl);
+        for (int i = threadIdx.x; i < SUR_LDS / 8; i += blockDim.x) dst[i] = src[i];
+    }
+    __syncthreads();
+    const long long ntiles = (n + 31) / 32;
+    const int lane = __lane_id(), wave = threadIdx.x >> 6;
+    for (long long tile = (long long)blockIdx.x * 4 + wave; tile < ntiles; tile += (long long)gridDim.x * 4) {
+        const long long r = tile * 32 + (lane & 31);
+        float v[4] = {0.f, 0.f, 0.f, 0.f};
+        if (r < n) {
+            if (lane < 32) for (int k = 0; k < 4; k++) v[k] = x[r * MLP_IN + k];
+            else v[0] = x[r * MLP_IN + 4];
+        }
+        float y0, y1;
+        mlp_tile(wg, wl, bias, pack4(v[0], v[1], v[2], v[3]), y0, y1);
+        if (r < n && lane < 32) { y[r * MLP_OUT] = y0; y[r * MLP_OUT + 1] = y1; }
+    }
+}
+
+// Training samples from the last Staeckel node phase: bound nodes with
+// finite actions, each kept with probability keep, into a ring buffer.
+// Row: inputs [5]; targets log(Jr/s + tf), log(Jz/s + tf); weight
+// 0.05 + NQ w f Ve^3 / sum_nodes(w f Ve^3), the node's share of its point's
+// density times the node count (mean about 1), so the loss spends its
+// effort where the density is made.
+constexpr int SUR_TW = 3;   // training-set row: two targets and a weight
+
+struct CollectArgs {
+    const double *R, *z, *Phi, *Ve, *Vc;
+    int p0, npts, quad;
+    NodeBuf nb;
+    const int *only;          // if set: only these nodes (judged ones)
+    unsigned int n_only;
+    double keep;
+    unsigned int seed;
+    float tfloor;
+    fj_df_params df;
+    const double *out6;       // [npts][6] raw moment sums of the node phase
+    float *X, *T;             // [cap][5], [cap][SUR_TW]
+    unsigned long long *pos;  // running write position (mod cap)
+    long long cap;
+};
+
+__global__ void __launch_bounds__(256) k_sur_collect(CollectArgs a) {
+    const long long tid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long long n = a.only ? (long long)a.n_only : (long long)a.npts * NQ;
+    bool take = false;
+    long long gid = 0;
+    if (tid < n) {
+        gid = a.only ? a.only[tid] : tid;
+        take = a.nb.flag[gid] == 1 && a.nb.Jr[gid] < 1e30 && a.nb.Jz[gid] < 1e30 &&
+               (double)hash32((unsigned int)gid ^ a.seed) < a.keep * 4294967296.0;
+    }
+    const unsigned long long m = __ballot(take);
+    if (!m) return;
+    const int lane = __lane_id(), leader = __ffsll((long long)m) - 1;
+    unsigned long long base = 0;
+    if (lane == leader) base = atomicAdd(a.pos, (unsigned long long)__popcll(m));
+    base = __shfl(base, leader);
+    if (!take) return;
+    const unsigned long long lower = lane == 0 ? 0ull : (m & ((~0ull) >> (64 - lane)));
+    const long long q = (long long)((base + __popcll(lower)) % (unsigned long long)a.cap);
+    const int p = a.p0 + (int)(gid / NQ), node = (int)(gid % NQ);
+    double V[3], w;
+    node_velocity(a.quad, node, a.Ve[p], V, w);
+    float f[MLP_IN];
+    sur_features(a.R[p], a.z[p], a.Vc[p], V, f);
+    for (int k = 0; k < MLP_IN; k++) a.X[q * MLP_IN + k] = f[k];
+    const double sc = sqrt(a.R[p] * a.R[p] + a.z[p] * a.z[p]) * a.Vc[p];
+    const double Jr = a.nb.Jr[gid], Jz = a.nb.Jz[gid];
+    a.T[q * SUR_TW] = (float)log(Jr / sc + a.tfloor);
+    a.T[q * SUR_TW + 1] = (float)log(Jz / sc + a.tfloor);
+    const double Ve = a.Ve[p], o0 = a.out6[(long long)p * 6];
+    const double share = o0 > 0 ? w * d_df(a.df, Jr, a.R[p] * V[1], Jz) * Ve * Ve * Ve / o0 : 0.0;
+#ifndef FJ_SUR_WEIGHTED
+#define FJ_SUR_WEIGHTED 1
+#endif
+    a.T[q * SUR_TW + 2] = FJ_SUR_WEIGHTED ? (float)(0.05 + NQ * share) : 1.f;
+}
+
+// Batch b of an epoch: rows (off + (j B + b) P) mod n of the training set,
+// normalised (P odd and coprime with n: every row once per epoch).
+__global__ void k_sur_batch(long long B, long long first, long long P, long long n, long long off, const float *X,
+                            const float *T, SurNorm nm, bf16_t *x16, float *t, float *wt) {
+    const long long b = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= B) return;
+    const long long idx = (off + (unsigned long long)(first + b) % (unsigned long long)n * (unsigned long long)P) % n;
+    for (int k = 0; k < MLP_IN; k++) x16[b * MLP_IN + k] = f2bf((X[idx * MLP_IN + k] - nm.mu[k]) / nm.sig[k]);
+    for (int k = 0; k < MLP_OUT; k++) t[b * MLP_OUT + k] = (T[idx * SUR_TW + k] - nm.tmu[k]) / nm.tsig[k];
+    wt[b] = T[idx * SUR_TW + 2];
+}
+
+// Per-block min and max of the normalised inputs over the training set.
+__global__ void k_sur_box(long long n, const float *X, SurNorm nm, float *part) {
+    __shared__ float lo[MLP_IN][256], hi[MLP_IN][256];
+    float l[MLP_IN], h[MLP_IN];
+    for (int k = 0; k < MLP_IN; k++) { l[k] = 3.4e38f; h[k] = -3.4e38f; }
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+        for (int k = 0; k < MLP_IN; k++) {
+            const float x = (X[i * MLP_IN + k] - nm.mu[k]) / nm.sig[k];
+            l[k] = fminf(l[k], x);
+            h[k] = fmaxf(h[k], x);
+        }
+    for (int k = 0; k < MLP_IN; k++) { lo[k][threadIdx.x] = l[k]; hi[k][threadIdx.x] = h[k]; }
+    __syncthreads();
+    for (int st = 128; st > 0; st >>= 1) {
+        if ((int)threadIdx.x < st)
+            for (int k = 0; k < MLP_IN; k++) {
+                lo[k][threadIdx.x] = fminf(lo[k][threadIdx.x], lo[k][threadIdx.x + st]);
+                hi[k][threadIdx.x] = fmaxf(hi[k][threadIdx.x], hi[k][threadIdx.x + st]);
+            }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0)
+        for (int k = 0; k < MLP_IN; k++) { part[blockIdx.x * 2 * MLP_IN + k] = lo[k][0]; part[blockIdx.x * 2 * MLP_IN + MLP_IN + k] = hi[k][0]; }
+}
+
+// Errors of the predicted actions at the judged nodes, after the Staeckel
+// pass: |J - J_St| / (Jr_St + Jz_St + |Jphi|), the fraction of the DF's
+// action combination h or g that the error moves (-1: unbound node).
+__global__ void k_sur_judge_err(SurArgs a, unsigned int n, float *err) {
+    const unsigned int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const long long gid = a.judge_node[i];
+    const int p = a.p0 + (int)(gid / NQ), node = (int)(gid % NQ);
+    double V[3], w;
+    node_velocity(a.quad, node, a.Ve[p], V, w);
+    const double Sr = a.nb.Jr[gid], Sz = a.nb.Jz[gid];
+    if (!(Sr < 1e30 && Sz < 1e30)) { err[2 * i] = -1.f; err[2 * i + 1] = -1.f; return; }
+    const double scale = Sr + Sz + fabs(a.R[p] * V[1]);
+    err[2 * i] = (float)(fabs(a.judge_J[2 * i] - Sr) / scale);
+    err[2 * i + 1] = (float)(fabs(a.judge_J[2 * i + 1] - Sz) / scale);
 }
 
 struct IntegArgs {
@@ -407,17 +771,6 @@ struct MomArgs {
     double *rho, *vrot, *sigR, *sigp, *sigz, *sigRz;   // [npts_total]
 };
 
-// df_hg (DF.cpp:65-76) and the rotation term (DF.cpp:114-119)
-__device__ inline double d_df(const fj_df_params &d, double Jr, double Jphi, double Jz) {
-    const double h = Jr + d.dphi_h * fabs(Jphi) + d.dz_h * Jz;
-    const double g = Jr + d.dphi_g * fabs(Jphi) + d.dz_g * Jz;
-    const double even = d.mass / fjm_powi(d.J0, 3) *
-                        fj_max(0., fjm_pow(1. + d.J0 / h, d.A) / (fjm_pow(1. + g / d.J0, d.B))) / fjm_powi(TPI, 3);
-    if (d.chi == 0.) return even;
-    const double t = fjm_tanh(d.chi * Jphi / d.J0);
-    if (d.rot_form == 1) return (1. - d.k) * even + d.k * t * even;
-    return even + d.k * t * even;
-}
 
 // k_moments: one block per point; fixed-order reduction of the 6 sums.
 __global__ void __launch_bounds__(256) k_moments(MomArgs ma) {
@@ -1164,6 +1517,27 @@ struct fj_ctx {
     int integ_lanes = 0, integ_lanes32 = 0;
     long long last_nodes = 0;
     fj_stats st{};
+    // action surrogate
+    double *d_pVc = nullptr;
+    bool sur_init = false, sur_norm = false;
+    int sur_on = 0;
+    double sur_judge = 0.01, sur_jzmin = 1e-3, sur_keep = 0;
+    unsigned int sur_seed = 1, sur_iter = 0;
+    Mlp mlp;
+    SurNorm nm{};
+    float *d_X = nullptr, *d_T = nullptr, *d_bt = nullptr, *d_bw = nullptr;   // set [cap][5], [cap][3]; batch
+    long long train_cap = 0;
+    unsigned long long *d_pos = nullptr;
+    double *d_loss = nullptr;
+    long long sur_nodes_cap = 0;
+    bf16_t *d_sx16 = nullptr;
+    float *d_sy = nullptr, *d_judge_J = nullptr;
+    int *d_sel = nullptr, *d_judge_node = nullptr;
+    double *d_judge_S = nullptr;
+    bf16_t *d_wpk = nullptr;   // packed weights of the fused inference (MlpPacked)
+    float *d_wbias = nullptr;
+    int sur_blocks = 0;
+    fj_surrogate_stats sst{};
     char err[512] = {0};
 };
 
@@ -1257,7 +1631,7 @@ int ensure_points(fj_ctx *c, int n) {
     int r = 0;
     if ((r = dalloc(c, &c->d_pR, n)) || (r = dalloc(c, &c->d_pz, n)) || (r = dalloc(c, &c->d_pPhi, n)) ||
         (r = dalloc(c, &c->d_pVe, n)) || (r = dalloc(c, &c->d_out6, (size_t)n * 6)) ||
-        (r = dalloc(c, &c->d_mom, (size_t)n * 6)))
+        (r = dalloc(c, &c->d_mom, (size_t)n * 6)) || (r = dalloc(c, &c->d_pVc, n)))
         return r;
     c->pt_cap = n;
     return 0;
@@ -1291,7 +1665,7 @@ int ensure_nodes(fj_ctx *c, long long nodes) {
         c->node_cap = nodes;
     }
     if (!c->d_cnt) {
-        if ((r = dalloc(c, &c->d_cnt, 8)) || (r = dalloc(c, &c->d_ctr, 1))) return r;
+        if ((r = dalloc(c, &c->d_cnt, 16)) || (r = dalloc(c, &c->d_ctr, 1))) return r;
     }
     const int l64 = integ_lanes(c, false), l32 = integ_lanes(c, true);
     if (l64 == FJ_E_HIP || l32 == FJ_E_HIP) return FJ_E_HIP;
@@ -1322,66 +1696,173 @@ void launch_integrals(fj_ctx *c, const TabArgs &ta, const IntegArgs &ia, int lan
     else k_integrals<<<lanes / 256, 256, integ_bytes(c, false), c->stream>>>(ta, ia);
 }
 
+// Staeckel actions of the tasks k_setup listed (counters d_cnt[0..1]): the
+// persistent integrator, then the FP64 full-limit pass for overflowed tasks.
+int run_integrals(fj_ctx *c, const TabArgs &ta, const NodeBuf &nb, bool mixed, float *t_int, float *t_ovf,
+                  unsigned int *novf_out) {
+    hipEvent_t e0, e1, e2;
+    HIPCHK(c, hipEventCreate(&e0));
+    HIPCHK(c, hipEventCreate(&e1));
+    HIPCHK(c, hipEventCreate(&e2));
+    const int lanes = integ_lanes(c, mixed);
+    IntegArgs ia{nb, c->d_list_u, c->d_list_v, c->d_cnt + 0, c->d_cnt + 1, c->d_cnt + 2,
+                 c->d_ovf, c->d_cnt + 3, c->d_ws, mixed ? QCAP_MIXED : QCAP, -1, c->d_ctr};
+    HIPCHK(c, hipEventRecord(e0, c->stream));
+    launch_integrals(c, ta, ia, lanes, mixed);
+    HIPCHK(c, hipGetLastError());
+    HIPCHK(c, hipEventRecord(e1, c->stream));
+    unsigned int novf = 0;
+    HIPCHK(c, hipMemcpyAsync(&novf, c->d_cnt + 3, sizeof novf, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    if (novf) {
+        // FP64 integrand, full 1000-interval limit; as many lanes as tasks
+        // (in blocks of 256), at most one full device
+        const int l64 = integ_lanes(c, false);
+        int fl = (int)((novf + 255) / 256 * 256);
+        if (fl > l64) fl = l64;
+        if (fl > c->wsfull_lanes) {
+            int r;
+            if ((r = dalloc(c, &c->d_wsfull, ws_doubles(QCAP_FULL, fl)))) return r;
+            c->wsfull_lanes = fl;
+        }
+        IntegArgs io = ia;
+        io.next = c->d_cnt + 4; io.ws = c->d_wsfull; io.cap = QCAP_FULL; io.ntask_override = (int)novf;
+        io.ovf_cnt = c->d_cnt + 5;
+        launch_integrals(c, ta, io, fl, false);
+        HIPCHK(c, hipGetLastError());
+    }
+    HIPCHK(c, hipEventRecord(e2, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    *t_int += elapsed(e0, e1);
+    *t_ovf += elapsed(e1, e2);
+    *novf_out = novf;
+    (void)hipEventDestroy(e0); (void)hipEventDestroy(e1); (void)hipEventDestroy(e2);
+    return 0;
+}
+
+int sur_ensure_nodes(fj_ctx *c, long long nn) {
+    if (nn <= c->sur_nodes_cap) return 0;
+    int r;
+    if ((r = dalloc(c, &c->d_sx16, (size_t)nn * MLP_IN)) || (r = dalloc(c, &c->d_sy, (size_t)nn * MLP_OUT)) ||
+        (r = dalloc(c, &c->d_sel, (size_t)nn)) || (r = dalloc(c, &c->d_judge_node, (size_t)nn)) ||
+        (r = dalloc(c, &c->d_judge_J, (size_t)nn * 2)) || (r = dalloc(c, &c->d_judge_S, (size_t)nn * 2)))
+        return r;
+    c->sur_nodes_cap = nn;
+    return 0;
+}
+
+// Judge statistics: median and 95th percentile of the judged nodes' errors.
+int sur_judge_stats(fj_ctx *c, const SurArgs &a, unsigned int nj) {
+    if (!nj) return 0;
+    float *d_err = reinterpret_cast<float *>(c->d_judge_S);
+    k_sur_judge_err<<<(nj + 255) / 256, 256, 0, c->stream>>>(a, nj, d_err);
+    HIPCHK(c, hipGetLastError());
+    std::vector<float> E(2 * (size_t)nj);
+    HIPCHK(c, hipMemcpyAsync(E.data(), d_err, E.size() * sizeof(float), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    for (int k = 0; k < 2; k++) {
+        std::vector<float> e;
+        e.reserve(nj);
+        for (unsigned int i = 0; i < nj; i++)
+            if (E[2 * i + k] >= 0) e.push_back(E[2 * i + k]);
+        if (e.empty()) continue;
+        std::sort(e.begin(), e.end());
+        const double med = e[e.size() / 2], p95 = e[(size_t)(0.95 * (e.size() - 1))];
+        if (k == 0) { c->sst.judge_err_r_med = med; c->sst.judge_err_r_p95 = p95; }
+        else { c->sst.judge_err_z_med = med; c->sst.judge_err_z_p95 = p95; }
+    }
+    return 0;
+}
+
+// Packed weights for the fused inference from the current fp32 parameters.
+int sur_pack(fj_ctx *c) {
+    k_mlp_pack<<<4 * 16 * 64 * 4 / 256, 256, 0, c->stream>>>(c->mlp.p, c->d_wpk, c->d_wbias);
+    HIPCHK(c, hipGetLastError());
+    if (!c->sur_blocks) {
+        hipDeviceProp_t prop;
+        HIPCHK(c, hipGetDeviceProperties(&prop, c->device));
+        int per_cu = 0;
+        HIPCHK(c, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_sur_infer, 256, SUR_LDS));
+        if (per_cu < 1) return set_err(c, FJ_E_HIP, "fused surrogate kernel does not fit (needs %d B of LDS)", SUR_LDS);
+        c->sur_blocks = prop.multiProcessorCount * per_cu;
+    }
+    return 0;
+}
+
+int sur_collect(fj_ctx *c, const NodeBuf &nb, int p0, int np, const int *only, unsigned int n_only, double keep) {
+    if (!c->sur_init || keep <= 0) return 0;
+    CollectArgs ca{c->d_pR, c->d_pz, c->d_pPhi, c->d_pVe, c->d_pVc, p0, np, c->quad, nb, only, n_only, keep,
+                   hash32(c->sur_seed * 977u + c->sur_iter * 131u + (unsigned)p0), c->nm.tfloor, c->df, c->d_out6,
+                   c->d_X, c->d_T, c->d_pos, c->train_cap};
+    const long long n = only ? (long long)n_only : (long long)np * NQ;
+    if (n > 0) k_sur_collect<<<(unsigned)((n + 255) / 256), 256, 0, c->stream>>>(ca);
+    HIPCHK(c, hipGetLastError());
+    return 0;
+}
+
 // Node phase for npts points already in d_pR/d_pz; results in d_out6 and d_mom.
 int run_nodes(fj_ctx *c, int npts) {
     if (!c->have_pot || !c->have_delta || !c->have_df)
         return set_err(c, FJ_E_STATE, "potential, Delta table and DF must be set before the node phase");
     const bool mixed = c->prec == FJ_PREC_MIXED;
+    const bool sur = c->sur_on && c->sur_init && c->sur_norm;
     const long long chunk = chunk_points(c, npts);
     int r = ensure_nodes(c, chunk * NQ);
     if (r) return r;
+    if (sur && (r = sur_ensure_nodes(c, chunk * NQ))) return r;
     const TabArgs ta = tab_args(c);
     const size_t lds = tab_bytes(c);
     hipEvent_t ev[8];
     for (int i = 0; i < 8; i++) HIPCHK(c, hipEventCreate(&ev[i]));
     HIPCHK(c, hipMemsetAsync(c->d_ctr, 0, sizeof(Counters), c->stream));
     memset(&c->st, 0, sizeof c->st);
+    if (sur) {
+        const long long nt = c->sst.n_train;
+        memset(&c->sst, 0, sizeof c->sst);
+        c->sst.n_train = nt;
+    }
+    c->sur_iter++;
     HIPCHK(c, hipEventRecord(ev[0], c->stream));
-    PointArgs pa{c->d_pR, c->d_pz, c->d_pPhi, c->d_pVe};
+    PointArgs pa{c->d_pR, c->d_pz, c->d_pPhi, c->d_pVe, c->d_pVc};
     k_points<<<(npts + 255) / 256, 256, lds, c->stream>>>(ta, pa, npts);
     HIPCHK(c, hipGetLastError());
     HIPCHK(c, hipEventRecord(ev[1], c->stream));
-    c->st.ms_points = 0;
-    float t_setup = 0, t_int = 0, t_ovf = 0, t_mom = 0;
-    const int lanes = integ_lanes(c, mixed);
-    c->st.integral_lanes = lanes;
+    float t_setup = 0, t_int = 0, t_ovf = 0, t_mom = 0, t_sur = 0;
+    c->st.integral_lanes = integ_lanes(c, mixed);
     for (long long p0 = 0; p0 < npts; p0 += chunk) {
         const int np = (int)((npts - p0) < chunk ? (npts - p0) : chunk);
         const long long nn = (long long)np * NQ;
         NodeBuf nb{c->d_nodef, nn, c->d_Jr, c->d_Jz, c->d_flag};
-        HIPCHK(c, hipMemsetAsync(c->d_cnt, 0, 8 * sizeof(unsigned int), c->stream));
+        HIPCHK(c, hipMemsetAsync(c->d_cnt, 0, 16 * sizeof(unsigned int), c->stream));
         SetupArgs sa{c->d_pR, c->d_pz, c->d_pPhi, c->d_pVe, (int)p0, np, c->quad, nb,
                      c->d_cnt + 0, c->d_cnt + 1, c->d_list_u, c->d_list_v, c->d_ctr,
-                     nullptr, nullptr, nullptr, nullptr};
+                     nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+        unsigned int nsel = 0, njudge = 0, nfall = 0;
+        SurArgs a{c->d_pR, c->d_pz, c->d_pPhi, c->d_pVe, c->d_pVc, (int)p0, np, c->quad, nb, c->nm,
+                  c->d_sx16, c->d_sy, c->sur_judge, c->sur_jzmin, hash32(c->sur_seed + 7919u * c->sur_iter),
+                  c->d_cnt + 6, c->d_sel, c->d_cnt + 7, c->d_judge_node, c->d_judge_J, c->d_cnt + 8};
+        if (sur) {
+            // surrogate for every bound node, Staeckel for fallbacks and the judged sample
+            HIPCHK(c, hipEventRecord(ev[2], c->stream));
+            k_sur_infer<<<c->sur_blocks, 256, SUR_LDS, c->stream>>>(a, c->d_wpk, c->d_wbias);
+            HIPCHK(c, hipGetLastError());
+            HIPCHK(c, hipEventRecord(ev[3], c->stream));
+            unsigned int cn[4];
+            HIPCHK(c, hipMemcpyAsync(cn, c->d_cnt + 6, sizeof cn, hipMemcpyDeviceToHost, c->stream));
+            HIPCHK(c, hipStreamSynchronize(c->stream));
+            nsel = cn[0]; njudge = cn[1]; nfall = cn[2];
+            c->sst.nodes_sur += cn[3];
+            t_sur += elapsed(ev[2], ev[3]);
+            sa.sel = c->d_sel;
+            sa.nsel = c->d_cnt + 6;
+        }
         HIPCHK(c, hipEventRecord(ev[2], c->stream));
-        k_setup<<<(unsigned)((nn + 255) / 256), 256, lds, c->stream>>>(ta, sa);
+        const long long nsetup = sur ? (long long)nsel : nn;
+        if (nsetup > 0) k_setup<<<(unsigned)((nsetup + 255) / 256), 256, lds, c->stream>>>(ta, sa);
         HIPCHK(c, hipGetLastError());
         HIPCHK(c, hipEventRecord(ev[3], c->stream));
-        IntegArgs ia{nb, c->d_list_u, c->d_list_v, c->d_cnt + 0, c->d_cnt + 1, c->d_cnt + 2,
-                     c->d_ovf, c->d_cnt + 3, c->d_ws, mixed ? QCAP_MIXED : QCAP, -1, c->d_ctr};
-        launch_integrals(c, ta, ia, lanes, mixed);
-        HIPCHK(c, hipGetLastError());
-        HIPCHK(c, hipEventRecord(ev[4], c->stream));
-        // overflow pass: tasks whose interval list outgrew QCAP, redone with the full limit
         unsigned int novf = 0;
-        HIPCHK(c, hipMemcpyAsync(&novf, c->d_cnt + 3, sizeof novf, hipMemcpyDeviceToHost, c->stream));
-        HIPCHK(c, hipStreamSynchronize(c->stream));
-        if (novf) {
-            // FP64 integrand, full 1000-interval limit; as many lanes as tasks
-            // (in blocks of 256), at most one full device
-            const int l64 = integ_lanes(c, false);
-            int fl = (int)((novf + 255) / 256 * 256);
-            if (fl > l64) fl = l64;
-            if (fl > c->wsfull_lanes) {
-                if ((r = dalloc(c, &c->d_wsfull, ws_doubles(QCAP_FULL, fl)))) return r;
-                c->wsfull_lanes = fl;
-            }
-            IntegArgs io = ia;
-            io.next = c->d_cnt + 4; io.ws = c->d_wsfull; io.cap = QCAP_FULL; io.ntask_override = (int)novf;
-            io.ovf_cnt = c->d_cnt + 5;
-            launch_integrals(c, ta, io, fl, false);
-            HIPCHK(c, hipGetLastError());
-        }
+        if ((r = run_integrals(c, ta, nb, mixed, &t_int, &t_ovf, &novf))) return r;
         HIPCHK(c, hipEventRecord(ev[5], c->stream));
         MomArgs ma{c->d_pPhi, c->d_pVe, c->d_pR, (int)p0, np, c->quad, nb, c->df, c->d_out6,
                    c->d_mom, c->d_mom + npts, c->d_mom + 2 * npts, c->d_mom + 3 * npts, c->d_mom + 4 * npts,
@@ -1390,15 +1871,23 @@ int run_nodes(fj_ctx *c, int npts) {
         else k_moments<<<np, 256, 0, c->stream>>>(ma);
         HIPCHK(c, hipGetLastError());
         HIPCHK(c, hipEventRecord(ev[6], c->stream));
+        // training samples: every Staeckel node phase (probability sur_keep), or the judged nodes
+        if (!sur) { if ((r = sur_collect(c, nb, (int)p0, np, nullptr, 0, c->sur_keep))) return r; }
+        else if (c->sur_keep > 0 && njudge) {
+            if ((r = sur_collect(c, nb, (int)p0, np, c->d_judge_node, njudge, 1.0))) return r;
+        }
         HIPCHK(c, hipStreamSynchronize(c->stream));
         t_setup += elapsed(ev[2], ev[3]);
-        t_int += elapsed(ev[3], ev[4]);
-        t_ovf += elapsed(ev[4], ev[5]);
         t_mom += elapsed(ev[5], ev[6]);
         unsigned int cnts[2];
         HIPCHK(c, hipMemcpy(cnts, c->d_cnt, sizeof cnts, hipMemcpyDeviceToHost));
         c->st.tasks += (long long)cnts[0] + cnts[1];
         c->st.overflow_tasks += novf;
+        if (sur) {
+            c->sst.fallback += nfall;
+            c->sst.judged += njudge;
+            if ((r = sur_judge_stats(c, a, njudge))) return r;
+        }
     }
     HIPCHK(c, hipEventRecord(ev[7], c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
@@ -1412,6 +1901,12 @@ int run_nodes(fj_ctx *c, int npts) {
     c->st.bound_nodes = (long long)h.bound;
     c->st.qk_calls = (long long)h.qk;
     c->st.brent_iters = (long long)h.brent;
+    if (sur) c->sst.ms_surrogate = t_sur;
+    if (c->sur_init) {
+        unsigned long long pos = 0;
+        HIPCHK(c, hipMemcpy(&pos, c->d_pos, sizeof pos, hipMemcpyDeviceToHost));
+        c->sst.n_train = (long long)(pos < (unsigned long long)c->train_cap ? pos : (unsigned long long)c->train_cap);
+    }
     for (int i = 0; i < 8; i++) (void)hipEventDestroy(ev[i]);
     return 0;
 }
@@ -1495,14 +1990,15 @@ int run_actions(fj_ctx *c, int n, const double *vR, const double *vphi, const do
     if (Dx) HIPCHK(c, hipMemcpy(dv + 3 * n, Dx, n * sizeof(double), hipMemcpyHostToDevice));
     const TabArgs ta = tab_args(c);
     const size_t lds = tab_bytes(c);
-    PointArgs pa{c->d_pR, c->d_pz, c->d_pPhi, c->d_pVe};
+    PointArgs pa{c->d_pR, c->d_pz, c->d_pPhi, c->d_pVe, c->d_pVc};
     k_points<<<(n + 255) / 256, 256, lds, c->stream>>>(ta, pa, n);
     NodeBuf nb{c->d_nodef, n, c->d_Jr, c->d_Jz, c->d_flag};
     c->last_nodes = n;
-    HIPCHK(c, hipMemsetAsync(c->d_cnt, 0, 8 * sizeof(unsigned int), c->stream));
+    HIPCHK(c, hipMemsetAsync(c->d_cnt, 0, 16 * sizeof(unsigned int), c->stream));
     HIPCHK(c, hipMemsetAsync(c->d_ctr, 0, sizeof(Counters), c->stream));
     SetupArgs sa{c->d_pR, c->d_pz, c->d_pPhi, c->d_pVe, 0, n, c->quad, nb, c->d_cnt + 0, c->d_cnt + 1,
-                 c->d_list_u, c->d_list_v, c->d_ctr, dv, dv + n, dv + 2 * n, Dx ? dv + 3 * n : nullptr};
+                 c->d_list_u, c->d_list_v, c->d_ctr, dv, dv + n, dv + 2 * n, Dx ? dv + 3 * n : nullptr,
+                 nullptr, nullptr};
     k_setup<<<(n + 255) / 256, 256, lds, c->stream>>>(ta, sa);
     HIPCHK(c, hipGetLastError());
     IntegArgs ia{nb, c->d_list_u, c->d_list_v, c->d_cnt + 0, c->d_cnt + 1, c->d_cnt + 2,
@@ -1559,6 +2055,13 @@ int fj_ctx_destroy(fj_ctx *c) {
                     c->d_ctr, c->d_ws, c->d_wsfull, c->d_pR, c->d_pz, c->d_pPhi, c->d_pVe, c->d_out6,
                     c->d_mom, c->d_rhl, c->d_gl, c->d_pw, c->d_pws, c->d_status};
     for (void *p : ptrs)
+        if (p) (void)hipFree(p);
+    void *sp[] = {c->d_pVc, c->d_X, c->d_T, c->d_bt, c->d_bw, c->d_pos, c->d_loss, c->d_sx16, c->d_sy, c->d_judge_J,
+                  c->d_sel, c->d_judge_node, c->d_judge_S, c->mlp.p, c->mlp.g, c->mlp.m, c->mlp.v, c->mlp.p16,
+                  c->mlp.x16, c->mlp.d16, c->mlp.dy16, c->mlp.pre, c->mlp.y, c->mlp.dh, c->mlp.dp32,
+                  c->mlp.loss_part, c->mlp.h16[0], c->mlp.h16[1], c->mlp.h16[2], c->mlp.h32[0], c->mlp.h32[1],
+                  c->mlp.h32[2], c->mlp.sk.part, c->d_wpk, c->d_wbias};
+    for (void *p : sp)
         if (p) (void)hipFree(p);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
@@ -1915,6 +2418,225 @@ int fj_debug_integral(fj_ctx *c, long long node, int type, int npsi, const doubl
     HIPCHK(c, hipStreamSynchronize(c->stream));
     HIPCHK(c, hipMemcpy(out, d + npsi, (npsi + 7) * sizeof(double), hipMemcpyDeviceToHost));
     (void)hipFree(d);
+    return 0;
+}
+
+// ---------------------------------------------------------------------------
+// Action surrogate
+// ---------------------------------------------------------------------------
+int fj_surrogate_init(fj_ctx *c, unsigned long long seed, long long capacity) {
+    if (!c || capacity < 1024) return FJ_E_ARG;
+    int r;
+    Mlp &m = c->mlp;
+    const long long cap = 1 << 20;   // rows per forward / training batch
+    if (!c->sur_init) {
+        const size_t N = MlpLayout::N, ch = (size_t)cap * MLP_H;
+        if ((r = dalloc(c, &m.p, N)) || (r = dalloc(c, &m.g, N)) || (r = dalloc(c, &m.m, N)) ||
+            (r = dalloc(c, &m.v, N)) || (r = dalloc(c, &m.p16, N)) || (r = dalloc(c, &m.x16, (size_t)cap * MLP_IN)) ||
+            (r = dalloc(c, &m.d16, ch)) || (r = dalloc(c, &m.dy16, (size_t)cap * MLP_OUT)) ||
+            (r = dalloc(c, &m.pre, ch)) || (r = dalloc(c, &m.y, (size_t)cap * MLP_OUT)) || (r = dalloc(c, &m.dh, ch)) ||
+            (r = dalloc(c, &m.dp32, ch)) || (r = dalloc(c, &m.loss_part, (size_t)cap * MLP_OUT / 256 + 1)) ||
+            (r = dalloc(c, &c->d_bt, (size_t)cap * MLP_OUT)) || (r = dalloc(c, &c->d_bw, (size_t)cap)) || (r = dalloc(c, &c->d_pos, 1)) ||
+            (r = dalloc(c, &c->d_loss, 1)) || (r = dalloc(c, &m.sk.part, (size_t)256 * MLP_H * MLP_H)))
+            return r;
+        m.sk.cap = (long long)256 * MLP_H * MLP_H;
+        for (int l = 0; l < 3; l++)
+            if ((r = dalloc(c, &m.h16[l], ch)) || (r = dalloc(c, &m.h32[l], ch))) return r;
+        if ((r = dalloc(c, &c->d_wpk, (size_t)MlpPacked::N16)) || (r = dalloc(c, &c->d_wbias, (size_t)3 * MLP_H + 2)))
+            return r;
+        m.cap = cap;
+    }
+    if (capacity != c->train_cap) {
+        if ((r = dalloc(c, &c->d_X, (size_t)capacity * MLP_IN)) || (r = dalloc(c, &c->d_T, (size_t)capacity * SUR_TW)))
+            return r;
+        c->train_cap = capacity;
+    }
+    std::vector<float> p;
+    mlp_init_params(p, seed);
+    HIPCHK(c, hipMemcpy(m.p, p.data(), p.size() * sizeof(float), hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemset(m.m, 0, p.size() * sizeof(float)));
+    HIPCHK(c, hipMemset(m.v, 0, p.size() * sizeof(float)));
+    HIPCHK(c, hipMemset(c->d_pos, 0, sizeof(unsigned long long)));
+    k_to_bf16<<<mlp_blocks(MlpLayout::N), 256, 0, c->stream>>>(MlpLayout::N, m.p, m.p16);
+    HIPCHK(c, hipGetLastError());
+    if ((r = sur_pack(c))) return r;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    m.step = 0;
+    c->sur_seed = (unsigned int)(seed ^ (seed >> 32));
+    c->sur_init = true;
+    c->sur_norm = false;
+    c->sur_on = 0;
+    memset(&c->sst, 0, sizeof c->sst);
+#ifndef FJ_SUR_TFLOOR
+#define FJ_SUR_TFLOOR 1e-6f
+#endif
+    c->nm.tfloor = FJ_SUR_TFLOOR;
+    return 0;
+}
+
+int fj_surrogate_collect(fj_ctx *c, double keep) {
+    if (!c || keep < 0 || keep > 1) return FJ_E_ARG;
+    if (!c->sur_init) return set_err(c, FJ_E_STATE, "fj_surrogate_init first");
+    c->sur_keep = keep;
+    return 0;
+}
+
+int fj_surrogate_mode(fj_ctx *c, int on, double judge_fraction, double jz_min) {
+    if (!c || judge_fraction < 0 || judge_fraction > 1) return FJ_E_ARG;
+    if (on && !(c->sur_init && c->sur_norm)) return set_err(c, FJ_E_STATE, "train the surrogate first");
+    c->sur_on = on;
+    c->sur_judge = judge_fraction;
+    c->sur_jzmin = jz_min;
+    return 0;
+}
+
+namespace {
+// Input / target normalisation and the training-set box, from the current set.
+int sur_normalise(fj_ctx *c, long long n) {
+    std::vector<float> X((size_t)n * MLP_IN), T((size_t)n * SUR_TW);
+    HIPCHK(c, hipMemcpy(X.data(), c->d_X, X.size() * sizeof(float), hipMemcpyDeviceToHost));
+    HIPCHK(c, hipMemcpy(T.data(), c->d_T, T.size() * sizeof(float), hipMemcpyDeviceToHost));
+    auto stats = [&](const std::vector<float> &A, int w, int k, float &mu, float &sg) {
+        double s = 0, s2 = 0;
+        for (long long i = 0; i < n; i++) { const double x = A[(size_t)i * w + k]; s += x; s2 += x * x; }
+        const double m = s / n, var = s2 / n - m * m;
+        mu = (float)m;
+        sg = (float)sqrt(var > 1e-12 ? var : 1e-12);
+    };
+    for (int k = 0; k < MLP_IN; k++) stats(X, MLP_IN, k, c->nm.mu[k], c->nm.sig[k]);
+    for (int k = 0; k < MLP_OUT; k++) stats(T, SUR_TW, k, c->nm.tmu[k], c->nm.tsig[k]);
+    const int nb = 256;
+    float *part = nullptr;
+    HIPCHK(c, hipMalloc((void **)&part, nb * 2 * MLP_IN * sizeof(float)));
+    k_sur_box<<<nb, 256, 0, c->stream>>>(n, c->d_X, c->nm, part);
+    std::vector<float> hp(nb * 2 * MLP_IN);
+    HIPCHK(c, hipMemcpyAsync(hp.data(), part, hp.size() * sizeof(float), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    (void)hipFree(part);
+    for (int k = 0; k < MLP_IN; k++) {
+        float lo = 3.4e38f, hi = -3.4e38f;
+        for (int b = 0; b < nb; b++) {
+            lo = fminf(lo, hp[b * 2 * MLP_IN + k]);
+            hi = fmaxf(hi, hp[b * 2 * MLP_IN + MLP_IN + k]);
+        }
+        c->nm.lo[k] = lo;
+        c->nm.hi[k] = hi;
+    }
+    c->sur_norm = true;
+    return 0;
+}
+}  // namespace
+
+int fj_surrogate_train(fj_ctx *c, int epochs, int batch, double lr, double *loss) {
+    if (!c || epochs < 0 || batch < 64 || !(lr > 0)) return FJ_E_ARG;
+    if (!c->sur_init) return set_err(c, FJ_E_STATE, "fj_surrogate_init first");
+    const long long n = c->sst.n_train;
+    if (n < 1024) return set_err(c, FJ_E_STATE, "only %lld training samples", n);
+    if (batch > c->mlp.cap) batch = (int)c->mlp.cap;
+    int r;
+    if (!c->sur_norm && (r = sur_normalise(c, n))) return r;
+    hipEvent_t a, b;
+    HIPCHK(c, hipEventCreate(&a));
+    HIPCHK(c, hipEventCreate(&b));
+    HIPCHK(c, hipEventRecord(a, c->stream));
+    const long long P = (long long)(2147483647ll % n);   // prime > n: coprime with n
+    double last = 0;
+    for (int e = 0; e < epochs; e++) {
+        const float lre = (float)(lr * 0.5 * (1 + cos(3.141592653589793 * e / (epochs > 1 ? epochs : 1))));
+        const long long off = (long long)(hash32(c->sur_seed + 104729u * (unsigned)(c->mlp.step + e)) % (unsigned)n);
+        HIPCHK(c, hipMemsetAsync(c->d_loss, 0, sizeof(double), c->stream));
+        for (long long j = 0; j < n; j += batch) {
+            const long long B = n - j < batch ? n - j : batch;
+            k_sur_batch<<<mlp_blocks(B), 256, 0, c->stream>>>(B, j, P, n, off, c->d_X, c->d_T, c->nm, c->mlp.x16,
+                                                             c->d_bt, c->d_bw);
+            mlp_forward(c->stream, c->mlp, B, c->mlp.x16, c->mlp.y);
+            mlp_backward(c->stream, c->mlp, B, c->d_bt, c->d_bw);
+            k_sum_into<<<1, 256, 0, c->stream>>>(mlp_blocks(B * MLP_OUT), c->mlp.loss_part, c->d_loss);
+            mlp_adam(c->stream, c->mlp, lre);
+        }
+        HIPCHK(c, hipGetLastError());
+        HIPCHK(c, hipMemcpyAsync(&last, c->d_loss, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        last /= (double)n * MLP_OUT;
+    }
+    if ((r = sur_pack(c))) return r;
+    HIPCHK(c, hipEventRecord(b, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    c->sst.ms_train = elapsed(a, b);
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    c->sst.last_loss = last;
+    if (loss) *loss = last;
+    return 0;
+}
+
+int fj_surrogate_get_stats(const fj_ctx *c, fj_surrogate_stats *out) {
+    if (!c || !out) return FJ_E_ARG;
+    *out = c->sst;
+    return 0;
+}
+
+int fj_surrogate_params(fj_ctx *c, float *get, const float *set) {
+    if (!c) return FJ_E_ARG;
+    if (!c->sur_init) return set_err(c, FJ_E_STATE, "fj_surrogate_init first");
+    const size_t N = MlpLayout::N;
+    if (set) {
+        HIPCHK(c, hipMemcpy(c->mlp.p, set, N * sizeof(float), hipMemcpyHostToDevice));
+        k_to_bf16<<<mlp_blocks((long long)N), 256, 0, c->stream>>>((long long)N, c->mlp.p, c->mlp.p16);
+        HIPCHK(c, hipGetLastError());
+        int r;
+        if ((r = sur_pack(c))) return r;
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+    }
+    if (get) HIPCHK(c, hipMemcpy(get, c->mlp.p, N * sizeof(float), hipMemcpyDeviceToHost));
+    return 0;
+}
+
+namespace {
+int sur_upload_inputs(fj_ctx *c, long long n, const float *x) {
+    float *dx = nullptr;
+    HIPCHK(c, hipMalloc((void **)&dx, (size_t)n * MLP_IN * sizeof(float)));
+    HIPCHK(c, hipMemcpy(dx, x, (size_t)n * MLP_IN * sizeof(float), hipMemcpyHostToDevice));
+    k_to_bf16<<<mlp_blocks(n * MLP_IN), 256, 0, c->stream>>>(n * MLP_IN, dx, c->mlp.x16);
+    HIPCHK(c, hipGetLastError());
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    (void)hipFree(dx);
+    return 0;
+}
+}  // namespace
+
+int fj_surrogate_forward(fj_ctx *c, long long n, const float *x, float *y) {
+    if (!c || n < 1 || !x || !y) return FJ_E_ARG;
+    if (!c->sur_init) return set_err(c, FJ_E_STATE, "fj_surrogate_init first");
+    if (n > c->mlp.cap) return set_err(c, FJ_E_ARG, "at most %lld rows", c->mlp.cap);
+    float *dx = nullptr;
+    HIPCHK(c, hipMalloc((void **)&dx, (size_t)n * MLP_IN * sizeof(float)));
+    HIPCHK(c, hipMemcpy(dx, x, (size_t)n * MLP_IN * sizeof(float), hipMemcpyHostToDevice));
+    k_mlp_infer_x<<<c->sur_blocks, 256, SUR_LDS, c->stream>>>(n, dx, c->d_wpk, c->d_wbias, c->mlp.y);
+    HIPCHK(c, hipGetLastError());
+    HIPCHK(c, hipMemcpyAsync(y, c->mlp.y, (size_t)n * MLP_OUT * sizeof(float), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    (void)hipFree(dx);
+    return 0;
+}
+
+int fj_surrogate_gradient(fj_ctx *c, long long n, const float *x, const float *t, float *grad, double *loss) {
+    if (!c || n < 1 || !x || !t || !grad) return FJ_E_ARG;
+    if (!c->sur_init) return set_err(c, FJ_E_STATE, "fj_surrogate_init first");
+    if (n > c->mlp.cap) return set_err(c, FJ_E_ARG, "at most %lld rows", c->mlp.cap);
+    int r;
+    if ((r = sur_upload_inputs(c, n, x))) return r;
+    HIPCHK(c, hipMemcpy(c->d_bt, t, (size_t)n * MLP_OUT * sizeof(float), hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemsetAsync(c->d_loss, 0, sizeof(double), c->stream));
+    mlp_forward(c->stream, c->mlp, n, c->mlp.x16, c->mlp.y);
+    mlp_backward(c->stream, c->mlp, n, c->d_bt);
+    k_sum_into<<<1, 256, 0, c->stream>>>(mlp_blocks(n * MLP_OUT), c->mlp.loss_part, c->d_loss);
+    HIPCHK(c, hipGetLastError());
+    double l = 0;
+    HIPCHK(c, hipMemcpyAsync(grad, c->mlp.g, MlpLayout::N * sizeof(float), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipMemcpyAsync(&l, c->d_loss, sizeof l, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    if (loss) *loss = l / ((double)n * MLP_OUT);
     return 0;
 }
 

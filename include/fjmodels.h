@@ -153,6 +153,51 @@ int fj_actions(fj_ctx *ctx, int n, const double *R, const double *z, const doubl
 
 int fj_get_stats(const fj_ctx *ctx, fj_stats *out);
 
+/* ------------------------------------------------------------------------
+ * Action surrogate.  No counterpart in the reference: it stands in for
+ * uvOrb (uvOrb.cpp:15-262) at the quadrature nodes of intermediate SCF
+ * iterations.  An MLP 5 -> 128 -> 128 -> 128 -> 2 (tanh) maps
+ * (log R, z/R, vR/vc, vz/vc, |vphi|/vc), vc^2 = R dPhi/dR + z dPhi/dz at the
+ * grid point, to (log(Jr + jf), log(Jz + jf)), jf = 1e-6; Jphi = R vphi stays
+ * exact.  BF16 weights and activations on the matrix cores with FP32
+ * accumulation, trained with Adam on the GPU on actions the Staeckel fudge
+ * computed at the nodes of earlier node phases.
+ * ------------------------------------------------------------------------ */
+typedef struct {
+    long long n_train;     /* samples in the training set */
+    long long nodes_sur;   /* bound nodes whose actions came from the network (last node phase) */
+    long long fallback;    /* bound nodes sent to the Staeckel fudge: outside the training box,
+                              |E| < 0.02 |Phi|, or predicted Jz below jz_min */
+    long long judged;      /* random sample computed both ways */
+    double judge_err_r_med, judge_err_r_p95, judge_err_z_med, judge_err_z_p95;  /* |J - J_St| / max(J_St, 1e-3 J0) */
+    double ms_surrogate;   /* inputs + inference + decisions of the last node phase */
+    double ms_train;       /* last fj_surrogate_train call */
+    double last_loss;      /* mean squared error of the last epoch (normalised targets) */
+} fj_surrogate_stats;
+
+/* Fresh network (Glorot-uniform weights from seed) and a training set of
+ * `capacity` samples (ring buffer). */
+int fj_surrogate_init(fj_ctx *ctx, unsigned long long seed, long long capacity);
+/* Staeckel node phases from now on add each bound node to the training set
+ * with probability keep (0 stops); in surrogate mode the judged nodes are
+ * added instead. */
+int fj_surrogate_collect(fj_ctx *ctx, double keep);
+/* `epochs` passes over the training set in batches (Adam, cosine-decayed
+ * rate lr); the first call fixes the input/target normalisation and the
+ * training box.  *loss: mean squared error of the last epoch. */
+int fj_surrogate_train(fj_ctx *ctx, int epochs, int batch, double lr, double *loss);
+/* on = 1: node phases take the actions from the network, with the Staeckel
+ * fudge for the fallback nodes and for a random `judge_fraction` of nodes. */
+int fj_surrogate_mode(fj_ctx *ctx, int on, double judge_fraction, double jz_min);
+int fj_surrogate_get_stats(const fj_ctx *ctx, fj_surrogate_stats *out);
+/* Network parameters as fp32 (W1[5][128], b1[128], W2[128][128], b2, W3, b3,
+ * W4[128][2], b4[2]: 34,050 values); get and/or set may be NULL. */
+int fj_surrogate_params(fj_ctx *ctx, float *get, const float *set);
+/* Raw network on normalised inputs x [n][5] -> y [n][2], and the gradient of
+ * the mean squared error against t [n][2] (n <= 1,048,576). */
+int fj_surrogate_forward(fj_ctx *ctx, long long n, const float *x, float *y);
+int fj_surrogate_gradient(fj_ctx *ctx, long long n, const float *x, const float *t, float *grad, double *loss);
+
 #ifdef __cplusplus
 }
 #endif
