@@ -91,6 +91,60 @@ def mixed_leg(ctx, model, prev, last, steps, pts):
             "counters": {k: int(np.mean([x[k] for x in st])) for k in ("qk_calls", "overflow_tasks", "integral_lanes")}}
 
 
+def surrogate_leg(ctx, model, guess_tables, n_iter=5, keep=0.5, epochs=10, judge=0.01):
+    """The same model converged twice from the guess potential, n_iter SCF
+    iterations each: plain FP64 Staeckel, and the surrogate schedule
+    (iterations 0-1 Staeckel with training samples, then training, the middle
+    iterations served by the network with a judged sample, the final
+    iteration Staeckel).  Reports both wall times (training included), the
+    final density coefficients' difference scaled by the monopole, and the
+    virial ratios."""
+    import numpy as np
+    import torch
+
+    def run(sur):
+        ctx.set_potential(*guess_tables)
+        model.total = [x.copy() for x in guess_tables]
+        model.philP = [tuple(x.copy() for x in guess_tables)]
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        t_it = []
+        for it in range(n_iter):
+            ti = time.perf_counter()
+            if sur and it == 0:
+                ctx.surrogate_init(seed=7, capacity=4 << 20)
+                ctx.surrogate_collect(keep)
+            if sur and it == 2:
+                ctx.surrogate_collect(0.0)
+                ctx.surrogate_train(epochs, batch=32768, lr=3e-3)
+                ctx.surrogate_collect(1.0)          # judged nodes join the set
+                ctx.surrogate_mode(True, judge=judge, jz_min=1e-3)
+            if sur and it == n_iter - 1:
+                ctx.surrogate_mode(False)
+            out = model.iterate()[0]
+            if sur and 2 <= it < n_iter - 2:
+                ctx.surrogate_train(1, batch=32768, lr=1e-3)   # fine-tune on the judged nodes
+            t_it.append(time.perf_counter() - ti)
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        vir = ctx.virial(out["rhl"], out["vrotl"], out["sigRl"], out["sigpl"], out["sigzl"])
+        st = ctx.surrogate_stats() if sur else None
+        ctx.surrogate_mode(False) if sur else None
+        return dt, t_it, out, vir, st
+
+    t_p, it_p, out_p, vir_p, _ = run(False)
+    t_s, it_s, out_s, vir_s, st = run(True)
+    rho_err = float(np.max(np.abs(out_s["rhl"] - out_p["rhl"]) / np.abs(out_p["rhl"][:, :1])))
+    return {"iterations": n_iter, "ms_plain": 1e3 * t_p, "ms_surrogate": 1e3 * t_s, "speedup": t_p / t_s,
+            "ms_per_iter_plain": [round(1e3 * x, 2) for x in it_p],
+            "ms_per_iter_surrogate": [round(1e3 * x, 2) for x in it_s],
+            "rho_l_err_vs_plain": rho_err, "virial_WK_plain": float(vir_p[4]), "virial_WK_surrogate": float(vir_s[4]),
+            "judge": {k: st[k] for k in ("judge_err_r_med", "judge_err_r_p95", "judge_err_z_med", "judge_err_z_p95",
+                                         "n_train", "nodes_sur", "fallback", "judged")},
+            "note": "training samples keep %.2f of the bound nodes of iterations 0-1, %d epochs; middle iterations "
+                    "served by the network with %.0f%% judged; final iteration Staeckel" % (keep, epochs, 100 * judge)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -98,6 +152,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision / block-scan leg")
+    ap.add_argument("--no-surrogate", action="store_true", help="skip the plain vs surrogate convergence leg")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run a 1 GiB device-to-device copy (reference for FETCH_SIZE/WRITE_SIZE counters)")
     args = ap.parse_args()
@@ -160,6 +215,8 @@ def main():
     }
     if not args.no_mixed:
         line["mixed"] = mixed_leg(ctx, model, prev, last, args.steps, pts)
+    if not args.no_surrogate:
+        line["surrogate"] = surrogate_leg(ctx, model, guess_tables)
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(npoly, ntheta, nr, ar, guess_tables, delta)
     if args.copy_calibration:
