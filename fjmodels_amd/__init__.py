@@ -6,10 +6,10 @@ mirror the reference's Potential / computeNewPhi / setDF / tabulateDelta
 surface.  This Python module is a thin ctypes binding used by tests and the
 benchmark; importing it does not touch the GPU.
 """
-from .capi import (Context, DFParams, FJError, df_params, FJ_QUAD_LOG22, FJ_QUAD_TANHSINH22,
+from .capi import (Batch, Context, DFParams, FJError, df_params, FJ_QUAD_LOG22, FJ_QUAD_TANHSINH22,
                    FJ_POISSON_QUADPACK, FJ_POISSON_BLOCKSCAN, FJ_PREC_FP64, FJ_PREC_MIXED, LIB_PATH)
 from .scf import log_grid, Model
 
-__all__ = ["Context", "DFParams", "FJError", "df_params", "log_grid", "Model", "LIB_PATH",
+__all__ = ["Batch", "Context", "DFParams", "FJError", "df_params", "log_grid", "Model", "LIB_PATH",
            "FJ_QUAD_LOG22", "FJ_QUAD_TANHSINH22", "FJ_POISSON_QUADPACK", "FJ_POISSON_BLOCKSCAN",
            "FJ_PREC_FP64", "FJ_PREC_MIXED"]

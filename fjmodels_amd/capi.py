@@ -28,6 +28,8 @@ EXPORTS = [
     "fj_eval_potential", "fj_actions", "fj_get_stats",
     "fj_surrogate_init", "fj_surrogate_collect", "fj_surrogate_train", "fj_surrogate_mode",
     "fj_surrogate_get_stats", "fj_surrogate_params", "fj_surrogate_forward", "fj_surrogate_gradient",
+    "fj_batch_create", "fj_batch_destroy", "fj_batch_setup", "fj_batch_autotune", "fj_batch_iterate",
+    "fj_batch_get", "fj_batch_get_stats",
 ]
 
 MLP_NPARAM = 5 * 128 + 128 + 2 * (128 * 128 + 128) + 128 * 2 + 2   # 34,050
@@ -53,6 +55,17 @@ class SurrogateStats(C.Structure):
     _fields_ = ([(n, C.c_longlong) for n in ("n_train", "nodes_sur", "fallback", "judged")] +
                 [(n, C.c_double) for n in ("judge_err_r_med", "judge_err_r_p95", "judge_err_z_med",
                                            "judge_err_z_p95", "ms_surrogate", "ms_train", "last_loss")])
+
+
+class ModelSpec(C.Structure):
+    _fields_ = [("nr", C.c_int), ("ar", _dp), ("ntheta", C.c_int), ("npoly", C.c_int), ("guess", C.c_int),
+                ("mass", C.c_double), ("r0", C.c_double), ("q", C.c_double), ("df", DFParams), ("quad", C.c_int)]
+
+
+class BatchStats(C.Structure):
+    _fields_ = [("ms_setup", C.c_double), ("ms_tune", C.c_double), ("ms_last_iter", C.c_double),
+                ("nodes", C.c_longlong), ("points", C.c_longlong), ("setup_block", C.c_int),
+                ("integ_blocks", C.c_int), ("graph", C.c_int)]
 
 
 def df_params(model="Hernquist", dphi_h=0.55, dz_h=0.55, dphi_g=1.0, dz_g=1.0,
@@ -104,6 +117,13 @@ def load():
         "fj_surrogate_forward": [v, C.c_longlong, C.POINTER(C.c_float), C.POINTER(C.c_float)],
         "fj_surrogate_gradient": [v, C.c_longlong, C.POINTER(C.c_float), C.POINTER(C.c_float),
                                   C.POINTER(C.c_float), C.POINTER(d)],
+        "fj_batch_create": [v, i, C.POINTER(ModelSpec), C.POINTER(v)],
+        "fj_batch_destroy": [v],
+        "fj_batch_setup": [v],
+        "fj_batch_autotune": [v],
+        "fj_batch_iterate": [v, i],
+        "fj_batch_get": [v, i] + [_dp] * 10,
+        "fj_batch_get_stats": [v, C.POINTER(BatchStats)],
     }
     for name, args in sig.items():
         f = getattr(lib, name)
@@ -318,3 +338,54 @@ class Context:
         self._chk(self.lib.fj_surrogate_gradient(self.h, len(x), x.ctypes.data_as(fp), t.ctypes.data_as(fp),
                                                  g.ctypes.data_as(fp), C.byref(loss)), "fj_surrogate_gradient")
         return g, loss.value
+
+
+class Batch:
+    """Many single-component models advanced together (fj_batch_*).  models:
+    dicts with ar, ntheta, npoly, model (guess), mass, r0, q, df, quad."""
+
+    def __init__(self, ctx, models):
+        self.ctx, self.lib = ctx, ctx.lib
+        self._ar = [_a(m["ar"]) for m in models]
+        specs = (ModelSpec * len(models))()
+        for k, m in enumerate(models):
+            specs[k] = ModelSpec(len(self._ar[k]), _p(self._ar[k]), m["ntheta"], m["npoly"], GUESS[m["model"]],
+                                 m.get("mass", 1.0), m.get("r0", 1.0), m.get("q", 1.0), m["df"], m.get("quad", 0))
+        self.shapes = [(len(self._ar[k]), m["npoly"]) for k, m in enumerate(models)]
+        h = C.c_void_p()
+        ctx._chk(self.lib.fj_batch_create(ctx.h, len(models), specs, C.byref(h)), "fj_batch_create")
+        self.h = h
+
+    def close(self):
+        if self.h:
+            self.lib.fj_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def setup(self):
+        self.ctx._chk(self.lib.fj_batch_setup(self.h), "fj_batch_setup")
+
+    def autotune(self):
+        self.ctx._chk(self.lib.fj_batch_autotune(self.h), "fj_batch_autotune")
+
+    def iterate(self, n=1):
+        self.ctx._chk(self.lib.fj_batch_iterate(self.h, n), "fj_batch_iterate")
+
+    def get(self, m):
+        nr, npoly = self.shapes[m]
+        t = [np.zeros((nr, npoly)) for _ in range(8)]
+        E, D = np.zeros(30), np.zeros(30)
+        self.ctx._chk(self.lib.fj_batch_get(self.h, m, *[_p(x) for x in t], _p(E), _p(D)), "fj_batch_get")
+        out = dict(zip(["rhl", "sigRl", "sigpl", "sigzl", "vrotl", "phil", "Pr", "Pr2"], t))
+        out["Egrid"], out["Dgrid"] = E, D
+        return out
+
+    def stats(self):
+        s = BatchStats()
+        self.ctx._chk(self.lib.fj_batch_get_stats(self.h, C.byref(s)), "fj_batch_get_stats")
+        return {n: getattr(s, n) for n, _ in BatchStats._fields_}

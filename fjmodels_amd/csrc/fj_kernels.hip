@@ -22,6 +22,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <type_traits>
 #include <vector>
 
 #include "../../include/fjmodels.h"
@@ -263,11 +264,10 @@ __device__ inline void node_velocity(int quad, int node, double Ve, double V[3],
     w = wa[i] * c_xw11[j] * wa[k];
 }
 
-// k_setup: everything of df()/uvOrb before the two QAGS integrals.
-__global__ void __launch_bounds__(256) k_setup(TabArgs ta, SetupArgs sa) {
-    extern __shared__ double smem[];
-    const LdsTabs L = stage_tables(ta, smem);
-    const long long tid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+// k_setup: everything of df()/uvOrb before the two QAGS integrals, for
+// node tid of the launch (every lane of a wave calls it: task appends are
+// wave-aggregated).
+__device__ inline void setup_node(const LdsTabs &L, const SetupArgs &sa, long long tid) {
     const bool explicit_v = sa.vR != nullptr;
     const long long nnodes = sa.sel ? (long long)*sa.nsel : explicit_v ? (long long)sa.npts : (long long)sa.npts * NQ;
     const bool valid = tid < nnodes;
@@ -303,6 +303,12 @@ __global__ void __launch_bounds__(256) k_setup(TabArgs ta, SetupArgs sa) {
     if (__lane_id() == 0 && nb_) atomicAdd(&sa.ctr->bound, nb_);
     if (valid && brent_it) atomicAdd(&sa.ctr->brent, (unsigned long long)brent_it);
     if (valid && nev) atomicAdd(&sa.ctr->grad, (unsigned long long)nev);
+}
+
+__global__ void __launch_bounds__(256) k_setup(TabArgs ta, SetupArgs sa) {
+    extern __shared__ double smem[];
+    const LdsTabs L = stage_tables(ta, smem);
+    setup_node(L, sa, (long long)blockIdx.x * blockDim.x + threadIdx.x);
 }
 
 // df_hg (DF.cpp:65-76) and the rotation term (DF.cpp:114-119)
@@ -662,12 +668,15 @@ struct IntegArgs {
     const int *list_u, *list_v;
     const unsigned int *cnt_u, *cnt_v;
     unsigned int *next;           // task counter
-    int *ovf_list;                // overflow task ids (task index into u..v space)
+    int *ovf_list;                // overflowed tasks as node * 2 + type
     unsigned int *ovf_cnt;
     double *ws;                   // per-lane workspace
     int cap;                      // intervals per lane
     int ntask_override;           // >= 0: tasks are ovf_list[0..n)
     Counters *ctr;
+    const unsigned int *ntask_dev;   // if set: tasks are ovf_list[0 .. *ntask_dev) (count on the device)
+    const PotView *pv;               // if set: task tables pv[pmodel[node / NQ]] (batched overflow pass)
+    const int *pmodel;
 };
 
 
@@ -680,8 +689,10 @@ struct IntegArgs {
 
 template <class FI, class FV>
 __device__ __forceinline__ void integrals_loop(const IntegArgs &ia, FI &f, const QagsWork &w, const FV &fv) {
-    const unsigned int nu = *ia.cnt_u, nv = *ia.cnt_v;
-    const unsigned int ntask = ia.ntask_override >= 0 ? (unsigned int)ia.ntask_override : nu + nv;
+    const bool from_list = ia.ntask_override >= 0 || ia.ntask_dev;
+    const unsigned int nu = from_list ? 0u : *ia.cnt_u, nv = from_list ? 0u : *ia.cnt_v;
+    const unsigned int ntask = ia.ntask_dev ? *ia.ntask_dev
+                               : ia.ntask_override >= 0 ? (unsigned int)ia.ntask_override : nu + nv;
     int task = -1;    // -1: needs a task, -2: queue drained
     long long node = 0;
     Qags q;
@@ -700,11 +711,19 @@ __device__ __forceinline__ void integrals_loop(const IntegArgs &ia, FI &f, const
                 const unsigned long long lower = (lane == 0) ? 0ull : (mask & ((~0ull) >> (64 - lane)));
                 const unsigned int t = base + (unsigned int)__popcll(lower);
                 if (t < ntask) {
-                    const unsigned int tt = ia.ntask_override >= 0 ? (unsigned int)ia.ovf_list[t] : t;
-                    task = (int)tt;
-                    const int type = tt < nu ? 0 : 1;
-                    node = type == 0 ? ia.list_u[tt] : ia.list_v[tt - nu];
+                    int type;
+                    if (from_list) {
+                        const int code = ia.ovf_list[t];   // node * 2 + type
+                        node = code >> 1;
+                        type = code & 1;
+                    } else {
+                        type = t < nu ? 0 : 1;
+                        node = type == 0 ? ia.list_u[t] : ia.list_v[t - nu];
+                    }
+                    task = 0;
                     load_jint(ia.nb, node, type, f);
+                    if constexpr (std::is_same<FI, JInt>::value)
+                        if (ia.pv) f.T = &ia.pv[ia.pmodel[node / NQ]];
                     qags_begin(q, 0, type == 0 ? PI : PIH, EPSABSINTEG, EPSRELINTEG, 1000);
                 } else {
                     task = -2;
@@ -719,7 +738,7 @@ __device__ __forceinline__ void integrals_loop(const IntegArgs &ia, FI &f, const
             if (q.state == 3) {
                 if (q.overflow) {
                     const unsigned int slot = atomicAdd(ia.ovf_cnt, 1u);
-                    ia.ovf_list[slot] = task;
+                    ia.ovf_list[slot] = (int)(node * 2 + f.type);
                     novf++;
                 } else {
                     (f.type == 0 ? ia.nb.Jr : ia.nb.Jz)[node] = f.action(q.result);
@@ -773,9 +792,15 @@ struct MomArgs {
 
 
 // k_moments: one block per point; fixed-order reduction of the 6 sums.
+__device__ inline void moments_point(const MomArgs &ma, int pl, double (*red)[256]);
+
 __global__ void __launch_bounds__(256) k_moments(MomArgs ma) {
     __shared__ double red[6][256];
-    const int pl = blockIdx.x;
+    moments_point(ma, blockIdx.x, red);
+}
+
+// The six sums of point p0 + pl (one block of 256 threads).
+__device__ inline void moments_point(const MomArgs &ma, int pl, double (*red)[256]) {
     const int p = ma.p0 + pl;
     const double R = ma.R[p], Ve = ma.Ve[p];
     const double jacob = Ve * Ve * Ve;
@@ -818,9 +843,14 @@ __global__ void __launch_bounds__(256) k_moments(MomArgs ma) {
 
 // Mixed precision: f(J) in FP32 with compensated FP32 sums; the pairwise
 // tree adds the (sum, compensation) pairs with an exact two-sum.
+__device__ inline void moments_point_f32(const MomArgs &ma, int pl, float (*red)[2][256]);
+
 __global__ void __launch_bounds__(256) k_moments_f32(MomArgs ma) {
     __shared__ float red[6][2][256];
-    const int pl = blockIdx.x;
+    moments_point_f32(ma, blockIdx.x, red);
+}
+
+__device__ inline void moments_point_f32(const MomArgs &ma, int pl, float (*red)[2][256]) {
     const int p = ma.p0 + pl;
     const double R = ma.R[p], Ve = ma.Ve[p];
     const float jacob = (float)(Ve * Ve * Ve);
@@ -1029,10 +1059,18 @@ __device__ void block_scan(double *v, int n, bool reverse, double *part) {
     __syncthreads();
 }
 
+__device__ inline void poisson_scan_col(int nr, int np_, const double *ar, const double *rhl, double *philP,
+                                        double *PrP, double *Pr2P, int np, double *Iint, double *Iext, double *part);
+
 __global__ void __launch_bounds__(PSCAN_THREADS) k_poisson_scan(int nr, int np_, const double *ar, const double *rhl,
                                                               double *philP, double *PrP, double *Pr2P) {
     __shared__ double Iint[PSCAN_MAXNR], Iext[PSCAN_MAXNR], part[PSCAN_THREADS];
-    const int np = blockIdx.x;
+    poisson_scan_col(nr, np_, ar, rhl, philP, PrP, Pr2P, blockIdx.x, Iint, Iext, part);
+}
+
+// Column np (order l = 2 np) of the block-scan Poisson solve, one block.
+__device__ inline void poisson_scan_col(int nr, int np_, const double *ar, const double *rhl, double *philP,
+                                        double *PrP, double *Pr2P, int np, double *Iint, double *Iext, double *part) {
     const int l = 2 * np;
     // segment i = [ar[i], ar[i+1]] stored at i+1 (internal) and i (external)
     for (int i = threadIdx.x; i < nr; i += blockDim.x) {
@@ -1378,11 +1416,16 @@ __device__ inline double dl_getrc(const PotView &T, double Lzsq, double R) {
 }
 
 // tabulateDelta prologue (tabulate.cpp:14-25), entries 0 and 1 of the radius loop.
+__device__ inline void delta_prep(const PotView &T, double *prep);
+
 __global__ void k_delta_prep(TabArgs ta, double *prep) {
     extern __shared__ double smem[];
     const LdsTabs L = stage_tables(ta, smem);
     if (threadIdx.x != 0) return;
-    const PotView &T = L.T;
+    delta_prep(L.T, prep);
+}
+
+__device__ inline void delta_prep(const PotView &T, double *prep) {
     const int NR = T.nr;
     const double Rmax = T.ar[NR - 1];
     const double Lzmax = Rmax * 0.98 * sqrt(-2 * pot_phi(T, Rmax, 0)), Rmin = 1e-3 * Rmax;
@@ -1399,22 +1442,30 @@ __global__ void k_delta_prep(TabArgs ta, double *prep) {
 }
 
 // tabulate.cpp:27-35 + Delta::getBestDelta (Delta.cpp:248-289), one lane per energy.
+__device__ inline void delta_energy(const PotView &T, int NG, const double *prep, double *scratch, int nE,
+                                    double *Eout, double *Dout, int *status);
+
 __global__ void k_delta(TabArgs ta, const double *prep, double *scratch, double *Eout, double *Dout, int *status) {
     extern __shared__ double smem[];
     const LdsTabs L = stage_tables(ta, smem);
     const int nE = blockIdx.x * blockDim.x + threadIdx.x;
-    const int NG = L.ngrid;
-    if (nE >= NG) return;
+    if (nE >= L.ngrid) return;
+    delta_energy(L.T, L.ngrid, prep, scratch + (size_t)nE * 400, nE, Eout, Dout, status);
+}
+
+// scratch: 400 doubles of this lane
+__device__ inline void delta_energy(const PotView &T, int NG, const double *prep, double *scratch, int nE,
+                                    double *Eout, double *Dout, int *status) {
     const double Lz1 = prep[0], Rg1 = prep[1], Ec1 = prep[2], Vm1 = prep[3];
     const double t = Vm1 * (nE + 1) / (double)NG;
     const double E = Ec1 + .5 * (t * t);
     DeltaLane d;
-    d.T = &L.T; d.E = E; d.Lz = .25 * Lz1; d.status = 0;
-    d.Rp = scratch + (size_t)nE * 400; d.zp = d.Rp + 200;
+    d.T = &T; d.E = E; d.Lz = .25 * Lz1; d.status = 0;
+    d.Rp = scratch; d.zp = d.Rp + 200;
     d.D2 = 0; d.sqDR = 0; d.Ri = 0; d.zi = 0; d.sv = 0; d.cv = 0; d.v0 = 0; d.N = 0;
     // Delta ctor: Rc = RcE(E, R0) (Utils.h:247-263)
     {
-        DLRcE F{&L.T, E};
+        DLRcE F{&T, E};
         double R0 = .8 * Rg1, f0 = F(R0);
         int guard = 0;
         while (f0 > 0 && guard++ < 4000) { R0 *= .8; f0 = F(R0); }
@@ -1466,6 +1517,265 @@ __global__ void k_delta(TabArgs ta, const double *prep, double *scratch, double 
     if (d.status) atomicOr(status, d.status);
 }
 
+
+// ===========================================================================
+// Batched models: every SCF phase of a batch of independent single-component
+// models (own grid, DF, Delta table) is one launch.  Blocks map to models
+// through prefix tables, so each block still stages its own model's tables in
+// LDS; the persistent integrator gives every model blocks in proportion to
+// its task count.  No host synchronisation inside an iteration (graph capture).
+// ===========================================================================
+struct ModelDev {
+    TabArgs ta;              // phil/Pr/E/D point at this model's installed tables
+    double *Pr2;
+    fj_df_params df;
+    int quad;
+    int po, np;              // first point (in the batch), points
+    int nr, ntheta, npoly;
+    int guess;               // FJ_GUESS_*
+    double mass, r0, q;
+    const double *ci, *si, *poly;   // theta nodes and projection weights [npoly][ntheta]
+    double *rhl, *vrotl, *sigRl, *sigpl, *sigzl, *sigRzl;   // projected [nr][npoly]
+    double *prep;            // 4 doubles: tabulateDelta prologue
+};
+
+__device__ inline int find_seg(const int *start, int M, int x) {   // largest m with start[m] <= x
+    int lo = 0, hi = M - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (start[mid] <= x) lo = mid; else hi = mid - 1;
+    }
+    return lo;
+}
+
+__device__ inline PotView global_potview(const TabArgs &ta) {
+    PotView T;
+    T.nr = ta.nr; T.npoly = ta.npoly; T.ar = ta.ar; T.phil = ta.phil; T.Pr = ta.Pr; T.lc = &c_legcoef;
+    T.lut = ta.lut; T.lg0 = ta.lg0; T.lsc = ta.lsc;
+    return T;
+}
+
+__global__ void kb_potviews(const ModelDev *md, int M, PotView *pv) {
+    const int m = blockIdx.x * blockDim.x + threadIdx.x;
+    if (m < M) pv[m] = global_potview(md[m].ta);
+}
+
+// counters of model m: cnt[8 m + 0] u tasks, 1 v tasks, 2 next task; cnt[8 M + 0]
+// overflowed tasks, 1 next overflow task, 2 second overflow count
+__global__ void kb_zero(unsigned int *p, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0u;
+}
+__global__ void kb_zero64(unsigned long long *p, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0ull;
+}
+
+__global__ void __launch_bounds__(256) kb_points(const ModelDev *md, const int *pmodel, PointArgs pa, int npts) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= npts) return;
+    const PotView T = global_potview(md[pmodel[i]].ta);
+    const double far = 10 * T.ar[T.nr - 1];
+    const double Phi_h = pot_phi(T, pa.R[i], pa.z[i]);
+    pa.Phi[i] = Phi_h;
+    pa.Ve[i] = sqrt(-2 * (Phi_h - pot_phi(T, far, far)));
+    if (pa.Vc) {
+        double dR, dz;
+        pot_phi_grad(T, pa.R[i], pa.z[i], &dR, &dz);
+        pa.Vc[i] = sqrt(fj_max(1e-30, pa.R[i] * dR + pa.z[i] * dz));
+    }
+}
+
+// Node setup of every model; model m owns blocks [bstart[m], bstart[m+1]).
+template <int BS>
+__global__ void __launch_bounds__(BS) kb_setup(const ModelDev *md, const int *bstart, int M, SetupArgs base,
+                                               unsigned int *cnt) {
+    extern __shared__ double smem[];
+    const int m = find_seg(bstart, M, blockIdx.x);
+    const ModelDev &d = md[m];
+    const LdsTabs L = stage_tables(d.ta, smem);
+    SetupArgs sa = base;
+    sa.p0 = 0;
+    sa.npts = d.po + d.np;
+    sa.quad = d.quad;
+    const long long no = (long long)d.po * NQ;
+    sa.list_u = base.list_u + no;
+    sa.list_v = base.list_v + no;
+    sa.cnt_u = cnt + 8 * m;
+    sa.cnt_v = cnt + 8 * m + 1;
+    const long long local = (long long)(blockIdx.x - bstart[m]) * BS + threadIdx.x;
+    setup_node(L, sa, local < (long long)d.np * NQ ? no + local : (long long)sa.npts * NQ);
+}
+
+// Blocks of the persistent integrator per model: one for every model with
+// tasks, the rest in proportion to the task counts (one thread, fixed order).
+__global__ void kb_assign(int M, const unsigned int *cnt, int B, int *block_model) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    unsigned long long tot = 0;
+    int with = 0;
+    for (int m = 0; m < M; m++) {
+        const unsigned int t = cnt[8 * m] + cnt[8 * m + 1];
+        tot += t;
+        with += t > 0;
+    }
+    int b = 0;
+    const int spare = B > with ? B - with : 0;
+    unsigned long long acc = 0;
+    int given = 0;
+    for (int m = 0; m < M && b < B; m++) {
+        const unsigned int t = cnt[8 * m] + cnt[8 * m + 1];
+        if (!t) continue;
+        acc += t;
+        const int upto = tot ? (int)((unsigned long long)spare * acc / tot) : 0;
+        int nb = 1 + (upto - given);
+        given = upto;
+        for (; nb > 0 && b < B; nb--) block_model[b++] = m;
+    }
+    for (; b < B; b++) block_model[b] = -1;
+}
+
+template <class FI>
+__device__ __forceinline__ IntegArgs batch_integ_args(const IntegArgs &base, const ModelDev &d, unsigned int *cnt,
+                                                      int m) {
+    IntegArgs ia = base;
+    const long long no = (long long)d.po * NQ;
+    ia.list_u = base.list_u + no;
+    ia.list_v = base.list_v + no;
+    ia.cnt_u = cnt + 8 * m;
+    ia.cnt_v = cnt + 8 * m + 1;
+    ia.next = cnt + 8 * m + 2;
+    return ia;
+}
+
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(FJ_INTEG_WAVES)))
+kb_integrals(const ModelDev *md, const int *block_model, IntegArgs base, unsigned int *cnt) {
+    extern __shared__ double smem[];
+    const int m = block_model[blockIdx.x];
+    if (m < 0) return;
+    const ModelDev &d = md[m];
+    const LdsTabs L = stage_tables(d.ta, smem);
+    const IntegArgs ia = batch_integ_args<JInt>(base, d, cnt, m);
+    const long long lane_g = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const QagsWork w = lane_work(ia.ws, ia.cap, lane_g, (long long)gridDim.x * blockDim.x);
+    const FvStrided fv{smem + tab_lds_doubles(d.ta.nr, d.ta.npoly, d.ta.ngrid) + threadIdx.x, (int)blockDim.x};
+    JInt f;
+    f.T = &L.T;
+    integrals_loop(ia, f, w, fv);
+}
+
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(FJ_INTEG32_WAVES)))
+kb_integrals_f32(const ModelDev *md, const int *block_model, IntegArgs base, unsigned int *cnt) {
+    extern __shared__ double smem[];
+    const int m = block_model[blockIdx.x];
+    if (m < 0) return;
+    const ModelDev &d = md[m];
+    const LdsTabs L = stage_tables(d.ta, smem);
+    float *f32 = reinterpret_cast<float *>(smem + tab_lds_doubles(d.ta.nr, d.ta.npoly, d.ta.ngrid));
+    const PotView32 T32 = stage_tables32(L, f32);
+    const IntegArgs ia = batch_integ_args<JInt32>(base, d, cnt, m);
+    const long long lane_g = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const QagsWork w = lane_work(ia.ws, ia.cap, lane_g, (long long)gridDim.x * blockDim.x);
+    const FvStridedF fv{f32 + tab32_lds_floats(d.ta.nr, d.ta.npoly) + threadIdx.x, (int)blockDim.x};
+    JInt32 f;
+    f.T = &T32;
+    integrals_loop(ia, f, w, fv);
+}
+
+// Overflowed tasks of every model, FP64, full interval limit, tables read
+// from global memory per task.
+__global__ void __launch_bounds__(256) kb_integrals_ovf(IntegArgs ia) {
+    extern __shared__ double smem[];
+    const long long lane_g = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const QagsWork w = lane_work(ia.ws, ia.cap, lane_g, (long long)gridDim.x * blockDim.x);
+    const FvStrided fv{smem + threadIdx.x, (int)blockDim.x};
+    JInt f;
+    f.T = ia.pv;
+    integrals_loop(ia, f, w, fv);
+}
+
+__global__ void __launch_bounds__(256) kb_moments(const ModelDev *md, const int *pmodel, MomArgs base, int mixed) {
+    __shared__ double red[6][256];
+    const int p = blockIdx.x;
+    const ModelDev &d = md[pmodel[p]];
+    MomArgs ma = base;
+    ma.p0 = 0;
+    ma.df = d.df;
+    ma.quad = d.quad;
+    if (mixed) moments_point_f32(ma, p, reinterpret_cast<float (*)[2][256]>(&red[0][0]));
+    else moments_point(ma, p, red);
+}
+
+// rho[0][0] fix + get_Ylm for every model; entries [rstart[m], rstart[m+1]).
+__global__ void kb_project(const ModelDev *md, const int *rstart, int M, const double *mom, int npts_total) {
+    const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= rstart[M]) return;
+    const int m = find_seg(rstart, M, idx);
+    const ModelDev &d = md[m];
+    const int loc = idx - rstart[m], np_ = d.npoly, ng = d.ntheta;
+    const int n = loc / np_, np = loc % np_;
+    const double *rho = mom + d.po, *vrot = mom + npts_total + d.po, *sigR = mom + 2 * npts_total + d.po,
+                 *sigp = mom + 3 * npts_total + d.po, *sigz = mom + 4 * npts_total + d.po,
+                 *sigRz = mom + 5 * npts_total + d.po;
+    const double *ar = d.ta.ar;
+    if (n == 0) {
+        if (np == 0) {
+            const double rho00 = (rho[2 * ng] - rho[1 * ng]) / (ar[2] - ar[1]) * (ar[0] - ar[1]) + rho[1 * ng];
+            d.rhl[0] = 2 * rho00; d.vrotl[0] = 2 * vrot[0]; d.sigRl[0] = 2 * sigR[0];
+            d.sigpl[0] = 2 * sigp[0]; d.sigzl[0] = 2 * sigz[0]; d.sigRzl[0] = 2 * sigRz[0];
+        } else {
+            d.rhl[np] = 0; d.vrotl[np] = 0; d.sigRl[np] = 0; d.sigpl[np] = 0; d.sigzl[np] = 0; d.sigRzl[np] = 0;
+        }
+        return;
+    }
+    double a = 0, b = 0, c = 0, e = 0, f = 0, g2 = 0;
+    for (int g = 0; g < ng; g++) {
+        const double wv = d.poly[np * ng + g];
+        const int i = n * ng + g;
+        a += wv * rho[i]; b += wv * vrot[i]; c += wv * sigR[i];
+        e += wv * sigp[i]; f += wv * sigz[i]; g2 += wv * sigRz[i];
+    }
+    d.rhl[loc] = a; d.vrotl[loc] = b; d.sigRl[loc] = c; d.sigpl[loc] = e; d.sigzl[loc] = f; d.sigRzl[loc] = g2;
+}
+
+// Guess density coefficients of every model (Potential::computeGuessRhl).
+__global__ void kb_guess(const ModelDev *md, const int *rstart, int M) {
+    const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= rstart[M]) return;
+    const int m = find_seg(rstart, M, idx);
+    const ModelDev &d = md[m];
+    const int loc = idx - rstart[m], np_ = d.npoly, ng = d.ntheta;
+    const int n = loc / np_, np = loc % np_;
+    const double q2 = d.q * d.q;
+    const double *ar = d.ta.ar;
+    double s = 0.;
+    for (int g = 0; g < ng; g++)
+        s += d.poly[np * ng + g] * d_rho_guess(d.guess, d.mass, d.r0, d.q, q2, ar[n] * d.si[g], ar[n] * d.ci[g]);
+    if (loc == 0) s = 2 * d_rho_guess(d.guess, d.mass, d.r0, d.q, q2, ar[0] * d.si[0], ar[0] * d.ci[0]);
+    d.rhl[loc] = s;
+}
+
+// Block-scan Poisson of every model into its installed tables; block (np, m).
+__global__ void __launch_bounds__(PSCAN_THREADS) kb_poisson(const ModelDev *md) {
+    __shared__ double Iint[PSCAN_MAXNR], Iext[PSCAN_MAXNR], part[PSCAN_THREADS];
+    const ModelDev &d = md[blockIdx.y];
+    if ((int)blockIdx.x >= d.npoly) return;
+    poisson_scan_col(d.nr, d.npoly, d.ta.ar, d.rhl, const_cast<double *>(d.ta.phil), const_cast<double *>(d.ta.Pr),
+                     d.Pr2, blockIdx.x, Iint, Iext, part);
+}
+
+__global__ void kb_delta_prep(const ModelDev *md, const PotView *pv, int M) {
+    const int m = blockIdx.x * blockDim.x + threadIdx.x;
+    if (m < M) delta_prep(pv[m], md[m].prep);
+}
+
+__global__ void kb_delta(const ModelDev *md, const PotView *pv, int M, double *scratch, int *status) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int NG = md[0].ta.ngrid;
+    if (i >= M * NG) return;
+    const int m = i / NG, nE = i % NG;
+    delta_energy(pv[m], NG, md[m].prep, scratch + (size_t)i * 400, nE, const_cast<double *>(md[m].ta.E),
+                 const_cast<double *>(md[m].ta.D), status);
+}
 
 // Diagnostic kernel: integrand samples and one QAGS run for one stored node.
 __global__ void k_debug_integral(TabArgs ta, NodeBuf nb, long long node, int type, const double *psi, int npsi,
@@ -2637,6 +2947,407 @@ int fj_surrogate_gradient(fj_ctx *c, long long n, const float *x, const float *t
     HIPCHK(c, hipMemcpyAsync(&l, c->d_loss, sizeof l, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     if (loss) *loss = l / ((double)n * MLP_OUT);
+    return 0;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Batched models (host side)
+// ---------------------------------------------------------------------------
+struct fj_batch {
+    fj_ctx *c = nullptr;
+    int M = 0, npts = 0, rows = 0, npoly_max = 0, cus = 0;
+    long long nodes = 0;
+    std::vector<ModelDev> hmd;
+    std::vector<void *> allocs;
+    ModelDev *d_md = nullptr;
+    PotView *d_pv = nullptr;
+    int *d_pmodel = nullptr, *d_bstart[2] = {nullptr, nullptr}, *d_rstart = nullptr, *d_block_model = nullptr;
+    int nblk[2] = {0, 0};                      // setup blocks for block sizes 128 and 256
+    double *d_R = nullptr, *d_z = nullptr, *d_Phi = nullptr, *d_Ve = nullptr, *d_Vc = nullptr, *d_out6 = nullptr,
+           *d_mom = nullptr;
+    double *d_nodef = nullptr, *d_Jr = nullptr, *d_Jz = nullptr;
+    int *d_flag = nullptr, *d_list_u = nullptr, *d_list_v = nullptr, *d_ovf = nullptr;
+    unsigned int *d_cnt = nullptr;
+    Counters *d_ctr = nullptr;
+    double *d_ws = nullptr, *d_wsfull = nullptr, *d_backup = nullptr;
+    long long ws_lanes = 0, wsfull_lanes = 0;
+    size_t lds_tab = 0, lds_integ = 0;
+    int setup_bs = 256, integ_blocks = 0, integ_max_blocks = 0, ovf_blocks = 64;
+    bool have_setup = false;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t gexec = nullptr;
+    fj_batch_stats st{};
+};
+
+namespace {
+
+template <class T>
+int balloc(fj_batch *b, T **p, size_t n) {
+    *p = nullptr;
+    const hipError_t e = hipMalloc((void **)p, (n ? n : 1) * sizeof(T));
+    if (e != hipSuccess) {
+        *p = nullptr;
+        return set_err(b->c, FJ_E_HIP, "hipMalloc of %zu bytes failed: %s", n * sizeof(T), hipGetErrorString(e));
+    }
+    b->allocs.push_back(*p);
+    return 0;
+}
+
+void batch_drop_graph(fj_batch *b) {
+    if (b->gexec) (void)hipGraphExecDestroy(b->gexec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    b->gexec = nullptr;
+    b->graph = nullptr;
+}
+
+// One SCF iteration of every model, stream-ordered, no host synchronisation.
+int batch_iteration(fj_batch *b) {
+    fj_ctx *c = b->c;
+    hipStream_t s = c->stream;
+    const int M = b->M;
+    const bool mixed = c->prec == FJ_PREC_MIXED;
+    kb_zero<<<(8 * M + 8 + 255) / 256, 256, 0, s>>>(b->d_cnt, 8 * M + 8);
+    kb_zero64<<<1, 64, 0, s>>>(reinterpret_cast<unsigned long long *>(b->d_ctr), (int)(sizeof(Counters) / 8));
+    PointArgs pa{b->d_R, b->d_z, b->d_Phi, b->d_Ve, b->d_Vc};
+    kb_points<<<(b->npts + 255) / 256, 256, 0, s>>>(b->d_md, b->d_pmodel, pa, b->npts);
+    NodeBuf nb{b->d_nodef, b->nodes, b->d_Jr, b->d_Jz, b->d_flag};
+    SetupArgs sa{b->d_R, b->d_z, b->d_Phi, b->d_Ve, 0, 0, 0, nb, nullptr, nullptr, b->d_list_u, b->d_list_v,
+                 b->d_ctr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+    if (b->setup_bs == 128)
+        kb_setup<128><<<b->nblk[0], 128, b->lds_tab, s>>>(b->d_md, b->d_bstart[0], M, sa, b->d_cnt);
+    else
+        kb_setup<256><<<b->nblk[1], 256, b->lds_tab, s>>>(b->d_md, b->d_bstart[1], M, sa, b->d_cnt);
+    kb_assign<<<1, 64, 0, s>>>(M, b->d_cnt, b->integ_blocks, b->d_block_model);
+    unsigned int *g = b->d_cnt + 8 * M;   // [0] overflowed, [1] next overflow task, [2] second overflow
+    IntegArgs ia{nb, b->d_list_u, b->d_list_v, nullptr, nullptr, nullptr, b->d_ovf, g,
+                 b->d_ws, mixed ? QCAP_MIXED : QCAP, -1, b->d_ctr};
+    if (mixed)
+        kb_integrals_f32<<<b->integ_blocks, 256, b->lds_integ, s>>>(b->d_md, b->d_block_model, ia, b->d_cnt);
+    else
+        kb_integrals<<<b->integ_blocks, 256, b->lds_integ, s>>>(b->d_md, b->d_block_model, ia, b->d_cnt);
+    IntegArgs io{nb, nullptr, nullptr, nullptr, nullptr, g + 1, b->d_ovf, g + 2, b->d_wsfull, QCAP_FULL, -1,
+                 b->d_ctr, g, b->d_pv, b->d_pmodel};
+    kb_integrals_ovf<<<b->ovf_blocks, 256, 20 * 256 * sizeof(double), s>>>(io);
+    MomArgs ma{b->d_Phi, b->d_Ve, b->d_R, 0, b->npts, 0, nb, fj_df_params{}, b->d_out6, b->d_mom,
+               b->d_mom + b->npts, b->d_mom + 2 * b->npts, b->d_mom + 3 * b->npts, b->d_mom + 4 * b->npts,
+               b->d_mom + 5 * b->npts};
+    kb_moments<<<b->npts, 256, 0, s>>>(b->d_md, b->d_pmodel, ma, mixed ? 1 : 0);
+    kb_project<<<(b->rows + 63) / 64, 64, 0, s>>>(b->d_md, b->d_rstart, M, b->d_mom, b->npts);
+    kb_poisson<<<dim3(b->npoly_max, M), PSCAN_THREADS, 0, s>>>(b->d_md);
+    HIPCHK(c, hipGetLastError());
+    return 0;
+}
+
+// Installed tables of every model <-> backup (autotuning restores them).
+int batch_save(fj_batch *b, bool restore) {
+    size_t off = 0;
+    for (int m = 0; m < b->M; m++) {
+        const ModelDev &d = b->hmd[m];
+        const size_t n = (size_t)d.nr * d.npoly;
+        double *tab[3] = {const_cast<double *>(d.ta.phil), const_cast<double *>(d.ta.Pr), d.Pr2};
+        for (int k = 0; k < 3; k++) {
+            double *bk = b->d_backup + off;
+            HIPCHK(b->c, hipMemcpyAsync(restore ? tab[k] : bk, restore ? bk : tab[k], n * sizeof(double),
+                                        hipMemcpyDeviceToDevice, b->c->stream));
+            off += n;
+        }
+    }
+    return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int fj_batch_create(fj_ctx *c, int M, const fj_model_spec *spec, fj_batch **out) {
+    if (!c || M < 1 || !spec || !out) return FJ_E_ARG;
+    *out = nullptr;
+    HIPCHK(c, hipSetDevice(c->device));
+    fj_batch *b = new fj_batch();
+    b->c = c;
+    b->M = M;
+    int r = 0;
+    const int NG = 30;
+    std::vector<int> pmodel, bstart[2], rstart;
+    size_t lds_max = 0;
+    for (int m = 0; m < M && !r; m++) {
+        const fj_model_spec &sp = spec[m];
+        if (sp.nr < 11 || !sp.ar || sp.ntheta < 1 || sp.npoly < 1 || sp.npoly > MAXPOLY || sp.nr > PSCAN_MAXNR) {
+            r = set_err(c, FJ_E_ARG, "model %d: need 11 <= nr <= %d, ntheta >= 1, 1 <= npoly <= %d", m, PSCAN_MAXNR,
+                        MAXPOLY);
+            break;
+        }
+        for (int i = 0; i < sp.nr; i++)
+            if (!(sp.ar[i] > 0) || (i && !(sp.ar[i] > sp.ar[i - 1]))) {
+                r = set_err(c, FJ_E_ARG, "model %d: radii must be positive and increasing", m);
+                break;
+            }
+        if (r) break;
+        ModelDev d{};
+        const int nr = sp.nr, np_ = sp.npoly, nt = sp.ntheta, nl = nr * np_;
+        double *ar, *phil, *Pr, *Pr2, *E, *D, *ci, *si, *poly, *co, *prep;
+        int *lut;
+        if ((r = balloc(b, &ar, nr)) || (r = balloc(b, &phil, nl)) || (r = balloc(b, &Pr, nl)) ||
+            (r = balloc(b, &Pr2, nl)) || (r = balloc(b, &E, NG)) || (r = balloc(b, &D, NG)) ||
+            (r = balloc(b, &ci, nt)) || (r = balloc(b, &si, nt)) || (r = balloc(b, &poly, (size_t)np_ * nt)) ||
+            (r = balloc(b, &co, (size_t)6 * nl)) || (r = balloc(b, &prep, 4)) || (r = balloc(b, &lut, NLUT)))
+            break;
+        std::vector<double> hc(nt), hw(nt), hs(nt), hp((size_t)np_ * nt), pol(np_);
+        host_gauleg(nt, 0, 1, hc.data(), hw.data());
+        for (int g = 0; g < nt; g++) {
+            hs[g] = sqrt(1 - hc[g] * hc[g]);
+            host_evenlegend(np_, hc[g], pol.data());
+            for (int k = 0; k < np_; k++) hp[(size_t)k * nt + g] = 2 * pol[k] * hw[g];
+        }
+        int hl[NLUT];
+        float lg0, lsc;
+        make_lut(nr, sp.ar, hl, &lg0, &lsc);
+        HIPCHK(c, hipMemcpy(ar, sp.ar, nr * sizeof(double), hipMemcpyHostToDevice));
+        HIPCHK(c, hipMemcpy(ci, hc.data(), nt * sizeof(double), hipMemcpyHostToDevice));
+        HIPCHK(c, hipMemcpy(si, hs.data(), nt * sizeof(double), hipMemcpyHostToDevice));
+        HIPCHK(c, hipMemcpy(poly, hp.data(), hp.size() * sizeof(double), hipMemcpyHostToDevice));
+        HIPCHK(c, hipMemcpy(lut, hl, sizeof hl, hipMemcpyHostToDevice));
+        d.ta.nr = nr; d.ta.npoly = np_; d.ta.ngrid = NG; d.ta.ar = ar; d.ta.phil = phil; d.ta.Pr = Pr;
+        d.ta.E = E; d.ta.D = D; d.ta.lut = lut; d.ta.lg0 = lg0; d.ta.lsc = lsc;
+        d.Pr2 = Pr2; d.df = sp.df; d.quad = sp.quad;
+        d.po = b->npts; d.np = nr * nt; d.nr = nr; d.ntheta = nt; d.npoly = np_;
+        d.guess = sp.guess; d.mass = sp.mass; d.r0 = sp.r0; d.q = sp.q;
+        d.ci = ci; d.si = si; d.poly = poly;
+        d.rhl = co; d.vrotl = co + nl; d.sigRl = co + 2 * nl; d.sigpl = co + 3 * nl; d.sigzl = co + 4 * nl;
+        d.sigRzl = co + 5 * nl;
+        d.prep = prep;
+        b->hmd.push_back(d);
+        // points and their coordinates
+        for (int n = 0; n < nr; n++)
+            for (int g = 0; g < nt; g++) pmodel.push_back(m);
+        bstart[0].push_back((int)0); bstart[1].push_back(0);
+        rstart.push_back(b->rows);
+        b->rows += nl;
+        b->npts += nr * nt;
+        b->npoly_max = np_ > b->npoly_max ? np_ : b->npoly_max;
+        const size_t lt = tab_lds_doubles(nr, np_, NG) * sizeof(double);
+        lds_max = lt > lds_max ? lt : lds_max;
+        // persistent integrator: FP64 (tables + 20 doubles per lane) or FP32 (+ FP32 tables, 20 floats per lane)
+        const size_t l64 = lt + 20 * 256 * sizeof(double);
+        const size_t l32 = lt + (tab32_lds_floats(nr, np_) + 20 * 256) * sizeof(float);
+        const size_t li = l64 > l32 ? l64 : l32;
+        b->lds_integ = li > b->lds_integ ? li : b->lds_integ;
+    }
+    if (r) { fj_batch_destroy(b); return r; }
+    b->nodes = (long long)b->npts * NQ;
+    if (b->nodes > (64ll << 20)) {
+        const long long n = b->nodes;
+        fj_batch_destroy(b);
+        return set_err(c, FJ_E_ARG, "batch of %lld nodes: at most %lld per batch (split the models)", n, 64ll << 20);
+    }
+    // block prefixes of the setup launch for block sizes 128 and 256
+    for (int k = 0; k < 2; k++) {
+        const int bs = k ? 256 : 128;
+        int acc = 0;
+        for (int m = 0; m < M; m++) {
+            bstart[k][m] = acc;
+            acc += (int)(((long long)b->hmd[m].np * NQ + bs - 1) / bs);
+        }
+        b->nblk[k] = acc;
+    }
+    rstart.push_back(b->rows);
+    std::vector<double> R(b->npts), z(b->npts);
+    for (int m = 0; m < M; m++) {
+        const ModelDev &d = b->hmd[m];
+        std::vector<double> hc(d.ntheta), hw(d.ntheta);
+        host_gauleg(d.ntheta, 0, 1, hc.data(), hw.data());
+        for (int n = 0; n < d.nr; n++)
+            for (int g = 0; g < d.ntheta; g++) {
+                R[d.po + n * d.ntheta + g] = spec[m].ar[n] * sqrt(1 - hc[g] * hc[g]);
+                z[d.po + n * d.ntheta + g] = spec[m].ar[n] * hc[g];
+            }
+    }
+    const long long nn = b->nodes;
+    b->lds_tab = lds_max;
+    if ((r = balloc(b, &b->d_md, M)) || (r = balloc(b, &b->d_pv, M)) || (r = balloc(b, &b->d_pmodel, b->npts)) ||
+        (r = balloc(b, &b->d_bstart[0], M)) || (r = balloc(b, &b->d_bstart[1], M)) ||
+        (r = balloc(b, &b->d_rstart, M + 1)) || (r = balloc(b, &b->d_R, b->npts)) || (r = balloc(b, &b->d_z, b->npts)) ||
+        (r = balloc(b, &b->d_Phi, b->npts)) || (r = balloc(b, &b->d_Ve, b->npts)) || (r = balloc(b, &b->d_Vc, b->npts)) ||
+        (r = balloc(b, &b->d_out6, (size_t)6 * b->npts)) || (r = balloc(b, &b->d_mom, (size_t)6 * b->npts)) ||
+        (r = balloc(b, &b->d_nodef, (size_t)NF_COUNT * nn)) || (r = balloc(b, &b->d_Jr, (size_t)nn)) ||
+        (r = balloc(b, &b->d_Jz, (size_t)nn)) || (r = balloc(b, &b->d_flag, (size_t)nn)) ||
+        (r = balloc(b, &b->d_list_u, (size_t)nn)) || (r = balloc(b, &b->d_list_v, (size_t)nn)) ||
+        (r = balloc(b, &b->d_ovf, (size_t)2 * nn)) || (r = balloc(b, &b->d_cnt, (size_t)8 * M + 8)) ||
+        (r = balloc(b, &b->d_ctr, 1)) || (r = balloc(b, &b->d_backup, (size_t)3 * b->rows))) {
+        fj_batch_destroy(b);
+        return r;
+    }
+    HIPCHK(c, hipMemcpy(b->d_md, b->hmd.data(), M * sizeof(ModelDev), hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(b->d_pmodel, pmodel.data(), pmodel.size() * sizeof(int), hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(b->d_bstart[0], bstart[0].data(), M * sizeof(int), hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(b->d_bstart[1], bstart[1].data(), M * sizeof(int), hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(b->d_rstart, rstart.data(), (M + 1) * sizeof(int), hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(b->d_R, R.data(), R.size() * sizeof(double), hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(b->d_z, z.data(), z.size() * sizeof(double), hipMemcpyHostToDevice));
+    // legendre constants (index-only values) for this device
+    const LegCoef lc = make_legcoef(MAXPOLY);
+    HIPCHK(c, hipMemcpyToSymbol(HIP_SYMBOL(c_legcoef), &lc, sizeof lc));
+    // persistent integrator: as many blocks as fit, at least one per model
+    hipDeviceProp_t prop;
+    HIPCHK(c, hipGetDeviceProperties(&prop, c->device));
+    int per_cu = 0;
+    HIPCHK(c, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kb_integrals, 256, b->lds_integ));
+    if (per_cu < 1) per_cu = 1;
+    b->cus = prop.multiProcessorCount;
+    b->integ_max_blocks = prop.multiProcessorCount * per_cu;
+    if (b->integ_max_blocks < M) b->integ_max_blocks = M;
+    b->integ_blocks = b->integ_max_blocks;
+    const long long lanes = (long long)b->integ_max_blocks * 256, flanes = (long long)b->ovf_blocks * 256;
+    if ((r = balloc(b, &b->d_block_model, (size_t)b->integ_max_blocks)) ||
+        (r = balloc(b, &b->d_ws, ws_doubles(QCAP, lanes))) || (r = balloc(b, &b->d_wsfull, ws_doubles(QCAP_FULL, flanes)))) {
+        fj_batch_destroy(b);
+        return r;
+    }
+    b->ws_lanes = lanes;
+    b->wsfull_lanes = flanes;
+    b->st.nodes = nn;
+    b->st.points = b->npts;
+    b->st.setup_block = b->setup_bs;
+    b->st.integ_blocks = b->integ_blocks;
+    kb_potviews<<<(M + 63) / 64, 64, 0, c->stream>>>(b->d_md, M, b->d_pv);
+    HIPCHK(c, hipGetLastError());
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    *out = b;
+    return 0;
+}
+
+int fj_batch_destroy(fj_batch *b) {
+    if (!b) return 0;
+    batch_drop_graph(b);
+    for (void *p : b->allocs)
+        if (p) (void)hipFree(p);
+    delete b;
+    return 0;
+}
+
+int fj_batch_setup(fj_batch *b) {
+    if (!b) return FJ_E_ARG;
+    fj_ctx *c = b->c;
+    hipEvent_t e0, e1;
+    HIPCHK(c, hipEventCreate(&e0));
+    HIPCHK(c, hipEventCreate(&e1));
+    HIPCHK(c, hipEventRecord(e0, c->stream));
+    kb_guess<<<(b->rows + 63) / 64, 64, 0, c->stream>>>(b->d_md, b->d_rstart, b->M);
+    kb_poisson<<<dim3(b->npoly_max, b->M), PSCAN_THREADS, 0, c->stream>>>(b->d_md);
+    kb_delta_prep<<<(b->M + 63) / 64, 64, 0, c->stream>>>(b->d_md, b->d_pv, b->M);
+    const int NG = 30, lanes = b->M * NG;
+    double *scr = nullptr;
+    int *dst = nullptr;
+    HIPCHK(c, hipMalloc((void **)&scr, (size_t)lanes * 400 * sizeof(double)));
+    HIPCHK(c, hipMalloc((void **)&dst, sizeof(int)));
+    HIPCHK(c, hipMemsetAsync(dst, 0, sizeof(int), c->stream));
+    kb_delta<<<(lanes + 63) / 64, 64, 0, c->stream>>>(b->d_md, b->d_pv, b->M, scr, dst);
+    HIPCHK(c, hipGetLastError());
+    HIPCHK(c, hipEventRecord(e1, c->stream));
+    int status = 0;
+    HIPCHK(c, hipMemcpyAsync(&status, dst, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    b->st.ms_setup = elapsed(e0, e1);
+    (void)hipFree(scr);
+    (void)hipFree(dst);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (status & 8) return set_err(c, FJ_E_DELTA, "Delta fit: no bracket for ds2dD2 (delta/Delta.cpp:276-281)");
+    b->have_setup = true;
+    batch_drop_graph(b);
+    return 0;
+}
+
+int fj_batch_autotune(fj_batch *b) {
+    if (!b) return FJ_E_ARG;
+    if (!b->have_setup) return set_err(b->c, FJ_E_STATE, "fj_batch_setup first");
+    fj_ctx *c = b->c;
+    int r;
+    hipEvent_t e0, e1, t0, t1;
+    HIPCHK(c, hipEventCreate(&e0)); HIPCHK(c, hipEventCreate(&e1));
+    HIPCHK(c, hipEventCreate(&t0)); HIPCHK(c, hipEventCreate(&t1));
+    HIPCHK(c, hipEventRecord(t0, c->stream));
+    if ((r = batch_save(b, false))) return r;
+    const int per_cu_max = b->integ_max_blocks / (b->cus > 0 ? b->cus : 1);
+    float best = 1e30f;
+    int best_bs = b->setup_bs, best_ib = b->integ_blocks;
+    for (int bs : {128, 256})
+        for (int k = 1; k <= (per_cu_max > 0 ? per_cu_max : 1); k++) {
+            int ib = k * b->cus;
+            if (ib < b->M) ib = b->M;
+            if (ib > b->integ_max_blocks) ib = b->integ_max_blocks;
+            b->setup_bs = bs;
+            b->integ_blocks = ib;
+            if ((r = batch_save(b, true))) return r;
+            HIPCHK(c, hipEventRecord(e0, c->stream));
+            if ((r = batch_iteration(b))) return r;
+            HIPCHK(c, hipEventRecord(e1, c->stream));
+            HIPCHK(c, hipStreamSynchronize(c->stream));
+            const float ms = elapsed(e0, e1);
+            if (ms < best) { best = ms; best_bs = bs; best_ib = ib; }
+        }
+    b->setup_bs = best_bs;
+    b->integ_blocks = best_ib;
+    if ((r = batch_save(b, true))) return r;
+    HIPCHK(c, hipEventRecord(t1, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    b->st.ms_tune = elapsed(t0, t1);
+    b->st.setup_block = best_bs;
+    b->st.integ_blocks = best_ib;
+    (void)hipEventDestroy(e0); (void)hipEventDestroy(e1); (void)hipEventDestroy(t0); (void)hipEventDestroy(t1);
+    batch_drop_graph(b);
+    return 0;
+}
+
+int fj_batch_iterate(fj_batch *b, int niter) {
+    if (!b || niter < 0) return FJ_E_ARG;
+    if (!b->have_setup) return set_err(b->c, FJ_E_STATE, "fj_batch_setup first");
+    fj_ctx *c = b->c;
+    int r;
+    if (!b->gexec) {
+        HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+        r = batch_iteration(b);
+        hipGraph_t g = nullptr;
+        const hipError_t e = hipStreamEndCapture(c->stream, &g);
+        if (r) { if (g) (void)hipGraphDestroy(g); return r; }
+        if (e != hipSuccess) return set_err(c, FJ_E_HIP, "graph capture: %s", hipGetErrorString(e));
+        b->graph = g;
+        HIPCHK(c, hipGraphInstantiate(&b->gexec, b->graph, nullptr, nullptr, 0));
+        b->st.graph = 1;
+    }
+    hipEvent_t e0, e1;
+    HIPCHK(c, hipEventCreate(&e0));
+    HIPCHK(c, hipEventCreate(&e1));
+    for (int it = 0; it < niter; it++) {
+        HIPCHK(c, hipEventRecord(e0, c->stream));
+        HIPCHK(c, hipGraphLaunch(b->gexec, c->stream));
+        HIPCHK(c, hipEventRecord(e1, c->stream));
+    }
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    if (niter) b->st.ms_last_iter = elapsed(e0, e1);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    return 0;
+}
+
+int fj_batch_get(fj_batch *b, int m, double *rhl, double *sigRl, double *sigpl, double *sigzl, double *vrotl,
+                 double *phil, double *Pr, double *Pr2, double *Egrid, double *Dgrid) {
+    if (!b || m < 0 || m >= b->M) return FJ_E_ARG;
+    fj_ctx *c = b->c;
+    const ModelDev &d = b->hmd[m];
+    const size_t n = (size_t)d.nr * d.npoly * sizeof(double);
+    const double *src[10] = {d.rhl, d.sigRl, d.sigpl, d.sigzl, d.vrotl, d.ta.phil, d.ta.Pr, d.Pr2, d.ta.E, d.ta.D};
+    double *dst[10] = {rhl, sigRl, sigpl, sigzl, vrotl, phil, Pr, Pr2, Egrid, Dgrid};
+    for (int k = 0; k < 10; k++)
+        if (dst[k]) HIPCHK(c, hipMemcpy(dst[k], src[k], k < 8 ? n : 30 * sizeof(double), hipMemcpyDeviceToHost));
+    return 0;
+}
+
+int fj_batch_get_stats(const fj_batch *b, fj_batch_stats *out) {
+    if (!b || !out) return FJ_E_ARG;
+    *out = b->st;
     return 0;
 }
 

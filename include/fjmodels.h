@@ -198,6 +198,47 @@ int fj_surrogate_params(fj_ctx *ctx, float *get, const float *set);
 int fj_surrogate_forward(fj_ctx *ctx, long long n, const float *x, float *y);
 int fj_surrogate_gradient(fj_ctx *ctx, long long n, const float *x, const float *t, float *grad, double *loss);
 
+/* ------------------------------------------------------------------------
+ * Batched models.  No counterpart in the reference (one model per run of
+ * fJmodels.cpp): many independent single-component models, each with its
+ * own grid, guess potential, DF and Delta table, advanced together.  Every
+ * phase of an SCF iteration is one launch over all models, the iteration is
+ * captured in a HIP graph, and fj_batch_autotune picks the launch shapes.
+ * The arithmetic follows the context's precision (fj_set_precision) and
+ * velocity rule of each model; the Poisson step is the block scan.
+ * ------------------------------------------------------------------------ */
+typedef struct fj_batch fj_batch;
+
+typedef struct {
+    int nr;                /* radii ar[nr] (copied), strictly increasing, nr >= 11 */
+    const double *ar;
+    int ntheta, npoly;     /* cos(theta) Gauss-Legendre nodes; even orders l < 2 npoly */
+    int guess;             /* FJ_GUESS_*, with mass, r0, q (SetModel, models.cpp:16-25) */
+    double mass, r0, q;
+    fj_df_params df;
+    int quad;              /* FJ_QUAD_* */
+} fj_model_spec;
+
+typedef struct {
+    double ms_setup, ms_tune, ms_last_iter;
+    long long nodes, points;
+    int setup_block, integ_blocks, graph;   /* chosen launch shapes; graph: iterations replay a HIP graph */
+} fj_batch_stats;
+
+int fj_batch_create(fj_ctx *ctx, int nmodels, const fj_model_spec *specs, fj_batch **out);
+int fj_batch_destroy(fj_batch *b);
+/* Guess potentials and Delta tables of every model. */
+int fj_batch_setup(fj_batch *b);
+/* Times the candidate launch shapes on one iteration (the models' state is
+ * restored afterwards) and keeps the fastest. */
+int fj_batch_autotune(fj_batch *b);
+/* niter SCF iterations of every model (graph replays after the first). */
+int fj_batch_iterate(fj_batch *b, int niter);
+/* Model m's state: any pointer may be NULL; tables [nr][npoly], Egrid/Dgrid [30]. */
+int fj_batch_get(fj_batch *b, int m, double *rhl, double *sigRl, double *sigpl, double *sigzl, double *vrotl,
+                 double *phil, double *Pr, double *Pr2, double *Egrid, double *Dgrid);
+int fj_batch_get_stats(const fj_batch *b, fj_batch_stats *out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,81 @@
+"""Batched models (fj_batch_*): every phase one launch over all models, the
+iteration replayed from a HIP graph, launch shapes from the autotuner.  Each
+model of the batch must give, bit for bit, what a single-model context gives
+on the same grid (guess potential, Delta table, block-scan Poisson, two SCF
+iterations), for FP64 node phases."""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+NAMES = ["rhl", "sigRl", "sigpl", "sigzl", "vrotl", "phil", "Pr", "Pr2"]
+
+
+def models():
+    import fjmodels_amd as fj
+    return [
+        {"ar": O.log_grid(16, 0.01, 100.0), "ntheta": 6, "npoly": 3, "model": "Hernquist",
+         "df": fj.df_params("Hernquist", dphi_h=0.7, dz_h=0.4)},
+        {"ar": O.log_grid(20, 0.005, 200.0), "ntheta": 4, "npoly": 4, "model": "NFW", "mass": 2.0, "r0": 1.5,
+         "df": fj.df_params("NFW", chi=1.0, rot_form=1, mass=2.0, r0=1.5)},
+        {"ar": O.log_grid(12, 0.05, 50.0), "ntheta": 8, "npoly": 2, "model": "Isochrone",
+         "df": fj.df_params("Isochrone", dphi_g=1.2, dz_g=0.8), "quad": fj.FJ_QUAD_TANHSINH22},
+    ]
+
+
+def single(ctx, m, niter):
+    import fjmodels_amd as fj
+    ctx.set_grid(m["ar"], m["ntheta"], m["npoly"])
+    rhl = ctx.guess_rhl(m["model"], m.get("mass", 1.0), m.get("r0", 1.0), m.get("q", 1.0))
+    tabs = ctx.poisson(rhl, fj.FJ_POISSON_BLOCKSCAN)
+    ctx.set_potential(*tabs)
+    E, D = ctx.tabulate_delta()
+    ctx.set_df(m["df"])
+    ctx.set_quad(m.get("quad", 0))
+    out = None
+    for _ in range(niter):
+        out = ctx.compute_new_phi(fj.FJ_POISSON_BLOCKSCAN)
+        ctx.set_potential(out["phil"], out["Pr"], out["Pr2"])
+    out["Egrid"], out["Dgrid"] = E, D
+    return out
+
+
+def test_batch_matches_single_models(ctx):
+    import fjmodels_amd as fj
+    ms = models()
+    b = fj.Batch(ctx, ms)
+    b.setup()
+    b.iterate(2)
+    st = b.stats()
+    print("batch stats", st)
+    assert st["graph"] == 1
+    got = [b.get(k) for k in range(len(ms))]
+    b.close()
+    for k, m in enumerate(ms):
+        ref = single(ctx, m, 2)
+        for n in NAMES + ["Egrid", "Dgrid"]:
+            assert np.array_equal(got[k][n], ref[n]), (k, n, np.max(np.abs(got[k][n] - ref[n])))
+
+
+def test_batch_autotune_keeps_results(ctx):
+    import fjmodels_amd as fj
+    ms = models()
+    b = fj.Batch(ctx, ms)
+    b.setup()
+    b.autotune()
+    st = b.stats()
+    print("autotune", st)
+    assert st["integ_blocks"] >= len(ms) and st["setup_block"] in (128, 256)
+    b.iterate(2)
+    got = [b.get(k) for k in range(len(ms))]
+    b.close()
+    b2 = fj.Batch(ctx, ms)
+    b2.setup()
+    b2.iterate(2)
+    ref = [b2.get(k) for k in range(len(ms))]
+    b2.close()
+    for k in range(len(ms)):
+        for n in NAMES:
+            assert np.array_equal(got[k][n], ref[n][n] if False else ref[k][n]), (k, n)
