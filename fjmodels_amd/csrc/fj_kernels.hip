@@ -22,7 +22,6 @@
 #include <string.h>
 
 #include <algorithm>
-#include <type_traits>
 #include <vector>
 
 #include "../../include/fjmodels.h"
@@ -267,7 +266,7 @@ __device__ inline void node_velocity(int quad, int node, double Ve, double V[3],
 // k_setup: everything of df()/uvOrb before the two QAGS integrals, for
 // node tid of the launch (every lane of a wave calls it: task appends are
 // wave-aggregated).
-__device__ inline void setup_node(const LdsTabs &L, const SetupArgs &sa, long long tid) {
+__device__ __forceinline__ void setup_node(const LdsTabs &L, const SetupArgs &sa, long long tid) {
     const bool explicit_v = sa.vR != nullptr;
     const long long nnodes = sa.sel ? (long long)*sa.nsel : explicit_v ? (long long)sa.npts : (long long)sa.npts * NQ;
     const bool valid = tid < nnodes;
@@ -687,7 +686,10 @@ struct IntegArgs {
 #define FJ_INTEG_WAVES 3   // waves per SIMD the register allocation of k_integrals aims at
 #endif
 
-template <class FI, class FV>
+// PerTaskTables: the tables of each task are ia.pv[ia.pmodel[node / NQ]]
+// (batched overflow pass); otherwise f.T stays the kernel's LDS view, which
+// keeps the view's fields in registers.
+template <bool PerTaskTables = false, class FI, class FV>
 __device__ __forceinline__ void integrals_loop(const IntegArgs &ia, FI &f, const QagsWork &w, const FV &fv) {
     const bool from_list = ia.ntask_override >= 0 || ia.ntask_dev;
     const unsigned int nu = from_list ? 0u : *ia.cnt_u, nv = from_list ? 0u : *ia.cnt_v;
@@ -722,8 +724,7 @@ __device__ __forceinline__ void integrals_loop(const IntegArgs &ia, FI &f, const
                     }
                     task = 0;
                     load_jint(ia.nb, node, type, f);
-                    if constexpr (std::is_same<FI, JInt>::value)
-                        if (ia.pv) f.T = &ia.pv[ia.pmodel[node / NQ]];
+                    if constexpr (PerTaskTables) f.T = &ia.pv[ia.pmodel[node / NQ]];
                     qags_begin(q, 0, type == 0 ? PI : PIH, EPSABSINTEG, EPSRELINTEG, 1000);
                 } else {
                     task = -2;
@@ -1690,7 +1691,7 @@ __global__ void __launch_bounds__(256) kb_integrals_ovf(IntegArgs ia) {
     const FvStrided fv{smem + threadIdx.x, (int)blockDim.x};
     JInt f;
     f.T = ia.pv;
-    integrals_loop(ia, f, w, fv);
+    integrals_loop<true>(ia, f, w, fv);
 }
 
 __global__ void __launch_bounds__(256) kb_moments(const ModelDev *md, const int *pmodel, MomArgs base, int mixed) {
