@@ -50,13 +50,29 @@ FJ_HD __forceinline__ double fj_min(double a, double b) { return a < b ? a : b; 
 // reference expressions (UtilsLeg.h:43-79), so every step rounds as there:
 //   evenlegend step np:  p = -pm2*el[np]*elm1[np]/ed[np] + pm1*(c2 - eb[np]);  p *= eg[np]
 //   legend step i:       P_i = (l2m1[i]*c*P_{i-1} - lm1[i]*P_{i-2}) / li[i]
+// ied and ili are the reciprocals 1/ed and 1/li, for fj_div_rn.
 struct LegCoef {
-    double el[MAXPOLY], elm1[MAXPOLY], ed[MAXPOLY], eb[MAXPOLY], eg[MAXPOLY];
-    double l2m1[2 * MAXPOLY], lm1[2 * MAXPOLY], li[2 * MAXPOLY];
+    double el[MAXPOLY], elm1[MAXPOLY], ed[MAXPOLY], eb[MAXPOLY], eg[MAXPOLY], ied[MAXPOLY];
+    double l2m1[2 * MAXPOLY], lm1[2 * MAXPOLY], li[2 * MAXPOLY], ili[2 * MAXPOLY];
 };
 
+// a / b rounded to nearest, given y = RN(1/b), with three fused
+// multiply-adds instead of a division.  q0 = RN(a y) is within 1.5 ulp of
+// a/b; one correction q1 = RN(q0 + (a - b q0) y) is within one ulp (its
+// error is 2^-52 of q0's plus the final rounding), so the remainder
+// a - b q1 is exact and Markstein's theorem makes q2 = RN(q1 + (a - b q1) y)
+// the correctly rounded quotient.  For a, b, a/b in the normal range (and
+// a == 0, where q0 carries the IEEE sign); tests/test_device_code_cpu.py
+// checks it against IEEE division.
+FJ_HD __forceinline__ double fj_div_rn(double a, double b, double y) {
+    const double q0 = a * y;
+    const double q1 = fma(fma(-q0, b, a), y, q0);
+    const double q2 = fma(fma(-q1, b, a), y, q1);
+    return a == 0.0 ? q0 : q2;
+}
+
 FJ_HD __forceinline__ double evenleg_step(const LegCoef *lc, int np, double pm2, double pm1, double c2) {
-    double p = -pm2 * lc->el[np] * lc->elm1[np] / lc->ed[np] + pm1 * (c2 - lc->eb[np]);
+    double p = fj_div_rn(-pm2 * lc->el[np] * lc->elm1[np], lc->ed[np], lc->ied[np]) + pm1 * (c2 - lc->eb[np]);
     p *= lc->eg[np];
     return p;
 }
@@ -76,7 +92,33 @@ struct PotView {
     const LegCoef *lc;
     const int *lut;      // [NLUT]
     float lg0, lsc;      // log2(ar[0]), NLUT / (log2(ar[nr-1]) - log2(ar[0]))
+    const double *rk;    // [nr][RK]: divisors of radial_interp per cell (fill_radk_cell)
 };
+
+// Per radial cell bot (bracket [ar[bot], ar[bot+1]]): the divisor of f1 and
+// its reciprocal, then for the quadratic correction with thr = bot-1 and with
+// thr = bot+2: the divisor of f2, its reciprocal, and f3.  Each is formed
+// with the operations of the quotient it serves in radial_interp, so the
+// interpolation weights keep their bits.
+constexpr int RK = 8;
+FJ_HD __forceinline__ void fill_radk_cell(int nr, const double *ar, int bot, double *k) {
+    const int top = bot + 1;
+    for (int i = 0; i < RK; i++) k[i] = 0.0;
+    if (top >= nr) return;
+    const double d = ar[top] - ar[bot];
+    k[0] = d;
+    k[1] = 1.0 / d;
+    if (bot > 0) {
+        const int thr = bot - 1;
+        const double den = (ar[thr] - ar[top]) * (ar[thr] - ar[bot]);
+        k[2] = den; k[3] = 1.0 / den; k[4] = (ar[thr] - ar[bot]) / (ar[top] - ar[bot]);
+    }
+    if (top + 1 < nr) {
+        const int thr = top + 1;
+        const double den = (ar[thr] - ar[top]) * (ar[thr] - ar[bot]);
+        k[5] = den; k[6] = 1.0 / den; k[7] = (ar[thr] - ar[bot]) / (ar[top] - ar[bot]);
+    }
+}
 
 FJ_HD __forceinline__ float fj_log2f(float x) {
 #if defined(__HIP_DEVICE_COMPILE__)
@@ -131,13 +173,16 @@ FJ_HD __forceinline__ Interp radial_interp(const PotView &T, double r) {
     const int top = bot + 1;
     I.bot = bot; I.top = top;
     const double db = r - ar[bot];
-    I.f1 = db / (ar[top] - ar[bot]);
+    const double *k = T.rk + RK * bot;
+    I.f1 = fj_div_rn(db, k[0], k[1]);                      // db / (ar[top] - ar[bot])
     if (top < 10) {
         I.quad = true;
-        I.thr = (I.f1 < 0.5 && bot > 0) ? bot - 1 : top + 1;
+        const bool below = I.f1 < 0.5 && bot > 0;
+        I.thr = below ? bot - 1 : top + 1;
         const double dt = r - ar[top];
-        I.f2 = dt * db / ((ar[I.thr] - ar[top]) * (ar[I.thr] - ar[bot]));
-        I.f3 = (ar[I.thr] - ar[bot]) / (ar[top] - ar[bot]);
+        const double *kq = k + (below ? 2 : 5);
+        I.f2 = fj_div_rn(dt * db, kq[0], kq[1]);           // dt db / ((ar[thr]-ar[top]) (ar[thr]-ar[bot]))
+        I.f3 = kq[2];                                      // (ar[thr]-ar[bot]) / (ar[top]-ar[bot])
     }
     return I;
 }
@@ -186,49 +231,52 @@ FJ_HD __forceinline__ double pot_phi(const PotView &T, double R, double z) {
 // dr (Potential.cpp:206-244, 261-276) with dlegend (UtilsLeg.h:51-57) and
 // the three-table intpo2 (UtilsLeg.h:127-158).  One radial search serves
 // all three; every value rounds as in the separate reference routines.
-FJ_HD __forceinline__ double pot_phi_grad(const PotView &T, double R, double z, double *dR, double *dz) {
+// With grad false only Phi is formed (the same bits as pot_phi).
+FJ_HD __forceinline__ double pot_eval(const PotView &T, double R, double z, bool grad, double *dR, double *dz) {
     const double cphi = z / sqrt(R * R + z * z);
     double r = sqrt(z * z + R * R);
     if (r < T.ar[0]) r = T.ar[0];
-    const double s = R / r;
-    double c = z / r;
-    if (c == 1) c -= 1e-8;
     const Interp I = radial_interp(T, r);
     const int np_ = T.npoly;
-    // dtheta: sum_j phip[j] dlegend(c, 2j), times -sqrt(1-c^2)
-    double dth = 0, Pm2 = 0, Pm1 = 1.0;   // P_{i-2}, P_{i-1}
-    for (int j = 0; j < np_; j++) {
-        double dpl = 0;
-        if (j > 0) {
-            const int n = 2 * j;
-            double P1 = Pm1, P0 = Pm2;
-            for (int i = n - 1; i <= n; i++) {
-                double Pi;
-                if (i == 1) Pi = c;
-                else Pi = (T.lc->l2m1[i] * c * P1 - T.lc->lm1[i] * P0) / T.lc->li[i];
-                P0 = P1; P1 = Pi;
+    if (grad) {
+        const double s = R / r;
+        double c = z / r;
+        if (c == 1) c -= 1e-8;
+        // dtheta: sum_j phip[j] dlegend(c, 2j), times -sqrt(1-c^2)
+        double dth = 0, Pm2 = 0, Pm1 = 1.0;   // P_{i-2}, P_{i-1}
+        for (int j = 0; j < np_; j++) {
+            double dpl = 0;
+            if (j > 0) {
+                const int n = 2 * j;
+                double P1 = Pm1, P0 = Pm2;
+                for (int i = n - 1; i <= n; i++) {
+                    double Pi;
+                    if (i == 1) Pi = c;
+                    else Pi = fj_div_rn(T.lc->l2m1[i] * c * P1 - T.lc->lm1[i] * P0, T.lc->li[i], T.lc->ili[i]);
+                    P0 = P1; P1 = Pi;
+                }
+                Pm2 = P0; Pm1 = P1;   // P_{n-1}, P_n
+                dpl = (T.lc->li[n] * Pm2 - T.lc->li[n] * c * Pm1) / (1 - c * c);
             }
-            Pm2 = P0; Pm1 = P1;   // P_{n-1}, P_n
-            dpl = (T.lc->li[n] * Pm2 - T.lc->li[n] * c * Pm1) / (1 - c * c);
+            dth += coef_at(T, I, T.phil, j) * dpl;
         }
-        dth += coef_at(T, I, T.phil, j) * dpl;
+        dth *= -sqrt(1 - c * c);
+        // dr: sum dphip[np] P_2np(cos(acos(c)))
+        const double ct = fjm_cos(fjm_acos(c)), ct2 = ct * ct;
+        double g1 = 0, pm2 = 1.0, pm1 = 1.5 * ct2 - .5;
+        for (int np = 0; np < np_; np++) {
+            double p;
+            if (np == 0) p = 1.0;
+            else if (np == 1) p = pm1;
+            else { p = evenleg_step(T.lc, np, pm2, pm1, ct2); pm2 = pm1; pm1 = p; }
+            double dph;
+            if (I.kep) dph = -(2 * np + 1) * T.phil[(T.nr - 1) * np_ + np] * fjm_powi(I.q, 2 * np + 1) / r;
+            else dph = interp_coef(I, T.Pr, np_, np);
+            g1 += dph * p;
+        }
+        *dR = dth * (c / r) + g1 * s;
+        *dz = dth * (-s / r) + g1 * c;
     }
-    dth *= -sqrt(1 - c * c);
-    // dr: sum dphip[np] P_2np(cos(acos(c)))
-    const double ct = fjm_cos(fjm_acos(c)), ct2 = ct * ct;
-    double g1 = 0, pm2 = 1.0, pm1 = 1.5 * ct2 - .5;
-    for (int np = 0; np < np_; np++) {
-        double p;
-        if (np == 0) p = 1.0;
-        else if (np == 1) p = pm1;
-        else { p = evenleg_step(T.lc, np, pm2, pm1, ct2); pm2 = pm1; pm1 = p; }
-        double dph;
-        if (I.kep) dph = -(2 * np + 1) * T.phil[(T.nr - 1) * np_ + np] * fjm_powi(I.q, 2 * np + 1) / r;
-        else dph = interp_coef(I, T.Pr, np_, np);
-        g1 += dph * p;
-    }
-    *dR = dth * (c / r) + g1 * s;
-    *dz = dth * (-s / r) + g1 * c;
     // Phi at (R, z): c from the unclamped radius
     const double cc2 = cphi * cphi;
     double phi = coef_at(T, I, T.phil, 0), a2 = 1.0, a1 = 1.5 * cc2 - .5;
@@ -239,6 +287,10 @@ FJ_HD __forceinline__ double pot_phi_grad(const PotView &T, double R, double z, 
         phi += coef_at(T, I, T.phil, np) * p;
     }
     return phi;
+}
+
+FJ_HD __forceinline__ double pot_phi_grad(const PotView &T, double R, double z, double *dR, double *dz) {
+    return pot_eval(T, R, z, true, dR, dz);
 }
 
 // ---------------------------------------------------------------------------
@@ -842,11 +894,13 @@ inline LegCoef make_legcoef(int npoly) {
         lc.ed[np] = (double)((l2 + 1) * (l2 - 1));
         lc.eb[np] = (l2 * l + l2 - 1) / (double)((l2 - 1) * (l2 + 3));
         lc.eg[np] = (l2 + 1) * (l2 + 3) / (double)((l + 1) * (l + 2));
+        lc.ied[np] = 1.0 / lc.ed[np];
     }
     for (int i = 0; i < 2 * MAXPOLY; i++) {
         lc.l2m1[i] = (double)(2 * i - 1);
         lc.lm1[i] = (double)(i - 1);
         lc.li[i] = (double)i;
+        lc.ili[i] = i > 0 ? 1.0 / (double)i : 0.0;
     }
     return lc;
 }

@@ -76,6 +76,7 @@ struct TabArgs {
     const double *ar, *phil, *Pr, *E, *D;
     const int *lut;
     float lg0, lsc;
+    const double *rk;   // [nr][RK] radial cell divisors (fill_radk_cell)
 };
 
 // Legendre recurrence constants (index-only values, the same for every grid)
@@ -83,7 +84,7 @@ struct TabArgs {
 __constant__ LegCoef c_legcoef;
 
 __host__ __device__ inline size_t tab_lds_doubles(int nr, int npoly, int ngrid) {
-    return (size_t)nr + 2 * (size_t)nr * npoly + 2 * (size_t)ngrid + NLUT / 2;
+    return (size_t)nr + 2 * (size_t)nr * npoly + 2 * (size_t)ngrid + (size_t)RK * nr + NLUT / 2;
 }
 
 struct LdsTabs {
@@ -94,17 +95,18 @@ struct LdsTabs {
 
 __device__ inline LdsTabs stage_tables(const TabArgs &a, double *smem) {
     const int nr = a.nr, np = a.npoly, ng = a.ngrid;
-    double *ar = smem, *phil = ar + nr, *Pr = phil + nr * np, *E = Pr + nr * np, *D = E + ng;
-    int *lut = reinterpret_cast<int *>(D + ng);
+    double *ar = smem, *phil = ar + nr, *Pr = phil + nr * np, *E = Pr + nr * np, *D = E + ng, *rk = D + ng;
+    int *lut = reinterpret_cast<int *>(rk + RK * nr);
     for (int i = threadIdx.x; i < nr; i += blockDim.x) ar[i] = a.ar[i];
     for (int i = threadIdx.x; i < nr * np; i += blockDim.x) { phil[i] = a.phil[i]; Pr[i] = a.Pr[i]; }
     for (int i = threadIdx.x; i < ng; i += blockDim.x) { E[i] = a.E[i]; D[i] = a.D[i]; }
+    for (int i = threadIdx.x; i < RK * nr; i += blockDim.x) rk[i] = a.rk[i];
     for (int i = threadIdx.x; i < NLUT; i += blockDim.x) lut[i] = a.lut[i];
     __syncthreads();
     LdsTabs t;
     t.T.nr = nr; t.T.npoly = np; t.T.ar = ar; t.T.phil = phil; t.T.Pr = Pr;
     t.T.lc = &c_legcoef;
-    t.T.lut = lut; t.T.lg0 = a.lg0; t.T.lsc = a.lsc;
+    t.T.lut = lut; t.T.lg0 = a.lg0; t.T.lsc = a.lsc; t.T.rk = rk;
     t.E = E; t.D = D; t.ngrid = ng;
     return t;
 }
@@ -304,7 +306,11 @@ __device__ __forceinline__ void setup_node(const LdsTabs &L, const SetupArgs &sa
     if (valid && nev) atomicAdd(&sa.ctr->grad, (unsigned long long)nev);
 }
 
-__global__ void __launch_bounds__(256) k_setup(TabArgs ta, SetupArgs sa) {
+#ifndef FJ_SETUP_WAVES
+#define FJ_SETUP_WAVES 4   // waves per SIMD the register allocation of k_setup aims at
+#endif
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(FJ_SETUP_WAVES)))
+k_setup(TabArgs ta, SetupArgs sa) {
     extern __shared__ double smem[];
     const LdsTabs L = stage_tables(ta, smem);
     setup_node(L, sa, (long long)blockIdx.x * blockDim.x + threadIdx.x);
@@ -1552,7 +1558,7 @@ __device__ inline int find_seg(const int *start, int M, int x) {   // largest m 
 __device__ inline PotView global_potview(const TabArgs &ta) {
     PotView T;
     T.nr = ta.nr; T.npoly = ta.npoly; T.ar = ta.ar; T.phil = ta.phil; T.Pr = ta.Pr; T.lc = &c_legcoef;
-    T.lut = ta.lut; T.lg0 = ta.lg0; T.lsc = ta.lsc;
+    T.lut = ta.lut; T.lg0 = ta.lg0; T.lsc = ta.lsc; T.rk = ta.rk;
     return T;
 }
 
@@ -1812,6 +1818,7 @@ struct fj_ctx {
     double *d_ci = nullptr, *d_si = nullptr, *d_poly = nullptr;
     int *d_lut = nullptr;
     float lut_lg0 = 0, lut_lsc = 0;
+    double *d_rk = nullptr;   // [nr][RK] radial cell divisors
     // node phase scratch
     long long node_cap = 0;
     double *d_nodef = nullptr, *d_Jr = nullptr, *d_Jz = nullptr;
@@ -1926,7 +1933,7 @@ TabArgs tab_args(const fj_ctx *c) {
     TabArgs t;
     t.nr = c->nr; t.npoly = c->npoly; t.ngrid = c->ngrid;
     t.ar = c->d_ar; t.phil = c->d_phil; t.Pr = c->d_Pr; t.E = c->d_E; t.D = c->d_D;
-    t.lut = c->d_lut; t.lg0 = c->lut_lg0; t.lsc = c->lut_lsc;
+    t.lut = c->d_lut; t.lg0 = c->lut_lg0; t.lsc = c->lut_lsc; t.rk = c->d_rk;
     return t;
 }
 size_t tab_bytes(const fj_ctx *c) { return tab_lds_doubles(c->nr, c->npoly, c->ngrid) * sizeof(double); }
@@ -2362,7 +2369,7 @@ int fj_ctx_destroy(fj_ctx *c) {
     if (!c) return 0;
     (void)hipSetDevice(c->device);
     void *ptrs[] = {c->d_ar, c->d_phil, c->d_Pr, c->d_Pr2, c->d_E, c->d_D, c->d_ci, c->d_si, c->d_poly, c->d_lut,
-                    c->d_nodef, c->d_Jr, c->d_Jz, c->d_flag, c->d_list_u, c->d_list_v, c->d_ovf, c->d_cnt,
+                    c->d_rk, c->d_nodef, c->d_Jr, c->d_Jz, c->d_flag, c->d_list_u, c->d_list_v, c->d_ovf, c->d_cnt,
                     c->d_ctr, c->d_ws, c->d_wsfull, c->d_pR, c->d_pz, c->d_pPhi, c->d_pVe, c->d_out6,
                     c->d_mom, c->d_rhl, c->d_gl, c->d_pw, c->d_pws, c->d_status};
     for (void *p : ptrs)
@@ -2409,7 +2416,8 @@ int fj_set_grid(fj_ctx *c, int nr, const double *ar, int ntheta, int npoly, int 
     if ((r = dalloc(c, &c->d_ar, nr)) || (r = dalloc(c, &c->d_phil, nl)) || (r = dalloc(c, &c->d_Pr, nl)) ||
         (r = dalloc(c, &c->d_Pr2, nl)) || (r = dalloc(c, &c->d_E, ngrid)) || (r = dalloc(c, &c->d_D, ngrid)) ||
         (r = dalloc(c, &c->d_ci, ntheta)) || (r = dalloc(c, &c->d_si, ntheta)) ||
-        (r = dalloc(c, &c->d_poly, (size_t)npoly * ntheta)) || (r = dalloc(c, &c->d_lut, NLUT)))
+        (r = dalloc(c, &c->d_poly, (size_t)npoly * ntheta)) || (r = dalloc(c, &c->d_lut, NLUT)) ||
+        (r = dalloc(c, &c->d_rk, (size_t)RK * nr)))
         return r;
     // grid-size dependent scratch is re-made on demand
     (void)dalloc(c, &c->d_rhl, 0); (void)dalloc(c, &c->d_gl, 0); (void)dalloc(c, &c->d_pws, 0);
@@ -2421,6 +2429,9 @@ int fj_set_grid(fj_ctx *c, int nr, const double *ar, int ntheta, int npoly, int 
     int lut[NLUT];
     make_lut(nr, ar, lut, &c->lut_lg0, &c->lut_lsc);
     HIPCHK(c, hipMemcpy(c->d_lut, lut, sizeof lut, hipMemcpyHostToDevice));
+    std::vector<double> rk((size_t)RK * nr);
+    for (int b = 0; b < nr; b++) fill_radk_cell(nr, ar, b, &rk[(size_t)RK * b]);
+    HIPCHK(c, hipMemcpy(c->d_rk, rk.data(), rk.size() * sizeof(double), hipMemcpyHostToDevice));
     HIPCHK(c, hipMemcpy(c->d_ar, ar, nr * sizeof(double), hipMemcpyHostToDevice));
     HIPCHK(c, hipMemcpy(c->d_ci, c->ci.data(), ntheta * sizeof(double), hipMemcpyHostToDevice));
     HIPCHK(c, hipMemcpy(c->d_si, c->si.data(), ntheta * sizeof(double), hipMemcpyHostToDevice));
@@ -3088,12 +3099,13 @@ int fj_batch_create(fj_ctx *c, int M, const fj_model_spec *spec, fj_batch **out)
         if (r) break;
         ModelDev d{};
         const int nr = sp.nr, np_ = sp.npoly, nt = sp.ntheta, nl = nr * np_;
-        double *ar, *phil, *Pr, *Pr2, *E, *D, *ci, *si, *poly, *co, *prep;
+        double *ar, *phil, *Pr, *Pr2, *E, *D, *ci, *si, *poly, *co, *prep, *rk;
         int *lut;
         if ((r = balloc(b, &ar, nr)) || (r = balloc(b, &phil, nl)) || (r = balloc(b, &Pr, nl)) ||
             (r = balloc(b, &Pr2, nl)) || (r = balloc(b, &E, NG)) || (r = balloc(b, &D, NG)) ||
             (r = balloc(b, &ci, nt)) || (r = balloc(b, &si, nt)) || (r = balloc(b, &poly, (size_t)np_ * nt)) ||
-            (r = balloc(b, &co, (size_t)6 * nl)) || (r = balloc(b, &prep, 4)) || (r = balloc(b, &lut, NLUT)))
+            (r = balloc(b, &co, (size_t)6 * nl)) || (r = balloc(b, &prep, 4)) || (r = balloc(b, &lut, NLUT)) ||
+            (r = balloc(b, &rk, (size_t)RK * nr)))
             break;
         std::vector<double> hc(nt), hw(nt), hs(nt), hp((size_t)np_ * nt), pol(np_);
         host_gauleg(nt, 0, 1, hc.data(), hw.data());
@@ -3105,13 +3117,16 @@ int fj_batch_create(fj_ctx *c, int M, const fj_model_spec *spec, fj_batch **out)
         int hl[NLUT];
         float lg0, lsc;
         make_lut(nr, sp.ar, hl, &lg0, &lsc);
+        std::vector<double> hrk((size_t)RK * nr);
+        for (int k = 0; k < nr; k++) fill_radk_cell(nr, sp.ar, k, &hrk[(size_t)RK * k]);
+        HIPCHK(c, hipMemcpy(rk, hrk.data(), hrk.size() * sizeof(double), hipMemcpyHostToDevice));
         HIPCHK(c, hipMemcpy(ar, sp.ar, nr * sizeof(double), hipMemcpyHostToDevice));
         HIPCHK(c, hipMemcpy(ci, hc.data(), nt * sizeof(double), hipMemcpyHostToDevice));
         HIPCHK(c, hipMemcpy(si, hs.data(), nt * sizeof(double), hipMemcpyHostToDevice));
         HIPCHK(c, hipMemcpy(poly, hp.data(), hp.size() * sizeof(double), hipMemcpyHostToDevice));
         HIPCHK(c, hipMemcpy(lut, hl, sizeof hl, hipMemcpyHostToDevice));
         d.ta.nr = nr; d.ta.npoly = np_; d.ta.ngrid = NG; d.ta.ar = ar; d.ta.phil = phil; d.ta.Pr = Pr;
-        d.ta.E = E; d.ta.D = D; d.ta.lut = lut; d.ta.lg0 = lg0; d.ta.lsc = lsc;
+        d.ta.E = E; d.ta.D = D; d.ta.lut = lut; d.ta.lg0 = lg0; d.ta.lsc = lsc; d.ta.rk = rk;
         d.Pr2 = Pr2; d.df = sp.df; d.quad = sp.quad;
         d.po = b->npts; d.np = nr * nt; d.nr = nr; d.ntheta = nt; d.npoly = np_;
         d.guess = sp.guess; d.mass = sp.mass; d.r0 = sp.r0; d.q = sp.q;

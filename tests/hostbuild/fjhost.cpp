@@ -42,6 +42,21 @@ int fjh_math(int which, int n, const double *x, double *y) {
     return 0;
 }
 
+// Potential view over host arrays, with the bracket table and the radial
+// cell divisors filled as the library fills them.
+struct HostPot {
+    LegCoef lc;
+    int lut[NLUT];
+    std::vector<double> rk;
+    PotView T;
+    HostPot(int nr, const double *ar, int npoly, const double *phil, const double *Pr, int nlc)
+        : lc(make_legcoef(nlc)), rk((size_t)RK * nr) {
+        for (int b = 0; b < nr; b++) fill_radk_cell(nr, ar, b, &rk[(size_t)RK * b]);
+        T = PotView{nr, npoly, ar, phil, Pr, &lc, lut, 0.f, 0.f, rk.data()};
+        make_lut(nr, ar, lut, &T.lg0, &T.lsc);
+    }
+};
+
 int fjh_qags(double (*f)(double, void *), void *p, double a, double b, double epsabs, double epsrel, int limit,
              int cap, int rule15, double *result, int *ier, int *nqk) {
     // lane 2 of 3 lanes sharing an interleaved workspace
@@ -55,10 +70,8 @@ int fjh_qags(double (*f)(double, void *), void *p, double a, double b, double ep
 
 // Radial bracket of radial_interp (bracket table + walk) at n radii.
 int fjh_bracket(int nr, const double *ar, int n, const double *r, int *bot) {
-    LegCoef lc = make_legcoef(1);
-    int lut[NLUT];
-    PotView T{nr, 1, ar, ar, ar, &lc, lut, 0.f, 0.f};
-    make_lut(nr, ar, lut, &T.lg0, &T.lsc);
+    HostPot H(nr, ar, 1, ar, ar, 1);
+    const PotView &T = H.T;
     for (int i = 0; i < n; i++) {
         const Interp I = radial_interp(T, r[i]);
         bot[i] = I.kep ? -1 : I.bot;
@@ -68,18 +81,14 @@ int fjh_bracket(int nr, const double *ar, int n, const double *r, int *bot) {
 
 double fjh_phi_grad(int nr, const double *ar, int npoly, const double *phil, const double *Pr, double R, double z,
                     double *dR, double *dz) {
-    LegCoef lc = make_legcoef(npoly);
-    int lut[NLUT];
-    PotView T{nr, npoly, ar, phil, Pr, &lc, lut, 0.f, 0.f};
-    make_lut(nr, ar, lut, &T.lg0, &T.lsc);
+    HostPot H(nr, ar, npoly, phil, Pr, npoly);
+    const PotView &T = H.T;
     return pot_phi_grad(T, R, z, dR, dz);
 }
 
 double fjh_phi(int nr, const double *ar, int npoly, const double *phil, const double *Pr, double R, double z) {
-    LegCoef lc = make_legcoef(npoly);
-    int lut[NLUT];
-    PotView T{nr, npoly, ar, phil, Pr, &lc, lut, 0.f, 0.f};
-    make_lut(nr, ar, lut, &T.lg0, &T.lsc);
+    HostPot H(nr, ar, npoly, phil, Pr, npoly);
+    const PotView &T = H.T;
     return pot_phi(T, R, z);
 }
 
@@ -87,10 +96,8 @@ double fjh_phi(int nr, const double *ar, int npoly, const double *phil, const do
 int fjh_nodes(int nr, const double *ar, int npoly, const double *phil, const double *Pr, int ng, const double *Eg,
               const double *Dg, int n, const double *R, const double *z, const double *Phi, const double *vR,
               const double *vphi, const double *vz, const double *Dx, double *out) {
-    LegCoef lc = make_legcoef(npoly);
-    int lut[NLUT];
-    PotView T{nr, npoly, ar, phil, Pr, &lc, lut, 0.f, 0.f};
-    make_lut(nr, ar, lut, &T.lg0, &T.lsc);
+    HostPot H(nr, ar, npoly, phil, Pr, npoly);
+    const PotView &T = H.T;
     for (int i = 0; i < n; i++) {
         NodeConst c{};
         double ubar, Deltau, vturn, jr, jz;
@@ -110,10 +117,8 @@ int fjh_nodes(int nr, const double *ar, int npoly, const double *phil, const dou
 int fjh_integral(int nr, const double *ar, int npoly, const double *phil, const double *Pr, int ng, const double *Eg,
                  const double *Dg, double R, double z, double Phi, const double *V, double Dx, int type, int npsi,
                  const double *psi, double *out) {
-    LegCoef lc = make_legcoef(npoly);
-    int lut[NLUT];
-    PotView T{nr, npoly, ar, phil, Pr, &lc, lut, 0.f, 0.f};
-    make_lut(nr, ar, lut, &T.lg0, &T.lsc);
+    HostPot H(nr, ar, npoly, phil, Pr, npoly);
+    const PotView &T = H.T;
     NodeConst c{};
     double ubar, Deltau, vturn, jr, jz;
     int wu, wv, bit = 0;
@@ -138,10 +143,8 @@ int fjh_rhofDF(int nr, const double *ar, int npoly, const double *phil, const do
                const double *Dg, const double *xa, const double *wa, const double *xp, const double *wp, double R,
                double z, double A, double B, double dphi_h, double dz_h, double dphi_g, double dz_g, double mass,
                double J0, double chi, double k, int rot_form, double *out6) {
-    LegCoef lc = make_legcoef(npoly);
-    int lut[NLUT];
-    PotView T{nr, npoly, ar, phil, Pr, &lc, lut, 0.f, 0.f};
-    make_lut(nr, ar, lut, &T.lg0, &T.lsc);
+    HostPot H(nr, ar, npoly, phil, Pr, npoly);
+    const PotView &T = H.T;
     std::vector<double> ws(ws_doubles(1000, 1));
     QagsWork w = lane_work(ws.data(), 1000, 0, 1);
     const double far = 10 * ar[nr - 1];
@@ -194,10 +197,8 @@ int fjh_actions(int nr, const double *ar, int npoly, const double *phil, const d
                 const double *Eg, const double *Dg, int n, const double *R, const double *z, const double *Phi,
                 const double *vR, const double *vphi, const double *vz, const double *Dx, int cap, double *Jr,
                 double *Jz, int *nqk) {
-    LegCoef lc = make_legcoef(npoly);
-    int lut[NLUT];
-    PotView T{nr, npoly, ar, phil, Pr, &lc, lut, 0.f, 0.f};
-    make_lut(nr, ar, lut, &T.lg0, &T.lsc);
+    HostPot H(nr, ar, npoly, phil, Pr, npoly);
+    const PotView &T = H.T;
     std::vector<double> ws(ws_doubles(cap, 1));
     QagsWork w = lane_work(ws.data(), cap, 0, 1);
     for (int i = 0; i < n; i++) {
@@ -225,4 +226,46 @@ int fjh_actions(int nr, const double *ar, int npoly, const double *phil, const d
     }
     return 0;
 }
+
+// fj_div_rn against IEEE division: n random quotients per divisor class.
+// mode 0: a, b with random significands and exponents in [-60, 60];
+// mode 1: b from the Legendre constants ed/li and radial-cell divisors of a
+// log grid, a random.  Returns the number of results that differ in any bit.
+long long fjh_divrn_check(long long n, unsigned long long seed, int mode) {
+    unsigned long long x = seed | 1;
+    auto next = [&x]() { x ^= x << 13; x ^= x >> 7; x ^= x << 17; return x; };
+    auto rnd = [&](int emin, int emax) {
+        const unsigned long long m = next() & 0x000fffffffffffffull;
+        const int e = emin + (int)(next() % (unsigned long long)(emax - emin + 1));
+        uint64_t u = ((uint64_t)(1023 + e) << 52) | m;
+        if (next() & 1) u |= 0x8000000000000000ull;
+        double d; memcpy(&d, &u, 8); return d;
+    };
+    std::vector<double> divs;
+    if (mode == 1) {
+        const LegCoef lc = make_legcoef(MAXPOLY);
+        for (int np = 2; np < MAXPOLY; np++) divs.push_back(lc.ed[np]);
+        for (int i = 1; i < 2 * MAXPOLY; i++) divs.push_back(lc.li[i]);
+        const int nr = 200;
+        std::vector<double> ar(nr), rk((size_t)RK * nr);
+        for (int i = 0; i < nr; i++) ar[i] = 1e-4 * pow(1e6, i / (double)(nr - 1));
+        for (int b = 0; b < nr; b++) fill_radk_cell(nr, ar.data(), b, &rk[(size_t)RK * b]);
+        for (int b = 0; b < nr - 1; b++) { divs.push_back(rk[RK * b]); if (b < 12) { if (b) divs.push_back(rk[RK * b + 2]); divs.push_back(rk[RK * b + 5]); } }
+    }
+    long long bad = 0;
+    for (long long i = 0; i < n; i++) {
+        const double b = mode == 0 ? rnd(-60, 60) : divs[i % divs.size()];
+        const double a = rnd(-60, 60);
+        const double y = 1.0 / b;
+        const double q = fj_div_rn(a, b, y), r = a / b;
+        if (memcmp(&q, &r, 8) != 0) bad++;
+    }
+    // zeros keep the IEEE sign
+    for (double a : {0.0, -0.0})
+        for (double b : {3.0, -3.0}) {
+            const double q = fj_div_rn(a, b, 1.0 / b), r = a / b;
+            if (memcmp(&q, &r, 8) != 0) bad++;
+        }
+    return bad;
 }
+}  // extern "C"

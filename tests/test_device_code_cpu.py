@@ -45,6 +45,8 @@ def H():
     lib.fjh_rhofDF.argtypes = [C.c_int, dp, C.c_int, dp, dp, C.c_int, dp, dp, dp, dp, dp, dp, C.c_double,
                                C.c_double] + [C.c_double] * 9 + [C.c_double, C.c_int, dp]
     lib.fjh_bracket.argtypes = [C.c_int, dp, C.c_int, dp, C.POINTER(C.c_int)]
+    lib.fjh_divrn_check.restype = C.c_longlong
+    lib.fjh_divrn_check.argtypes = [C.c_longlong, C.c_ulonglong, C.c_int]
     lib.fjh_qags.argtypes = [O.FN, C.c_void_p, C.c_double, C.c_double, C.c_double, C.c_double, C.c_int, C.c_int,
                              C.c_int, dp, C.POINTER(C.c_int), C.POINTER(C.c_int)]
     return lib
@@ -176,3 +178,13 @@ def test_radial_bracket_matches_bisection(H, kind):
                 top = m
         ref[i] = b
     assert np.array_equal(bot, ref), np.where(bot != ref)[0][:5]
+
+
+@pytest.mark.parametrize("mode", [0, 1], ids=["random", "path_divisors"])
+def test_reciprocal_division_is_ieee_division(H, mode):
+    """fj_div_rn(a, b, RN(1/b)) (three fused multiply-adds) gives the bits of
+    IEEE a/b: on random operands over 2^-60..2^60 and on every divisor the
+    potential evaluation divides by (the evenlegend and legend constants, the
+    radial cell widths and quadratic-correction products of a 200-radius log
+    grid); zeros keep the quotient's sign."""
+    assert H.fjh_divrn_check(4_000_000, 12345 + mode, mode) == 0

@@ -41,10 +41,14 @@ rng = np.random.default_rng(7)
 
 CASES = [
     (0, "exp", lambda: np.concatenate([rng.uniform(-700, 700, 400), rng.uniform(-1, 1, 400)]), mpmath.exp, 2),
-    (1, "expm1", lambda: rng.uniform(-0.5, 0.5, 500), mpmath.expm1, 2),
-    (2, "sinh", lambda: np.concatenate([rng.uniform(-1, 1, 300), rng.uniform(-60, 60, 300), 10 ** rng.uniform(-8, 0, 100)]),
+    (1, "expm1", lambda: np.concatenate([rng.uniform(-0.5, 0.5, 500), rng.uniform(-40, 40, 300),
+                                         10 ** rng.uniform(-12, 0, 100), -10 ** rng.uniform(-12, 0, 100),
+                                         rng.uniform(690, 709, 50)]), mpmath.expm1, 2),
+    (2, "sinh", lambda: np.concatenate([rng.uniform(-1, 1, 300), rng.uniform(-60, 60, 300), 10 ** rng.uniform(-8, 0, 100),
+                                        rng.uniform(700, 710, 50), -rng.uniform(700, 710, 50)]),
      mpmath.sinh, 3),
-    (3, "cosh", lambda: rng.uniform(-60, 60, 500), mpmath.cosh, 3),
+    (3, "cosh", lambda: np.concatenate([rng.uniform(-60, 60, 500), rng.uniform(-1, 1, 200), rng.uniform(700, 710, 50)]),
+     mpmath.cosh, 3),
     (4, "sin", lambda: np.concatenate([rng.uniform(-10, 10, 500), 10 ** rng.uniform(-8, 0, 100)]), mpmath.sin, 2),
     (5, "cos", lambda: rng.uniform(-10, 10, 500), mpmath.cos, 2),
     (6, "acos", lambda: np.concatenate([rng.uniform(-1, 1, 600), 1 - 10 ** rng.uniform(-12, -1, 100)]), mpmath.acos, 3),
