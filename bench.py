@@ -26,13 +26,25 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 PEAK_FP64 = 78.6   # TFLOP/s, MI355X
+# HBM bytes per k_integrals launch of this workload (timed launches of the default
+# schedule): rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE runs, scaled by what the
+# counters read for a 1 GiB device copy (fetch x 2.0, write x 1.0); the runs and
+# their commands are in profiles/k_integrals_hernquist_40x20_fp64.md.
+K_INTEGRALS_TRAFFIC_BYTES = 180.1e9
 
 
 def f_qk21(npoly):
     return 21 * (11 * npoly + 25) + 190
 
 
-def cpu_baseline(npoly, ntheta, nr, ar, tables, delta, sample_every=25, threads=None):
+def line_tasks_bytes(stats):
+    """Algorithmic bytes of k_integrals: per action integral the 12 node
+    constants read and the action written (doubles)."""
+    import numpy as np
+    return float(np.mean([s["tasks"] for s in stats])) * 13 * 8
+
+
+def cpu_baseline(npoly, ntheta, nr, ar, tables, delta, sample_every=8, threads=None):
     import numpy as np
     from oracle import oracle as O
     threads = threads or min(16, os.cpu_count() or 1)
@@ -145,6 +157,70 @@ def surrogate_leg(ctx, model, guess_tables, n_iter=5, keep=0.5, epochs=10, judge
                     "served by the network with %.0f%% judged; final iteration Staeckel" % (keep, epochs, 100 * judge)}
 
 
+def batch_leg(ctx, n_models=8, iters=2, seed=5):
+    """Batched sweep (a bounded sample of the 256-model sweep): n_models
+    single-component models of random type (Hernquist, Isochrone, NFW), DF
+    parameters (dphi, dz in [0.3, 1.2], chi in [0, 2], README rotation form)
+    and uneven grids (nr in 24..40, ntheta in 12..20, l_max 8..12), mixed
+    precision node phase and block-scan Poisson.  Every phase of an iteration
+    is one launch over all models, replayed from a HIP graph, with launch
+    shapes from the autotuner.  Compared with the same models advanced one
+    after another through a single-model context (same precision and
+    Poisson mode).  Times cover the SCF iterations only (setup, Delta tables
+    and tuning are reported apart)."""
+    import numpy as np
+    import torch
+    import fjmodels_amd as fj
+    rng = np.random.default_rng(seed)
+    models = []
+    for k in range(n_models):
+        kind = ("Hernquist", "Isochrone", "NFW")[k % 3]
+        nr, nt, npoly = int(rng.integers(24, 41)), int(rng.integers(12, 21)), int(rng.integers(5, 8))
+        df = fj.df_params(kind, dphi_h=float(rng.uniform(0.3, 1.2)), dz_h=float(rng.uniform(0.3, 1.2)),
+                          chi=float(rng.uniform(0, 2)), rot_form=1)
+        models.append({"ar": fj.log_grid(nr, 0.01, 100.0), "ntheta": nt, "npoly": npoly, "model": kind, "df": df})
+    pts = sum(len(m["ar"]) * m["ntheta"] for m in models)
+    ctx.set_precision(fj.FJ_PREC_MIXED)
+    t0 = time.perf_counter()
+    b = fj.Batch(ctx, models)
+    b.setup()
+    t_setup = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    b.autotune()
+    t_tune = time.perf_counter() - t0
+    b.iterate(1)                       # graph capture + first replay
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    b.iterate(iters)
+    torch.cuda.synchronize()
+    t_b = (time.perf_counter() - t0) / iters
+    st = b.stats()
+    b.close()
+    t_seq = 0.0
+    for m in models:
+        ctx.set_grid(m["ar"], m["ntheta"], m["npoly"])
+        rhl = ctx.guess_rhl(m["model"], 1.0, 1.0, 1.0)
+        ctx.set_potential(*ctx.poisson(rhl, fj.FJ_POISSON_BLOCKSCAN))
+        ctx.tabulate_delta()
+        ctx.set_df(m["df"])
+        out = ctx.compute_new_phi(fj.FJ_POISSON_BLOCKSCAN)
+        ctx.set_potential(out["phil"], out["Pr"], out["Pr2"])
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(iters):
+            out = ctx.compute_new_phi(fj.FJ_POISSON_BLOCKSCAN)
+            ctx.set_potential(out["phil"], out["Pr"], out["Pr2"])
+        torch.cuda.synchronize()
+        t_seq += (time.perf_counter() - t0) / iters
+    ctx.set_precision(fj.FJ_PREC_FP64)
+    return {"models": n_models, "grid_points": pts, "ms_per_iter_batched": 1e3 * t_b,
+            "ms_per_iter_one_by_one": 1e3 * t_seq, "speedup": t_seq / t_b,
+            "value_batched": pts / t_b, "unit": "grid-pt DF moments/s",
+            "s_setup": t_setup, "s_autotune": t_tune,
+            "tuned": {"setup_block": st["setup_block"], "integ_blocks": st["integ_blocks"], "graph": st["graph"]},
+            "precision": "mixed (FP32 integrands/DF, FP64 roots/Delta)", "poisson": "blockscan"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -153,6 +229,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision / block-scan leg")
     ap.add_argument("--no-surrogate", action="store_true", help="skip the plain vs surrogate convergence leg")
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched-models leg")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run a 1 GiB device-to-device copy (reference for FETCH_SIZE/WRITE_SIZE counters)")
     args = ap.parse_args()
@@ -205,7 +282,9 @@ def main():
                    "nr": nr, "ntheta": ntheta, "npoly": npoly, "nodes_per_point": 10648},
         "roofline": {"bound": "mfma", "pipe": "VALU FP64 (MI355X FP64 vector and matrix peaks are equal)",
                      "kernel": "k_integrals", "achieved": achieved, "peak": PEAK_FP64, "unit": "TFLOP/s",
-                     "frac": achieved / PEAK_FP64, "traffic": None,
+                     "frac": achieved / PEAK_FP64, "traffic": K_INTEGRALS_TRAFFIC_BYTES,
+                     "traffic_note": "HBM bytes per launch from rocprofv3 FETCH_SIZE/WRITE_SIZE (calibrated); "
+                                     "algorithmic bytes (node records) %.3g" % (line_tasks_bytes(stats)),
                      "flops_per_launch": flops, "ms_per_launch": t_int * 1e3},
         "phases_ms": {k: float(np.mean([s[k] for s in stats])) for k in
                       ("ms_points", "ms_setup", "ms_integrals", "ms_overflow", "ms_moments", "ms_project",
@@ -217,6 +296,8 @@ def main():
         line["mixed"] = mixed_leg(ctx, model, prev, last, args.steps, pts)
     if not args.no_surrogate:
         line["surrogate"] = surrogate_leg(ctx, model, guess_tables)
+    if not args.no_batch:
+        line["batch"] = batch_leg(ctx)
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(npoly, ntheta, nr, ar, guess_tables, delta)
     if args.copy_calibration:
