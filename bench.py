@@ -103,16 +103,32 @@ def mixed_leg(ctx, model, prev, last, steps, pts):
             "counters": {k: int(np.mean([x[k] for x in st])) for k in ("qk_calls", "overflow_tasks", "integral_lanes")}}
 
 
-def surrogate_leg(ctx, model, guess_tables, n_iter=5, keep=0.5, epochs=10, judge=0.01):
-    """The same model converged twice from the guess potential, n_iter SCF
-    iterations each: plain FP64 Staeckel, and the surrogate schedule
-    (iterations 0-1 Staeckel with training samples, then training, the middle
-    iterations served by the network with a judged sample, the final
-    iteration Staeckel).  Reports both wall times (training included), the
-    final density coefficients' difference scaled by the monopole, and the
-    virial ratios."""
+def surrogate_leg(ctx, model, guess_tables, tol=1e-4, max_iter=30, keep=0.15, epochs=16, judge=0.05,
+                  max_sur=8, sur_floor=4e-3, gate=0.05):
+    """The Hernquist model converged twice from the guess potential to the
+    same tolerance: successive density coefficients differ by less than `tol`
+    of the monopole (the driver's convergence test on rho_l).
+
+    plain: Staeckel actions every iteration.
+    surrogate: iterations 0-1 Staeckel, keeping a random `keep` of the bound
+    nodes' actions as training samples; then the network is trained
+    (`epochs` passes) and serves (Jr, Jz) while a random `judge` fraction of
+    the nodes is recomputed with Staeckel, scored, added to the training set
+    and used for one fine-tuning pass after each iteration; once the served
+    iterations change rho_l by less than `sur_floor` (the network's noise
+    level) or after `max_sur` of them, Staeckel iterations run to `tol`.
+    The judge gates the network: a served iteration whose judged median
+    relative error of Jr or Jz exceeds `gate` is discarded (the previous
+    potential is restored) and the run continues with Staeckel.
+
+    Reports the wall time of both runs (training included), the iteration
+    counts, the difference of the two converged rho_l (scaled by the
+    monopole) and both virial ratios."""
     import numpy as np
     import torch
+
+    def drho(a, b):
+        return float(np.max(np.abs(a["rhl"] - b["rhl"]) / np.abs(a["rhl"][:, :1])))
 
     def run(sur):
         ctx.set_potential(*guess_tables)
@@ -120,41 +136,72 @@ def surrogate_leg(ctx, model, guess_tables, n_iter=5, keep=0.5, epochs=10, judge
         model.philP = [tuple(x.copy() for x in guess_tables)]
         torch.cuda.synchronize()
         t0 = time.perf_counter()
-        t_it = []
-        for it in range(n_iter):
-            ti = time.perf_counter()
+        hist, kinds, t_train = [], [], 0.0
+        prev, out = None, None
+        phase = "collect" if sur else "plain"
+        n_sur = 0
+        for it in range(max_iter):
             if sur and it == 0:
-                ctx.surrogate_init(seed=7, capacity=4 << 20)
+                ctx.surrogate_init(seed=7, capacity=1 << 20)
                 ctx.surrogate_collect(keep)
-            if sur and it == 2:
-                ctx.surrogate_collect(0.0)
+            if phase == "collect" and it == 2:
+                ctx.surrogate_collect(1.0)            # from now on: the judged nodes
+                tt = time.perf_counter()
                 ctx.surrogate_train(epochs, batch=32768, lr=3e-3)
-                ctx.surrogate_collect(1.0)          # judged nodes join the set
+                t_train += time.perf_counter() - tt
                 ctx.surrogate_mode(True, judge=judge, jz_min=1e-3)
-            if sur and it == n_iter - 1:
-                ctx.surrogate_mode(False)
+                phase = "sur"
+            if phase == "sur":
+                saved = ([x.copy() for x in model.total], [tuple(x.copy() for x in p) for p in model.philP])
             out = model.iterate()[0]
-            if sur and 2 <= it < n_iter - 2:
-                ctx.surrogate_train(1, batch=32768, lr=1e-3)   # fine-tune on the judged nodes
-            t_it.append(time.perf_counter() - ti)
+            if phase == "sur":
+                js = ctx.surrogate_stats()
+                if max(js["judge_err_r_med"], js["judge_err_z_med"]) > gate:
+                    kinds.append("R")                 # rejected by the judge
+                    ctx.surrogate_mode(False)
+                    ctx.surrogate_collect(0.0)
+                    model.total, model.philP = saved
+                    ctx.set_potential(*model.total)
+                    phase = "plain"
+                    continue
+            kinds.append("S" if phase == "sur" else "P")
+            d = drho(out, prev) if prev is not None else None
+            hist.append(d)
+            prev = out
+            if phase == "sur":
+                n_sur += 1
+                if (d is not None and d < sur_floor) or n_sur >= max_sur:
+                    ctx.surrogate_mode(False)
+                    ctx.surrogate_collect(0.0)
+                    phase = "plain"
+                else:
+                    tt = time.perf_counter()
+                    ctx.surrogate_train(1, batch=32768, lr=1e-3)
+                    t_train += time.perf_counter() - tt
+            elif phase == "plain" and d is not None and d < tol:
+                break
         torch.cuda.synchronize()
         dt = time.perf_counter() - t0
         vir = ctx.virial(out["rhl"], out["vrotl"], out["sigRl"], out["sigpl"], out["sigzl"])
         st = ctx.surrogate_stats() if sur else None
-        ctx.surrogate_mode(False) if sur else None
-        return dt, t_it, out, vir, st
+        if sur:
+            ctx.surrogate_mode(False)
+        return dt, out, vir, st, hist, "".join(kinds), t_train
 
-    t_p, it_p, out_p, vir_p, _ = run(False)
-    t_s, it_s, out_s, vir_s, st = run(True)
-    rho_err = float(np.max(np.abs(out_s["rhl"] - out_p["rhl"]) / np.abs(out_p["rhl"][:, :1])))
-    return {"iterations": n_iter, "ms_plain": 1e3 * t_p, "ms_surrogate": 1e3 * t_s, "speedup": t_p / t_s,
-            "ms_per_iter_plain": [round(1e3 * x, 2) for x in it_p],
-            "ms_per_iter_surrogate": [round(1e3 * x, 2) for x in it_s],
-            "rho_l_err_vs_plain": rho_err, "virial_WK_plain": float(vir_p[4]), "virial_WK_surrogate": float(vir_s[4]),
-            "judge": {k: st[k] for k in ("judge_err_r_med", "judge_err_r_p95", "judge_err_z_med", "judge_err_z_p95",
-                                         "n_train", "nodes_sur", "fallback", "judged")},
-            "note": "training samples keep %.2f of the bound nodes of iterations 0-1, %d epochs; middle iterations "
-                    "served by the network with %.0f%% judged; final iteration Staeckel" % (keep, epochs, 100 * judge)}
+    t_p, out_p, vir_p, _, h_p, k_p, _ = run(False)
+    t_s, out_s, vir_s, st, h_s, k_s, t_tr = run(True)
+    return {"tol_rho_l": tol, "ms_plain": 1e3 * t_p, "ms_surrogate": 1e3 * t_s, "speedup": t_p / t_s,
+            "iterations_plain": len(k_p), "iterations_surrogate_run": k_s, "ms_training": 1e3 * t_tr,
+            "rho_l_diff_converged": drho(out_s, out_p),
+            "virial_WK_plain": float(vir_p[4]), "virial_WK_surrogate": float(vir_s[4]),
+            "d_rho_l_plain": [None if x is None else float("%.3g" % x) for x in h_p],
+            "d_rho_l_surrogate_run": [None if x is None else float("%.3g" % x) for x in h_s],
+            "judge_last_served_iteration": {k: st[k] for k in ("judge_err_r_med", "judge_err_r_p95", "judge_err_z_med",
+                                                             "judge_err_z_p95", "n_train", "nodes_sur", "fallback",
+                                                             "judged")},
+            "note": "P = Staeckel iteration, S = served by the network (%.0f%% of nodes judged), R = served iteration "
+                    "rejected by the judge (median error above %.0f%%) and redone with Staeckel; training: %d epochs "
+                    "after iteration 1, then one pass per served iteration" % (100 * judge, 100 * gate, epochs)}
 
 
 def batch_leg(ctx, n_models=8, iters=2, seed=5):

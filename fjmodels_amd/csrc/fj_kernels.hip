@@ -330,13 +330,15 @@ __device__ inline double d_df(const fj_df_params &d, double Jr, double Jphi, dou
 
 // ---------------------------------------------------------------------------
 // Action surrogate in the node phase.  Inputs of node (R, z, V):
-// (log R, z/R, vR/vc, vz/vc, |vphi|/vc); outputs log(J / s + tfloor) for
-// J = Jr, Jz with the action scale s = r vc of the grid point.
+// (log R, z/R, vR/vc, vz/vc, |vphi|/vc) with vc the circular-speed scale of
+// the grid point.  Outputs: log(Jr / s + tfloor) with the action scale
+// s = r vc of the grid point, and log((Jz + zf s) / (L - |Lz| + zf s)), Jz
+// relative to its spherical-limit value (L = |r x V|, Lz = R vphi).
 // ---------------------------------------------------------------------------
 struct SurNorm {
     float mu[MLP_IN], sig[MLP_IN], tmu[MLP_OUT], tsig[MLP_OUT];
     float lo[MLP_IN], hi[MLP_IN];   // training-set box of the normalised inputs
-    float tfloor;                   // targets are log(J / s + tfloor), s = r vc
+    float tfloor;                   // Jr target: log(Jr / s + tfloor)
 };
 
 __device__ inline void sur_features(double R, double z, double vc, const double V[3], float f[MLP_IN]) {
@@ -345,6 +347,15 @@ __device__ inline void sur_features(double R, double z, double vc, const double 
     f[2] = (float)(V[0] / vc);
     f[3] = (float)(V[2] / vc);
     f[4] = (float)(fabs(V[1]) / vc);
+}
+
+// Jz of the spherical limit, L - |Lz|: the Jz target is the logarithm of
+// (Jz + zf s) / (L - |Lz| + zf s), a correction near zero.
+constexpr float SUR_ZFLOOR = 1e-6f;
+__device__ inline double sur_jz_sph(double R, double z, const double V[3]) {
+    const double Lm = z * V[0] - R * V[2];
+    const double L = sqrt((R * R + z * z) * V[1] * V[1] + Lm * Lm);
+    return fmax(0.0, L - fabs(R * V[1]));
 }
 
 struct SurArgs {
@@ -408,12 +419,13 @@ __global__ void __launch_bounds__(256) k_sur_post(SurArgs a) {
             const float sc = (float)(sqrt(R * R + z * z) * a.Vc[p]);
             const float lr = a.y[gid * MLP_OUT] * a.nm.tsig[0] + a.nm.tmu[0];
             const float lz = a.y[gid * MLP_OUT + 1] * a.nm.tsig[1] + a.nm.tmu[1];
-            Jr = fmaxf(0.f, sc * (expf(lr) - a.nm.tfloor));
-            Jz = fmaxf(0.f, sc * (expf(lz) - a.nm.tfloor));
-            a.nb.Jr[gid] = Jr;
-            a.nb.Jz[gid] = Jz;
             double V[3], w;
             node_velocity(a.quad, node, a.Ve[p], V, w);
+            const float zs = (float)sur_jz_sph(R, z, V) + sc * SUR_ZFLOOR;
+            Jr = fmaxf(0.f, sc * (expf(lr) - a.nm.tfloor));
+            Jz = fmaxf(0.f, zs * expf(lz) - sc * SUR_ZFLOOR);
+            a.nb.Jr[gid] = Jr;
+            a.nb.Jz[gid] = Jz;
             const float aJphi = (float)fabs(R * V[1]);
             if (Jz < (float)a.jz_min * (Jr + Jz + aJphi)) fall = true;
             else if ((double)hash32((unsigned int)gid ^ a.seed) < a.judge_frac * 4294967296.0) judge = true;
@@ -490,8 +502,9 @@ __global__ void __launch_bounds__(256) k_sur_infer(SurArgs a, const bf16_t *__re
             if (flag == 1) {
                 const double R = a.R[p], z = a.z[p];
                 const float sc = (float)(sqrt(R * R + z * z) * a.Vc[p]);
+                const float zs = (float)sur_jz_sph(R, z, V) + sc * SUR_ZFLOOR;
                 Jr = fmaxf(0.f, sc * (expf(y0 * a.nm.tsig[0] + a.nm.tmu[0]) - a.nm.tfloor));
-                Jz = fmaxf(0.f, sc * (expf(y1 * a.nm.tsig[1] + a.nm.tmu[1]) - a.nm.tfloor));
+                Jz = fmaxf(0.f, zs * expf(y1 * a.nm.tsig[1] + a.nm.tmu[1]) - sc * SUR_ZFLOOR);
                 const float aJphi = (float)fabs(R * V[1]);
                 if (Jz < (float)a.jz_min * (Jr + Jz + aJphi)) fall = true;
                 else {
@@ -605,7 +618,7 @@ __global__ void __launch_bounds__(256) k_sur_collect(CollectArgs a) {
     const double sc = sqrt(a.R[p] * a.R[p] + a.z[p] * a.z[p]) * a.Vc[p];
     const double Jr = a.nb.Jr[gid], Jz = a.nb.Jz[gid];
     a.T[q * SUR_TW] = (float)log(Jr / sc + a.tfloor);
-    a.T[q * SUR_TW + 1] = (float)log(Jz / sc + a.tfloor);
+    a.T[q * SUR_TW + 1] = (float)log((Jz + sc * SUR_ZFLOOR) / (sur_jz_sph(a.R[p], a.z[p], V) + sc * SUR_ZFLOOR));
     const double Ve = a.Ve[p], o0 = a.out6[(long long)p * 6];
     const double share = o0 > 0 ? w * d_df(a.df, Jr, a.R[p] * V[1], Jz) * Ve * Ve * Ve / o0 : 0.0;
 #ifndef FJ_SUR_WEIGHTED

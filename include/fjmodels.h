@@ -158,9 +158,10 @@ int fj_get_stats(const fj_ctx *ctx, fj_stats *out);
  * uvOrb (uvOrb.cpp:15-262) at the quadrature nodes of intermediate SCF
  * iterations.  An MLP 5 -> 128 -> 128 -> 128 -> 2 (tanh) maps
  * (log R, z/R, vR/vc, vz/vc, |vphi|/vc), vc^2 = R dPhi/dR + z dPhi/dz at the
- * grid point, to (log(Jr + jf), log(Jz + jf)), jf = 1e-6; Jphi = R vphi stays
- * exact.  BF16 weights and activations on the matrix cores with FP32
- * accumulation, trained with Adam on the GPU on actions the Staeckel fudge
+ * grid point, to (log(Jr/s + 1e-6), log((Jz + 1e-6 s) / (L - |Lz| + 1e-6 s)))
+ * with s = r vc and L = |r x v|: Jz relative to its spherical-limit value;
+ * Jphi = R vphi stays exact.  BF16 weights and activations on the matrix
+ * cores with FP32 accumulation, trained with Adam on the GPU on actions the Staeckel fudge
  * computed at the nodes of earlier node phases.
  * ------------------------------------------------------------------------ */
 typedef struct {
