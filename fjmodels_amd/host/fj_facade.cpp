@@ -154,6 +154,8 @@ fJParams readParam(const char *fName) {
         else if (key == "rmin") P.rmin = val();
         else if (key == "rmax") P.rmax = val();
         else if (key == "tol") P.tol = val();
+        else if (key == "merge") P.merge = val();
+        else if (key == "delta_refresh") P.delta_refresh = (int)val();
         else if (key == "quad") {
             const std::string s = str();
             if (s == "log22") P.quad = FJ_QUAD_LOG22;
@@ -440,6 +442,31 @@ double potentialDistance(double **o, double **n) {
     return maxd;
 }
 
+void savePhi(double **philOLD, double **PrOLD, double **Pr2OLD) {
+    for (int i = 0; i < NR; i++)
+        for (int j = 0; j < NPOLY; j++) {
+            philOLD[i][j] = phil[i][j];
+            PrOLD[i][j] = Pr[i][j];
+            Pr2OLD[i][j] = Pr2[i][j];
+        }
+}
+
+double mergePhi(double **philOLD, double **PrOLD, double **Pr2OLD, double f) {
+    const double fa = 1 + f;
+    double maxd = 0;
+    for (int i = 0; i < NR; i++)
+        for (int j = 0; j < NPOLY; j++) {
+            const double d = fabs((phil[i][j] - philOLD[i][j]) / phil[i][0]);
+            maxd = maxd > d ? maxd : d;
+            phil[i][j] = fa * phil[i][j] - f * philOLD[i][j];
+            Pr[i][j] = fa * Pr[i][j] - f * PrOLD[i][j];
+            Pr2[i][j] = fa * Pr2[i][j] - f * Pr2OLD[i][j];
+        }
+    printf("-- Merge Phi: max distance %f \n", maxd);
+    syncPotential();
+    return maxd;
+}
+
 // ---------------------------------------------------------------------------
 // Output files
 // ---------------------------------------------------------------------------
@@ -516,6 +543,22 @@ OutFile readOut(const std::string &path) {
     }
     if (!in) throw Error(FJ_E_ARG, "truncated file " + path);
     return o;
+}
+
+void loadPotential(Potential *p, const OutFile &o) {
+    if (o.nr != NR || o.npoly != NPOLY) throw Error(FJ_E_ARG, "the file's grid differs from the current grid");
+    for (int i = 0; i < NR; i++)
+        if (fabs(o.ar[i] - ar[i]) > 1e-5 * ar[i]) throw Error(FJ_E_ARG, "the file's radii differ from the current grid");
+    const size_t b = sizeof(double) * NR * NPOLY;
+    memcpy(p->philP[0], o.m[6].data(), b);
+    memcpy(p->PrP[0], o.m[7].data(), b);
+    memcpy(p->Pr2P[0], o.m[8].data(), b);
+    if (p->comp == 1) {
+        memcpy(phil[0], o.m[6].data(), b);
+        memcpy(Pr[0], o.m[7].data(), b);
+        memcpy(Pr2[0], o.m[8].data(), b);
+        syncPotential();
+    }
 }
 
 }  // namespace fjm

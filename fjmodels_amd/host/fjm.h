@@ -57,6 +57,10 @@ struct fJParams {
     //   tol: stop when the potential changes by less than tol
     //        (mergePhi's distance, include/UtilsLeg.h:266-278); 0 = run itermax
     //   poisson quadpack|blockscan
+    //   merge f: after each computeNewPhi, Phi <- (1+f) Phi - f Phi_old
+    //        (mergePhi, include/UtilsLeg.h:266-278; one component); 0 = off
+    //   delta_refresh 1: tabulate Delta again from each new potential (the
+    //        reference tabulates it once, from the guess: fJmodels.cpp:127)
     int rot_form = 0, rot_form_2 = 0;
     double rot_k = UNSET, rot_k_2 = UNSET;
     int nr = 60, ntheta = 30, npoly = 22, ngrid = 30;
@@ -64,6 +68,8 @@ struct fJParams {
     int quad = FJ_QUAD_LOG22;
     int poisson = FJ_POISSON_QUADPACK;
     double tol = 0;
+    double merge = 0;
+    int delta_refresh = 0;
 };
 
 // Number of components found by checkParameterFile (readParam.cpp:125-150).
@@ -168,6 +174,18 @@ std::vector<double> vir2(Potential *p, double **rhlH = nullptr, double **vrotlH 
 // mergePhi's distance (include/UtilsLeg.h:266-278): max over (n, np) of
 // |new - old| / |new[n][0]|.
 double potentialDistance(double **philOld, double **philNew);
+
+// savePhi (include/UtilsLeg.h:256-263): copies of the global phil, Pr, Pr2.
+void savePhi(double **philOLD, double **PrOLD, double **Pr2OLD);
+// mergePhi (include/UtilsLeg.h:266-278): the distance of the global phil
+// from philOLD, then phil, Pr, Pr2 <- (1+f) new - f old; prints the
+// distance as the reference does and installs the merged potential.
+double mergePhi(double **philOLD, double **PrOLD, double **Pr2OLD, double f);
+
+// Installs the potential tables of a file written by writeOut (readOut) as
+// component p's tables and, for component 1, as the global potential: the
+// state after the iteration that wrote the file, for resuming a run.
+void loadPotential(Potential *p, const struct OutFile &o);
 
 // ---------------------------------------------------------------------------
 // Output: io/writeOut.cpp

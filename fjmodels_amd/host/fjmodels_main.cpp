@@ -5,10 +5,18 @@
 // computeNewPhi, write models/*.out and print the virial diagnostic.
 //
 //   fjmodels [param.txt] [--outdir DIR] [--device N]
+//            [--resume FILE [--resume2 FILE2]] [--first-iter K]
 //
 // Besides the reference's keys the parameter file may set nr, rmin, rmax,
 // ntheta, npoly, ngrid (grid), quad (log22 | tanhsinh22), rot_form / rot_k
-// (rotation term) and tol (stop when the potential changes by less).
+// (rotation term), tol (stop when the potential changes by less), merge
+// (mergePhi's over-relaxation factor, one component) and delta_refresh
+// (tabulate Delta from every new potential).
+//
+// --resume continues a run from the potential in a file it wrote (the .f64
+// sidecar reproduces the uninterrupted run bit for bit; FILE2: component 2),
+// after the guess potential and Delta table are rebuilt as at the start;
+// --first-iter K numbers the iterations from K (the run ends at itermax).
 #include <math.h>
 #include <stdio.h>
 #include <string.h>
@@ -23,6 +31,11 @@ using namespace fjm;
 
 namespace {
 
+struct Resume {
+    std::string file1, file2;
+    int first = 0;
+};
+
 double now() {
     using C = std::chrono::steady_clock;
     return std::chrono::duration<double>(C::now().time_since_epoch()).count();
@@ -32,7 +45,7 @@ void copy_mat(double **dst, double **src) { memcpy(dst[0], src[0], sizeof(double
 
 // twoComp (fJmodels.cpp:38-81); the virial diagnostic runs on the total
 // density, which the reference does not print for two components.
-void twoComp(fJParams P, const std::string &dir) {
+void twoComp(fJParams P, const std::string &dir, const Resume &R) {
     printf("\nSetting up two-component f(J) model...\n");
     double **rhl2 = mat(NR, NPOLY), **vrotl2 = mat(NR, NPOLY), **sigRl2 = mat(NR, NPOLY),
            **sigpl2 = mat(NR, NPOLY), **sigzl2 = mat(NR, NPOLY), **sigRzl2 = mat(NR, NPOLY);
@@ -48,8 +61,14 @@ void twoComp(fJParams P, const std::string &dir) {
     p2.computePhil(p2.rhlP);
     updatePhil(&p, &p2);
     tabulateDelta(&p);
+    if (!R.file1.empty()) {
+        if (R.file2.empty()) throw Error(FJ_E_ARG, "two components: --resume2 names component 2's file");
+        loadPotential(&p, readOut(R.file1));
+        loadPotential(&p2, readOut(R.file2));
+        updatePhil(&p, &p2);
+    }
     double **old = mat(NR, NPOLY);
-    for (int k = 0; k < P.itermax; k++) {
+    for (int k = R.first; k < P.itermax; k++) {
         printf("\n Iter:%d\n", k);
         const double t0 = now();
         copy_mat(old, phil);
@@ -58,6 +77,7 @@ void twoComp(fJParams P, const std::string &dir) {
         setDF(&P, &p, 2);
         computeNewPhi(&p2, rhl2, sigRl2, sigpl2, sigzl2, sigRzl2, vrotl2);
         updatePhil(&p, &p2);
+        if (P.delta_refresh) tabulateDelta(&p);
         writeOut(P, k, 1, rhl, sigRl, sigpl, sigzl, sigRzl, vrotl, p.philP, p.PrP, p.Pr2P, dir);
         writeOut(P, k, 2, rhl2, sigRl2, sigpl2, sigzl2, sigRzl2, vrotl2, p2.philP, p2.PrP, p2.Pr2P, dir);
         const double d = potentialDistance(old, phil);
@@ -72,7 +92,7 @@ void twoComp(fJParams P, const std::string &dir) {
 }
 
 // oneComp (fJmodels.cpp:86-144)
-void oneComp(fJParams P, const std::string &dir) {
+void oneComp(fJParams P, const std::string &dir, const Resume &R) {
     printf("\nSetting up one-component f(J) model...\n");
     SetModel(P);
     Potential p;
@@ -81,14 +101,18 @@ void oneComp(fJParams P, const std::string &dir) {
     p.computePhil();
     tabulateDelta(&p);
     setDF(&P, &p, 1);
-    double **old = mat(NR, NPOLY);
-    for (int k = 0; k < P.itermax; k++) {
+    if (!R.file1.empty()) loadPotential(&p, readOut(R.file1));
+    double **old = mat(NR, NPOLY), **PrOld = mat(NR, NPOLY), **Pr2Old = mat(NR, NPOLY);
+    for (int k = R.first; k < P.itermax; k++) {
         printf("\n Iter:%d\n", k);
         const double t0 = now();
         copy_mat(old, phil);
+        if (P.merge != 0) savePhi(old, PrOld, Pr2Old);
         computeNewPhi(&p, rhl, sigRl, sigpl, sigzl, sigRzl, vrotl);
+        if (P.merge != 0) mergePhi(old, PrOld, Pr2Old, P.merge);
         writeOut(P, k, 1, rhl, sigRl, sigpl, sigzl, sigRzl, vrotl, phil, Pr, Pr2, dir);
         vir2(&p);
+        if (P.delta_refresh) tabulateDelta(&p);
         const double d = potentialDistance(old, phil);
         printf("-- iteration %d: %.3f s, potential change %.3e\n", k, now() - t0, d);
         if (P.tol > 0 && d < P.tol) {
@@ -97,6 +121,8 @@ void oneComp(fJParams P, const std::string &dir) {
         }
     }
     free_mat(old);
+    free_mat(PrOld);
+    free_mat(Pr2Old);
 }
 
 }  // namespace
@@ -106,12 +132,17 @@ int main(int argc, char **argv) {
     const char *param = "param.txt";
     std::string dir = "models";
     int device = 0;
+    Resume R;
     for (int i = 1; i < argc; i++) {
         if (!strcmp(argv[i], "--outdir") && i + 1 < argc) dir = argv[++i];
         else if (!strcmp(argv[i], "--device") && i + 1 < argc) device = atoi(argv[++i]);
+        else if (!strcmp(argv[i], "--resume") && i + 1 < argc) R.file1 = argv[++i];
+        else if (!strcmp(argv[i], "--resume2") && i + 1 < argc) R.file2 = argv[++i];
+        else if (!strcmp(argv[i], "--first-iter") && i + 1 < argc) R.first = atoi(argv[++i]);
         else if (argv[i][0] != '-') param = argv[i];
         else {
-            fprintf(stderr, "usage: %s [param.txt] [--outdir DIR] [--device N]\n", argv[0]);
+            fprintf(stderr, "usage: %s [param.txt] [--outdir DIR] [--device N] [--resume FILE [--resume2 FILE2]] "
+                            "[--first-iter K]\n", argv[0]);
             return 2;
         }
     }
@@ -123,8 +154,8 @@ int main(int argc, char **argv) {
         set_poisson_mode(P.poisson);
         mkdir(dir.c_str(), 0755);
         printf("=\n======================================================================\n");
-        if (components == 1) oneComp(P, dir);
-        else twoComp(P, dir);
+        if (components == 1) oneComp(P, dir, R);
+        else twoComp(P, dir, R);
     } catch (const Error &e) {
         fprintf(stderr, "%s\n", e.what());
         return e.code ? e.code : 1;

@@ -64,3 +64,80 @@ def test_driver_matches_oracle(tmp_path):
             assert np.array_equal(got[n], ref[n]), (it, n, np.max(np.abs(got[n] - ref[n])))
         assert os.path.exists(tmp_path / f"e2e_{it}.out")
         p.set_tables(ref["phil"], ref["Pr"], ref["Pr2"])
+
+
+def run_driver(pf, outdir, *extra):
+    exe = os.path.join(ROOT, "fjmodels_amd", "fjmodels")
+    print("running", exe, *extra, flush=True)
+    r = subprocess.run([exe, str(pf), "--outdir", str(outdir), *extra], capture_output=True, text=True, timeout=300)
+    print(r.stdout[-1500:], r.stderr[-1500:])
+    assert r.returncode == 0
+    return r.stdout
+
+
+def test_driver_resume_reproduces_uninterrupted_run(tmp_path):
+    """--resume from the .f64 sidecar of iteration 0, numbered from 1, gives
+    the bits of iteration 1 of the uninterrupted run (guess potential and
+    Delta table rebuilt as at the start, potential tables from the file)."""
+    pf = tmp_path / "param.txt"
+    pf.write_text(PARAM)
+    a, b = tmp_path / "a", tmp_path / "b"
+    a.mkdir()
+    b.mkdir()
+    run_driver(pf, a)
+    run_driver(pf, b, "--resume", str(a / "e2e_0.out.f64"), "--first-iter", "1")
+    assert not (b / "e2e_0.out.f64").exists()
+    assert (a / "e2e_1.out.f64").read_bytes() == (b / "e2e_1.out.f64").read_bytes()
+    assert (a / "e2e_1.out").read_bytes() == (b / "e2e_1.out").read_bytes()
+
+
+def test_driver_merge_matches_oracle(tmp_path):
+    """merge 0.25: after each computeNewPhi the potential tables become
+    1.25 new - 0.25 old (mergePhi, include/UtilsLeg.h:266-278); the written
+    tables and the next iteration use them.  Against the oracle with the
+    same update, bit for bit, and the printed distance."""
+    pf = tmp_path / "param.txt"
+    pf.write_text(PARAM + "merge 0.25\n")
+    out = run_driver(pf, tmp_path)
+    nr, npoly, nth = 12, 3, 4
+    p = O.Pot(O.log_grid(nr, 0.05, 50.0), npoly, nth)
+    rhl = p.guess_rhl("Hernquist")
+    old = p.compute_phil(rhl)
+    p.set_tables(*old)
+    assert p.tabulate_delta() == 0
+    dfo = O.df_params("Hernquist")
+    dists = []
+    for it in range(2):
+        ref = p.compute_new_phi(dfo, nthreads=16)
+        dists.append(np.max(np.abs((ref["phil"] - old[0]) / ref["phil"][:, :1])))
+        merged = [1.25 * ref[n] - 0.25 * o for n, o in zip(("phil", "Pr", "Pr2"), old)]
+        ar, got = read_f64(tmp_path / f"e2e_{it}.out.f64", nr, npoly)
+        for n in ("rhl", "sigRl", "sigzl"):
+            assert np.array_equal(got[n], ref[n]), (it, n)
+        for n, m in zip(("phil", "Pr", "Pr2"), merged):
+            assert np.array_equal(got[n], m), (it, n, np.max(np.abs(got[n] - m)))
+        p.set_tables(*merged)
+        old = merged
+    printed = [float(l.split()[-1]) for l in out.splitlines() if l.startswith("-- Merge Phi: max distance")]
+    assert len(printed) == 2 and np.allclose(printed, dists, rtol=1e-5, atol=1e-6), (printed, dists)
+
+
+def test_driver_delta_refresh_matches_oracle(tmp_path):
+    """delta_refresh 1: the Delta table is tabulated again from every new
+    potential (the reference keeps the guess potential's table); against the
+    oracle doing the same, bit for bit."""
+    pf = tmp_path / "param.txt"
+    pf.write_text(PARAM + "delta_refresh 1\n")
+    run_driver(pf, tmp_path)
+    nr, npoly, nth = 12, 3, 4
+    p = O.Pot(O.log_grid(nr, 0.05, 50.0), npoly, nth)
+    p.set_tables(*p.compute_phil(p.guess_rhl("Hernquist")))
+    assert p.tabulate_delta() == 0
+    dfo = O.df_params("Hernquist")
+    for it in range(2):
+        ref = p.compute_new_phi(dfo, nthreads=16)
+        _, got = read_f64(tmp_path / f"e2e_{it}.out.f64", nr, npoly)
+        for n in NAMES:
+            assert np.array_equal(got[n], ref[n]), (it, n, np.max(np.abs(got[n] - ref[n])))
+        p.set_tables(ref["phil"], ref["Pr"], ref["Pr2"])
+        assert p.tabulate_delta() == 0
