@@ -25,7 +25,8 @@ EXPORTS = [
     "fj_set_precision", "fj_set_potential", "fj_get_potential", "fj_set_delta_table",
     "fj_get_delta_table", "fj_set_df", "fj_guess_rhl", "fj_poisson", "fj_moments", "fj_rhofDF",
     "fj_project_legendre", "fj_compute_new_phi", "fj_tabulate_delta", "fj_virial",
-    "fj_eval_potential", "fj_actions", "fj_get_stats",
+    "fj_eval_potential", "fj_actions", "fj_df_points", "fj_sigma_df", "fj_line_profile",
+    "fj_velocity_distribution", "fj_get_stats",
     "fj_surrogate_init", "fj_surrogate_collect", "fj_surrogate_train", "fj_surrogate_mode",
     "fj_surrogate_get_stats", "fj_surrogate_params", "fj_surrogate_forward", "fj_surrogate_gradient",
     "fj_batch_create", "fj_batch_destroy", "fj_batch_setup", "fj_batch_autotune", "fj_batch_iterate",
@@ -107,6 +108,10 @@ def load():
         "fj_virial": [v] + [_dp] * 6,
         "fj_eval_potential": [v, i, _dp, _dp, _dp, _dp, _dp],
         "fj_actions": [v, i] + [_dp] * 8,
+        "fj_df_points": [v, i] + [_dp] * 6,
+        "fj_sigma_df": [v, _dp, i, _dp, _dp, _dp],
+        "fj_line_profile": [v, _dp, d, d, i, _dp, _dp],
+        "fj_velocity_distribution": [v, i, d, d, i, _dp, _dp],
         "fj_get_stats": [v, C.POINTER(Stats)],
         "fj_surrogate_init": [v, C.c_ulonglong, C.c_longlong],
         "fj_surrogate_collect": [v, d],
@@ -286,6 +291,33 @@ class Context:
         self._chk(self.lib.fj_actions(self.h, n, _p(R), _p(z), _p(vR), _p(vphi), _p(vz), _p(D),
                                       _p(Jr), _p(Jz)), "fj_actions")
         return Jr, Jz
+
+    def df_points(self, R, z, vR, vphi, vz):
+        R, z, vR, vphi, vz = (_a(x) for x in (R, z, vR, vphi, vz))
+        f = np.zeros(len(R))
+        self._chk(self.lib.fj_df_points(self.h, len(R), _p(R), _p(z), _p(vR), _p(vphi), _p(vz), _p(f)),
+                  "fj_df_points")
+        return f
+
+    def sigma_df(self, rhl, R, z):
+        rhl, R, z = _a(rhl), _a(np.atleast_1d(R)), _a(np.atleast_1d(z))
+        out = np.zeros(len(R))
+        self._chk(self.lib.fj_sigma_df(self.h, _p(rhl), len(R), _p(R), _p(z), _p(out)), "fj_sigma_df")
+        return out
+
+    def line_profile(self, rhl, R, z, vx):
+        rhl, vx = _a(rhl), _a(np.atleast_1d(vx))
+        out = np.zeros(len(vx))
+        self._chk(self.lib.fj_line_profile(self.h, _p(rhl), R, z, len(vx), _p(vx), _p(out)), "fj_line_profile")
+        return out
+
+    def velocity_distribution(self, which, R, z, v):
+        """which: 0 P(vphi), 1 P(vz), 2 P(vR)."""
+        v = _a(np.atleast_1d(v))
+        out = np.zeros(len(v))
+        self._chk(self.lib.fj_velocity_distribution(self.h, which, R, z, len(v), _p(v), _p(out)),
+                  "fj_velocity_distribution")
+        return out
 
     def stats(self):
         s = Stats()

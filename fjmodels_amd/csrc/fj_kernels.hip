@@ -26,6 +26,7 @@
 
 #include "../../include/fjmodels.h"
 #include "fj_device.hpp"
+#include "fj_observe.h"
 #include "fj_node.hpp"
 #include "fj_mixed.hpp"
 #include "fj_mlp.hpp"
@@ -326,6 +327,17 @@ __device__ inline double d_df(const fj_df_params &d, double Jr, double Jphi, dou
     const double t = fjm_tanh(d.chi * Jphi / d.J0);
     if (d.rot_form == 1) return (1. - d.k) * even + d.k * t * even;
     return even + d.k * t * even;
+}
+
+// df() (DF.cpp:78-124) at explicit points whose actions are in Jr, Jz:
+// 0 where H = V^2/2 + Phi >= 0, else the double-power-law DF.
+__global__ void __launch_bounds__(256) k_df_points(int n, const double *R, const double *Phi, const double *vR,
+                                                   const double *vphi, const double *vz, const double *Jr,
+                                                   const double *Jz, fj_df_params df, double *f) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double H = .5 * (vR[i] * vR[i] + vphi[i] * vphi[i] + vz[i] * vz[i]) + Phi[i];   // DF.cpp:83
+    f[i] = H >= 0 ? 0.0 : d_df(df, Jr[i], R[i] * vphi[i], Jz[i]);
 }
 
 // ---------------------------------------------------------------------------
@@ -2658,6 +2670,195 @@ int fj_actions(fj_ctx *c, int n, const double *R, const double *z, const double 
     if ((r = run_actions(c, n, vR, vphi, vz, Delta))) return r;
     HIPCHK(c, hipMemcpy(Jr, c->d_Jr, n * sizeof(double), hipMemcpyDeviceToHost));
     HIPCHK(c, hipMemcpy(Jz, c->d_Jz, n * sizeof(double), hipMemcpyDeviceToHost));
+    return 0;
+}
+
+// ---------------------------------------------------------------------------
+// Observables (integ/Integ.cpp:33-43, 63-138, 165-236): f(J) at explicit
+// phase-space points on the device, the quadrature sums on the host in the
+// serial order of the reference's loops (GaussQuad.cpp:21-71, 124-134).
+// ---------------------------------------------------------------------------
+int fj_df_points(fj_ctx *c, int n, const double *R, const double *z, const double *vR, const double *vphi,
+                 const double *vz, double *f) {
+    if (!c || n < 1 || !R || !z || !vR || !vphi || !vz || !f) return FJ_E_ARG;
+    if (!c->have_df) return set_err(c, FJ_E_STATE, "no DF set");
+    int r = ensure_points(c, n);
+    if (r) return r;
+    HIPCHK(c, hipMemcpy(c->d_pR, R, n * sizeof(double), hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(c->d_pz, z, n * sizeof(double), hipMemcpyHostToDevice));
+    if ((r = run_actions(c, n, vR, vphi, vz, nullptr))) return r;
+    double *d = nullptr;
+    HIPCHK(c, hipMalloc((void **)&d, (size_t)4 * n * sizeof(double)));
+    hipError_t e = hipMemcpy(d, vR, n * sizeof(double), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(d + n, vphi, n * sizeof(double), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(d + 2 * n, vz, n * sizeof(double), hipMemcpyHostToDevice);
+    if (e == hipSuccess) {
+        k_df_points<<<(n + 255) / 256, 256, 0, c->stream>>>(n, c->d_pR, c->d_pPhi, d, d + n, d + 2 * n, c->d_Jr,
+                                                           c->d_Jz, c->df, d + 3 * n);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    if (e == hipSuccess) e = hipMemcpy(f, d + 3 * n, n * sizeof(double), hipMemcpyDeviceToHost);
+    (void)hipFree(d);
+    if (e != hipSuccess) return set_err(c, FJ_E_HIP, "fj_df_points: %s", hipGetErrorString(e));
+    return 0;
+}
+
+namespace {
+
+struct QuadTables {
+    double xpt11[22], xw11[22], xpt01[22], xw01[22];
+};
+
+int quad_tables(fj_ctx *c, QuadTables &q) {
+    HIPCHK(c, hipMemcpyFromSymbol(q.xpt11, HIP_SYMBOL(c_xpt11), sizeof q.xpt11));
+    HIPCHK(c, hipMemcpyFromSymbol(q.xw11, HIP_SYMBOL(c_xw11), sizeof q.xw11));
+    HIPCHK(c, hipMemcpyFromSymbol(q.xpt01, HIP_SYMBOL(c_xpt01), sizeof q.xpt01));
+    HIPCHK(c, hipMemcpyFromSymbol(q.xw01, HIP_SYMBOL(c_xw01), sizeof q.xw01));
+    return 0;
+}
+
+// topbottom (include/Utils.h:103-128): 0 if x is outside the table beyond
+// the 1e-5 tolerance (bot and top then keep their values).
+int host_topbottom(const double *xm, int n, double x, int *botx, int *topx) {
+    int top = n - 1, bot = 0;
+    if ((xm[top] - x) * (x - xm[bot]) < 0.) {
+        if (fabs(xm[top] - x) < 1e-5 * fabs(x - xm[bot])) { *topx = top; *botx = top - 1; return 1; }
+        if (fabs(x - xm[bot]) < 1e-5 * fabs(x - xm[top])) { *topx = bot + 1; *botx = bot; return 1; }
+        return 0;
+    }
+    while (top - bot > 1) {
+        const int m = (top + bot) / 2;
+        if ((xm[top] - x) * (x - xm[m]) >= 0) bot = m;
+        else top = m;
+    }
+    *topx = top; *botx = bot;
+    return 1;
+}
+
+// ev_dens (include/UtilsLeg.h:323-334) with the one-table int_dens (285-298).
+double host_ev_dens(const fj_ctx *c, const double *rhl, double R, double z) {
+    const int NR = c->nr, NP = c->npoly;
+    double pol[MAXPOLY], densp[MAXPOLY];
+    const double cc = z / sqrt(R * R + z * z), r = sqrt(R * R + z * z);
+    host_evenlegend(NP, cc, pol);
+    if (r > c->ar[NR - 1]) {
+        for (int i = 0; i < NP; i++) densp[i] = 0;
+    } else {
+        int top = 0, bot = 0;
+        host_topbottom(c->ar.data(), NR, r, &bot, &top);
+        const double f = (r - c->ar[bot]) / (c->ar[top] - c->ar[bot]);
+        for (int i = 0; i < NP; i++) densp[i] = f * rhl[top * NP + i] + (1 - f) * rhl[bot * NP + i];
+    }
+    double dens = .5 * densp[0];
+    for (int np = 1; np < NP; np++) {
+        const double fac = .5 * (4 * np + 1);
+        dens += fac * densp[np] * pol[np];
+    }
+    return dens;
+}
+
+// Phi at (R, z) and the escape speed of rhofDF (Integ.cpp:165-166).
+int host_phi_ve(fj_ctx *c, double R, double z, double *Phi, double *Ve) {
+    const double far = 10 * c->ar[c->nr - 1];
+    const double Rs[2] = {R, far}, zs[2] = {z, far};
+    double ph[2];
+    int r;
+    if ((r = fj_eval_potential(c, 2, Rs, zs, ph, nullptr, nullptr))) return r;
+    *Phi = ph[0];
+    *Ve = sqrt(-2 * (ph[0] - ph[1]));
+    return 0;
+}
+
+}  // namespace
+
+int fj_sigma_df(fj_ctx *c, const double *rhl, int n, const double *R, const double *z, double *sigma) {
+    if (!c || !rhl || n < 1 || !R || !z || !sigma) return FJ_E_ARG;
+    if (c->nr <= 0) return set_err(c, FJ_E_STATE, "no grid set");
+    QuadTables q;
+    int r;
+    if ((r = quad_tables(c, q))) return r;
+    const double rmax = c->ar[c->nr - 1];
+    for (int m = 0; m < n; m++) {
+        double sum = 0.;
+        for (int i = 0; i < 22; i++) {
+            const double rr = R[m] + q.xpt01[i] * (rmax - R[m]);
+            const double jacob = rmax - R[m];
+            sum += q.xw01[i] * (jacob * host_ev_dens(c, rhl, rr, z[m]));
+        }
+        sigma[m] = 2 * sum;
+    }
+    return 0;
+}
+
+int fj_line_profile(fj_ctx *c, const double *rhl, double R, double z, int n, const double *vx, double *out) {
+    if (!c || !rhl || n < 1 || !vx || !out) return FJ_E_ARG;
+    if (!c->have_pot || !c->have_delta || !c->have_df)
+        return set_err(c, FJ_E_STATE, "potential, Delta table and DF must be set");
+    QuadTables q;
+    int r;
+    double Phi, Ve, Sigma;
+    if ((r = quad_tables(c, q)) || (r = host_phi_ve(c, R, z, &Phi, &Ve)) || (r = fj_sigma_df(c, rhl, 1, &R, &z, &Sigma)))
+        return r;
+    const double armax = c->ar[c->nr - 1];
+    const long long nn = 22 * 22 * 22;
+    const long long np = nn * n;
+    if (np > (1ll << 30)) return set_err(c, FJ_E_ARG, "too many velocities in one call");
+    std::vector<double> Rv(np, R), zv(np, z), v0(np), v1(np), v2(np), jac(np), f(np);
+    for (int m = 0; m < n; m++)
+        for (int i = 0; i < 22; i++)
+            for (int j = 0; j < 22; j++)
+                for (int k = 0; k < 22; k++) {
+                    const long long t = m * nn + (i * 22 + j) * 22 + k;
+                    double V[3];
+                    fjob_lp_node(R, Ve, armax, vx[m], q.xpt11[i], q.xpt11[j], q.xpt11[k], V, &jac[t]);
+                    v0[t] = V[0]; v1[t] = V[1]; v2[t] = V[2];
+                }
+    if ((r = fj_df_points(c, (int)np, Rv.data(), zv.data(), v0.data(), v1.data(), v2.data(), f.data()))) return r;
+    for (int m = 0; m < n; m++) {
+        double sum = 0;
+        for (int i = 0; i < 22; i++)
+            for (int j = 0; j < 22; j++)
+                for (int k = 0; k < 22; k++) {
+                    const long long t = m * nn + (i * 22 + j) * 22 + k;
+                    sum += q.xw11[i] * q.xw11[j] * q.xw11[k] * (jac[t] * f[t]);
+                }
+        out[m] = sum / Sigma;
+    }
+    return 0;
+}
+
+int fj_velocity_distribution(fj_ctx *c, int which, double R, double z, int n, const double *v, double *out) {
+    if (!c || which < 0 || which > 2 || n < 1 || !v || !out) return FJ_E_ARG;
+    if (!c->have_pot || !c->have_delta || !c->have_df)
+        return set_err(c, FJ_E_STATE, "potential, Delta table and DF must be set");
+    QuadTables q;
+    int r;
+    double Phi, Ve;
+    if ((r = quad_tables(c, q)) || (r = host_phi_ve(c, R, z, &Phi, &Ve))) return r;
+    // Int2D_0101 (Pvphi: both axes [0,1]) or Int2D_0111 (second axis [-1,1])
+    const double *xb = which == 0 ? q.xpt01 : q.xpt11, *wb = which == 0 ? q.xw01 : q.xw11;
+    const long long nn = 22 * 22, np = nn * n;
+    std::vector<double> Rv(np, R), zv(np, z), v0(np), v1(np), v2(np), f(np);
+    for (int m = 0; m < n; m++)
+        for (int i = 0; i < 22; i++)
+            for (int j = 0; j < 22; j++) {
+                const long long t = m * nn + i * 22 + j;
+                double V[3];
+                fjob_pv_node(which, Ve, v[m], q.xpt01[i], xb[j], V);
+                v0[t] = V[0]; v1[t] = V[1]; v2[t] = V[2];
+            }
+    if ((r = fj_df_points(c, (int)np, Rv.data(), zv.data(), v0.data(), v1.data(), v2.data(), f.data()))) return r;
+    const double jacob = Ve * Ve;
+    for (int m = 0; m < n; m++) {
+        double sum = 0;
+        for (int i = 0; i < 22; i++)
+            for (int j = 0; j < 22; j++) {
+                const long long t = m * nn + i * 22 + j;
+                sum += q.xw01[i] * wb[j] * (jacob * f[t]);
+            }
+        out[m] = sum;
+    }
     return 0;
 }
 

@@ -20,7 +20,8 @@
  * formed with one fma, so it needs no branch on |x|; sinh and cosh from one
  * expm1 and one division, again without a branch; log via the atanh series
  * of (m-1)/(m+1); asin via its Taylor series on |x| <= 1/2 and the
- * half-angle identity beyond.
+ * half-angle identity beyond; atan by two half-angle reductions and its Taylor
+ * series.
  */
 #ifndef FJ_MATH_H
 #define FJ_MATH_H
@@ -232,6 +233,47 @@ FJM_FN double fjm_log(double x) {
     const double ln2_hi = 6.93147180369123816490e-01, ln2_lo = 1.90821492927058770002e-10;
     const double de = (double)e;
     return FJM_FMA(de, ln2_hi, FJM_FMA(de, ln2_lo, lm));
+}
+
+/* atan(x): |x| > 1 by pi/2 - atan(1/x); two halvings t -> t / (1 + sqrt(1 + t^2))
+ * (atan t = 2 atan of the result) bring |t| below tan(pi/16) < 0.2, where the Taylor
+ * series to t^25 is exact to far below an ulp. */
+FJM_FN double fjm_atan(double x) {
+    if (x != x) return x;
+    const double ax = x < 0 ? -x : x;
+    const int inv = ax > 1.0;
+    double t = inv ? 1.0 / ax : ax;
+    t = t / (1.0 + sqrt(1.0 + t * t));
+    t = t / (1.0 + sqrt(1.0 + t * t));
+    const double t2 = t * t;
+    double p = 1.0 / 25.0;
+    p = FJM_FMA(p, t2, -1.0 / 23.0);
+    p = FJM_FMA(p, t2, 1.0 / 21.0);
+    p = FJM_FMA(p, t2, -1.0 / 19.0);
+    p = FJM_FMA(p, t2, 1.0 / 17.0);
+    p = FJM_FMA(p, t2, -1.0 / 15.0);
+    p = FJM_FMA(p, t2, 1.0 / 13.0);
+    p = FJM_FMA(p, t2, -1.0 / 11.0);
+    p = FJM_FMA(p, t2, 1.0 / 9.0);
+    p = FJM_FMA(p, t2, -1.0 / 7.0);
+    p = FJM_FMA(p, t2, 1.0 / 5.0);
+    p = FJM_FMA(p, t2, -1.0 / 3.0);
+    double r = 4.0 * FJM_FMA(t * t2, p, t);
+    if (inv) r = (1.57079632679489655800e+00 - r) + 6.12323399573676603587e-17;
+    return x < 0 ? -r : r;
+}
+
+/* atan2(y, x) for finite arguments; atan2(0, 0) = 0 */
+FJM_FN double fjm_atan2(double y, double x) {
+    const double pi_hi = 3.14159265358979311600e+00, pi_lo = 1.22464679914735317720e-16;
+    if (x > 0) return fjm_atan(y / x);
+    if (x < 0) {
+        const double a = fjm_atan(y / x);
+        return y >= 0 ? (a + pi_lo) + pi_hi : (a - pi_lo) - pi_hi;
+    }
+    if (y > 0) return 1.57079632679489655800e+00;
+    if (y < 0) return -1.57079632679489655800e+00;
+    return 0.0;
 }
 
 /* asinh(x) */

@@ -156,6 +156,7 @@ fJParams readParam(const char *fName) {
         else if (key == "tol") P.tol = val();
         else if (key == "merge") P.merge = val();
         else if (key == "delta_refresh") P.delta_refresh = (int)val();
+        else if (key == "lineprofile") P.lineprofile = (int)val();
         else if (key == "quad") {
             const std::string s = str();
             if (s == "log22") P.quad = FJ_QUAD_LOG22;
@@ -525,6 +526,62 @@ std::string writeOut(const fJParams &f, int iter, unsigned comp, double **rhlH, 
         s << std::setprecision(17);
         write_body(s, f, comp, M);
     }
+    return path;
+}
+
+double SigmaDF(double R, double z) {
+    double s;
+    check(fj_sigma_df(ctx(), rhl[0], 1, &R, &z, &s), "fj_sigma_df");
+    return s;
+}
+
+void line_profile(double R, double z, int n, const double *vx, double *out) {
+    check(fj_line_profile(ctx(), rhl[0], R, z, n, vx, out), "fj_line_profile");
+}
+
+double line_profile(double R, double z, double vx, Potential *p) {
+    (void)p;
+    double o;
+    line_profile(R, z, 1, &vx, &o);
+    return o;
+}
+
+static double vdist(int which, double R, double z, double v) {
+    double o;
+    check(fj_velocity_distribution(ctx(), which, R, z, 1, &v, &o), "fj_velocity_distribution");
+    return o;
+}
+double P_vp(double R, double z, double vphi, Potential *p) { (void)p; return vdist(0, R, z, vphi); }
+double P_vz(double R, double z, double vz, Potential *p) { (void)p; return vdist(1, R, z, vz); }
+double P_vr(double R, double z, double vr, Potential *p) { (void)p; return vdist(2, R, z, vr); }
+
+std::string writeLineProfiles(const fJParams &f, int iter, unsigned comp, Potential *p, const std::string &dir) {
+    (void)p;
+    const std::string &user = comp == 1 ? f.outName : f.outName2;
+    std::string name = user != "null" ? user + "_" + toString(iter) + ".out" : composeName(f, iter, comp);
+    name.erase(name.end() - 3, name.end());
+    name += "lp";
+    const std::string path = dir + "/" + name;
+    std::ofstream o(path.c_str(), std::ios::out);
+    if (!o.is_open()) throw Error(FJ_E_ARG, "cannot open " + path);
+    const int size_lp = 150, half = size_lp / 2;
+    const int rad[3] = {0, NR * 2 / 3, NR * 5 / 6};
+    const double far = 10 * ar[NR - 1];
+    double Rs[4] = {ar[rad[0]], ar[rad[1]], ar[rad[2]], far}, zs[4] = {ar[0], ar[0], ar[0], far}, ph[4];
+    check(fj_eval_potential(ctx(), 4, Rs, zs, ph, nullptr, nullptr), "fj_eval_potential");
+    double ve[3];
+    for (int k = 0; k < 3; k++) ve[k] = sqrt(-2 * (ph[k] - ph[3]));
+    std::vector<double> v(size_lp), vx(size_lp), lp[3];
+    for (int i = 0; i < half; i++) v[i] = -pow(10., -3. + (half - 1 - i) * (3.) / (half - 1));
+    for (int i = half; i < size_lp; i++) v[i] = pow(10., -3. + (i - half) * (3.) / (half - 1));
+    for (int i = 0; i < size_lp; i++) vx[i] = 3. * v[i];
+    for (int k = 0; k < 3; k++) {
+        lp[k].resize(size_lp);
+        line_profile(ar[rad[k]], ar[0], size_lp, vx.data(), lp[k].data());
+    }
+    for (int i = 0; i < size_lp; i++)
+        o << ve[0] * 3. * v[i] << " " << lp[0][i] << " " << ve[1] * 3. * v[i] << " " << lp[1][i] << " "
+          << ve[2] * 3. * v[i] << " " << lp[2][i] << std::endl;
     return path;
 }
 

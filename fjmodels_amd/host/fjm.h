@@ -61,6 +61,8 @@ struct fJParams {
     //        (mergePhi, include/UtilsLeg.h:266-278; one component); 0 = off
     //   delta_refresh 1: tabulate Delta again from each new potential (the
     //        reference tabulates it once, from the guess: fJmodels.cpp:127)
+    //   lineprofile 1: write the .lp file of writeOut.cpp:159-186 (the
+    //        reference's LINEPROFILE build) after each iteration
     int rot_form = 0, rot_form_2 = 0;
     double rot_k = UNSET, rot_k_2 = UNSET;
     int nr = 60, ntheta = 30, npoly = 22, ngrid = 30;
@@ -70,6 +72,7 @@ struct fJParams {
     double tol = 0;
     double merge = 0;
     int delta_refresh = 0;
+    int lineprofile = 0;
 };
 
 // Number of components found by checkParameterFile (readParam.cpp:125-150).
@@ -174,6 +177,25 @@ std::vector<double> vir2(Potential *p, double **rhlH = nullptr, double **vrotlH 
 // mergePhi's distance (include/UtilsLeg.h:266-278): max over (n, np) of
 // |new - old| / |new[n][0]|.
 double potentialDistance(double **philOld, double **philNew);
+
+// ---------------------------------------------------------------------------
+// Observables: integ/Integ.cpp (the reference's quadratures, summed in the
+// serial order of their loops; f(J) on the GPU)
+// ---------------------------------------------------------------------------
+// SigmaDF (Integ.cpp:168-172) from the global rhl.
+double SigmaDF(double R, double z);
+// line_profile (Integ.cpp:188-197); the second form takes n velocities.
+double line_profile(double R, double z, double vx, Potential *p);
+void line_profile(double R, double z, int n, const double *vx, double *out);
+// P_vp, P_vz, P_vr (Integ.cpp:202-236).
+double P_vp(double R, double z, double vphi, Potential *p);
+double P_vz(double R, double z, double vz, Potential *p);
+double P_vr(double R, double z, double vr, Potential *p);
+// The .lp file of writeOut.cpp:159-186 (LINEPROFILE): 150 line-of-sight
+// velocities, three radii (ar[0], ar[2 nr/3], ar[5 nr/6] at z = ar[0]), next
+// to the .out file of (iter, comp).  Returns its path.
+std::string writeLineProfiles(const fJParams &fJP, int iter, unsigned comp, Potential *p,
+                              const std::string &dir = "models");
 
 // savePhi (include/UtilsLeg.h:256-263): copies of the global phil, Pr, Pr2.
 void savePhi(double **philOLD, double **PrOLD, double **Pr2OLD);

@@ -10,8 +10,9 @@
 // Besides the reference's keys the parameter file may set nr, rmin, rmax,
 // ntheta, npoly, ngrid (grid), quad (log22 | tanhsinh22), rot_form / rot_k
 // (rotation term), tol (stop when the potential changes by less), merge
-// (mergePhi's over-relaxation factor, one component) and delta_refresh
-// (tabulate Delta from every new potential).
+// (mergePhi's over-relaxation factor, one component), delta_refresh
+// (tabulate Delta from every new potential) and lineprofile (write the
+// .lp line-profile file of every iteration).
 //
 // --resume continues a run from the potential in a file it wrote (the .f64
 // sidecar reproduces the uninterrupted run bit for bit; FILE2: component 2),
@@ -111,6 +112,7 @@ void oneComp(fJParams P, const std::string &dir, const Resume &R) {
         computeNewPhi(&p, rhl, sigRl, sigpl, sigzl, sigRzl, vrotl);
         if (P.merge != 0) mergePhi(old, PrOld, Pr2Old, P.merge);
         writeOut(P, k, 1, rhl, sigRl, sigpl, sigzl, sigRzl, vrotl, phil, Pr, Pr2, dir);
+        if (P.lineprofile) writeLineProfiles(P, k, 1, &p, dir);
         vir2(&p);
         if (P.delta_refresh) tabulateDelta(&p);
         const double d = potentialDistance(old, phil);

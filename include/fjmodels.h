@@ -151,6 +151,26 @@ int fj_eval_potential(fj_ctx *ctx, int n, const double *R, const double *z,
 int fj_actions(fj_ctx *ctx, int n, const double *R, const double *z, const double *vR,
                const double *vphi, const double *vz, const double *Delta, double *Jr, double *Jz);
 
+/* df() (DF.cpp:78-124) at n phase-space points: 0 where H = v^2/2 + Phi(R,z)
+ * >= 0, else the DF of fj_set_df at the Staeckel actions (Delta from the
+ * table, as df() takes it). */
+int fj_df_points(fj_ctx *ctx, int n, const double *R, const double *z, const double *vR,
+                 const double *vphi, const double *vz, double *f);
+
+/* Observables of integ/Integ.cpp, the reference's quadratures summed in the
+ * serial order of their loops (GaussQuad.cpp:21-71, 124-134):
+ *   fj_sigma_df   SigmaDF (Integ.cpp:168-172, SigmaInteg 33-43): 2 Int1D_01
+ *                 of ev_dens (UtilsLeg.h:323-334, from rhl [nr][npoly]) along
+ *                 r from R to ar[nr-1], at n points (R, z);
+ *   fj_line_profile  line_profile (Integ.cpp:188-197, lineProfile 64-92) at n
+ *                 line-of-sight velocities vx: Int3D_111111 / SigmaDF;
+ *   fj_velocity_distribution  P_vp, P_vz, P_vr (Integ.cpp:202-236, Pvphi /
+ *                 Pvz / Pvr 97-138; which = 0, 1, 2) at n values v of the
+ *                 fixed component. */
+int fj_sigma_df(fj_ctx *ctx, const double *rhl, int n, const double *R, const double *z, double *sigma);
+int fj_line_profile(fj_ctx *ctx, const double *rhl, double R, double z, int n, const double *vx, double *out);
+int fj_velocity_distribution(fj_ctx *ctx, int which, double R, double z, int n, const double *v, double *out);
+
 int fj_get_stats(const fj_ctx *ctx, fj_stats *out);
 
 /* ------------------------------------------------------------------------

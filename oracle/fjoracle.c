@@ -15,6 +15,7 @@
  * one implementation on both sides makes the reference's discrete decisions
  * reproducible bit for bit).  Only tests and the checker use this library. */
 #include "../fjmodels_amd/csrc/fj_math.h"
+#include "../fjmodels_amd/csrc/fj_observe.h"
 
 #include <float.h>
 #include <math.h>
@@ -1694,4 +1695,80 @@ void fjo_vir2(const fjo_pot *p, const double *rhl, const double *vrotl, const do
     out[5] = KRR; out[6] = WRR; out[7] = WRR / KRR;
     out[8] = Kzz; out[9] = Wzz; out[10] = Wzz / Kzz;
     free(ci); free(wi);
+}
+
+/* ---- observables (integ/Integ.cpp) --------------------------------------- */
+
+/* ev_dens (include/UtilsLeg.h:323-334) with the one-table int_dens. */
+static double ev_dens1(const fjo_pot *p, const double *rhl, double R, double z) {
+    const int NP = p->npoly;
+    double pol[64], densp[64];
+    double c = z / sqrt(R * R + z * z), r = sqrt(R * R + z * z);
+    fjo_evenlegend(NP, c, pol);
+    int_dens1(p, r, rhl, densp);
+    double dens = .5 * densp[0];
+    for (int np = 1; np < NP; np++) {
+        double fac = .5 * (4 * np + 1);
+        dens += fac * densp[np] * pol[np];
+    }
+    return dens;
+}
+
+/* SigmaDF (Integ.cpp:168-172) = 2 Int1D_01 (GaussQuad.cpp:31-39) of
+ * SigmaInteg (Integ.cpp:33-43): r from R to ar[nr-1]. */
+double fjo_sigma_df(const fjo_pot *p, const double *rhl, double R, double z) {
+    const double rmax = p->ar[p->nr - 1];
+    double sum = 0.;
+    for (int i = 0; i < 22; i++) {
+        double r = R + xpt01[i] * (rmax - R);
+        double jacob = rmax - R;
+        sum += xw01[i] * (jacob * ev_dens1(p, rhl, r, z));
+    }
+    return 2 * sum;
+}
+
+static void phi_ve(const fjo_pot *p, double R, double z, double *Phi, double *Ve) {
+    double far = 10 * p->ar[p->nr - 1];
+    *Phi = fjo_phi(p, R, z);
+    *Ve = sqrt(-2 * (*Phi - fjo_phi(p, far, far)));
+}
+
+/* line_profile (Integ.cpp:188-197): Int3D_111111 (GaussQuad.cpp:124-134) of
+ * lineProfile (Integ.cpp:64-92) in serial (i, j, k) order, over SigmaDF. */
+double fjo_line_profile(const fjo_pot *p, const fjo_df *d, const double *rhl, double R, double z, double vx) {
+    double Phi, Ve;
+    phi_ve(p, R, z, &Phi, &Ve);
+    double Sigma = fjo_sigma_df(p, rhl, R, z);
+    double x[3] = {R, z, Phi}, sum = 0;
+    for (int i = 0; i < 22; i++)
+        for (int j = 0; j < 22; j++)
+            for (int k = 0; k < 22; k++) {
+                double V[3], jacob;
+                fjob_lp_node(R, Ve, p->ar[p->nr - 1], vx, xpt11[i], xpt11[j], xpt11[k], V, &jacob);
+                sum += xw11[i] * xw11[j] * xw11[k] * (jacob * fjo_df_eval(p, d, x, V));
+            }
+    return sum / Sigma;
+}
+
+/* P_vp / P_vz / P_vr (Integ.cpp:202-236): Int2D_0101 / Int2D_0111
+ * (GaussQuad.cpp:52-71) of Pvphi / Pvz / Pvr (Integ.cpp:97-138), serial
+ * (i, j) order; which = 0, 1, 2. */
+double fjo_vdist(const fjo_pot *p, const fjo_df *d, int which, double R, double z, double v) {
+    double Phi, Ve;
+    phi_ve(p, R, z, &Phi, &Ve);
+    const double *xb = which == 0 ? xpt01 : xpt11, *wb = which == 0 ? xw01 : xw11;
+    double x[3] = {R, z, Phi}, sum = 0;
+    for (int i = 0; i < 22; i++)
+        for (int j = 0; j < 22; j++) {
+            double V[3];
+            fjob_pv_node(which, Ve, v, xpt01[i], xb[j], V);
+            sum += xw01[i] * wb[j] * (Ve * Ve * fjo_df_eval(p, d, x, V));
+        }
+    return sum;
+}
+
+/* The velocity rules of include/GaussPts.h:76-136 (xpt11, xw11, xpt01, xw01). */
+void fjo_quad_tables(double *x11, double *w11, double *x01, double *w01) {
+    memcpy(x11, xpt11, sizeof xpt11); memcpy(w11, xw11, sizeof xw11);
+    memcpy(x01, xpt01, sizeof xpt01); memcpy(w01, xw01, sizeof xw01);
 }

@@ -89,6 +89,14 @@ int fjo_actions(const fjo_pot *p, double Delta, double R, double z, double Phi0,
                 double vR, double vphi, double vz, double *Jr, double *Jz);
 double fjo_df_eval(const fjo_pot *p, const fjo_df *df, const double *x, const double *v);
 double fjo_df_hg(const fjo_df *df, double Jr, double Jphi, double Jz);
+/* Observables (integ/Integ.cpp:33-43, 63-138, 165-236): projected density
+ * SigmaDF, line profile at line-of-sight velocity vx, velocity distributions
+ * P_vp / P_vz / P_vr (which 0 / 1 / 2) at the fixed component v. */
+double fjo_sigma_df(const fjo_pot *p, const double *rhl, double R, double z);
+/* Copies of the velocity rules xpt11, xw11, xpt01, xw01 (22 doubles each). */
+void fjo_quad_tables(double *x11, double *w11, double *x01, double *w01);
+double fjo_line_profile(const fjo_pot *p, const fjo_df *df, const double *rhl, double R, double z, double vx);
+double fjo_vdist(const fjo_pot *p, const fjo_df *df, int which, double R, double z, double v);
 double fjo_rhofDF(const fjo_pot *p, const fjo_df *df, double R, double z, double *out6,
                   double *vrot, double *sigR, double *sigp, double *sigz, double *sigRz);
 /* Grid moments at every (ar[n]*si, ar[n]*ci) point; arrays are [nr][ngauss]. */

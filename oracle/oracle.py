@@ -76,6 +76,10 @@ def _load():
         "fjo_df_eval": (d, [v, C.POINTER(DFParams), _dp, _dp]),
         "fjo_df_hg": (d, [C.POINTER(DFParams), d, d, d]),
         "fjo_rhofDF": (d, [v, C.POINTER(DFParams), d, d, _dp, _dp, _dp, _dp, _dp, _dp]),
+        "fjo_sigma_df": (d, [v, _dp, d, d]),
+        "fjo_quad_tables": (None, [_dp, _dp, _dp, _dp]),
+        "fjo_line_profile": (d, [v, C.POINTER(DFParams), _dp, d, d, d]),
+        "fjo_vdist": (d, [v, C.POINTER(DFParams), C.c_int, d, d, d]),
         "fjo_compute_rhl_grid": (None, [v, C.POINTER(DFParams), i] + [_dp] * 6),
         "fjo_compute_new_phi": (i, [v, C.POINTER(DFParams), i] + [_dp] * 9),
         "fjo_vir2": (None, [v] + [_dp] * 6),
@@ -191,6 +195,23 @@ class Pot:
         lib.fjo_actions(self.h, Delta, R, z, Phi0, vR, vphi, vz, C.byref(Jr), C.byref(Jz))
         return Jr.value, Jz.value
 
+    def df_eval(self, df, x, v):
+        """df() (DF.cpp:78-124) at x = (R, z, Phi), velocity v."""
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        v = np.ascontiguousarray(v, dtype=np.float64)
+        return lib.fjo_df_eval(self.h, C.byref(df), ptr(x), ptr(v))
+
+    def sigma_df(self, rhl, R, z):
+        rhl = np.ascontiguousarray(rhl, dtype=np.float64)
+        return lib.fjo_sigma_df(self.h, ptr(rhl), R, z)
+
+    def line_profile(self, df, rhl, R, z, vx):
+        rhl = np.ascontiguousarray(rhl, dtype=np.float64)
+        return lib.fjo_line_profile(self.h, C.byref(df), ptr(rhl), R, z, vx)
+
+    def velocity_distribution(self, df, which, R, z, v):
+        return lib.fjo_vdist(self.h, C.byref(df), which, R, z, v)
+
     def rhofDF(self, df, R, z):
         out = np.zeros(6)
         v = [C.c_double() for _ in range(5)]
@@ -238,6 +259,13 @@ def counters():
 
 def reset_counters():
     lib.fjo_counters_reset()
+
+
+def quad_tables():
+    """(xpt11, xw11, xpt01, xw01) of include/GaussPts.h:76-136."""
+    t = [np.zeros(22) for _ in range(4)]
+    lib.fjo_quad_tables(*[ptr(x) for x in t])
+    return t
 
 
 def guess_model(nr, rmin, rmax, npoly, ngauss, model="Hernquist", mass=1.0, r0=1.0, q=1.0,

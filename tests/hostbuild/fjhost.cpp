@@ -36,6 +36,8 @@ int fjh_math(int which, int n, const double *x, double *y) {
         case 8: r = fjm_log(a); break;
         case 9: r = fjm_tanh(a); break;
         case 10: r = fjm_pow(a, y[i]); break;
+        case 11: r = fjm_atan(a); break;
+        case 12: r = fjm_atan2(a, y[i]); break;
         }
         y[i] = r;
     }

@@ -78,3 +78,20 @@ def test_pow_integer_and_fractional(H):
     ref = [mpmath.power(mpmath.mpf(float(a)), mpmath.mpf(float(b))) for a, b in zip(x, y)]
     rel = np.abs(got / np.array([float(v) for v in ref]) - 1)
     assert rel.max() < 5e-15
+
+
+def test_atan_and_atan2(H):
+    """fjm_atan over 10^-8 .. 10^8 and both signs, fjm_atan2 over all four
+    quadrants and the axes, against mpmath."""
+    x = np.concatenate([rng.uniform(-1, 1, 400), 10 ** rng.uniform(-8, 8, 300), -10 ** rng.uniform(-8, 8, 300)])
+    got = run(H, 11, x)
+    u = ulps(got, [mpmath.atan(mpmath.mpf(float(v))) for v in x])
+    print("atan max ulp", u.max())
+    assert u.max() <= 3, (u.max(), x[np.argmax(u)])
+    y = np.concatenate([rng.uniform(-5, 5, 600), [1.0, -1.0, 2.0, 0.0]])
+    xx = np.concatenate([rng.uniform(-5, 5, 600), [0.0, 0.0, -3.0, 1.0]])
+    got = run(H, 12, y, xx)
+    ref = [mpmath.atan2(mpmath.mpf(float(a)), mpmath.mpf(float(b))) for a, b in zip(y, xx)]
+    u = ulps(got, ref)
+    print("atan2 max ulp", u.max())
+    assert u.max() <= 3, (u.max(), y[np.argmax(u)], xx[np.argmax(u)])

@@ -141,3 +141,29 @@ def test_driver_delta_refresh_matches_oracle(tmp_path):
             assert np.array_equal(got[n], ref[n]), (it, n, np.max(np.abs(got[n] - ref[n])))
         p.set_tables(ref["phil"], ref["Pr"], ref["Pr2"])
         assert p.tabulate_delta() == 0
+
+
+def test_driver_line_profiles(tmp_path):
+    """lineprofile 1: the .lp file of writeOut.cpp:159-186 (150 velocities,
+    radii ar[0], ar[2 nr/3], ar[5 nr/6] at z = ar[0]) after iteration 0; its
+    values (6 digits) against the oracle's line_profile with the density and
+    potential of that iteration."""
+    pf = tmp_path / "param.txt"
+    pf.write_text(PARAM.replace("itermax 2", "itermax 1") + "lineprofile 1\n")
+    run_driver(pf, tmp_path)
+    rows = [list(map(float, l.split())) for l in (tmp_path / "e2e_0.lp").read_text().splitlines()]
+    assert len(rows) == 150 and all(len(r) == 6 for r in rows)
+    nr, npoly, nth = 12, 3, 4
+    p = O.Pot(O.log_grid(nr, 0.05, 50.0), npoly, nth)
+    p.set_tables(*p.compute_phil(p.guess_rhl("Hernquist")))
+    assert p.tabulate_delta() == 0
+    dfo = O.df_params("Hernquist")
+    ref = p.compute_new_phi(dfo, nthreads=16)
+    p.set_tables(ref["phil"], ref["Pr"], ref["Pr2"])
+    half = 75
+    v = [-10 ** (-3 + (half - 1 - i) * 3 / (half - 1)) for i in range(half)] + \
+        [10 ** (-3 + (i - half) * 3 / (half - 1)) for i in range(half, 150)]
+    for i in (0, 74, 75, 149):
+        for col, k in ((1, 0), (3, nr * 2 // 3), (5, nr * 5 // 6)):
+            lp = p.line_profile(dfo, ref["rhl"], p.ar[k], p.ar[0], 3 * v[i])
+            assert abs(rows[i][col] - lp) <= 5e-6 * abs(lp), (i, col, rows[i][col], lp)
