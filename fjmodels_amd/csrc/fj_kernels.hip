@@ -1679,38 +1679,65 @@ __device__ __forceinline__ IntegArgs batch_integ_args(const IntegArgs &base, con
     return ia;
 }
 
+// The next model after m (cyclically) whose task queue is not drained, or -1.
+__device__ inline int kb_next_model(unsigned int *cnt, int M, int m) {
+    for (int k = 1; k <= M; k++) {
+        const int c = (m + k) % M;
+        const unsigned int tot = cnt[8 * c] + cnt[8 * c + 1];
+        if (atomicAdd(&cnt[8 * c + 2], 0u) < tot) return c;
+    }
+    return -1;
+}
+
+// Integrals of every model: a block starts on the model kb_assign gave it
+// and, once that model's queue is drained, moves on to the next model with
+// tasks left (re-staging its tables), so that no model's tail idles the
+// device.  The hop bound only guards the loop: every visit either takes
+// tasks or finds the queue drained for good.
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(FJ_INTEG_WAVES)))
-kb_integrals(const ModelDev *md, const int *block_model, IntegArgs base, unsigned int *cnt) {
+kb_integrals(const ModelDev *md, const int *block_model, int M, IntegArgs base, unsigned int *cnt) {
     extern __shared__ double smem[];
-    const int m = block_model[blockIdx.x];
-    if (m < 0) return;
-    const ModelDev &d = md[m];
-    const LdsTabs L = stage_tables(d.ta, smem);
-    const IntegArgs ia = batch_integ_args<JInt>(base, d, cnt, m);
+    __shared__ int s_m;
+    int m = block_model[blockIdx.x];
     const long long lane_g = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    const QagsWork w = lane_work(ia.ws, ia.cap, lane_g, (long long)gridDim.x * blockDim.x);
-    const FvStrided fv{smem + tab_lds_doubles(d.ta.nr, d.ta.npoly, d.ta.ngrid) + threadIdx.x, (int)blockDim.x};
-    JInt f;
-    f.T = &L.T;
-    integrals_loop(ia, f, w, fv);
+    for (int hop = 0; hop < 4 * M + 8 && m >= 0; hop++) {
+        const ModelDev &d = md[m];
+        const LdsTabs L = stage_tables(d.ta, smem);
+        const IntegArgs ia = batch_integ_args<JInt>(base, d, cnt, m);
+        const QagsWork w = lane_work(ia.ws, ia.cap, lane_g, (long long)gridDim.x * blockDim.x);
+        const FvStrided fv{smem + tab_lds_doubles(d.ta.nr, d.ta.npoly, d.ta.ngrid) + threadIdx.x, (int)blockDim.x};
+        JInt f;
+        f.T = &L.T;
+        integrals_loop(ia, f, w, fv);
+        __syncthreads();
+        if (threadIdx.x == 0) s_m = kb_next_model(cnt, M, m);
+        __syncthreads();
+        m = s_m;
+    }
 }
 
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(FJ_INTEG32_WAVES)))
-kb_integrals_f32(const ModelDev *md, const int *block_model, IntegArgs base, unsigned int *cnt) {
+kb_integrals_f32(const ModelDev *md, const int *block_model, int M, IntegArgs base, unsigned int *cnt) {
     extern __shared__ double smem[];
-    const int m = block_model[blockIdx.x];
-    if (m < 0) return;
-    const ModelDev &d = md[m];
-    const LdsTabs L = stage_tables(d.ta, smem);
-    float *f32 = reinterpret_cast<float *>(smem + tab_lds_doubles(d.ta.nr, d.ta.npoly, d.ta.ngrid));
-    const PotView32 T32 = stage_tables32(L, f32);
-    const IntegArgs ia = batch_integ_args<JInt32>(base, d, cnt, m);
+    __shared__ int s_m;
+    int m = block_model[blockIdx.x];
     const long long lane_g = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    const QagsWork w = lane_work(ia.ws, ia.cap, lane_g, (long long)gridDim.x * blockDim.x);
-    const FvStridedF fv{f32 + tab32_lds_floats(d.ta.nr, d.ta.npoly) + threadIdx.x, (int)blockDim.x};
-    JInt32 f;
-    f.T = &T32;
-    integrals_loop(ia, f, w, fv);
+    for (int hop = 0; hop < 4 * M + 8 && m >= 0; hop++) {
+        const ModelDev &d = md[m];
+        const LdsTabs L = stage_tables(d.ta, smem);
+        float *f32 = reinterpret_cast<float *>(smem + tab_lds_doubles(d.ta.nr, d.ta.npoly, d.ta.ngrid));
+        const PotView32 T32 = stage_tables32(L, f32);
+        const IntegArgs ia = batch_integ_args<JInt32>(base, d, cnt, m);
+        const QagsWork w = lane_work(ia.ws, ia.cap, lane_g, (long long)gridDim.x * blockDim.x);
+        const FvStridedF fv{f32 + tab32_lds_floats(d.ta.nr, d.ta.npoly) + threadIdx.x, (int)blockDim.x};
+        JInt32 f;
+        f.T = &T32;
+        integrals_loop(ia, f, w, fv);
+        __syncthreads();
+        if (threadIdx.x == 0) s_m = kb_next_model(cnt, M, m);
+        __syncthreads();
+        m = s_m;
+    }
 }
 
 // Overflowed tasks of every model, FP64, full interval limit, tables read
@@ -3250,9 +3277,9 @@ int batch_iteration(fj_batch *b) {
     IntegArgs ia{nb, b->d_list_u, b->d_list_v, nullptr, nullptr, nullptr, b->d_ovf, g,
                  b->d_ws, mixed ? QCAP_MIXED : QCAP, -1, b->d_ctr};
     if (mixed)
-        kb_integrals_f32<<<b->integ_blocks, 256, b->lds_integ, s>>>(b->d_md, b->d_block_model, ia, b->d_cnt);
+        kb_integrals_f32<<<b->integ_blocks, 256, b->lds_integ, s>>>(b->d_md, b->d_block_model, b->M, ia, b->d_cnt);
     else
-        kb_integrals<<<b->integ_blocks, 256, b->lds_integ, s>>>(b->d_md, b->d_block_model, ia, b->d_cnt);
+        kb_integrals<<<b->integ_blocks, 256, b->lds_integ, s>>>(b->d_md, b->d_block_model, b->M, ia, b->d_cnt);
     IntegArgs io{nb, nullptr, nullptr, nullptr, nullptr, g + 1, b->d_ovf, g + 2, b->d_wsfull, QCAP_FULL, -1,
                  b->d_ctr, g, b->d_pv, b->d_pmodel};
     kb_integrals_ovf<<<b->ovf_blocks, 256, 20 * 256 * sizeof(double), s>>>(io);
