@@ -1620,7 +1620,7 @@ __global__ void __launch_bounds__(256) kb_points(const ModelDev *md, const int *
 
 // Node setup of every model; model m owns blocks [bstart[m], bstart[m+1]).
 template <int BS>
-__global__ void __launch_bounds__(BS) kb_setup(const ModelDev *md, const int *bstart, int M, SetupArgs base,
+__global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(FJ_SETUP_WAVES))) kb_setup(const ModelDev *md, const int *bstart, int M, SetupArgs base,
                                                unsigned int *cnt) {
     extern __shared__ double smem[];
     const int m = find_seg(bstart, M, blockIdx.x);
