@@ -115,14 +115,19 @@ __device__ inline LdsTabs stage_tables(const TabArgs &a, double *smem) {
 // FP32 copies of the radial grid, Phi_l table and even-Legendre step
 // constants for the mixed-precision integrand, after the FP64 tables.
 __host__ __device__ inline size_t tab32_lds_floats(int nr, int npoly) {
-    return (size_t)nr + (size_t)nr * npoly + 3 * MAXPOLY;
+    return (size_t)2 * nr + (size_t)nr * npoly + 3 * MAXPOLY;
 }
 
 __device__ inline PotView32 stage_tables32(const LdsTabs &L, float *f) {
     const int nr = L.T.nr, np = L.T.npoly;
     float *ar = f, *phil = ar + nr, *la = phil + nr * np, *lb = la + MAXPOLY, *lg = lb + MAXPOLY;
+    float *lo = lg + MAXPOLY;
     for (int i = threadIdx.x; i < nr; i += blockDim.x) ar[i] = (float)L.T.ar[i];
     for (int i = threadIdx.x; i < nr * np; i += blockDim.x) phil[i] = (float)L.T.phil[i];
+    for (int i = threadIdx.x; i < nr; i += blockDim.x) {
+        const double p0 = L.T.phil[i * np];
+        lo[i] = (float)(p0 - (double)(float)p0);
+    }
     for (int i = threadIdx.x; i < MAXPOLY; i += blockDim.x) {
         const LegCoef *lc = L.T.lc;
         la[i] = i >= 2 ? (float)(-lc->el[i] * lc->elm1[i] / lc->ed[i]) : 0.f;
@@ -131,7 +136,7 @@ __device__ inline PotView32 stage_tables32(const LdsTabs &L, float *f) {
     }
     __syncthreads();
     PotView32 T;
-    T.nr = nr; T.npoly = np; T.ar = ar; T.phil = phil; T.la = la; T.lb = lb; T.lg = lg;
+    T.nr = nr; T.npoly = np; T.ar = ar; T.phil = phil; T.phil0lo = lo; T.la = la; T.lb = lb; T.lg = lg;
     T.lut = L.T.lut; T.lg0 = L.T.lg0; T.lsc = L.T.lsc;
     return T;
 }
@@ -181,6 +186,17 @@ __device__ inline void load_jint(const NodeBuf &nb, long long i, int type, JInt3
     f.type = type;
 #pragma unroll
     for (int m = 0; m < JK; m++) f.k[m] = (float)g.k[m];
+    // energy shift (JInt32): c = fl(E)
+    const float c = (float)g.k[1];
+    const double cd = (double)c;
+    f.c = c;
+    f.k[1] = (float)(g.k[1] - cd);
+    if (type == 0) {
+        f.k[9] = (float)(g.k[9] - cd * (g.k[7] + g.k[6]));      // (sh1sq + sv2) (Phi(u1) - c)
+    } else {
+        f.k[6] = (float)(g.k[6] + cd);                          // Iv at E - c
+        f.k[7] = (float)(g.k[7] - cd * (1 + g.k[5]));           // (1 + shu2) (Phi(pi/2) - c)
+    }
     const double rtwo = 1.4142135623730951;
     f.pre = type == 0 ? rtwo * g.k[0] * g.k[3] : 2 * rtwo * g.k[0] * (PIH - g.k[2]);
 }
@@ -1020,24 +1036,104 @@ struct IExtT {   // QAGIU transform of I_external: x = a + (1-t)/t, (f(x)/t)/t
     }
 };
 
-// computeInts + computePhil (Potential.cpp:110-200), one lane per (n, np).
-__global__ void k_poisson(int nr, int np_, const double *ar, const double *rhl, double *ws, int cap,
-                          double *philP, double *PrP, double *Pr2P, int *status) {
-    const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+// Kronrod rule of qk_rule with the integrand spread over the lanes of one
+// wave: lane s < 2(NK-1) evaluates the off-centre abscissa s of qk_rule's
+// visiting order, lane 2(NK-1) the centre; the values meet in `sh` (LDS, one
+// wave per block) and every lane then takes the sums in qk_rule's order, so
+// the result carries qk_rule's bits and is the same in all lanes.
+template <int NK, class F>
+__device__ __forceinline__ QKOut qk_rule_wave(const F &f, double a, double b, double *sh) {
+    const double *xgk = NK == 11 ? kXGK21 : kXGK15;
+    const double *wgk = NK == 11 ? kWGK21 : kWGK15;
+    const double *wg = NK == 11 ? kWG10 : kWG7;
+    constexpr int NG = (NK - 1) / 2;
+    constexpr int NV = 2 * (NK - 1);
+    const double center = 0.5 * (a + b), half = 0.5 * (b - a), ahalf = fabs(half);
+    const int t = threadIdx.x;
+    if (t <= NV) {
+        double x = center;
+        if (t < NV) {
+            const int m = t >> 1;
+            const int jj = m < NG ? 2 * m + 1 : 2 * (m - NG);
+            const double absc = half * xgk[jj];
+            x = (t & 1) ? center + absc : center - absc;
+        }
+        sh[t] = f(x);
+    }
+    __syncthreads();
+    const double fc = sh[NV];
+    double rg = 0, rk = fc * wgk[NK - 1], rabs = fabs(rk);
+    if (NK % 2 == 0) rg = fc * wg[NK / 2 - 1];
+#pragma unroll
+    for (int m = 0; m < NK - 1; m++) {
+        const int jj = m < NG ? 2 * m + 1 : 2 * (m - NG);
+        const double f1 = sh[2 * m], fx = sh[2 * m + 1];
+        const double fs = f1 + fx;
+        if (m < NG) rg += wg[m] * fs;
+        rk += wgk[jj] * fs;
+        rabs += wgk[jj] * (fabs(f1) + fabs(fx));
+    }
+    const double mean = rk * 0.5;
+    double rasc = wgk[NK - 1] * fabs(fc - mean);
+#pragma unroll
+    for (int j = 0; j < NK - 1; j++) {
+        const int m = (j & 1) ? (j >> 1) : NG + (j >> 1);
+        rasc += wgk[j] * (fabs(sh[2 * m] - mean) + fabs(sh[2 * m + 1] - mean));
+    }
+    __syncthreads();   // sh is rewritten by the next rule
+    const double err = (rk - rg) * half;
+    QKOut o;
+    o.result = rk * half;
+    o.resabs = rabs * ahalf;
+    o.resasc = rasc * ahalf;
+    o.abserr = qk_rescale(err, o.resabs, o.resasc);
+    return o;
+}
+
+// qags_solve with qk_rule_wave.  Every lane of the wave runs the same
+// (uniform) QAGS state; the workspace stores are the same value at the same
+// address from all lanes.
+template <int NK, class F>
+__device__ __forceinline__ double qags_solve_wave(const F &f, double a, double b, double epsabs, double epsrel,
+                                                  int limit, const QagsWork &w, int cap, double *sh,
+                                                  int *ier = nullptr) {
+    Qags q;
+    qags_begin(q, a, b, epsabs, epsrel, limit);
+    while (q.state != 3) {
+        const QKOut r = qk_rule_wave<NK>(f, q.next_a, q.next_b, sh);
+        qags_consume(q, w, cap, r);
+    }
+    if (ier) *ier = q.overflow ? -1 : q.ier;
+    return q.result;
+}
+
+// computeInts + computePhil (Potential.cpp:110-200), one wave per (n, np):
+// the 21 (QK21, internal integral) or 15 (QK15 of QAGIU's transform,
+// external integral) integrand values of every rule are taken by the wave's
+// lanes at once; the QAGS bookkeeping is the lane-serial one, so the
+// coefficients are those of one lane per integral, bit for bit.
+__global__ void __launch_bounds__(64) k_poisson(int nr, int np_, const double *ar, const double *rhl, double *ws,
+                                                int cap, double *philP, double *PrP, double *Pr2P, int *status) {
+    __shared__ double sh[32];
+    const int idx = blockIdx.x;
     if (idx >= nr * np_) return;
     const int n = idx / np_, np = idx % np_;
     const int l = 2 * np;
-    const QagsWork w = lane_work(ws, cap, idx, (long long)gridDim.x * blockDim.x);
+    const QagsWork w = lane_work(ws, cap, idx, (long long)gridDim.x);
     IInt fi{ar, rhl, nr, np_, np, l};
-    const double Iint = qags_solve<11>(fi, 0., ar[n], 1e-4, 1e-3, 1000, w, cap);
+    const double Iint = qags_solve_wave<11>(fi, 0., ar[n], 1e-4, 1e-3, 1000, w, cap, sh);
     IExtT fe{fi, ar[n]};
     double epsabs = 1e-4, epsrel = 1e-3, Iext = 0;
     int ier = 1, guard = 0;
     while (ier != 0) {
-        Iext = qags_solve<8>(fe, 0.0, 1.0, epsabs, epsrel, 1000, w, cap, &ier);
+        Iext = qags_solve_wave<8>(fe, 0.0, 1.0, epsabs, epsrel, 1000, w, cap, sh, &ier);
         epsabs *= 2; epsrel *= 2;
-        if (isnan(Iext) || ++guard > 200) { atomicOr(status, 1); break; }
+        if (isnan(Iext) || ++guard > 200) {
+            if (threadIdx.x == 0) atomicOr(status, 1);
+            break;
+        }
     }
+    if (threadIdx.x != 0) return;
     const double G = 1, rn = ar[n];
     philP[idx] = -TPI * G * (Iint / fjm_powi(rn, l + 1) + Iext * fjm_powi(rn, l));
     PrP[idx] = -TPI * G * (-(l + 1) * Iint / fjm_powi(rn, l + 2) + l * Iext * fjm_powi(rn, l - 1));
@@ -2332,8 +2428,8 @@ int run_poisson(fj_ctx *c, int mode, const double *d_rhl, double *d_phil, double
     }
     HIPCHK(c, hipMemsetAsync(c->d_status, 0, sizeof(int), c->stream));
     HIPCHK(c, hipEventRecord(a, c->stream));
-    k_poisson<<<(n + 63) / 64, 64, 0, c->stream>>>(c->nr, c->npoly, c->d_ar, d_rhl, c->d_pws, QCAP_FULL, d_phil,
-                                                   d_Pr, d_Pr2, c->d_status);
+    k_poisson<<<n, 64, 0, c->stream>>>(c->nr, c->npoly, c->d_ar, d_rhl, c->d_pws, QCAP_FULL, d_phil, d_Pr, d_Pr2,
+                                       c->d_status);
     HIPCHK(c, hipGetLastError());
     HIPCHK(c, hipEventRecord(b, c->stream));
     int status = 0;

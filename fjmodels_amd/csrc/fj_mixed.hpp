@@ -17,15 +17,22 @@ namespace fj {
 
 // Potential tables in FP32 (LDS).  Even Legendre step as
 // P_2np = (la[np] P_2np-4 + P_2np-2 (c^2 - lb[np])) lg[np]  (UtilsLeg.h:68-79).
+// The monopole column is held as a float pair: phil[n][0] + phil0lo[n] is the
+// FP64 value to 2^-48, so that a potential difference Phi - c with c close to
+// Phi keeps FP32 relative accuracy.
 struct PotView32 {
     int nr, npoly;
-    const float *ar, *phil;
+    const float *ar, *phil, *phil0lo;
     const float *la, *lb, *lg;
     const int *lut;    // bracket table of PotView (fj_device.hpp)
     float lg0, lsc;
 };
 
-__device__ __forceinline__ float pot_phi32(const PotView32 &T, float R, float z) {
+// Phi(R, z) - shift.  The shift enters through the monopole, whose
+// interpolation runs on differences of the float-pair table, so that at the
+// bottom of a deep potential (shift close to Phi, E - Phi << |Phi|) the
+// result carries no cancellation.
+__device__ __forceinline__ float pot_phi32(const PotView32 &T, float R, float z, float shift) {
     const float r2 = R * R + z * z;
     const float rs = sqrtf(r2);
     const float c = z / rs;
@@ -36,8 +43,9 @@ __device__ __forceinline__ float pot_phi32(const PotView32 &T, float R, float z)
     if (r >= ar[nr - 1]) {
         // Keplerian tail: phil[nr-1][l] (ar[nr-1]/r)^(l+1), l = 2np
         const float q = ar[nr - 1] / r, q2 = q * q;
-        float qp = q;
-        for (int np = 0; np < np_; np++) {
+        coef[0] = fmaf(T.phil[(nr - 1) * np_], q, -shift) + T.phil0lo[nr - 1] * q;
+        float qp = q * q2;
+        for (int np = 1; np < np_; np++) {
             coef[np] = T.phil[(nr - 1) * np_ + np] * qp;
             qp *= q2;
         }
@@ -58,7 +66,17 @@ __device__ __forceinline__ float pot_phi32(const PotView32 &T, float R, float z)
             f2 = (r - ar[top]) * db / ((ar[thr] - ar[top]) * (ar[thr] - ar[bot]));
             f3 = (ar[thr] - ar[bot]) / (ar[top] - ar[bot]);
         }
-        for (int np = 0; np < np_; np++) {
+        {
+            // monopole minus shift: b - shift + f1 (t - b) + f2 (h - b - f3 (t - b))
+            const float *lo = T.phil0lo;
+            const float b = T.phil[bot * np_];
+            const float bc = (b - shift) + lo[bot];
+            const float tb = (T.phil[top * np_] - b) + (lo[top] - lo[bot]);
+            float v = bc + f1 * tb;
+            if (quad) v += f2 * (((T.phil[thr * np_] - b) + (lo[thr] - lo[bot])) - f3 * tb);
+            coef[0] = v;
+        }
+        for (int np = 1; np < np_; np++) {
             const float t = T.phil[top * np_ + np], b = T.phil[bot * np_ + np];
             float v = b + f1 * (t - b);
             if (quad) v += f2 * (T.phil[thr * np_ + np] - b - f3 * (t - b));
@@ -84,10 +102,15 @@ __device__ __forceinline__ void sinhcosh32(float u, float *s, float *c) {
     *c = 0.5f * (ep + 1.f / ep);
 }
 
-// Juint / Jvint in FP32 with JInt's packing of the node constants.
+// Juint / Jvint in FP32 with JInt's packing of the node constants, with the
+// energies shifted by c = fl(E): k[1] = E - c and the potential constants
+// (K2 of Ju; Iv, KV of Jv) taken at Phi - c, so that every term of the
+// integrand is of the size of E - Phi (the integrands are invariant under
+// the common shift of E and Phi).
 struct JInt32 {
     const PotView32 *T;
     float k[JK];
+    float c;
     double pre;   // FP64 prefactor of the action: sqrt(2) Delta Deltau, or 2 sqrt(2) Delta (pi/2 - vturn)
     int type;
     __device__ double operator()(double psi_d) const {
@@ -101,7 +124,7 @@ struct JInt32 {
             float shu, chu;
             sinhcosh32(u, &shu, &chu);
             const float shu2 = shu * shu;
-            const float Phi = pot_phi32(*T, D * shu * k[4], D * chu * k[5]);
+            const float Phi = pot_phi32(*T, D * shu * k[4], D * chu * k[5], c);
             const float dU = (shu2 + k[6]) * Phi - k[9];
             F = E * (shu2 - k[7]) + k[8] - dU - k[10] * (1.f / shu2 - k[11]);
         } else {
@@ -109,7 +132,7 @@ struct JInt32 {
             float s, cs;
             sincosf(v, &s, &cs);
             const float sv2 = s * s;
-            const float Phi = pot_phi32(*T, D * k[3] * s, D * k[4] * cs);
+            const float Phi = pot_phi32(*T, D * k[3] * s, D * k[4] * cs, c);
             const float dV = k[7] - (k[5] + sv2) * Phi;
             F = E * sv2 + k[6] + dV - .5f * k[8] / (D * D * sv2);
         }
