@@ -326,11 +326,35 @@ __device__ __forceinline__ void setup_node(const LdsTabs &L, const SetupArgs &sa
 #ifndef FJ_SETUP_WAVES
 #define FJ_SETUP_WAVES 4   // waves per SIMD the register allocation of k_setup aims at
 #endif
+
+// Lane order of k_setup over the npts x NQ nodes of a chunk: tiles of 64
+// grid points x NQ nodes, and inside a tile lane r takes velocity node
+// r / 64 of point r % 64 (of the tile's points, fewer in the last tile).  A
+// wave then follows one velocity node, in units of the local escape speed,
+// at 64 neighbouring grid points: orbits of like shape, whose Brent
+// iteration counts are alike.  The node records stay point-major (node
+// p * NQ + q), so this changes which lane computes a node, never a result.
+#ifndef FJ_SETUP_TILED
+#define FJ_SETUP_TILED 1
+#endif
+__device__ inline long long setup_lane_node(long long tid, int npts) {
+    if (tid >= (long long)npts * NQ) return tid;          // past the last node: stays invalid
+    const long long tile = 64ll * NQ;
+    const long long T = tid / tile, r = tid - T * tile;
+    const int left = npts - (int)T * 64;
+    const int tp = left < 64 ? left : 64;                   // points in this tile, >= 1
+    const long long q = r / tp, pt = r - q * tp;            // q < NQ, pt < tp
+    return (T * 64 + pt) * NQ + q;
+}
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(FJ_SETUP_WAVES)))
 k_setup(TabArgs ta, SetupArgs sa) {
     extern __shared__ double smem[];
     const LdsTabs L = stage_tables(ta, smem);
-    setup_node(L, sa, (long long)blockIdx.x * blockDim.x + threadIdx.x);
+    long long tid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+#if FJ_SETUP_TILED
+    if (!sa.sel && !sa.vR) tid = setup_lane_node(tid, sa.npts);
+#endif
+    setup_node(L, sa, tid);
 }
 
 // df_hg (DF.cpp:65-76) and the rotation term (DF.cpp:114-119)
