@@ -205,7 +205,7 @@ def surrogate_leg(ctx, model, guess_tables, tol=1e-4, max_iter=30, keep=0.15, ep
 
 
 def batch_leg(ctx, n_models=8, iters=2, seed=5):
-    """Batched sweep (a bounded sample of the 256-model sweep): n_models
+    """Batched models against the same models one by one: n_models
     single-component models of random type (Hernquist, Isochrone, NFW), DF
     parameters (dphi, dz in [0.3, 1.2], chi in [0, 2], README rotation form)
     and uneven grids (nr in 24..40, ntheta in 12..20, l_max 8..12), mixed
@@ -268,6 +268,32 @@ def batch_leg(ctx, n_models=8, iters=2, seed=5):
             "precision": "mixed (FP32 integrands/DF, FP64 roots/Delta)", "poisson": "blockscan"}
 
 
+def sweep_leg(ctx, n_models=32, seed=0, iters=2):
+    """Model sweep (a bounded sample of the 256-model sweep): the first
+    n_models of the seeded draw of fjmodels_amd.draw_models (types uniform
+    over Hernquist/Isochrone/NFW, dphi, dz in [0.3, 1.2], chi in [0, 2],
+    nr in 24..96, ntheta in 12..32, l_max 8/10/12), mixed precision node
+    phase, block-scan Poisson, packed into batches of at most 64 M nodes.
+    The timed iterations are graph replays after the first (capture)."""
+    import fjmodels_amd as fj
+    from fjmodels_amd import sweep as S
+    full = fj.draw_models(256, seed=seed)
+    full_pts = sum(len(m["ar"]) * m["ntheta"] for m in full)
+    ctx.set_precision(fj.FJ_PREC_MIXED)
+    try:
+        def progress(done, total, sec):
+            print("sweep: batch %d of %d done, %.1f s" % (done, total, sec), file=sys.stderr, flush=True)
+        _, st = fj.Sweep(ctx, full[:n_models]).run(iters, progress)
+    finally:
+        ctx.set_precision(fj.FJ_PREC_FP64)
+    st["unit"] = "grid-pt DF moments/s"
+    st["full_sweep"] = {"models": 256, "grid_points": full_pts, "nodes": full_pts * S.NODES_PER_POINT,
+                        "batches": len(S.partition(full))}
+    st["precision"] = "mixed (FP32 integrands/DF, FP64 roots/Delta)"
+    st["poisson"] = "blockscan"
+    return st
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -277,6 +303,8 @@ def main():
     ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision / block-scan leg")
     ap.add_argument("--no-surrogate", action="store_true", help="skip the plain vs surrogate convergence leg")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-models leg")
+    ap.add_argument("--sweep", type=int, default=32,
+                    help="models of the seeded 256-model sweep draw to run (0: skip the sweep leg)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run a 1 GiB device-to-device copy (reference for FETCH_SIZE/WRITE_SIZE counters)")
     args = ap.parse_args()
@@ -345,6 +373,8 @@ def main():
         line["surrogate"] = surrogate_leg(ctx, model, guess_tables)
     if not args.no_batch:
         line["batch"] = batch_leg(ctx)
+    if args.sweep > 0:
+        line["sweep"] = sweep_leg(ctx, args.sweep)
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(npoly, ntheta, nr, ar, guess_tables, delta)
     if args.copy_calibration:
