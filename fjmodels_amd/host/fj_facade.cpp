@@ -167,6 +167,11 @@ fJParams readParam(const char *fName) {
             if (s == "quadpack") P.poisson = FJ_POISSON_QUADPACK;
             else if (s == "blockscan") P.poisson = FJ_POISSON_BLOCKSCAN;
             else throw Error(FJ_E_ARG, "poisson must be quadpack or blockscan");
+        } else if (key == "precision") {
+            const std::string s = str();
+            if (s == "fp64") P.precision = FJ_PREC_FP64;
+            else if (s == "mixed") P.precision = FJ_PREC_MIXED;
+            else throw Error(FJ_E_ARG, "precision must be fp64 or mixed");
         }
     }
     checkParameterFile(&P);
@@ -242,6 +247,8 @@ fj_ctx *context() { return ctx(); }
 void set_quad_rule(int preset) { check(fj_set_quad_rule(ctx(), preset), "fj_set_quad_rule"); }
 
 void set_poisson_mode(int mode) { g_poisson = mode; }
+
+void set_precision(int mode) { check(fj_set_precision(ctx(), mode), "fj_set_precision"); }
 
 void SetModel(const fJParams &f, unsigned comp) {
     if (comp == 1) { g_mass = f.mass; g_r0 = f.r0; g_q = f.q; }

@@ -57,6 +57,8 @@ struct fJParams {
     //   tol: stop when the potential changes by less than tol
     //        (mergePhi's distance, include/UtilsLeg.h:266-278); 0 = run itermax
     //   poisson quadpack|blockscan
+    //   precision fp64|mixed: node phase in FP64 (default) or with FP32
+    //        action integrands and DF (fj_set_precision)
     //   merge f: after each computeNewPhi, Phi <- (1+f) Phi - f Phi_old
     //        (mergePhi, include/UtilsLeg.h:266-278; one component); 0 = off
     //   delta_refresh 1: tabulate Delta again from each new potential (the
@@ -69,6 +71,7 @@ struct fJParams {
     double rmin = UNSET, rmax = 200.;
     int quad = FJ_QUAD_LOG22;
     int poisson = FJ_POISSON_QUADPACK;
+    int precision = FJ_PREC_FP64;
     double tol = 0;
     double merge = 0;
     int delta_refresh = 0;
@@ -101,6 +104,8 @@ fj_ctx *context();
 void set_quad_rule(int preset);
 // Poisson solver of computePhil / computeNewPhi (FJ_POISSON_QUADPACK default).
 void set_poisson_mode(int mode);
+// Arithmetic of the node phase (FJ_PREC_FP64 default, FJ_PREC_MIXED).
+void set_precision(int mode);
 
 // SetModel (pot/models.cpp:16-25): mass, r0, q of component comp.
 void SetModel(const fJParams &fJP, unsigned comp = 1);

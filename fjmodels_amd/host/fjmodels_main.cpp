@@ -154,6 +154,7 @@ int main(int argc, char **argv) {
         SetGrid(P.rmax, P.nr, P.ntheta, P.npoly, P.ngrid, P.rmin > 0 ? P.rmin : 0, device);
         set_quad_rule(P.quad);
         set_poisson_mode(P.poisson);
+        set_precision(P.precision);
         mkdir(dir.c_str(), 0755);
         printf("=\n======================================================================\n");
         if (components == 1) oneComp(P, dir, R);
