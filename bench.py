@@ -351,7 +351,7 @@ def main():
         "value": value, "unit": "grid-pt DF moments/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms,
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "f64", "data": "synthetic (Hernquist guess potential, cfg grid)",
+        "dtype": "f64", "data": "synthetic (Hernquist guess potential, 40 x 20 grid)",
         "config": {"workload": "single-component Hernquist DPL f(J), 40 log radii [0.01,100] x 20 GL cos(theta), "
                                "l<=8, 22^3 velocity nodes/point, guess potential + Delta table, SCF iterations",
                    "nr": nr, "ntheta": ntheta, "npoly": npoly, "nodes_per_point": 10648},

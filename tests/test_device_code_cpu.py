@@ -55,7 +55,7 @@ def H():
 @pytest.fixture(scope="module")
 def models():
     out = {}
-    out["cfg1"] = O.guess_model(40, 0.01, 100.0, 5, 20)[0]
+    out["hq40"] = O.guess_model(40, 0.01, 100.0, 5, 20)[0]
     out["small"] = O.guess_model(12, 0.05, 50.0, 3, 4)[0]
     return out
 
@@ -66,7 +66,7 @@ def tabs(p):
     return phil, Pr, E, D
 
 
-@pytest.mark.parametrize("name", ["cfg1", "small"])
+@pytest.mark.parametrize("name", ["hq40", "small"])
 def test_potential_bitwise(H, models, name):
     p = models[name]
     phil, Pr, _, _ = tabs(p)
@@ -90,14 +90,14 @@ def point_nodes(p, R, z):
     return Phi, V[b].copy(), np.array([p.deltafn(h) for h in H_[b]])
 
 
-@pytest.mark.parametrize("name,pts", [("cfg1", [(0.866, 0.5), (60.0, 40.0), (0.011, 0.004)]),
+@pytest.mark.parametrize("name,pts", [("hq40", [(0.866, 0.5), (60.0, 40.0), (0.011, 0.004)]),
                                       ("small", [(47.198880926701165, 16.50047391037859), (0.9, 2.0)])])
 def test_actions_bitwise(H, models, name, pts):
     p = models[name]
     phil, Pr, E, D = tabs(p)
     for R, z in pts:
         Phi, V, Dl = point_nodes(p, R, z)
-        if name == "cfg1":   # a sample keeps the CPU time down
+        if name == "hq40":   # a sample keeps the CPU time down
             V, Dl = V[::7].copy(), Dl[::7].copy()
         n = len(V)
         Jr, Jz = np.zeros(n), np.zeros(n)
@@ -110,7 +110,7 @@ def test_actions_bitwise(H, models, name, pts):
 
 
 @pytest.mark.parametrize("name,R,z", [("small", 47.198880926701165, 16.50047391037859), ("small", 2.0, 1.0),
-                                      ("cfg1", 0.866, 0.5)])
+                                      ("hq40", 0.866, 0.5)])
 def test_rhofDF_sums_bitwise(H, models, name, R, z):
     p = models[name]
     phil, Pr, E, D = tabs(p)

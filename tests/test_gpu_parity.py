@@ -24,17 +24,17 @@ def relerr(a, b, floor=0.0):
 
 
 @pytest.fixture(scope="module")
-def cfg1():
-    """cfg1 grid (40 log radii in [0.01,100], 20 theta nodes, l<=8) with the
+def hq40():
+    """Hernquist grid (40 log radii in [0.01,100], 20 theta nodes, l<=8) with the
     Hernquist guess potential and the oracle's Delta table."""
     p, rhl = O.guess_model(40, 0.01, 100.0, 5, 20)
     return p, rhl
 
 
 @pytest.fixture(scope="module")
-def gctx(ctx, cfg1):
+def gctx(ctx, hq40):
     import fjmodels_amd as fj
-    p, _ = cfg1
+    p, _ = hq40
     ctx.set_grid(p.ar, p.ngauss, p.npoly, p.ngrid)
     ctx.set_potential(*p.tables())
     ctx.set_delta(*p.delta())
@@ -43,8 +43,8 @@ def gctx(ctx, cfg1):
     return ctx
 
 
-def test_potential_eval(gctx, cfg1):
-    p, _ = cfg1
+def test_potential_eval(gctx, hq40):
+    p, _ = hq40
     rng = np.random.default_rng(1)
     r = 10 ** rng.uniform(-3.5, 3, 400)
     th = rng.uniform(0, np.pi, 400)
@@ -57,8 +57,8 @@ def test_potential_eval(gctx, cfg1):
     assert np.array_equal(phi, ref[:, 0]) and np.array_equal(dR, ref[:, 1]) and np.array_equal(dz, ref[:, 2])
 
 
-def test_guess_and_poisson(gctx, cfg1):
-    p, rhl = cfg1
+def test_guess_and_poisson(gctx, hq40):
+    p, rhl = hq40
     g = gctx.guess_rhl("Hernquist")
     # l > 0 coefficients of a spherical model are rounding noise: scale by the monopole
     scale = np.abs(rhl[:, :1])
@@ -71,8 +71,8 @@ def test_guess_and_poisson(gctx, cfg1):
         assert np.array_equal(a, b), e
 
 
-def test_actions_sample(gctx, cfg1):
-    p, _ = cfg1
+def test_actions_sample(gctx, hq40):
+    p, _ = hq40
     rng = np.random.default_rng(2)
     n = 400
     r = 10 ** rng.uniform(-2, 2, n)
@@ -101,8 +101,8 @@ def moment_errors(got, ref):
     return e
 
 
-def test_rhofDF_points(gctx, cfg1):
-    p, _ = cfg1
+def test_rhofDF_points(gctx, hq40):
+    p, _ = hq40
     R = np.array([0.866, 0.05, 10.0, 60.0])
     z = np.array([0.5, 0.02, 3.0, 40.0])
     out6, mom = gctx.rhofDF(R, z)
@@ -119,10 +119,10 @@ def test_rhofDF_points(gctx, cfg1):
     assert worst < RTOL
 
 
-def test_all_nodes_of_one_point(gctx, cfg1):
+def test_all_nodes_of_one_point(gctx, hq40):
     """Every velocity node of one grid point: GPU actions vs oracle actions at
     the same Delta; reports how many nodes differ beyond 1e-9."""
-    p, _ = cfg1
+    p, _ = hq40
     R, z = 0.866, 0.5
     Phi = p.phi(R, z)
     far = 10 * p.ar[-1]
@@ -147,12 +147,12 @@ def test_all_nodes_of_one_point(gctx, cfg1):
     assert len(bad) == 0
 
 
-def test_delta_table(gctx, cfg1):
+def test_delta_table(gctx, hq40):
     """tabulateDelta is ill-conditioned: 1-ulp perturbations of the potential
     tables move the oracle's own Delta^2 entries by up to ~30% (the shell-orbit
     ellipse fit depends on where the adaptive RK8PD samples the orbit); the
     bit-reproducible arithmetic makes the GPU table equal to the oracle's."""
-    p, _ = cfg1
+    p, _ = hq40
     E, D = gctx.tabulate_delta()
     rE, rD = p.delta()
     d = np.abs(D / rD - 1)
