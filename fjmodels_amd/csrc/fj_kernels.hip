@@ -1597,11 +1597,14 @@ __device__ inline void delta_prep(const PotView &T, double *prep) {
 __device__ inline void delta_energy(const PotView &T, int NG, const double *prep, double *scratch, int nE,
                                     double *Eout, double *Dout, int *status);
 
+// One energy per wave (block of 64, lane 0 works): the energies' nested
+// Brent / RK8PD loops take different trip counts, and lanes of one wave
+// would wait for each other at every loop level.
 __global__ void k_delta(TabArgs ta, const double *prep, double *scratch, double *Eout, double *Dout, int *status) {
     extern __shared__ double smem[];
     const LdsTabs L = stage_tables(ta, smem);
-    const int nE = blockIdx.x * blockDim.x + threadIdx.x;
-    if (nE >= L.ngrid) return;
+    const int nE = blockIdx.x;
+    if (threadIdx.x != 0 || nE >= L.ngrid) return;
     delta_energy(L.T, L.ngrid, prep, scratch + (size_t)nE * 400, nE, Eout, Dout, status);
 }
 
@@ -1947,10 +1950,11 @@ __global__ void kb_delta_prep(const ModelDev *md, const PotView *pv, int M) {
     if (m < M) delta_prep(pv[m], md[m].prep);
 }
 
+// One (model, energy) per wave, as k_delta.
 __global__ void kb_delta(const ModelDev *md, const PotView *pv, int M, double *scratch, int *status) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = blockIdx.x;
     const int NG = md[0].ta.ngrid;
-    if (i >= M * NG) return;
+    if (threadIdx.x != 0 || i >= M * NG) return;
     const int m = i / NG, nE = i % NG;
     delta_energy(pv[m], NG, md[m].prep, scratch + (size_t)i * 400, nE, const_cast<double *>(md[m].ta.E),
                  const_cast<double *>(md[m].ta.D), status);
@@ -3052,7 +3056,7 @@ int fj_tabulate_delta(fj_ctx *c, double *Egrid, double *Dgrid) {
     k_delta_prep<<<1, 64, tab_bytes(c), c->stream>>>(ta, prep);
     hipError_t e = hipGetLastError();
     if (e == hipSuccess) {
-        k_delta<<<(NG + 63) / 64, 64, tab_bytes(c), c->stream>>>(ta, prep, scr, E, D, dst);
+        k_delta<<<NG, 64, tab_bytes(c), c->stream>>>(ta, prep, scr, E, D, dst);
         e = hipGetLastError();
     }
     int status = 0;
@@ -3620,7 +3624,7 @@ int fj_batch_setup(fj_batch *b) {
     HIPCHK(c, hipMalloc((void **)&scr, (size_t)lanes * 400 * sizeof(double)));
     HIPCHK(c, hipMalloc((void **)&dst, sizeof(int)));
     HIPCHK(c, hipMemsetAsync(dst, 0, sizeof(int), c->stream));
-    kb_delta<<<(lanes + 63) / 64, 64, 0, c->stream>>>(b->d_md, b->d_pv, b->M, scr, dst);
+    kb_delta<<<lanes, 64, 0, c->stream>>>(b->d_md, b->d_pv, b->M, scr, dst);
     HIPCHK(c, hipGetLastError());
     HIPCHK(c, hipEventRecord(e1, c->stream));
     int status = 0;
