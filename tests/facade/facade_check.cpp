@@ -21,6 +21,7 @@ int main(int argc, char **argv) {
         printf("A2 %.17g B2 %.17g mass2 %.17g\n", P.alpha_2, P.beta_2, P.mass_2);
         printf("grid %d %d %d %d %.17g %.17g\n", P.nr, P.ntheta, P.npoly, P.ngrid, P.rmin, P.rmax);
         printf("rot %d %.17g tol %.17g quad %d\n", P.rot_form, P.rot_k, P.tol, P.quad);
+        printf("modes poisson %d precision %d\n", P.poisson, P.precision);
         printf("name1 %s\n", composeName(P, 3, 1).c_str());
         printf("name2 %s\n", composeName(P, 3, 2).c_str());
         SetGrid(P.rmax, P.nr, P.ntheta, P.npoly, P.ngrid, P.rmin > 0 ? P.rmin : 0);

@@ -37,6 +37,8 @@ rmin 0.05
 rmax 50
 rot_form 1
 tol 1e-5
+poisson blockscan
+precision mixed
 quad tanhsinh22
 
 #2:model NFW
@@ -80,6 +82,17 @@ def test_param_file(run):
     assert kv["A2"].split() == ["0", "B2", "2", "mass2", "2"]
     assert kv["grid"].split() == ["12", "4", "3", "30", "0.050000000000000003", "50"]
     assert kv["rot"].split()[0] == "1" and float(kv["rot"].split()[3]) == 1e-5
+    assert kv["modes"].split() == ["poisson", "1", "precision", "1"]     # FJ_POISSON_BLOCKSCAN, FJ_PREC_MIXED
+
+
+def test_bad_mode_keys_are_rejected(tmp_path):
+    subprocess.run(["make", "-s", "-C", FAC], check=True)
+    for line in ("precision half", "poisson simpson"):
+        pf = tmp_path / "p.txt"
+        pf.write_text("model Hernquist\nchi 0\n" + line + "\n")
+        r = subprocess.run([os.path.join(FAC, "facade_check"), str(pf), str(tmp_path)], capture_output=True,
+                           text=True)
+        assert r.returncode != 0 and "must be" in (r.stdout + r.stderr), line
 
 
 def test_compose_name(run):
