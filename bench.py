@@ -365,7 +365,8 @@ def main():
                       ("ms_points", "ms_setup", "ms_integrals", "ms_overflow", "ms_moments", "ms_project",
                        "ms_poisson")},
         "counters": {k: int(np.mean([s[k] for s in stats])) for k in
-                     ("bound_nodes", "tasks", "qk_calls", "overflow_tasks", "brent_iters", "integral_lanes")},
+                     ("bound_nodes", "tasks", "qk_calls", "overflow_tasks", "brent_iters", "integral_lanes",
+                      "unbound_nodes", "bracket_failures", "nonfinite_nodes")},
     }
     if not args.no_mixed:
         line["mixed"] = mixed_leg(ctx, model, prev, last, args.steps, pts)

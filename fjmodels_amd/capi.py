@@ -49,7 +49,8 @@ class Stats(C.Structure):
                                            "ms_moments", "ms_project", "ms_poisson", "ms_total")] +
                 [(n, C.c_longlong) for n in ("points", "nodes", "bound_nodes", "tasks", "qk_calls",
                                              "overflow_tasks", "brent_iters")] +
-                [("integral_lanes", C.c_int), ("setup_lanes", C.c_int)])
+                [("integral_lanes", C.c_int), ("setup_lanes", C.c_int)] +
+                [(n, C.c_longlong) for n in ("unbound_nodes", "bracket_failures", "nonfinite_nodes")])
 
 
 class SurrogateStats(C.Structure):

@@ -217,6 +217,9 @@ struct FvStridedF {
 // Device counters
 struct Counters {
     unsigned long long bound, tasks_u, tasks_v, qk, overflow, brent, grad;
+    unsigned long long unbound;        // E >= 0 after the spheroidal transform: Jr = Jz = 1e31
+    unsigned long long bracket_fail;   // a turning-point bracket walk stopped by its iteration guard
+    unsigned long long nonfinite;      // nodes whose f(J) is NaN or infinite
 };
 
 // Wave-aggregated append to a task list.
@@ -292,7 +295,7 @@ __device__ __forceinline__ void setup_node(const LdsTabs &L, const SetupArgs &sa
     const long long gid = sa.sel ? (valid ? (long long)sa.sel[tid] : 0) : tid;
     bool want_u = false, want_v = false;
     long long nev = 0;
-    int brent_it = 0;
+    int brent_it = 0, flag = 0, bfail = 0;
     if (valid) {
         const int pl = explicit_v ? (int)gid : (int)(gid / NQ), node = explicit_v ? 0 : (int)(gid % NQ);
         const int p = sa.p0 + pl;
@@ -300,14 +303,13 @@ __device__ __forceinline__ void setup_node(const LdsTabs &L, const SetupArgs &sa
         double V[3], w;
         if (explicit_v) { V[0] = sa.vR[p]; V[1] = sa.vphi[p]; V[2] = sa.vz[p]; }
         else node_velocity(sa.quad, node, Ve, V, w);
-        int flag = 0;
         double Jr = 0, Jz = 0;
         NodeConst c;
         double ubar = 0, Deltau = 0, vturn = 0;
         int wu = 0, wv = 0;
         const double Dx = (explicit_v && sa.Dx) ? sa.Dx[p] : 0.0;
         flag = node_setup(L.T, L.E, L.D, L.ngrid, R, z, Phi_h, V, Dx, !explicit_v, c, ubar, Deltau, vturn,
-                          wu, wv, Jr, Jz, nev, brent_it);
+                          wu, wv, Jr, Jz, nev, brent_it, &bfail);
         want_u = wu; want_v = wv;
         if (flag == 1) store_const(sa.nb, gid, c, ubar, Deltau, vturn);
         sa.nb.flag[gid] = flag;
@@ -319,6 +321,9 @@ __device__ __forceinline__ void setup_node(const LdsTabs &L, const SetupArgs &sa
     // counters (wave-aggregated)
     const unsigned long long nb_ = __popcll(__ballot(valid && (want_u || want_v || nev > 0)));
     if (__lane_id() == 0 && nb_) atomicAdd(&sa.ctr->bound, nb_);
+    const unsigned long long nu_ = __popcll(__ballot(valid && flag == 2));
+    if (__lane_id() == 0 && nu_) atomicAdd(&sa.ctr->unbound, nu_);
+    if (valid && bfail) atomicAdd(&sa.ctr->bracket_fail, 1ull);
     if (valid && brent_it) atomicAdd(&sa.ctr->brent, (unsigned long long)brent_it);
     if (valid && nev) atomicAdd(&sa.ctr->grad, (unsigned long long)nev);
 }
@@ -860,7 +865,19 @@ struct MomArgs {
     fj_df_params df;
     double *out6;                       // [npts_total][6]
     double *rho, *vrot, *sigR, *sigp, *sigz, *sigRz;   // [npts_total]
+    unsigned long long *nonfinite;      // if set: count of nodes with a NaN or infinite f(J)
 };
+
+// Adds the block's count of non-finite f(J) values to *ctr (one atomic per block).
+__device__ inline void count_nonfinite(unsigned long long *ctr, int bad) {
+    if (!ctr) return;
+    __shared__ int nb_bad;
+    if (threadIdx.x == 0) nb_bad = 0;
+    __syncthreads();
+    if (bad) atomicAdd(&nb_bad, bad);
+    __syncthreads();
+    if (threadIdx.x == 0 && nb_bad) atomicAdd(ctr, (unsigned long long)nb_bad);
+}
 
 
 // k_moments: one block per point; fixed-order reduction of the 6 sums.
@@ -877,6 +894,7 @@ __device__ inline void moments_point(const MomArgs &ma, int pl, double (*red)[25
     const double R = ma.R[p], Ve = ma.Ve[p];
     const double jacob = Ve * Ve * Ve;
     double s[6] = {0, 0, 0, 0, 0, 0};
+    int bad = 0;
     for (int node = threadIdx.x; node < NQ; node += blockDim.x) {
         const long long gid = (long long)pl * NQ + node;
         const int flag = ma.nb.flag[gid];
@@ -885,6 +903,7 @@ __device__ inline void moments_point(const MomArgs &ma, int pl, double (*red)[25
         node_velocity(ma.quad, node, Ve, V, w);
         const double Jphi = R * V[1];
         const double DF = d_df(ma.df, ma.nb.Jr[gid], Jphi, ma.nb.Jz[gid]);
+        bad += !isfinite(DF);
         const double jd = jacob * DF;
         s[0] += w * jd;
         s[1] += w * (jd * V[1]);
@@ -893,6 +912,7 @@ __device__ inline void moments_point(const MomArgs &ma, int pl, double (*red)[25
         s[4] += w * (jd * V[2] * V[2]);
         s[5] += w * (jd * V[0] * V[2]);
     }
+    count_nonfinite(ma.nonfinite, bad);
     for (int m = 0; m < 6; m++) red[m][threadIdx.x] = s[m];
     __syncthreads();
     for (int st = blockDim.x / 2; st > 0; st >>= 1) {
@@ -934,6 +954,7 @@ __device__ inline void moments_point_f32(const MomArgs &ma, int pl, float (*red)
     d.J0 = (float)ma.df.J0; d.invJ0 = (float)(1.0 / ma.df.J0);
     d.chi = (float)ma.df.chi; d.k = (float)ma.df.k; d.rot_form = ma.df.rot_form;
     KSum s[6];
+    int bad = 0;
     for (int node = threadIdx.x; node < NQ; node += blockDim.x) {
         const long long gid = (long long)pl * NQ + node;
         if (ma.nb.flag[gid] == 0) continue;
@@ -941,6 +962,7 @@ __device__ inline void moments_point_f32(const MomArgs &ma, int pl, float (*red)
         node_velocity(ma.quad, node, Ve, Vd, wd);
         const float V0 = (float)Vd[0], V1 = (float)Vd[1], V2 = (float)Vd[2], w = (float)wd;
         const float DF = df32(d, (float)ma.nb.Jr[gid], (float)(R * Vd[1]), (float)ma.nb.Jz[gid]);
+        bad += !isfinite(DF);
         const float jd = jacob * DF;
         s[0].add(w * jd);
         s[1].add(w * (jd * V1));
@@ -949,6 +971,7 @@ __device__ inline void moments_point_f32(const MomArgs &ma, int pl, float (*red)
         s[4].add(w * (jd * V2 * V2));
         s[5].add(w * (jd * V0 * V2));
     }
+    count_nonfinite(ma.nonfinite, bad);
     for (int m = 0; m < 6; m++) { red[m][0][threadIdx.x] = s[m].s; red[m][1][threadIdx.x] = s[m].c; }
     __syncthreads();
     for (int st = blockDim.x / 2; st > 0; st >>= 1) {
@@ -2360,7 +2383,7 @@ int run_nodes(fj_ctx *c, int npts) {
         HIPCHK(c, hipEventRecord(ev[5], c->stream));
         MomArgs ma{c->d_pPhi, c->d_pVe, c->d_pR, (int)p0, np, c->quad, nb, c->df, c->d_out6,
                    c->d_mom, c->d_mom + npts, c->d_mom + 2 * npts, c->d_mom + 3 * npts, c->d_mom + 4 * npts,
-                   c->d_mom + 5 * npts};
+                   c->d_mom + 5 * npts, &c->d_ctr->nonfinite};
         if (mixed) k_moments_f32<<<np, 256, 0, c->stream>>>(ma);
         else k_moments<<<np, 256, 0, c->stream>>>(ma);
         HIPCHK(c, hipGetLastError());
@@ -2395,6 +2418,9 @@ int run_nodes(fj_ctx *c, int npts) {
     c->st.bound_nodes = (long long)h.bound;
     c->st.qk_calls = (long long)h.qk;
     c->st.brent_iters = (long long)h.brent;
+    c->st.unbound_nodes = (long long)h.unbound;
+    c->st.bracket_failures = (long long)h.bracket_fail;
+    c->st.nonfinite_nodes = (long long)h.nonfinite;
     if (sur) c->sst.ms_surrogate = t_sur;
     if (c->sur_init) {
         unsigned long long pos = 0;
@@ -3409,7 +3435,7 @@ int batch_iteration(fj_batch *b) {
     kb_integrals_ovf<<<b->ovf_blocks, 256, 20 * 256 * sizeof(double), s>>>(io);
     MomArgs ma{b->d_Phi, b->d_Ve, b->d_R, 0, b->npts, 0, nb, fj_df_params{}, b->d_out6, b->d_mom,
                b->d_mom + b->npts, b->d_mom + 2 * b->npts, b->d_mom + 3 * b->npts, b->d_mom + 4 * b->npts,
-               b->d_mom + 5 * b->npts};
+               b->d_mom + 5 * b->npts, &b->d_ctr->nonfinite};
     kb_moments<<<b->npts, 256, 0, s>>>(b->d_md, b->d_pmodel, ma, mixed ? 1 : 0);
     kb_project<<<(b->rows + 63) / 64, 64, 0, s>>>(b->d_md, b->d_rstart, M, b->d_mom, b->npts);
     kb_poisson<<<dim3(b->npoly_max, M), PSCAN_THREADS, 0, s>>>(b->d_md);

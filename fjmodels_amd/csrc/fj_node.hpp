@@ -151,7 +151,7 @@ FJ_HD inline JInt make_jint(const PotView *T, const NodeConst &c, double ubar, d
 FJ_HD inline int node_setup(const PotView &T, const double *Eg, const double *Dg, int ng, double R, double z,
                             double Phi_h, const double *V, double Dx, bool check_H, NodeConst &c, double &ubar,
                             double &Deltau, double &vturn, int &want_u, int &want_v, double &Jr, double &Jz,
-                            long long &nev, int &brent_it) {
+                            long long &nev, int &brent_it, int *bracket_fail = nullptr) {
     int flag = 0;
     want_u = 0; want_v = 0; Jr = 0; Jz = 0; ubar = 0; Deltau = 0; vturn = 0;
     const double H = .5 * (V[0] * V[0] + V[1] * V[1] + V[2] * V[2]) + Phi_h;   // DF.cpp:83
@@ -189,6 +189,9 @@ FJ_HD inline int node_setup(const PotView &T, const double *Eg, const double *Dg
             while (dut < 0 && guard++ < 4000) { uo = utry; dfuo = dut; utry *= .8; dut = du(utry); }
             ui = utry; dfui = dut;
         }
+        // a bracket walk that ran into its iteration guard (the reference would
+        // loop on): counted, the node continues as the reference would after it
+        bool bfail = guard >= 4000;
         int it = 0;
         const double umid = brent_solve(du, ui, uo, dfui, dfuo, &it);
         brent_it += it;
@@ -269,6 +272,7 @@ FJ_HD inline int node_setup(const PotView &T, const double *Eg, const double *Dg
             }
             uinner = ut_; fuinner = ut;
         }
+        bfail = bfail || guard >= 4000;
         if (!noroot) {
             int i1 = 0, i2 = 0;
             const double a = brent_solve(U, uinner, u1, fuinner, fu1, &i1);
@@ -314,6 +318,7 @@ FJ_HD inline int node_setup(const PotView &T, const double *Eg, const double *Dg
         }
         if (fabs(vturn - PIH) < TINY) Jz = 0;
         else want_v = 1;
+        if (bracket_fail) *bracket_fail = bfail || guard >= 4000;
                     }
 
     return flag;

@@ -64,6 +64,12 @@ typedef struct {
     double ms_points, ms_setup, ms_integrals, ms_overflow, ms_moments, ms_project, ms_poisson, ms_total;
     long long points, nodes, bound_nodes, tasks, qk_calls, overflow_tasks, brent_iters;
     int integral_lanes, setup_lanes;
+    /* node diagnostics (the reference passes these cases on silently):
+     * unbound_nodes: H < 0 but E >= 0 in spheroidal coordinates, Jr = Jz = 1e31 (uvOrb.cpp:38-40);
+     * bracket_failures: a turning-point bracket walk (uvOrb.cpp:109-224) stopped by its iteration
+     *   guard, where the reference would keep walking;
+     * nonfinite_nodes: nodes whose f(J) is NaN or infinite (they poison the moment sums). */
+    long long unbound_nodes, bracket_failures, nonfinite_nodes;
 } fj_stats;
 
 /* Context on HIP device `device` (no reference counterpart: the reference

@@ -1474,7 +1474,7 @@ int fjo_actions(const fjo_pot *p, double Delta, double R, double z, double Phi0,
     (void)chu2;
     o.u = u; o.v = v; o.shu2 = shu2; o.sv = sv; o.cv = cv; o.sv2 = sv2;
     o.E = .5 * p2 + .5 * o.Lzsq / (R * R) + Phi0;
-    if (o.E >= 0 || isnan(o.E)) { *Jr = 1e31; *Jz = 1e31; return 1; }
+    if (o.E >= 0 || isnan(o.E)) { *Jr = 1e31; *Jz = 1e31; g_cnt.unbound_nodes++; return 1; }
     g_cnt.bound_nodes++;
     Getu0(&o, 58);
     o.sh1sq = fjm_sinh(o.umid) * fjm_sinh(o.umid); o.ch1sq = fjm_cosh(o.umid) * fjm_cosh(o.umid);

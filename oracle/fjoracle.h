@@ -115,6 +115,7 @@ void fjo_vir2(const fjo_pot *p, const double *rhl, const double *vrotl, const do
 /* Node-work counters of the last fjo_rhofDF/fjo_actions calls on this thread. */
 typedef struct {
     long long nodes, bound_nodes, phi_evals, grad_evals, qk21_calls, brent_iters;
+    long long unbound_nodes;   /* E >= 0 after the spheroidal transform (Jr = Jz = 1e31) */
 } fjo_counters;
 void fjo_counters_get(fjo_counters *c);
 void fjo_counters_reset(void);

@@ -27,7 +27,7 @@ class DFParams(C.Structure):
 
 class Counters(C.Structure):
     _fields_ = [(n, C.c_longlong) for n in
-                ("nodes", "bound_nodes", "phi_evals", "grad_evals", "qk21_calls", "brent_iters")]
+                ("nodes", "bound_nodes", "phi_evals", "grad_evals", "qk21_calls", "brent_iters", "unbound_nodes")]
 
 
 def build():

@@ -110,6 +110,7 @@ def test_rhofDF_points(gctx, hq40):
     print("stats", st)
     dfo = O.df_params("Hernquist")
     worst = 0
+    O.reset_counters()
     for i in range(len(R)):
         _, ref, _ = p.rhofDF(dfo, R[i], z[i])
         e = moment_errors(out6[i], ref)
@@ -117,6 +118,12 @@ def test_rhofDF_points(gctx, hq40):
         worst = max(worst, e.max())
         assert np.array_equal(out6[i], ref), (i, e)
     assert worst < RTOL
+    # node diagnostics: the same bound / unbound split as the oracle, no
+    # bracket walk stopped by its guard, no non-finite f(J)
+    oc = O.counters()
+    assert st["bound_nodes"] == oc["bound_nodes"] and st["unbound_nodes"] == oc["unbound_nodes"], (st, oc)
+    assert st["bracket_failures"] == 0 and st["nonfinite_nodes"] == 0
+    assert st["qk_calls"] == oc["qk21_calls"] and st["brent_iters"] == oc["brent_iters"]
 
 
 def test_all_nodes_of_one_point(gctx, hq40):
