@@ -39,6 +39,17 @@ constexpr double TINY = 1e-12;
 constexpr double DBL_EPS = 2.220446049250313e-16;
 constexpr double DBL_MINV = 2.2250738585072014e-308;
 constexpr double DBL_MAXV = 1.7976931348623157e+308;
+
+// Machine constants of the QAGS stepper (QUADPACK's d1mach / r1mach): the
+// double instance is dqagse, the float instance QUADPACK's single-precision qagse.
+template <class T> struct QLim;
+template <> struct QLim<double> {
+    static constexpr double eps = DBL_EPS, uflow = DBL_MINV, oflow = DBL_MAXV;
+};
+template <> struct QLim<float> {
+    static constexpr float eps = 1.1920928955078125e-07f, uflow = 1.1754943508222875e-38f,
+                           oflow = 3.4028234663852886e+38f;
+};
 constexpr int MAXPOLY = 32;
 
 // MAX/MIN of include/Utils.h:47-48 (a NaN first argument compares false).
@@ -426,17 +437,20 @@ __constant__ static const double kWG7[4] = {
     0.129484966168869693270611432679082, 0.279705391489276667901467771423780,
     0.381830050505118944950369775488975, 0.417959183673469387755102040816327};
 
-struct QKOut { double result, abserr, resabs, resasc; };
+template <class T>
+struct QKOutT { T result, abserr, resabs, resasc; };
+using QKOut = QKOutT<double>;
 
-FJ_HD __forceinline__ double qk_rescale(double err, double resabs, double resasc) {
+template <class T>
+FJ_HD __forceinline__ T qk_rescale(T err, T resabs, T resasc) {
     err = fabs(err);
     if (resasc != 0 && err != 0) {
-        const double t = 200 * err / resasc;
-        const double scale = t * sqrt(t);   // (200 err/resasc)^1.5
+        const T t = 200 * err / resasc;
+        const T scale = t * sqrt(t);   // (200 err/resasc)^1.5
         err = scale < 1 ? resasc * scale : resasc;
     }
-    if (resabs > DBL_MINV / (50 * DBL_EPS)) {
-        const double min_err = 50 * DBL_EPS * resabs;
+    if (resabs > QLim<T>::uflow / (50 * QLim<T>::eps)) {
+        const T min_err = 50 * QLim<T>::eps * resabs;
         if (min_err > err) err = min_err;
     }
     return err;
@@ -522,39 +536,49 @@ struct Lanes {
 // lane_work()).  Addresses are formed at each access from the uniform base
 // and stride and the lane index, so they do not occupy registers across the
 // integrand evaluations.
-struct QagsWork {
-    double *ws;      // buffer base (uniform)
+template <class T>
+struct QagsWorkT {
+    T *ws;           // buffer base (uniform)
     long long ld;    // lanes sharing the buffer (uniform)
     int n1;          // cap + 1 (uniform)
     int lane;
-    FJ_HD Lanes<double> list(int k) const { return {ws + (long long)k * n1 * ld + lane, ld}; }
-    FJ_HD Lanes<double> alist() const { return list(0); }
-    FJ_HD Lanes<double> blist() const { return list(1); }
-    FJ_HD Lanes<double> rlist() const { return list(2); }
-    FJ_HD Lanes<double> elist() const { return list(3); }
-    FJ_HD Lanes<double> rlist2() const { return {ws + 4ll * n1 * ld + lane, ld}; }
-    FJ_HD Lanes<double> res3la() const { return {ws + (4ll * n1 + 56) * ld + lane, ld}; }
+    FJ_HD Lanes<T> list(int k) const { return {ws + (long long)k * n1 * ld + lane, ld}; }
+    FJ_HD Lanes<T> alist() const { return list(0); }
+    FJ_HD Lanes<T> blist() const { return list(1); }
+    FJ_HD Lanes<T> rlist() const { return list(2); }
+    FJ_HD Lanes<T> elist() const { return list(3); }
+    FJ_HD Lanes<T> rlist2() const { return {ws + 4ll * n1 * ld + lane, ld}; }
+    FJ_HD Lanes<T> res3la() const { return {ws + (4ll * n1 + 56) * ld + lane, ld}; }
     FJ_HD Lanes<int> iord() const {
         return {reinterpret_cast<int *>(ws + (4ll * n1 + 60) * ld) + lane, ld};
     }
 };
+using QagsWork = QagsWorkT<double>;
 
-struct Qags {
+template <class T>
+struct QagsT {
     // inputs
-    double a0, b0, epsabs, epsrel;
+    T a0, b0, epsabs, epsrel;
     int limit;
     // stepping state: 0 first rule, 1 left half pending, 2 right half pending, 3 done
     int state;
-    double next_a, next_b;
+    T next_a, next_b;
     // dqagse locals
-    double area, area1, error1, defab1, defabs, dres, erlarg, erlast, errbnd, errmax,
+    T area, area1, error1, defab1, defabs, dres, erlarg, erlast, errbnd, errmax,
         errsum, ertest, small, correc, result, abserr;
     int ier, ierro, iroff1, iroff2, iroff3, ksgn, ktmin, last, maxerr, nres, nrmax, numrl2, extrap, noext;
     int overflow;  // workspace capacity reached before limit
 };
+using Qags = QagsT<double>;
 
-FJ_HD void qags_begin(Qags &q, double a, double b, double epsabs, double epsrel, int limit);
-FJ_HD void qags_consume(Qags &q, const QagsWork &w, int cap, const QKOut &r);
+// T is taken from q only (the bounds and tolerances convert to it).
+template <class T> struct QNoDeduce { using type = T; };
+template <class T>
+FJ_HD __forceinline__ void qags_begin(QagsT<T> &q, typename QNoDeduce<T>::type a, typename QNoDeduce<T>::type b,
+                                      typename QNoDeduce<T>::type epsabs, typename QNoDeduce<T>::type epsrel,
+                                      int limit);
+template <class T>
+FJ_HD __forceinline__ void qags_consume(QagsT<T> &q, const QagsWorkT<T> &w, int cap, const QKOutT<T> &r);
 
 template <int NK, class F>
 FJ_HD __forceinline__ double qags_solve(F &f, double a, double b, double epsabs, double epsrel, int limit,
@@ -578,10 +602,11 @@ FJ_HD __forceinline__ double qags_solve(F &f, double a, double b, double epsabs,
 namespace fj {
 
 // QUADPACK dqpsrt (1-based lists).
-FJ_HD __forceinline__ void qpsrt(int limit, int last, int &maxerr, double &ermax, const Lanes<double> &elist,
+template <class T>
+FJ_HD __forceinline__ void qpsrt(int limit, int last, int &maxerr, T &ermax, const Lanes<T> &elist,
                                       const Lanes<int> &iord, int &nrmax) {
     int i, j, k, ido, ibeg, jbnd, isucc, jupbn;
-    double errmax, errmin;
+    T errmax, errmin;
     if (last <= 2) { iord[1] = 1; iord[2] = 2; goto L90; }
     errmax = elist[maxerr];
     if (nrmax != 1) {
@@ -624,9 +649,10 @@ L90:
 }
 
 // QUADPACK dqelg (epsilon algorithm), epstab and res3la 1-based.
-FJ_HD __forceinline__ void qelg(int &n, const Lanes<double> &epstab, double &result, double &abserr,
-                                     const Lanes<double> &res3la, int &nres) {
-    const double epmach = DBL_EPS, oflow = DBL_MAXV;
+template <class T>
+FJ_HD __forceinline__ void qelg(int &n, const Lanes<T> &epstab, T &result, T &abserr,
+                                     const Lanes<T> &res3la, int &nres) {
+    const T epmach = QLim<T>::eps, oflow = QLim<T>::oflow;
     nres++;
     abserr = oflow;
     result = epstab[n];
@@ -639,33 +665,33 @@ FJ_HD __forceinline__ void qelg(int &n, const Lanes<double> &epstab, double &res
         int k1 = n;
         for (int i = 1; i <= newelm; i++) {
             const int k2 = k1 - 1, k3 = k1 - 2;
-            double res = epstab[k1 + 2];
-            const double e0 = epstab[k3], e1 = epstab[k2], e2 = res;
-            const double e1abs = fabs(e1), delta2 = e2 - e1, err2 = fabs(delta2);
-            const double tol2 = fmax(fabs(e2), e1abs) * epmach;
-            const double delta3 = e1 - e0, err3 = fabs(delta3);
-            const double tol3 = fmax(e1abs, fabs(e0)) * epmach;
+            T res = epstab[k1 + 2];
+            const T e0 = epstab[k3], e1 = epstab[k2], e2 = res;
+            const T e1abs = fabs(e1), delta2 = e2 - e1, err2 = fabs(delta2);
+            const T tol2 = fmax(fabs(e2), e1abs) * epmach;
+            const T delta3 = e1 - e0, err3 = fabs(delta3);
+            const T tol3 = fmax(e1abs, fabs(e0)) * epmach;
             if (!(err2 > tol2 || err3 > tol3)) {
                 result = res;
                 abserr = err2 + err3;
-                abserr = fmax(abserr, 0.5e+01 * epmach * fabs(result));
+                abserr = fmax(abserr, T(0.5e+01) * epmach * fabs(result));
                 return;
             }
-            const double e3 = epstab[k1];
+            const T e3 = epstab[k1];
             epstab[k1] = e1;
-            const double delta1 = e1 - e3, err1 = fabs(delta1);
-            const double tol1 = fmax(e1abs, fabs(e3)) * epmach;
+            const T delta1 = e1 - e3, err1 = fabs(delta1);
+            const T tol1 = fmax(e1abs, fabs(e3)) * epmach;
             bool shrink = (err1 <= tol1 || err2 <= tol2 || err3 <= tol3);
-            double ss = 0;
+            T ss = 0;
             if (!shrink) {
-                ss = 1.0 / delta1 + 1.0 / delta2 - 1.0 / delta3;
-                if (!(fabs(ss * e1) > 0.1e-03)) shrink = true;
+                ss = T(1.0) / delta1 + T(1.0) / delta2 - T(1.0) / delta3;
+                if (!(fabs(ss * e1) > T(0.1e-03))) shrink = true;
             }
             if (shrink) { n = i + i - 1; break; }
-            res = e1 + 1.0 / ss;
+            res = e1 + T(1.0) / ss;
             epstab[k1] = res;
             k1 = k1 - 2;
-            const double error = err2 + fabs(res - e2) + err3;
+            const T error = err2 + fabs(res - e2) + err3;
             if (!(error > abserr)) { abserr = error; result = res; }
         }
         if (n == limexp) n = 2 * (limexp / 2) - 1;
@@ -684,30 +710,35 @@ FJ_HD __forceinline__ void qelg(int &n, const Lanes<double> &epstab, double &res
             abserr = oflow;
         }
     }
-    abserr = fmax(abserr, 0.5e+01 * epmach * fabs(result));
+    abserr = fmax(abserr, T(0.5e+01) * epmach * fabs(result));
 }
 
-FJ_HD __forceinline__ void qags_begin(Qags &q, double a, double b, double epsabs, double epsrel, int limit) {
+template <class T>
+FJ_HD __forceinline__ void qags_begin(QagsT<T> &q, typename QNoDeduce<T>::type a, typename QNoDeduce<T>::type b,
+                                      typename QNoDeduce<T>::type epsabs, typename QNoDeduce<T>::type epsrel,
+                                      int limit) {
     q.a0 = a; q.b0 = b; q.epsabs = epsabs; q.epsrel = epsrel; q.limit = limit;
     q.state = 0; q.next_a = a; q.next_b = b;
     q.ier = 0; q.last = 0; q.result = 0; q.abserr = 0; q.overflow = 0;
-    if (epsabs <= 0 && epsrel < fmax(50 * DBL_EPS, 0.5e-28)) { q.ier = 6; q.state = 3; }
+    if (epsabs <= 0 && epsrel < fmax(50 * QLim<T>::eps, T(0.5e-28))) { q.ier = 6; q.state = 3; }
 }
 
 // Sets up the bisection of interval maxerr for loop index q.last: the rule
 // runs on the left half [a1, b1] first.  The interval stays in the workspace
 // until both halves are done, so its ends are read back there.
-FJ_HD __forceinline__ void qags_start_iter(Qags &q, const QagsWork &w, int cap) {
+template <class T>
+FJ_HD __forceinline__ void qags_start_iter(QagsT<T> &q, const QagsWorkT<T> &w, int cap) {
     if (q.last > cap) { q.overflow = 1; q.state = 3; return; }
-    const double a = w.alist()[q.maxerr], b = w.blist()[q.maxerr];
+    const T a = w.alist()[q.maxerr], b = w.blist()[q.maxerr];
     q.erlast = q.errmax;
-    q.next_a = a; q.next_b = 0.5 * (a + b);
+    q.next_a = a; q.next_b = T(0.5) * (a + b);
     q.state = 1;
 }
 
 // dqagse labels 100..140
-FJ_HD __forceinline__ void qags_finish(Qags &q, const QagsWork &w, int from115) {
-    const double oflow = DBL_MAXV;
+template <class T>
+FJ_HD __forceinline__ void qags_finish(QagsT<T> &q, const QagsWorkT<T> &w, int from115) {
+    const T oflow = QLim<T>::oflow;
     int go115 = from115, go130 = 0;
     if (!go115) {
         if (q.abserr == oflow) go115 = 1;
@@ -720,13 +751,13 @@ FJ_HD __forceinline__ void qags_finish(Qags &q, const QagsWork &w, int from115) 
             else if (q.area == 0.0) go130 = 1;
         }
         if (!go115 && !go130) {
-            if (q.ksgn == -1 && fmax(fabs(q.result), fabs(q.area)) <= q.defabs * 0.1e-01) go130 = 1;
-            else if (0.1e-01 > (q.result / q.area) || (q.result / q.area) > 0.1e+03 || q.errsum > fabs(q.area))
+            if (q.ksgn == -1 && fmax(fabs(q.result), fabs(q.area)) <= q.defabs * T(0.1e-01)) go130 = 1;
+            else if (T(0.1e-01) > (q.result / q.area) || (q.result / q.area) > T(0.1e+03) || q.errsum > fabs(q.area))
                 q.ier = 6;
         }
     }
     if (go115) {
-        double s = 0.0;
+        T s = 0;
         for (int k = 1; k <= q.last; k++) s = s + w.rlist()[k];
         q.result = s;
         q.abserr = q.errsum;
@@ -735,17 +766,18 @@ FJ_HD __forceinline__ void qags_finish(Qags &q, const QagsWork &w, int from115) 
     q.state = 3;
 }
 
-FJ_HD __forceinline__ void qags_consume(Qags &q, const QagsWork &w, int cap, const QKOut &r) {
-    const double epmach = DBL_EPS, uflow = DBL_MINV, oflow = DBL_MAXV;
+template <class T>
+FJ_HD __forceinline__ void qags_consume(QagsT<T> &q, const QagsWorkT<T> &w, int cap, const QKOutT<T> &r) {
+    const T epmach = QLim<T>::eps, uflow = QLim<T>::uflow, oflow = QLim<T>::oflow;
     if (q.state == 0) {
         q.result = r.result; q.abserr = r.abserr; q.defabs = r.resabs;
-        const double resabs = r.resasc;
+        const T resabs = r.resasc;
         q.dres = fabs(q.result);
         q.errbnd = fmax(q.epsabs, q.epsrel * q.dres);
         q.last = 1;
         w.alist()[1] = q.a0; w.blist()[1] = q.b0;
         w.rlist()[1] = q.result; w.elist()[1] = q.abserr; w.iord()[1] = 1;
-        if (q.abserr <= 1.0e+02 * epmach * q.defabs && q.abserr > q.errbnd) q.ier = 2;
+        if (q.abserr <= T(1.0e+02) * epmach * q.defabs && q.abserr > q.errbnd) q.ier = 2;
         if (q.limit == 1) q.ier = 1;
         if (q.ier != 0 || (q.abserr <= q.errbnd && q.abserr != resabs) || q.abserr == 0.0) { q.state = 3; return; }
         w.rlist2()[1] = q.result;
@@ -754,7 +786,7 @@ FJ_HD __forceinline__ void qags_consume(Qags &q, const QagsWork &w, int cap, con
         q.ierro = 0; q.iroff1 = 0; q.iroff2 = 0; q.iroff3 = 0;
         q.erlarg = 0; q.ertest = 0; q.small = 0; q.correc = 0;
         q.ksgn = -1;
-        if (q.dres >= (1.0 - 0.5e+02 * epmach) * q.defabs) q.ksgn = 1;
+        if (q.dres >= (T(1.0) - T(0.5e+02) * epmach) * q.defabs) q.ksgn = 1;
         q.last = 2;
         qags_start_iter(q, w, cap);
         return;
@@ -767,15 +799,15 @@ FJ_HD __forceinline__ void qags_consume(Qags &q, const QagsWork &w, int cap, con
     }
     // state 2: both halves known -- body of the dqagse loop for index last
     const int last = q.last, limit = q.limit;
-    const double area2 = r.result, error2 = r.abserr, defab2 = r.resasc;
-    const double area1 = q.area1, error1 = q.error1;
-    const double a1 = w.alist()[q.maxerr], b2 = w.blist()[q.maxerr], b1 = 0.5 * (a1 + b2), a2 = b1;
-    const double area12 = area1 + area2;
-    const double erro12 = error1 + error2;
+    const T area2 = r.result, error2 = r.abserr, defab2 = r.resasc;
+    const T area1 = q.area1, error1 = q.error1;
+    const T a1 = w.alist()[q.maxerr], b2 = w.blist()[q.maxerr], b1 = T(0.5) * (a1 + b2), a2 = b1;
+    const T area12 = area1 + area2;
+    const T erro12 = error1 + error2;
     q.errsum = q.errsum + erro12 - q.errmax;
     q.area = q.area + area12 - w.rlist()[q.maxerr];
     if (!(q.defab1 == error1 || defab2 == error2)) {
-        if (!(fabs(w.rlist()[q.maxerr] - area12) > 0.1e-04 * fabs(area12) || erro12 < 0.99 * q.errmax)) {
+        if (!(fabs(w.rlist()[q.maxerr] - area12) > T(0.1e-04) * fabs(area12) || erro12 < T(0.99) * q.errmax)) {
             if (q.extrap) q.iroff2++;
             else q.iroff1++;
         }
@@ -787,7 +819,7 @@ FJ_HD __forceinline__ void qags_consume(Qags &q, const QagsWork &w, int cap, con
     if (q.iroff1 + q.iroff2 >= 10 || q.iroff3 >= 20) q.ier = 2;
     if (q.iroff2 >= 5) q.ierro = 3;
     if (last == limit) q.ier = 1;
-    if (fmax(fabs(a1), fabs(b2)) <= (1.0 + 0.1e+03 * epmach) * (fabs(a2) + 0.1e+04 * uflow)) q.ier = 4;
+    if (fmax(fabs(a1), fabs(b2)) <= (T(1.0) + T(0.1e+03) * epmach) * (fabs(a2) + T(0.1e+04) * uflow)) q.ier = 4;
     if (error2 > error1) {
         w.alist()[q.maxerr] = a2;
         w.alist()[last] = a1;
@@ -808,7 +840,7 @@ FJ_HD __forceinline__ void qags_consume(Qags &q, const QagsWork &w, int cap, con
     if (q.ier != 0) { qags_finish(q, w, 0); return; }
     bool next = false;
     if (last == 2) {
-        q.small = fabs(q.b0 - q.a0) * 0.375;
+        q.small = fabs(q.b0 - q.a0) * T(0.375);
         q.erlarg = q.errsum;
         q.ertest = q.errbnd;
         w.rlist2()[2] = q.area;
@@ -838,10 +870,10 @@ FJ_HD __forceinline__ void qags_consume(Qags &q, const QagsWork &w, int cap, con
         if (!next) {
             q.numrl2++;
             w.rlist2()[q.numrl2] = q.area;
-            double reseps, abseps;
+            T reseps, abseps;
             qelg(q.numrl2, w.rlist2(), reseps, abseps, w.res3la(), q.nres);
             q.ktmin++;
-            if (q.ktmin > 5 && q.abserr < 0.1e-02 * q.errsum) q.ier = 5;
+            if (q.ktmin > 5 && q.abserr < T(0.1e-02) * q.errsum) q.ier = 5;
             if (abseps < q.abserr) {
                 q.ktmin = 0;
                 q.abserr = abseps;
@@ -856,7 +888,7 @@ FJ_HD __forceinline__ void qags_consume(Qags &q, const QagsWork &w, int cap, con
             q.errmax = w.elist()[q.maxerr];
             q.nrmax = 1;
             q.extrap = 0;
-            q.small = q.small * 0.5;
+            q.small = q.small * T(0.5);
             q.erlarg = q.errsum;
             next = true;
         }
@@ -878,8 +910,9 @@ inline size_t ws_doubles(int cap, long long nlanes) {
     return (size_t)(ws_lane_doubles(cap) * nlanes + ((long long)(cap + 1) * nlanes + 1) / 2);
 }
 
-FJ_HD inline QagsWork lane_work(double *ws, int cap, long long lane, long long nlanes) {
-    QagsWork w;
+template <class T>
+FJ_HD inline QagsWorkT<T> lane_work(T *ws, int cap, long long lane, long long nlanes) {
+    QagsWorkT<T> w;
     w.ws = ws; w.ld = nlanes; w.n1 = cap + 1; w.lane = (int)lane;
     return w;
 }

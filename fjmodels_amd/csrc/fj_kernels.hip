@@ -22,6 +22,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <type_traits>
 #include <vector>
 
 #include "../../include/fjmodels.h"
@@ -41,7 +42,10 @@ constexpr int QCAP = 64;                // QAGS intervals kept per lane before o
 // Mixed mode: FP32 integrands whose QAGS passes this many intervals are
 // limited by their own rounding (near-circular or near-planar orbits, where
 // F(u) is a small difference of large terms); they are redone in FP64.
-constexpr int QCAP_MIXED = 24;
+#ifndef FJ_QCAP_MIXED
+#define FJ_QCAP_MIXED 24
+#endif
+constexpr int QCAP_MIXED = FJ_QCAP_MIXED;
 constexpr int QCAP_FULL = 1000;         // the reference's workspace limit (Utils.h:55)
 
 // include/GaussPts.h:76-136
@@ -765,15 +769,18 @@ struct IntegArgs {
 // PerTaskTables: the tables of each task are ia.pv[ia.pmodel[node / NQ]]
 // (batched overflow pass); otherwise f.T stays the kernel's LDS view, which
 // keeps the view's fields in registers.
-template <bool PerTaskTables = false, class FI, class FV>
-__device__ __forceinline__ void integrals_loop(const IntegArgs &ia, FI &f, const QagsWork &w, const FV &fv) {
+// The real type R of the workspace picks the stepper: double is dqagse with
+// the FP64 rule (qk_rule), float QUADPACK's single-precision qagse with the
+// FP32 rule (qk_rule32, mixed precision).
+template <bool PerTaskTables = false, class FI, class FV, class R>
+__device__ __forceinline__ void integrals_loop(const IntegArgs &ia, FI &f, const QagsWorkT<R> &w, const FV &fv) {
     const bool from_list = ia.ntask_override >= 0 || ia.ntask_dev;
     const unsigned int nu = from_list ? 0u : *ia.cnt_u, nv = from_list ? 0u : *ia.cnt_v;
     const unsigned int ntask = ia.ntask_dev ? *ia.ntask_dev
                                : ia.ntask_override >= 0 ? (unsigned int)ia.ntask_override : nu + nv;
     int task = -1;    // -1: needs a task, -2: queue drained
     long long node = 0;
-    Qags q;
+    QagsT<R> q;
     unsigned long long nqk = 0, novf = 0;
     while (true) {
         // fetch (wave-aggregated)
@@ -809,7 +816,9 @@ __device__ __forceinline__ void integrals_loop(const IntegArgs &ia, FI &f, const
         }
         if (__ballot(task >= 0) == 0) break;
         if (task >= 0) {
-            const QKOut r = qk_rule<11>(f, q.next_a, q.next_b, fv);
+            QKOutT<R> r;
+            if constexpr (std::is_same<R, float>::value) r = qk_rule32<11>(f, q.next_a, q.next_b, fv);
+            else r = qk_rule<11>(f, q.next_a, q.next_b, fv);
             nqk++;
             qags_consume(q, w, ia.cap, r);
             if (q.state == 3) {
@@ -842,7 +851,7 @@ k_integrals(TabArgs ta, IntegArgs ia) {
 
 // Mixed precision: FP32 integrand, FP64 QAGS.
 #ifndef FJ_INTEG32_WAVES
-#define FJ_INTEG32_WAVES 3
+#define FJ_INTEG32_WAVES 4
 #endif
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(FJ_INTEG32_WAVES)))
 k_integrals_f32(TabArgs ta, IntegArgs ia) {
@@ -851,8 +860,9 @@ k_integrals_f32(TabArgs ta, IntegArgs ia) {
     float *f32 = reinterpret_cast<float *>(smem + tab_lds_doubles(ta.nr, ta.npoly, ta.ngrid));
     const PotView32 T32 = stage_tables32(L, f32);
     const long long lane_g = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    const QagsWork w = lane_work(ia.ws, ia.cap, lane_g, (long long)gridDim.x * blockDim.x);
-    const FvStridedF fv{f32 + tab32_lds_floats(ta.nr, ta.npoly) + threadIdx.x, (int)blockDim.x};
+    const QagsWorkT<float> w =
+        lane_work(reinterpret_cast<float *>(ia.ws), ia.cap, lane_g, (long long)gridDim.x * blockDim.x);
+    const FvStrided32 fv{f32 + tab32_lds_floats(ta.nr, ta.npoly) + threadIdx.x, (int)blockDim.x};
     JInt32 f;
     f.T = &T32;
     integrals_loop(ia, f, w, fv);
@@ -1874,8 +1884,9 @@ kb_integrals_f32(const ModelDev *md, const int *block_model, int M, IntegArgs ba
         float *f32 = reinterpret_cast<float *>(smem + tab_lds_doubles(d.ta.nr, d.ta.npoly, d.ta.ngrid));
         const PotView32 T32 = stage_tables32(L, f32);
         const IntegArgs ia = batch_integ_args<JInt32>(base, d, cnt, m);
-        const QagsWork w = lane_work(ia.ws, ia.cap, lane_g, (long long)gridDim.x * blockDim.x);
-        const FvStridedF fv{f32 + tab32_lds_floats(d.ta.nr, d.ta.npoly) + threadIdx.x, (int)blockDim.x};
+        const QagsWorkT<float> w =
+            lane_work(reinterpret_cast<float *>(ia.ws), ia.cap, lane_g, (long long)gridDim.x * blockDim.x);
+        const FvStrided32 fv{f32 + tab32_lds_floats(d.ta.nr, d.ta.npoly) + threadIdx.x, (int)blockDim.x};
         JInt32 f;
         f.T = &T32;
         integrals_loop(ia, f, w, fv);
@@ -3376,7 +3387,8 @@ struct fj_batch {
     Counters *d_ctr = nullptr;
     double *d_ws = nullptr, *d_wsfull = nullptr, *d_backup = nullptr;
     long long ws_lanes = 0, wsfull_lanes = 0;
-    size_t lds_tab = 0, lds_integ = 0;
+    size_t lds_tab = 0, lds_integ = 0, lds_integ32 = 0;   // LDS of the FP64 / FP32 persistent integrators
+    int per_cu64 = 1, per_cu32 = 1;                         // their resident blocks per CU
     int setup_bs = 256, integ_blocks = 0, integ_max_blocks = 0, ovf_blocks = 64;
     bool have_setup = false;
     hipGraph_t graph = nullptr;
@@ -3422,14 +3434,18 @@ int batch_iteration(fj_batch *b) {
         kb_setup<128><<<b->nblk[0], 128, b->lds_tab, s>>>(b->d_md, b->d_bstart[0], M, sa, b->d_cnt);
     else
         kb_setup<256><<<b->nblk[1], 256, b->lds_tab, s>>>(b->d_md, b->d_bstart[1], M, sa, b->d_cnt);
-    kb_assign<<<1, 64, 0, s>>>(M, b->d_cnt, b->integ_blocks, b->d_block_model);
+    // integrator blocks: the chosen count, at most what is resident at this precision
+    int ib = b->cus * (mixed ? b->per_cu32 : b->per_cu64);
+    if (b->integ_blocks < ib) ib = b->integ_blocks;
+    if (ib < M) ib = M;
+    kb_assign<<<1, 64, 0, s>>>(M, b->d_cnt, ib, b->d_block_model);
     unsigned int *g = b->d_cnt + 8 * M;   // [0] overflowed, [1] next overflow task, [2] second overflow
     IntegArgs ia{nb, b->d_list_u, b->d_list_v, nullptr, nullptr, nullptr, b->d_ovf, g,
                  b->d_ws, mixed ? QCAP_MIXED : QCAP, -1, b->d_ctr};
     if (mixed)
-        kb_integrals_f32<<<b->integ_blocks, 256, b->lds_integ, s>>>(b->d_md, b->d_block_model, b->M, ia, b->d_cnt);
+        kb_integrals_f32<<<ib, 256, b->lds_integ32, s>>>(b->d_md, b->d_block_model, b->M, ia, b->d_cnt);
     else
-        kb_integrals<<<b->integ_blocks, 256, b->lds_integ, s>>>(b->d_md, b->d_block_model, b->M, ia, b->d_cnt);
+        kb_integrals<<<ib, 256, b->lds_integ, s>>>(b->d_md, b->d_block_model, b->M, ia, b->d_cnt);
     IntegArgs io{nb, nullptr, nullptr, nullptr, nullptr, g + 1, b->d_ovf, g + 2, b->d_wsfull, QCAP_FULL, -1,
                  b->d_ctr, g, b->d_pv, b->d_pmodel};
     kb_integrals_ovf<<<b->ovf_blocks, 256, 20 * 256 * sizeof(double), s>>>(io);
@@ -3539,8 +3555,8 @@ int fj_batch_create(fj_ctx *c, int M, const fj_model_spec *spec, fj_batch **out)
         // persistent integrator: FP64 (tables + 20 doubles per lane) or FP32 (+ FP32 tables, 20 floats per lane)
         const size_t l64 = lt + 20 * 256 * sizeof(double);
         const size_t l32 = lt + (tab32_lds_floats(nr, np_) + 20 * 256) * sizeof(float);
-        const size_t li = l64 > l32 ? l64 : l32;
-        b->lds_integ = li > b->lds_integ ? li : b->lds_integ;
+        b->lds_integ = l64 > b->lds_integ ? l64 : b->lds_integ;
+        b->lds_integ32 = l32 > b->lds_integ32 ? l32 : b->lds_integ32;
     }
     if (r) { fj_batch_destroy(b); return r; }
     b->nodes = (long long)b->npts * NQ;
@@ -3599,9 +3615,12 @@ int fj_batch_create(fj_ctx *c, int M, const fj_model_spec *spec, fj_batch **out)
     // persistent integrator: as many blocks as fit, at least one per model
     hipDeviceProp_t prop;
     HIPCHK(c, hipGetDeviceProperties(&prop, c->device));
-    int per_cu = 0;
-    HIPCHK(c, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kb_integrals, 256, b->lds_integ));
-    if (per_cu < 1) per_cu = 1;
+    int pc64 = 0, pc32 = 0;
+    HIPCHK(c, hipOccupancyMaxActiveBlocksPerMultiprocessor(&pc64, kb_integrals, 256, b->lds_integ));
+    HIPCHK(c, hipOccupancyMaxActiveBlocksPerMultiprocessor(&pc32, kb_integrals_f32, 256, b->lds_integ32));
+    b->per_cu64 = pc64 < 1 ? 1 : pc64;
+    b->per_cu32 = pc32 < 1 ? 1 : pc32;
+    const int per_cu = b->per_cu64 > b->per_cu32 ? b->per_cu64 : b->per_cu32;
     b->cus = prop.multiProcessorCount;
     b->integ_max_blocks = prop.multiProcessorCount * per_cu;
     if (b->integ_max_blocks < M) b->integ_max_blocks = M;
