@@ -110,7 +110,7 @@ def test_rotating_nfw_mixed_precision(ctx, nfw96):
     The mean rotation is the small odd part of the DF near the centre
     (tanh(chi R vphi / J0) ~ 1e-4 at r = 1e-3), a difference of the prograde
     and retrograde halves, so its FP32 error there is the largest: about
-    4e-4 of its monopole, against 3e-6 for rho_l and 1e-6 for the
+    1e-3 of its monopole, against 3e-6 for rho_l and 1e-6 for the
     dispersions."""
     import fjmodels_amd as fj
     p = nfw96
