@@ -105,12 +105,15 @@ def mixed_leg(ctx, model, prev, last, steps, pts):
 
 def surrogate_leg(ctx, model, guess_tables, tol=1e-4, max_iter=30, keep=0.15, epochs=16, judge=0.05,
                   max_sur=8, sur_floor=4e-3, gate=0.05):
-    """The Hernquist model converged twice from the guess potential to the
-    same tolerance: successive density coefficients differ by less than `tol`
-    of the monopole (the driver's convergence test on rho_l).
+    """The Hernquist model converged three times from the guess potential to
+    the same tolerance: successive density coefficients differ by less than
+    `tol` of the monopole (the driver's convergence test on rho_l).  All
+    three use the QUADPACK Poisson solve.
 
-    plain: Staeckel actions every iteration.
-    surrogate: iterations 0-1 Staeckel, keeping a random `keep` of the bound
+    plain: FP64 Staeckel actions every iteration.
+    mixed: the same with the mixed-precision node phase.
+    surrogate (with the mixed node phase for its Staeckel iterations):
+    iterations 0-1 Staeckel, keeping a random `keep` of the bound
     nodes' actions as training samples; then the network is trained
     (`epochs` passes) and serves (Jr, Jz) while a random `judge` fraction of
     the nodes is recomputed with Staeckel, scored, added to the training set
@@ -121,16 +124,18 @@ def surrogate_leg(ctx, model, guess_tables, tol=1e-4, max_iter=30, keep=0.15, ep
     relative error of Jr or Jz exceeds `gate` is discarded (the previous
     potential is restored) and the run continues with Staeckel.
 
-    Reports the wall time of both runs (training included), the iteration
-    counts, the difference of the two converged rho_l (scaled by the
-    monopole) and both virial ratios."""
+    Reports the wall time of the three runs (training included), the
+    iteration counts, the differences of the converged rho_l from the plain
+    run's (scaled by the monopole) and the virial ratios."""
     import numpy as np
     import torch
+    import fjmodels_amd as fj
 
     def drho(a, b):
         return float(np.max(np.abs(a["rhl"] - b["rhl"]) / np.abs(a["rhl"][:, :1])))
 
-    def run(sur):
+    def run(sur, mixed):
+        ctx.set_precision(fj.FJ_PREC_MIXED if mixed else fj.FJ_PREC_FP64)
         ctx.set_potential(*guess_tables)
         model.total = [x.copy() for x in guess_tables]
         model.philP = [tuple(x.copy() for x in guess_tables)]
@@ -186,20 +191,26 @@ def surrogate_leg(ctx, model, guess_tables, tol=1e-4, max_iter=30, keep=0.15, ep
         st = ctx.surrogate_stats() if sur else None
         if sur:
             ctx.surrogate_mode(False)
+        ctx.set_precision(fj.FJ_PREC_FP64)
         return dt, out, vir, st, hist, "".join(kinds), t_train
 
-    t_p, out_p, vir_p, _, h_p, k_p, _ = run(False)
-    t_s, out_s, vir_s, st, h_s, k_s, t_tr = run(True)
-    return {"tol_rho_l": tol, "ms_plain": 1e3 * t_p, "ms_surrogate": 1e3 * t_s, "speedup": t_p / t_s,
-            "iterations_plain": len(k_p), "iterations_surrogate_run": k_s, "ms_training": 1e3 * t_tr,
-            "rho_l_diff_converged": drho(out_s, out_p),
-            "virial_WK_plain": float(vir_p[4]), "virial_WK_surrogate": float(vir_s[4]),
+    t_p, out_p, vir_p, _, h_p, k_p, _ = run(False, False)
+    t_m, out_m, vir_m, _, h_m, k_m, _ = run(False, True)
+    t_s, out_s, vir_s, st, h_s, k_s, t_tr = run(True, True)
+    return {"tol_rho_l": tol, "ms_plain": 1e3 * t_p, "ms_mixed": 1e3 * t_m, "ms_surrogate": 1e3 * t_s,
+            "speedup": t_p / t_s, "speedup_mixed_only": t_p / t_m,
+            "iterations_plain": len(k_p), "iterations_mixed": len(k_m), "iterations_surrogate_run": k_s,
+            "ms_training": 1e3 * t_tr,
+            "rho_l_diff_converged": drho(out_s, out_p), "rho_l_diff_converged_mixed": drho(out_m, out_p),
+            "virial_WK_plain": float(vir_p[4]), "virial_WK_mixed": float(vir_m[4]),
+            "virial_WK_surrogate": float(vir_s[4]),
             "d_rho_l_plain": [None if x is None else float("%.3g" % x) for x in h_p],
             "d_rho_l_surrogate_run": [None if x is None else float("%.3g" % x) for x in h_s],
             "judge_last_served_iteration": {k: st[k] for k in ("judge_err_r_med", "judge_err_r_p95", "judge_err_z_med",
                                                              "judge_err_z_p95", "n_train", "nodes_sur", "fallback",
                                                              "judged")},
-            "note": "P = Staeckel iteration, S = served by the network (%.0f%% of nodes judged), R = served iteration "
+            "note": "speedup: plain FP64 over surrogate + mixed node phase; P = Staeckel iteration (mixed), "
+                    "S = served by the network (%.0f%% of nodes judged), R = served iteration "
                     "rejected by the judge (median error above %.0f%%) and redone with Staeckel; training: %d epochs "
                     "after iteration 1, then one pass per served iteration" % (100 * judge, 100 * gate, epochs)}
 
