@@ -143,58 +143,6 @@ struct JInt32 {
     __device__ double action(double integral) const { return pre * integral / PI; }
 };
 
-// The 2(NK-1) off-centre values of one FP32 rule, element k at p[k * ld] (LDS).
-struct FvStrided32 {
-    float *p;
-    int ld;
-    __device__ float &operator[](int k) const { return p[k * ld]; }
-};
-
-// qk_rule in FP32 (QUADPACK's single-precision qk21 / qk15): the same
-// visiting order and sums, with the abscissae and weights rounded to float.
-template <int NK, class F>
-__device__ __forceinline__ QKOutT<float> qk_rule32(const F &f, float a, float b, const FvStrided32 &fv) {
-    const double *xgk = NK == 11 ? kXGK21 : kXGK15;
-    const double *wgk = NK == 11 ? kWGK21 : kWGK15;
-    const double *wg = NK == 11 ? kWG10 : kWG7;
-    constexpr int NG = (NK - 1) / 2;
-    const float center = 0.5f * (a + b), half = 0.5f * (b - a), ahalf = fabsf(half);
-    const float fc = f(center);
-    float rg = 0.f, rk = fc * (float)wgk[NK - 1], rabs = fabsf(rk);
-    if (NK % 2 == 0) rg = fc * (float)wg[NK / 2 - 1];
-    float f1 = 0.f;
-#pragma unroll 1
-    for (int s = 0; s < 2 * (NK - 1); s++) {
-        const int m = s >> 1;
-        const int jj = m < NG ? 2 * m + 1 : 2 * (m - NG);
-        const float absc = half * (float)xgk[jj];
-        const float fx = f((s & 1) ? center + absc : center - absc);
-        fv[s] = fx;
-        if (s & 1) {
-            const float fs = f1 + fx, wk = (float)wgk[jj];
-            if (m < NG) rg += (float)wg[m] * fs;
-            rk += wk * fs;
-            rabs += wk * (fabsf(f1) + fabsf(fx));
-        } else {
-            f1 = fx;
-        }
-    }
-    const float mean = rk * 0.5f;
-    float rasc = (float)wgk[NK - 1] * fabsf(fc - mean);
-#pragma unroll
-    for (int j = 0; j < NK - 1; j++) {
-        const int m = (j & 1) ? (j >> 1) : NG + (j >> 1);
-        rasc += (float)wgk[j] * (fabsf(fv[2 * m] - mean) + fabsf(fv[2 * m + 1] - mean));
-    }
-    const float err = (rk - rg) * half;
-    QKOutT<float> o;
-    o.result = rk * half;
-    o.resabs = rabs * ahalf;
-    o.resasc = rasc * ahalf;
-    o.abserr = qk_rescale(err, o.resabs, o.resasc);
-    return o;
-}
-
 // DF (df_hg, DF.cpp:65-76, and the rotation term DF.cpp:114-119) in FP32.
 struct DF32 {
     float A, B, dphi_h, dz_h, dphi_g, dz_g, norm, J0, invJ0, chi, k;

@@ -70,6 +70,37 @@ int fjh_qags(double (*f)(double, void *), void *p, double a, double b, double ep
     return 0;
 }
 
+// The float instance of the stepper (QUADPACK's single-precision qagse with
+// the FP32 QK21 rule of the mixed-precision kernels) on a float integrand.
+struct CFnF {
+    float (*f)(float, void *);
+    void *p;
+    float operator()(float x) const { return f(x, p); }
+};
+struct FvF {
+    float v[20];
+    float &operator[](int k) { return v[k]; }
+};
+int fjh_qags32(float (*f)(float, void *), void *p, float a, float b, float epsabs, float epsrel, int limit,
+               int cap, float *result, int *ier, int *nqk) {
+    std::vector<float> ws((size_t)3 * (5 * (cap + 1) + 60));   // 3 lanes, interleaved
+    const QagsWorkT<float> w = lane_work(ws.data(), cap, 1, 3);
+    const CFnF fn{f, p};
+    QagsT<float> q;
+    qags_begin(q, a, b, epsabs, epsrel, limit);
+    FvF fv;
+    int n = 0;
+    while (q.state != 3) {
+        const QKOutT<float> r = qk_rule32<11>(fn, q.next_a, q.next_b, fv);
+        n++;
+        qags_consume(q, w, cap, r);
+    }
+    *result = q.result;
+    *ier = q.overflow ? -1 : q.ier;
+    *nqk = n;
+    return 0;
+}
+
 // Radial bracket of radial_interp (bracket table + walk) at n radii.
 int fjh_bracket(int nr, const double *ar, int n, const double *r, int *bot) {
     HostPot H(nr, ar, 1, ar, ar, 1);

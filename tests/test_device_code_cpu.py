@@ -31,6 +31,9 @@ def P(a):
     return np.ascontiguousarray(a, dtype=np.float64).ctypes.data_as(dp)
 
 
+FNF = C.CFUNCTYPE(C.c_float, C.c_float, C.c_void_p)
+
+
 @pytest.fixture(scope="module")
 def H():
     import subprocess
@@ -49,6 +52,8 @@ def H():
     lib.fjh_divrn_check.argtypes = [C.c_longlong, C.c_ulonglong, C.c_int]
     lib.fjh_qags.argtypes = [O.FN, C.c_void_p, C.c_double, C.c_double, C.c_double, C.c_double, C.c_int, C.c_int,
                              C.c_int, dp, C.POINTER(C.c_int), C.POINTER(C.c_int)]
+    lib.fjh_qags32.argtypes = [FNF, C.c_void_p, C.c_float, C.c_float, C.c_float, C.c_float, C.c_int, C.c_int,
+                               C.POINTER(C.c_float), C.POINTER(C.c_int), C.POINTER(C.c_int)]
     return lib
 
 
@@ -188,3 +193,38 @@ def test_reciprocal_division_is_ieee_division(H, mode):
     radial cell widths and quadratic-correction products of a 200-radius log
     grid); zeros keep the quotient's sign."""
     assert H.fjh_divrn_check(4_000_000, 12345 + mode, mode) == 0
+
+
+def test_qags_float_instance(H):
+    """The float instance of the QAGS stepper with the FP32 QK21 rule (the
+    mixed-precision kernels' bookkeeping: QUADPACK's single-precision qagse)
+    against scipy's QUADPACK in double precision on integrands that subdivide
+    and extrapolate: the result meets the requested tolerance (plus FP32
+    rounding of the integrand), or the stepper reports roundoff (ier 2), the
+    workspace cap (-1, the kernels' FP64 fallback) or ier 5 (divergent
+    extrapolation), as dqagse would."""
+    from scipy.integrate import quad
+    import math
+    fs = [lambda x: x * x * math.exp(-x),
+          lambda x: math.log(x) / math.sqrt(x) if x > 0 else 0.0,
+          lambda x: math.sqrt(abs(x - 0.3137)),
+          lambda x: math.sqrt(max(0.0, math.cos(3 * x) + 0.2)) * math.sin(x),
+          lambda x: 1.0 / (1e-3 + (x - 0.5) ** 2)]
+    stats = []
+    for f in fs:
+        cb = FNF(lambda x, prm: float(f(x)))
+        for a, b in ((0.0, 1.0), (0.0, 3.0)):
+            ref = quad(f, a, b, epsabs=1e-12, epsrel=1e-12, limit=1000)[0]
+            for epsrel in (1e-3, 1e-5):
+                r, ier, nqk = C.c_float(), C.c_int(), C.c_int()
+                H.fjh_qags32(cb, None, a, b, 1e-5, epsrel, 1000, 64, C.byref(r), C.byref(ier), C.byref(nqk))
+                err = abs(r.value - ref)
+                tol = max(1e-5, epsrel * abs(ref)) + 1e-6 * abs(ref) + 1e-6
+                stats.append((ier.value, nqk.value, err / tol))
+                assert ier.value in (0, 2, 5, -1), (f, a, b, ier.value)
+                if ier.value == 0:
+                    assert err <= 3 * tol, (f, a, b, epsrel, r.value, ref)
+                else:
+                    assert err <= 100 * tol, (f, a, b, epsrel, ier.value, r.value, ref)
+    print(stats)
+    assert sum(1 for s in stats if s[0] == 0) >= len(stats) // 2
