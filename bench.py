@@ -26,6 +26,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 PEAK_FP64 = 78.6   # TFLOP/s, MI355X
+PEAK_FP32 = 157.3  # TFLOP/s, MI355X vector FP32
 # HBM bytes per k_integrals launch of this workload (timed launches of the default
 # schedule): rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE runs, scaled by what the
 # counters read for a 1 GiB device copy (fetch x 2.0, write x 1.0); the runs and
@@ -70,12 +71,16 @@ def cpu_baseline(npoly, ntheta, nr, ar, tables, delta, sample_every=8, threads=N
                       f"oracle C restatement, {threads} threads, {dt:.1f} s"}
 
 
-def mixed_leg(ctx, model, prev, last, steps, pts):
-    """Mixed precision (FP32 action integrands and DF, FP64 roots, Delta and
-    Poisson) with the exact block-scan Poisson solve.  Accuracy: one iteration
-    from the same potential as the last FP64 iteration, every coefficient's
-    difference scaled by the monopole at its radius; speed: `steps`
-    iterations continuing from there."""
+def mixed_leg(ctx, model, prev, last, steps, pts, npoly):
+    """Mixed precision (FP32 action integrands, Kronrod rules, QAGS
+    bookkeeping and DF; FP64 roots, Delta and Poisson) with the exact
+    block-scan Poisson solve.  Accuracy: one iteration from the same
+    potential as the last FP64 iteration, every coefficient's difference
+    scaled by the monopole at its radius; speed: `steps` iterations
+    continuing from there.  Roofline: the counted QK21 flops of the FP32
+    integrator (rules of the FP32 pass: all rules less those of the FP64
+    overflow pass, which the counters do not split, so the figure is a
+    bound from above) over its launch time, against the FP32 vector peak."""
     import numpy as np
     import torch
     import fjmodels_amd as fj
@@ -96,8 +101,14 @@ def mixed_leg(ctx, model, prev, last, steps, pts):
     dt = (time.perf_counter() - t0) / steps
     ctx.set_precision(fj.FJ_PREC_FP64)
     model.mode = fj.FJ_POISSON_QUADPACK
+    qk = float(np.mean([x["qk_calls"] for x in st]))
+    t_int = float(np.mean([x["ms_integrals"] for x in st])) * 1e-3
+    ach = qk * f_qk21(npoly) / t_int / 1e12
     return {"value": pts / dt, "ms_per_step": 1e3 * dt, "poisson": "blockscan",
             "err_vs_fp64": err,
+            "roofline": {"bound": "mfma", "pipe": "VALU FP32", "kernel": "k_integrals_f32",
+                         "achieved": ach, "peak": PEAK_FP32, "unit": "TFLOP/s", "frac": ach / PEAK_FP32,
+                         "ms_per_launch": t_int * 1e3},
             "phases_ms": {k: float(np.mean([x[k] for x in st])) for k in
                           ("ms_setup", "ms_integrals", "ms_overflow", "ms_moments", "ms_poisson")},
             "counters": {k: int(np.mean([x[k] for x in st])) for k in ("qk_calls", "overflow_tasks", "integral_lanes")}}
@@ -380,7 +391,7 @@ def main():
                       "unbound_nodes", "bracket_failures", "nonfinite_nodes")},
     }
     if not args.no_mixed:
-        line["mixed"] = mixed_leg(ctx, model, prev, last, args.steps, pts)
+        line["mixed"] = mixed_leg(ctx, model, prev, last, args.steps, pts, npoly)
     if not args.no_surrogate:
         line["surrogate"] = surrogate_leg(ctx, model, guess_tables)
     if not args.no_batch:
