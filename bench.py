@@ -77,10 +77,11 @@ def mixed_leg(ctx, model, prev, last, steps, pts, npoly):
     block-scan Poisson solve.  Accuracy: one iteration from the same
     potential as the last FP64 iteration, every coefficient's difference
     scaled by the monopole at its radius; speed: `steps` iterations
-    continuing from there.  Roofline: the counted QK21 flops of the FP32
-    integrator (rules of the FP32 pass: all rules less those of the FP64
-    overflow pass, which the counters do not split, so the figure is a
-    bound from above) over its launch time, against the FP32 vector peak."""
+    continuing from there.  Roofline: the counted QK21 flops of all rules of
+    the iteration over the FP32 integrator's launch time, against the FP32
+    vector peak; the counters do not split off the rules of the FP64
+    overflow pass (0.9 % of the tasks), so this bounds the FP32 pass from
+    above."""
     import numpy as np
     import torch
     import fjmodels_amd as fj
