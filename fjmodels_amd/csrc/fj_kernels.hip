@@ -205,19 +205,6 @@ __device__ inline void load_jint(const NodeBuf &nb, long long i, int type, JInt3
     f.pre = type == 0 ? rtwo * g.k[0] * g.k[3] : 2 * rtwo * g.k[0] * (PIH - g.k[2]);
 }
 
-// The 20 off-centre values of one QK21 rule kept as floats (mixed mode: the
-// integrand is an FP32 value, so the float holds it exactly).
-struct FvRefF {
-    float *p;
-    __device__ FvRefF &operator=(double x) { *p = (float)x; return *this; }
-    __device__ operator double() const { return (double)*p; }
-};
-struct FvStridedF {
-    float *p;
-    int ld;
-    __device__ FvRefF operator[](int k) const { return FvRefF{p + k * ld}; }
-};
-
 // Device counters
 struct Counters {
     unsigned long long bound, tasks_u, tasks_v, qk, overflow, brent, grad;
