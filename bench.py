@@ -115,8 +115,8 @@ def mixed_leg(ctx, model, prev, last, steps, pts, npoly):
             "counters": {k: int(np.mean([x[k] for x in st])) for k in ("qk_calls", "overflow_tasks", "integral_lanes")}}
 
 
-def surrogate_leg(ctx, model, guess_tables, tol=1e-4, max_iter=30, keep=0.15, epochs=16, judge=0.05,
-                  max_sur=8, sur_floor=4e-3, gate=0.05):
+def surrogate_leg(ctx, model, guess_tables, tol=1e-4, max_iter=30, keep=0.15, epochs=4, judge=0.05,
+                  max_sur=8, sur_floor=4e-3, gate=0.05, collect_from=3, serve_from=5):
     """The Hernquist model converged three times from the guess potential to
     the same tolerance: successive density coefficients differ by less than
     `tol` of the monopole (the driver's convergence test on rho_l).  All
@@ -125,9 +125,10 @@ def surrogate_leg(ctx, model, guess_tables, tol=1e-4, max_iter=30, keep=0.15, ep
     plain: FP64 Staeckel actions every iteration.
     mixed: the same with the mixed-precision node phase.
     surrogate (with the mixed node phase for its Staeckel iterations):
-    iterations 0-1 Staeckel, keeping a random `keep` of the bound
-    nodes' actions as training samples; then the network is trained
-    (`epochs` passes) and serves (Jr, Jz) while a random `judge` fraction of
+    Staeckel iterations, keeping from iteration `collect_from` on a random
+    `keep` of the bound nodes' actions as training samples; at iteration
+    `serve_from` the network is trained (`epochs` passes) and serves
+    (Jr, Jz) while a random `judge` fraction of
     the nodes is recomputed with Staeckel, scored, added to the training set
     and used for one fine-tuning pass after each iteration; once the served
     iterations change rho_l by less than `sur_floor` (the network's noise
@@ -158,10 +159,10 @@ def surrogate_leg(ctx, model, guess_tables, tol=1e-4, max_iter=30, keep=0.15, ep
         phase = "collect" if sur else "plain"
         n_sur = 0
         for it in range(max_iter):
-            if sur and it == 0:
+            if sur and it == collect_from:
                 ctx.surrogate_init(seed=7, capacity=1 << 20)
                 ctx.surrogate_collect(keep)
-            if phase == "collect" and it == 2:
+            if phase == "collect" and it == serve_from:
                 ctx.surrogate_collect(1.0)            # from now on: the judged nodes
                 tt = time.perf_counter()
                 ctx.surrogate_train(epochs, batch=32768, lr=3e-3)
@@ -224,7 +225,8 @@ def surrogate_leg(ctx, model, guess_tables, tol=1e-4, max_iter=30, keep=0.15, ep
             "note": "speedup: plain FP64 over surrogate + mixed node phase; P = Staeckel iteration (mixed), "
                     "S = served by the network (%.0f%% of nodes judged), R = served iteration "
                     "rejected by the judge (median error above %.0f%%) and redone with Staeckel; training: %d epochs "
-                    "after iteration 1, then one pass per served iteration" % (100 * judge, 100 * gate, epochs)}
+                    "before iteration %d, then one pass per served iteration" % (100 * judge, 100 * gate, epochs,
+                                                                                  serve_from)}
 
 
 def batch_leg(ctx, n_models=8, iters=2, seed=5):
