@@ -162,34 +162,43 @@ __global__ void k_tanh_grad(long long n, const float *dH, const float *h, float 
     dp16[i] = f2bf(v);
 }
 
-// db[c] = sum_r X[r][c] for c < N (fixed order: 256 strided partials, then a tree)
-__global__ void k_colsum(long long M, int N, int ldx, const float *X, float *db) {
+// Column sums db[c] = sum_r X[r][c] of an M x N row-major matrix (ld ldx,
+// N <= 256), in two fixed-order stages.  Stage 1: block b sums rows
+// [b R, (b+1) R) with row-coalesced reads (thread t takes column t % N of
+// every (256 / N)-th row from t / N) into part[b][0..N).  Stage 2: one block
+// per column adds the nb partials (a 256-wide tree).
+__device__ __forceinline__ float as_f32(float x) { return x; }
+__device__ __forceinline__ float as_f32(bf16_t x) { return bf2f(x); }
+template <class T>
+__global__ void __launch_bounds__(256) k_colsum_part(long long M, int N, int ldx, long long R, const T *X,
+                                                     float *part) {
     __shared__ float red[256];
-    const int c = blockIdx.x;
+    const int t = threadIdx.x, rstep = 256 / N, c = t % N, r0 = t / N;
     float s = 0.f;
-    for (long long r = threadIdx.x; r < M; r += 256) s += X[r * ldx + c];
-    red[threadIdx.x] = s;
-    __syncthreads();
-    for (int st = 128; st > 0; st >>= 1) {
-        if ((int)threadIdx.x < st) red[threadIdx.x] += red[threadIdx.x + st];
-        __syncthreads();
+    if (r0 < rstep) {
+        const long long rb = (long long)blockIdx.x * R, re = rb + R < M ? rb + R : M;
+        for (long long r = rb + r0; r < re; r += rstep) s += as_f32(X[r * ldx + c]);
     }
-    if (threadIdx.x == 0) db[c] = red[0];
+    red[t] = s;
+    __syncthreads();
+    if (t < N) {
+        float v = 0.f;
+        for (int k = 0; k < rstep; k++) v += red[k * N + t];
+        part[(long long)blockIdx.x * N + t] = v;
+    }
 }
-
-// bf16 copy of an fp32 column-sum source: db from a bf16 matrix
-__global__ void k_colsum_bf(long long M, int N, int ldx, const bf16_t *X, float *db) {
+__global__ void __launch_bounds__(256) k_colsum_fin(int nb, int N, const float *part, float *db) {
     __shared__ float red[256];
-    const int c = blockIdx.x;
+    const int c = blockIdx.x, t = threadIdx.x;
     float s = 0.f;
-    for (long long r = threadIdx.x; r < M; r += 256) s += bf2f(X[r * ldx + c]);
-    red[threadIdx.x] = s;
+    for (int b = t; b < nb; b += 256) s += part[(long long)b * N + c];
+    red[t] = s;
     __syncthreads();
     for (int st = 128; st > 0; st >>= 1) {
-        if ((int)threadIdx.x < st) red[threadIdx.x] += red[threadIdx.x + st];
+        if (t < st) red[t] += red[t + st];
         __syncthreads();
     }
-    if (threadIdx.x == 0) db[c] = red[0];
+    if (t == 0) db[c] = red[0];
 }
 
 // Adam step on n fp32 parameters; writes the bf16 copy used by the GEMMs.
@@ -257,15 +266,36 @@ struct Mlp {
 
 inline int mlp_blocks(long long n, int t = 256) { return (int)((n + t - 1) / t); }
 
-// C[i][j] = sum_z part[z][i][j] (fixed order), C with leading dimension ldc
-__global__ void k_splitk_sum(int M, int N, int ldc, int nz, const float *part, float *C) {
-    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= (long long)M * N) return;
-    const long long r = i / N;
-    const int c = (int)(i % N);
+// db[0..N) = column sums of X [M][N] (ld ldx), through the split-K buffer.
+template <class T>
+inline void colsum(hipStream_t s, long long M, int N, int ldx, const T *X, float *db, SplitK &sk) {
+    long long nb = (M + 127) / 128;
+    if (nb > 256) nb = 256;
+    if (nb * N > sk.cap) nb = sk.cap / N;
+    if (nb < 1) nb = 1;
+    const long long R = (M + nb - 1) / nb;
+    k_colsum_part<T><<<(unsigned)nb, 256, 0, s>>>(M, N, ldx, R, X, sk.part);
+    k_colsum_fin<<<N, 256, 0, s>>>((int)nb, N, sk.part, db);
+}
+
+// C[i][j] = sum_z part[z][i][j] (fixed order), C with leading dimension
+// ldc.  A block takes 64 consecutive outputs; its four waves sum the z in
+// four interleaved groups (wave w: z = w, w + 4, ...), which then add in
+// wave order.
+__global__ void __launch_bounds__(256) k_splitk_sum(int M, int N, int ldc, int nz, const float *part, float *C) {
+    __shared__ float red[4][64];
+    const int t = threadIdx.x, o = t & 63, w = t >> 6;
+    const long long i = (long long)blockIdx.x * 64 + o, MN = (long long)M * N;
     float s = 0.f;
-    for (int z = 0; z < nz; z++) s += part[(long long)z * M * N + i];
-    C[r * ldc + c] = s;
+    if (i < MN)
+        for (int z = w; z < nz; z += 4) s += part[(long long)z * MN + i];
+    red[w][o] = s;
+    __syncthreads();
+    if (w == 0 && i < MN) {
+        const long long r = i / N;
+        const int c = (int)(i % N);
+        C[r * ldc + c] = ((red[0][o] + red[1][o]) + red[2][o]) + red[3][o];
+    }
 }
 
 template <bool TA, bool TB>
@@ -275,9 +305,9 @@ inline void gemm(hipStream_t s, int M, int N, int K, const bf16_t *A, int lda, c
     // few output tiles and a long K (weight gradients): split K over blocks
     const long long tiles = (long long)grid.x * grid.y;
     int nz = 1;
-    if (sk && !acc && tiles < 256 && K >= 2048) {
-        nz = (int)((256 + tiles - 1) / tiles);
-        const int maxz = K / 1024;
+    if (sk && !acc && tiles < 1024 && K >= 2048) {
+        nz = (int)((1024 + tiles - 1) / tiles);   // about four blocks per CU
+        const int maxz = K / 256;
         if (nz > maxz) nz = maxz;
         while (nz > 1 && (long long)nz * M * N > sk->cap) nz--;
     }
@@ -288,7 +318,7 @@ inline void gemm(hipStream_t s, int M, int N, int K, const bf16_t *A, int lda, c
     const int kc = ((K + nz - 1) / nz + GK - 1) / GK * GK;
     grid.z = nz;
     k_gemm_bf16<TA, TB><<<grid, 256, 0, s>>>(M, N, K, A, lda, B, ldb, sk->part, N, 0, kc, (long long)M * N);
-    k_splitk_sum<<<(unsigned)(((long long)M * N + 255) / 256), 256, 0, s>>>(M, N, ldc, nz, sk->part, C);
+    k_splitk_sum<<<(unsigned)(((long long)M * N + 63) / 64), 256, 0, s>>>(M, N, ldc, nz, sk->part, C);
 }
 
 // Forward of rows x16 [M][5] (normalised, bf16) -> y [M][2] (normalised
@@ -319,12 +349,12 @@ inline void mlp_backward(hipStream_t s, Mlp &n, long long M, const float *tgt, c
     // output layer
     gemm<true, false>(s, MLP_H, MLP_OUT, (int)M, n.h16[2], MLP_H, n.dy16, MLP_OUT, n.g + L::W4, MLP_OUT, false,
                       &n.sk);
-    k_colsum_bf<<<MLP_OUT, 256, 0, s>>>(M, MLP_OUT, MLP_OUT, n.dy16, n.g + L::B4);
+    colsum(s, M, MLP_OUT, MLP_OUT, n.dy16, n.g + L::B4, n.sk);
     gemm<false, true>(s, (int)M, MLP_H, MLP_OUT, n.dy16, MLP_OUT, n.p16 + L::W4, MLP_OUT, n.dh, MLP_H);
     const int Wo[3] = {L::W1, L::W2, L::W3}, Bo[3] = {L::B1, L::B2, L::B3};
     for (int l = 2; l >= 0; l--) {
         k_tanh_grad<<<mlp_blocks(M * MLP_H), 256, 0, s>>>(M * MLP_H, n.dh, n.h32[l], n.dp32, n.d16);
-        k_colsum<<<MLP_H, 256, 0, s>>>(M, MLP_H, MLP_H, n.dp32, n.g + Bo[l]);
+        colsum(s, M, MLP_H, MLP_H, n.dp32, n.g + Bo[l], n.sk);
         const bf16_t *in = l > 0 ? n.h16[l - 1] : n.x16;
         const int K = l > 0 ? MLP_H : MLP_IN;
         gemm<true, false>(s, K, MLP_H, (int)M, in, K, n.d16, MLP_H, n.g + Wo[l], MLP_H, false, &n.sk);
