@@ -134,8 +134,9 @@ def surrogate_leg(ctx, model, guess_tables, tol=1e-4, max_iter=30, keep=0.15, ep
     iterations change rho_l by less than `sur_floor` (the network's noise
     level) or after `max_sur` of them, Staeckel iterations run to `tol`.
     The judge gates the network: a served iteration whose judged median
-    relative error of Jr or Jz exceeds `gate` is discarded (the previous
-    potential is restored) and the run continues with Staeckel.
+    error of Jr or Jz, relative to the node's total action and weighted by
+    the node's share of the density sum, exceeds `gate` is discarded (the
+    previous potential is restored) and the run continues with Staeckel.
 
     Reports the wall time of the three runs (training included), the
     iteration counts, the differences of the converged rho_l from the plain
@@ -174,7 +175,7 @@ def surrogate_leg(ctx, model, guess_tables, tol=1e-4, max_iter=30, keep=0.15, ep
             out = model.iterate()[0]
             if phase == "sur":
                 js = ctx.surrogate_stats()
-                if max(js["judge_err_r_med"], js["judge_err_z_med"]) > gate:
+                if max(js["judge_werr_r_med"], js["judge_werr_z_med"]) > gate:
                     kinds.append("R")                 # rejected by the judge
                     ctx.surrogate_mode(False)
                     ctx.surrogate_collect(0.0)
@@ -219,12 +220,14 @@ def surrogate_leg(ctx, model, guess_tables, tol=1e-4, max_iter=30, keep=0.15, ep
             "virial_WK_surrogate": float(vir_s[4]),
             "d_rho_l_plain": [None if x is None else float("%.3g" % x) for x in h_p],
             "d_rho_l_surrogate_run": [None if x is None else float("%.3g" % x) for x in h_s],
-            "judge_last_served_iteration": {k: st[k] for k in ("judge_err_r_med", "judge_err_r_p95", "judge_err_z_med",
+            "judge_last_served_iteration": {k: st[k] for k in ("judge_werr_r_med", "judge_werr_z_med",
+                                                             "judge_err_r_med", "judge_err_r_p95", "judge_err_z_med",
                                                              "judge_err_z_p95", "n_train", "nodes_sur", "fallback",
                                                              "judged")},
             "note": "speedup: plain FP64 over surrogate + mixed node phase; P = Staeckel iteration (mixed), "
                     "S = served by the network (%.0f%% of nodes judged), R = served iteration "
-                    "rejected by the judge (median error above %.0f%%) and redone with Staeckel; training: %d epochs "
+                    "rejected by the judge (density-weighted median error above %.0f%%) and redone with Staeckel; "
+                    "training: %d epochs "
                     "before iteration %d, then one pass per served iteration" % (100 * judge, 100 * gate, epochs,
                                                                                   serve_from)}
 

@@ -56,7 +56,8 @@ class Stats(C.Structure):
 class SurrogateStats(C.Structure):
     _fields_ = ([(n, C.c_longlong) for n in ("n_train", "nodes_sur", "fallback", "judged")] +
                 [(n, C.c_double) for n in ("judge_err_r_med", "judge_err_r_p95", "judge_err_z_med",
-                                           "judge_err_z_p95", "ms_surrogate", "ms_train", "last_loss")])
+                                           "judge_err_z_p95", "ms_surrogate", "ms_train", "last_loss",
+                                           "judge_werr_r_med", "judge_werr_z_med")])
 
 
 class ModelSpec(C.Structure):

@@ -421,6 +421,7 @@ struct SurArgs {
     int *judge_node;
     float *judge_J;         // [2 * judged]
     unsigned int *nfall;
+    fj_df_params df;        // weights of the judged nodes' errors
 };
 
 __host__ __device__ inline unsigned int hash32(unsigned int x) {
@@ -715,18 +716,24 @@ __global__ void k_sur_box(long long n, const float *X, SurNorm nm, float *part) 
 // Errors of the predicted actions at the judged nodes, after the Staeckel
 // pass: |J - J_St| / (Jr_St + Jz_St + |Jphi|), the fraction of the DF's
 // action combination h or g that the error moves (-1: unbound node).
+// err[2i], err[2i+1]: the network's Jr, Jz errors at judged node i over the
+// node's total action; err[2n + i]: the node's weight in the density sum,
+// quadrature weight x Ve^3 x f(J) with the Staeckel actions.
 __global__ void k_sur_judge_err(SurArgs a, unsigned int n, float *err) {
     const unsigned int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const long long gid = a.judge_node[i];
     const int p = a.p0 + (int)(gid / NQ), node = (int)(gid % NQ);
     double V[3], w;
-    node_velocity(a.quad, node, a.Ve[p], V, w);
+    const double Ve = a.Ve[p];
+    node_velocity(a.quad, node, Ve, V, w);
     const double Sr = a.nb.Jr[gid], Sz = a.nb.Jz[gid];
-    if (!(Sr < 1e30 && Sz < 1e30)) { err[2 * i] = -1.f; err[2 * i + 1] = -1.f; return; }
-    const double scale = Sr + Sz + fabs(a.R[p] * V[1]);
+    if (!(Sr < 1e30 && Sz < 1e30)) { err[2 * i] = -1.f; err[2 * i + 1] = -1.f; err[2 * n + i] = 0.f; return; }
+    const double Jphi = a.R[p] * V[1];
+    const double scale = Sr + Sz + fabs(Jphi);
     err[2 * i] = (float)(fabs(a.judge_J[2 * i] - Sr) / scale);
     err[2 * i + 1] = (float)(fabs(a.judge_J[2 * i + 1] - Sz) / scale);
+    err[2 * n + i] = (float)(w * Ve * Ve * Ve * d_df(a.df, Sr, Jphi, Sz));
 }
 
 struct IntegArgs {
@@ -2272,19 +2279,26 @@ int sur_judge_stats(fj_ctx *c, const SurArgs &a, unsigned int nj) {
     float *d_err = reinterpret_cast<float *>(c->d_judge_S);
     k_sur_judge_err<<<(nj + 255) / 256, 256, 0, c->stream>>>(a, nj, d_err);
     HIPCHK(c, hipGetLastError());
-    std::vector<float> E(2 * (size_t)nj);
+    std::vector<float> E(3 * (size_t)nj);
     HIPCHK(c, hipMemcpyAsync(E.data(), d_err, E.size() * sizeof(float), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     for (int k = 0; k < 2; k++) {
-        std::vector<float> e;
+        std::vector<std::pair<float, float>> e;   // (error, density weight)
         e.reserve(nj);
         for (unsigned int i = 0; i < nj; i++)
-            if (E[2 * i + k] >= 0) e.push_back(E[2 * i + k]);
+            if (E[2 * i + k] >= 0) e.push_back({E[2 * i + k], E[2 * (size_t)nj + i]});
         if (e.empty()) continue;
         std::sort(e.begin(), e.end());
-        const double med = e[e.size() / 2], p95 = e[(size_t)(0.95 * (e.size() - 1))];
-        if (k == 0) { c->sst.judge_err_r_med = med; c->sst.judge_err_r_p95 = p95; }
-        else { c->sst.judge_err_z_med = med; c->sst.judge_err_z_p95 = p95; }
+        const double med = e[e.size() / 2].first, p95 = e[(size_t)(0.95 * (e.size() - 1))].first;
+        // weighted median: the error below which half of the judged density weight lies
+        double wtot = 0, acc = 0, wmed = e.back().first;
+        for (const auto &x : e) wtot += x.second;
+        for (const auto &x : e) {
+            acc += x.second;
+            if (acc >= 0.5 * wtot) { wmed = x.first; break; }
+        }
+        if (k == 0) { c->sst.judge_err_r_med = med; c->sst.judge_err_r_p95 = p95; c->sst.judge_werr_r_med = wmed; }
+        else { c->sst.judge_err_z_med = med; c->sst.judge_err_z_p95 = p95; c->sst.judge_werr_z_med = wmed; }
     }
     return 0;
 }
@@ -2355,7 +2369,7 @@ int run_nodes(fj_ctx *c, int npts) {
         unsigned int nsel = 0, njudge = 0, nfall = 0;
         SurArgs a{c->d_pR, c->d_pz, c->d_pPhi, c->d_pVe, c->d_pVc, (int)p0, np, c->quad, nb, c->nm,
                   c->d_sx16, c->d_sy, c->sur_judge, c->sur_jzmin, hash32(c->sur_seed + 7919u * c->sur_iter),
-                  c->d_cnt + 6, c->d_sel, c->d_cnt + 7, c->d_judge_node, c->d_judge_J, c->d_cnt + 8};
+                  c->d_cnt + 6, c->d_sel, c->d_cnt + 7, c->d_judge_node, c->d_judge_J, c->d_cnt + 8, c->df};
         if (sur) {
             // surrogate for every bound node, Staeckel for fallbacks and the judged sample
             HIPCHK(c, hipEventRecord(ev[2], c->stream));

@@ -196,10 +196,15 @@ typedef struct {
     long long fallback;    /* bound nodes sent to the Staeckel fudge: outside the training box,
                               |E| < 0.02 |Phi|, or predicted Jz below jz_min */
     long long judged;      /* random sample computed both ways */
-    double judge_err_r_med, judge_err_r_p95, judge_err_z_med, judge_err_z_p95;  /* |J - J_St| / max(J_St, 1e-3 J0) */
+    /* errors of the judged nodes' network actions, |J - J_St| / (Jr_St + Jz_St + |Jphi|): median and
+     * 95th percentile over the judged nodes */
+    double judge_err_r_med, judge_err_r_p95, judge_err_z_med, judge_err_z_p95;
     double ms_surrogate;   /* inputs + inference + decisions of the last node phase */
     double ms_train;       /* last fj_surrogate_train call */
     double last_loss;      /* mean squared error of the last epoch (normalised targets) */
+    /* the same errors' median weighted by each judged node's share of the density sum
+     * (quadrature weight x Ve^3 x f(J_St)) */
+    double judge_werr_r_med, judge_werr_z_med;
 } fj_surrogate_stats;
 
 /* Fresh network (Glorot-uniform weights from seed) and a training set of
