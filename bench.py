@@ -338,7 +338,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 "
-                                   "(multi-GPU sharding is described in DESIGN.md, not built)"}))
+                                   "(sharding the grid over GPUs is not built)"}))
         sys.exit(2)
 
     import numpy as np
