@@ -139,7 +139,8 @@ def surrogate_leg(ctx, model, guess_tables, tol=1e-4, max_iter=30, keep=0.15, ep
     previous potential is restored) and the run continues with Staeckel.
 
     Reports the wall time of the three runs (training included), the
-    iteration counts, the differences of the converged rho_l from the plain
+    median wall time of one Staeckel and one served iteration (the judge's
+    rejected ones included), the iteration counts, the differences of the converged rho_l from the plain
     run's (scaled by the monopole) and the virial ratios."""
     import numpy as np
     import torch
@@ -155,7 +156,7 @@ def surrogate_leg(ctx, model, guess_tables, tol=1e-4, max_iter=30, keep=0.15, ep
         model.philP = [tuple(x.copy() for x in guess_tables)]
         torch.cuda.synchronize()
         t0 = time.perf_counter()
-        hist, kinds, t_train = [], [], 0.0
+        hist, kinds, t_train, t_kind = [], [], 0.0, {}
         prev, out = None, None
         phase = "collect" if sur else "plain"
         n_sur = 0
@@ -172,7 +173,10 @@ def surrogate_leg(ctx, model, guess_tables, tol=1e-4, max_iter=30, keep=0.15, ep
                 phase = "sur"
             if phase == "sur":
                 saved = ([x.copy() for x in model.total], [tuple(x.copy() for x in p) for p in model.philP])
+            ti = time.perf_counter()
             out = model.iterate()[0]
+            ti = 1e3 * (time.perf_counter() - ti)
+            t_kind.setdefault("S" if phase == "sur" else "P", []).append(ti)
             if phase == "sur":
                 js = ctx.surrogate_stats()
                 if max(js["judge_werr_r_med"], js["judge_werr_z_med"]) > gate:
@@ -206,15 +210,17 @@ def surrogate_leg(ctx, model, guess_tables, tol=1e-4, max_iter=30, keep=0.15, ep
         if sur:
             ctx.surrogate_mode(False)
         ctx.set_precision(fj.FJ_PREC_FP64)
-        return dt, out, vir, st, hist, "".join(kinds), t_train
+        return dt, out, vir, st, hist, "".join(kinds), t_train, {k: float(np.median(v)) for k, v in t_kind.items()}
 
-    t_p, out_p, vir_p, _, h_p, k_p, _ = run(False, False)
-    t_m, out_m, vir_m, _, h_m, k_m, _ = run(False, True)
-    t_s, out_s, vir_s, st, h_s, k_s, t_tr = run(True, True)
+    t_p, out_p, vir_p, _, h_p, k_p, _, i_p = run(False, False)
+    t_m, out_m, vir_m, _, h_m, k_m, _, i_m = run(False, True)
+    t_s, out_s, vir_s, st, h_s, k_s, t_tr, i_s = run(True, True)
     return {"tol_rho_l": tol, "ms_plain": 1e3 * t_p, "ms_mixed": 1e3 * t_m, "ms_surrogate": 1e3 * t_s,
             "speedup": t_p / t_s, "speedup_mixed_only": t_p / t_m,
             "iterations_plain": len(k_p), "iterations_mixed": len(k_m), "iterations_surrogate_run": k_s,
             "ms_training": 1e3 * t_tr,
+            "ms_median_iteration": {"plain": i_p["P"], "mixed": i_m["P"],
+                                    "surrogate_run_staeckel": i_s["P"], "surrogate_run_served": i_s.get("S")},
             "rho_l_diff_converged": drho(out_s, out_p), "rho_l_diff_converged_mixed": drho(out_m, out_p),
             "virial_WK_plain": float(vir_p[4]), "virial_WK_mixed": float(vir_m[4]),
             "virial_WK_surrogate": float(vir_s[4]),
