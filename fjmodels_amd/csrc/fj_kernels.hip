@@ -843,7 +843,9 @@ k_integrals(TabArgs ta, IntegArgs ia) {
     integrals_loop(ia, f, w, fv);
 }
 
-// Mixed precision: FP32 integrand, FP64 QAGS.
+// Mixed precision: FP32 integrand, Kronrod rule and QAGS bookkeeping (the
+// float instance of the stepper, QUADPACK's qagse); tasks that reach the
+// FP32 interval cap go to the FP64 overflow pass.
 #ifndef FJ_INTEG32_WAVES
 #define FJ_INTEG32_WAVES 4
 #endif

@@ -43,7 +43,11 @@ enum { FJ_GUESS_HERNQUIST = 0, FJ_GUESS_ISOCHRONE = 1, FJ_GUESS_NFW = 2, FJ_GUES
  * (Potential.cpp:161-206), or exact piecewise-linear integration (block scan). */
 enum { FJ_POISSON_QUADPACK = 0, FJ_POISSON_BLOCKSCAN = 1 };
 
-/* Arithmetic of the node phase (DF evaluation and velocity quadrature). */
+/* Arithmetic of the node phase (DF evaluation and velocity quadrature).
+ * FJ_PREC_MIXED: FP32 action integrands, Kronrod rules, QAGS bookkeeping
+ * and DF (tasks that reach the FP32 interval cap rerun in FP64); FP64 Delta,
+ * spheroidal coordinates, turning-point roots and the Poisson solve.
+ * Results are not bitwise those of FJ_PREC_FP64. */
 enum { FJ_PREC_FP64 = 0, FJ_PREC_MIXED = 1 };
 
 /* DF parameters: df/DF.cpp:18-20,36-63 (alpha[2], dphi_h, dz_h, dphi_g, dz_g)
