@@ -31,7 +31,7 @@ PEAK_FP32 = 157.3  # TFLOP/s, MI355X vector FP32
 # schedule): rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE runs, scaled by what the
 # counters read for a 1 GiB device copy (fetch x 2.0, write x 1.0); the runs and
 # their commands are in profiles/k_integrals_hernquist_40x20_fp64.md.
-K_INTEGRALS_TRAFFIC_BYTES = 203.1e9
+K_INTEGRALS_TRAFFIC_BYTES = 185.1e9
 
 
 def f_qk21(npoly):

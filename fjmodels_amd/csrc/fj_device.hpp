@@ -579,13 +579,30 @@ FJ_HD __forceinline__ QKOutT<float> qk_rule32(const F &f, float a, float b, FV &
 // [next_a, next_b] and hands the result to qags_consume().  Lists are
 // 1-based arrays of length cap+1 in a per-lane workspace.
 // ---------------------------------------------------------------------------
-// One lane's view of a list stored interleaved across lanes: element i at
-// p[i * ld], so a wave touching the same index reads one contiguous run.
+// Workspace layout.  FJ_WS_CHUNK_BYTES 0: lists interleaved element by
+// element across lanes (element i of lane L at i * ld + L), so a wave
+// touching the same index reads one contiguous run.  FJ_WS_CHUNK_BYTES B (a
+// power of two): interleaved by B-byte chunks (element i of lane L at
+// (i / CH) * CH * ld + L * CH + i % CH, CH = B / sizeof(T)), so a lane
+// walking consecutive entries (the epsilon table in qelg, the error order in
+// qpsrt) stays within one chunk for CH steps.
+#ifndef FJ_WS_CHUNK_BYTES
+#define FJ_WS_CHUNK_BYTES 64
+#endif
+template <class T>
+FJ_HD constexpr int ws_chunk() { return FJ_WS_CHUNK_BYTES ? FJ_WS_CHUNK_BYTES / (int)sizeof(T) : 1; }
+FJ_HD constexpr long long ws_round(long long n, int ch) { return (n + ch - 1) / ch * ch; }
+
+// One lane's view of a list: p is the lane's first element (chunk).
 template <class T>
 struct Lanes {
     T *p;
     long long ld;
-    FJ_HD T &operator[](int i) const { return p[(long long)i * ld]; }
+    FJ_HD T &operator[](int i) const {
+        constexpr unsigned CH = (unsigned)ws_chunk<T>();
+        const unsigned u = (unsigned)i;
+        return p[(long long)(u / CH) * CH * ld + (u % CH)];
+    }
 };
 
 // A lane's QAGS workspace inside a buffer shared by nlanes lanes (layout at
@@ -598,15 +615,20 @@ struct QagsWorkT {
     long long ld;    // lanes sharing the buffer (uniform)
     int n1;          // cap + 1 (uniform)
     int lane;
-    FJ_HD Lanes<T> list(int k) const { return {ws + (long long)k * n1 * ld + lane, ld}; }
+    static constexpr int CH = ws_chunk<T>();
+    FJ_HD long long l1() const { return ws_round(n1, CH); }   // per-lane length of a list
+    FJ_HD Lanes<T> at(long long off) const { return {ws + off * ld + (long long)lane * CH, ld}; }
+    FJ_HD Lanes<T> list(int k) const { return at(k * l1()); }
     FJ_HD Lanes<T> alist() const { return list(0); }
     FJ_HD Lanes<T> blist() const { return list(1); }
     FJ_HD Lanes<T> rlist() const { return list(2); }
     FJ_HD Lanes<T> elist() const { return list(3); }
-    FJ_HD Lanes<T> rlist2() const { return {ws + 4ll * n1 * ld + lane, ld}; }
-    FJ_HD Lanes<T> res3la() const { return {ws + (4ll * n1 + 56) * ld + lane, ld}; }
+    FJ_HD Lanes<T> rlist2() const { return at(4 * l1()); }
+    FJ_HD Lanes<T> res3la() const { return at(4 * l1() + ws_round(56, CH)); }
     FJ_HD Lanes<int> iord() const {
-        return {reinterpret_cast<int *>(ws + (4ll * n1 + 60) * ld) + lane, ld};
+        return {reinterpret_cast<int *>(ws + (4 * l1() + ws_round(56, CH) + ws_round(4, CH)) * ld) +
+                    (long long)lane * ws_chunk<int>(),
+                ld};
     }
 };
 using QagsWork = QagsWorkT<double>;
@@ -958,12 +980,17 @@ FJ_HD __forceinline__ void qags_consume(QagsT<T> &q, const QagsWorkT<T> &w, int 
 
 namespace fj {
 
-// QAGS workspace of nlanes lanes, interleaved by lane: per lane the doubles
-// alist, blist, rlist, elist [cap+1], epsilon table [56], res3la [4], then
-// the ints iord [cap+1]; element i of a list of lane L sits at i * nlanes + L.
-FJ_HD inline long long ws_lane_doubles(int cap) { return 4 * (long long)(cap + 1) + 56 + 4; }
+// QAGS workspace of nlanes lanes, interleaved by lane (element by element
+// or by chunks, FJ_WS_CHUNK_BYTES): per lane the doubles alist, blist,
+// rlist, elist [cap+1], epsilon table [56], res3la [4], then the ints
+// iord [cap+1], each rounded up to whole chunks.  The float instance of the
+// same cap takes no more bytes than the double one.
+FJ_HD inline long long ws_lane_doubles(int cap) {
+    constexpr int CH = ws_chunk<double>();
+    return 4 * ws_round(cap + 1, CH) + ws_round(56, CH) + ws_round(4, CH);
+}
 inline size_t ws_doubles(int cap, long long nlanes) {
-    return (size_t)(ws_lane_doubles(cap) * nlanes + ((long long)(cap + 1) * nlanes + 1) / 2);
+    return (size_t)(ws_lane_doubles(cap) * nlanes + (ws_round(cap + 1, ws_chunk<int>()) * nlanes + 1) / 2);
 }
 
 template <class T>

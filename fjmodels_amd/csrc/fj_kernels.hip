@@ -47,6 +47,8 @@ constexpr int QCAP = 64;                // QAGS intervals kept per lane before o
 #endif
 constexpr int QCAP_MIXED = FJ_QCAP_MIXED;
 constexpr int QCAP_FULL = 1000;         // the reference's workspace limit (Utils.h:55)
+// The float workspace of the mixed path lives in the buffer sized for QCAP doubles.
+static_assert(QCAP_MIXED <= QCAP, "FP32 interval cap above the workspace's");
 
 // include/GaussPts.h:76-136
 __constant__ double c_xpt11[22] = {

@@ -83,8 +83,8 @@ struct FvF {
 };
 int fjh_qags32(float (*f)(float, void *), void *p, float a, float b, float epsabs, float epsrel, int limit,
                int cap, float *result, int *ier, int *nqk) {
-    std::vector<float> ws((size_t)3 * (5 * (cap + 1) + 60));   // 3 lanes, interleaved
-    const QagsWorkT<float> w = lane_work(ws.data(), cap, 1, 3);
+    std::vector<double> wsd(ws_doubles(cap, 3));   // 3 lanes, sized as the kernels size it
+    const QagsWorkT<float> w = lane_work(reinterpret_cast<float *>(wsd.data()), cap, 1, 3);
     const CFnF fn{f, p};
     QagsT<float> q;
     qags_begin(q, a, b, epsabs, epsrel, limit);
