@@ -50,7 +50,8 @@ class Stats(C.Structure):
                 [(n, C.c_longlong) for n in ("points", "nodes", "bound_nodes", "tasks", "qk_calls",
                                              "overflow_tasks", "brent_iters")] +
                 [("integral_lanes", C.c_int), ("setup_lanes", C.c_int)] +
-                [(n, C.c_longlong) for n in ("unbound_nodes", "bracket_failures", "nonfinite_nodes")])
+                [(n, C.c_longlong) for n in ("unbound_nodes", "bracket_failures", "nonfinite_nodes",
+                                             "phi_evals", "grad_evals")])
 
 
 class SurrogateStats(C.Structure):

@@ -213,6 +213,7 @@ struct Counters {
     unsigned long long unbound;        // E >= 0 after the spheroidal transform: Jr = Jz = 1e31
     unsigned long long bracket_fail;   // a turning-point bracket walk stopped by its iteration guard
     unsigned long long nonfinite;      // nodes whose f(J) is NaN or infinite
+    unsigned long long phi;            // potential evaluations of k_setup (gradient ones included)
 };
 
 // Wave-aggregated append to a task list.
@@ -288,7 +289,7 @@ __device__ __forceinline__ void setup_node(const LdsTabs &L, const SetupArgs &sa
     const long long gid = sa.sel ? (valid ? (long long)sa.sel[tid] : 0) : tid;
     bool want_u = false, want_v = false;
     long long nev = 0;
-    int brent_it = 0, flag = 0, bfail = 0;
+    int brent_it = 0, flag = 0, bfail = 0, nphi = 0;
     if (valid) {
         const int pl = explicit_v ? (int)gid : (int)(gid / NQ), node = explicit_v ? 0 : (int)(gid % NQ);
         const int p = sa.p0 + pl;
@@ -302,7 +303,7 @@ __device__ __forceinline__ void setup_node(const LdsTabs &L, const SetupArgs &sa
         int wu = 0, wv = 0;
         const double Dx = (explicit_v && sa.Dx) ? sa.Dx[p] : 0.0;
         flag = node_setup(L.T, L.E, L.D, L.ngrid, R, z, Phi_h, V, Dx, !explicit_v, c, ubar, Deltau, vturn,
-                          wu, wv, Jr, Jz, nev, brent_it, &bfail);
+                          wu, wv, Jr, Jz, nev, brent_it, &bfail, &nphi);
         want_u = wu; want_v = wv;
         if (flag == 1) store_const(sa.nb, gid, c, ubar, Deltau, vturn);
         sa.nb.flag[gid] = flag;
@@ -319,6 +320,7 @@ __device__ __forceinline__ void setup_node(const LdsTabs &L, const SetupArgs &sa
     if (valid && bfail) atomicAdd(&sa.ctr->bracket_fail, 1ull);
     if (valid && brent_it) atomicAdd(&sa.ctr->brent, (unsigned long long)brent_it);
     if (valid && nev) atomicAdd(&sa.ctr->grad, (unsigned long long)nev);
+    if (valid && nphi) atomicAdd(&sa.ctr->phi, (unsigned long long)nphi);
 }
 
 #ifndef FJ_SETUP_WAVES
@@ -1992,24 +1994,6 @@ __global__ void kb_delta(const ModelDev *md, const PotView *pv, int M, double *s
                  const_cast<double *>(md[m].ta.D), status);
 }
 
-// Diagnostic kernel: integrand samples and one QAGS run for one stored node.
-__global__ void k_debug_integral(TabArgs ta, NodeBuf nb, long long node, int type, const double *psi, int npsi,
-                                 double *ws, double *out) {
-    extern __shared__ double smem[];
-    const LdsTabs L = stage_tables(ta, smem);
-    if (threadIdx.x != 0) return;
-    JInt f;
-    f.T = &L.T;
-    load_jint(nb, node, type, f);
-    for (int i = 0; i < npsi; i++) out[i] = f(psi[i]);
-    const QagsWork w = lane_work(ws, QCAP_FULL, 0, 1);
-    int ier = 0, nqk = 0;
-    out[npsi] = qags_solve<11>(f, 0, type == 0 ? PI : PIH, EPSABSINTEG, EPSRELINTEG, 1000, w, QCAP_FULL, &ier, &nqk);
-    out[npsi + 1] = ier;
-    out[npsi + 2] = nqk;
-    const QKOut r = qk_rule<11>(f, 0, type == 0 ? PI : PIH, FvLocal<20>{});
-    out[npsi + 3] = r.result; out[npsi + 4] = r.abserr; out[npsi + 5] = r.resabs; out[npsi + 6] = r.resasc;
-}
 }  // namespace
 
 // ===========================================================================
@@ -2037,6 +2021,7 @@ struct fj_ctx {
     double *d_wsfull = nullptr; long long wsfull_lanes = 0;
     // point buffers
     int pt_cap = 0;
+    double *d_eval = nullptr; int eval_cap = 0;   // fj_eval_potential: R, z, Phi, dR, dz [eval_cap] each
     double *d_pR = nullptr, *d_pz = nullptr, *d_pPhi = nullptr, *d_pVe = nullptr, *d_out6 = nullptr, *d_mom = nullptr;
     // grid buffers
     double *d_rhl = nullptr, *d_gl = nullptr, *d_pw = nullptr, *d_pws = nullptr; int *d_status = nullptr;
@@ -2437,6 +2422,10 @@ int run_nodes(fj_ctx *c, int npts) {
     c->st.unbound_nodes = (long long)h.unbound;
     c->st.bracket_failures = (long long)h.bracket_fail;
     c->st.nonfinite_nodes = (long long)h.nonfinite;
+    // k_points: Phi(R,z) and Phi(10 r_max, 10 r_max), plus the gradient at (R,z) for the surrogate's Vc
+    const long long pts_grad = c->d_pVc ? npts : 0;
+    c->st.grad_evals = (long long)h.grad + pts_grad;
+    c->st.phi_evals = (long long)h.phi + 21 * (long long)h.qk + 2 * (long long)npts + pts_grad;
     if (sur) c->sst.ms_surrogate = t_sur;
     if (c->sur_init) {
         unsigned long long pos = 0;
@@ -2589,7 +2578,7 @@ int fj_ctx_destroy(fj_ctx *c) {
     void *ptrs[] = {c->d_ar, c->d_phil, c->d_Pr, c->d_Pr2, c->d_E, c->d_D, c->d_ci, c->d_si, c->d_poly, c->d_lut,
                     c->d_rk, c->d_nodef, c->d_Jr, c->d_Jz, c->d_flag, c->d_list_u, c->d_list_v, c->d_ovf, c->d_cnt,
                     c->d_ctr, c->d_ws, c->d_wsfull, c->d_pR, c->d_pz, c->d_pPhi, c->d_pVe, c->d_out6,
-                    c->d_mom, c->d_rhl, c->d_gl, c->d_pw, c->d_pws, c->d_status};
+                    c->d_mom, c->d_rhl, c->d_gl, c->d_pw, c->d_pws, c->d_status, c->d_eval};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     void *sp[] = {c->d_pVc, c->d_X, c->d_T, c->d_bt, c->d_bw, c->d_pos, c->d_loss, c->d_sx16, c->d_sy, c->d_judge_J,
@@ -2835,20 +2824,27 @@ int fj_compute_new_phi(fj_ctx *c, int mode, double *rhl, double *sigRl, double *
 int fj_eval_potential(fj_ctx *c, int n, const double *R, const double *z, double *phi, double *dR, double *dz) {
     if (!c || n < 1 || !R || !z || !phi || (dR && !dz) || (!dR && dz)) return FJ_E_ARG;
     if (!c->have_pot) return set_err(c, FJ_E_STATE, "no potential set");
-    double *d = nullptr;
-    HIPCHK(c, hipMalloc((void **)&d, (size_t)n * 5 * sizeof(double)));
-    HIPCHK(c, hipMemcpy(d, R, n * sizeof(double), hipMemcpyHostToDevice));
-    HIPCHK(c, hipMemcpy(d + n, z, n * sizeof(double), hipMemcpyHostToDevice));
+    // context-owned buffer, grown on demand: repeated calls allocate nothing
+    if (n > c->eval_cap) {
+        const int cap = n > 1024 ? n : 1024;
+        int r = dalloc(c, &c->d_eval, (size_t)cap * 5);
+        if (r) { c->eval_cap = 0; return r; }
+        c->eval_cap = cap;
+    }
+    double *d = c->d_eval;
+    const size_t cap = (size_t)c->eval_cap, b = (size_t)n * sizeof(double);
+    HIPCHK(c, hipMemcpyAsync(d, R, b, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(d + cap, z, b, hipMemcpyHostToDevice, c->stream));
     const TabArgs ta = tab_args(c);
-    k_eval<<<(n + 255) / 256, 256, tab_bytes(c), c->stream>>>(ta, n, d, d + n, d + 2 * n, dR ? d + 3 * n : nullptr,
-                                                              dR ? d + 4 * n : nullptr);
-    hipError_t e = hipGetLastError();
-    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
-    if (e == hipSuccess) e = hipMemcpy(phi, d + 2 * n, n * sizeof(double), hipMemcpyDeviceToHost);
-    if (e == hipSuccess && dR) e = hipMemcpy(dR, d + 3 * n, n * sizeof(double), hipMemcpyDeviceToHost);
-    if (e == hipSuccess && dR) e = hipMemcpy(dz, d + 4 * n, n * sizeof(double), hipMemcpyDeviceToHost);
-    (void)hipFree(d);
-    if (e != hipSuccess) return set_err(c, FJ_E_HIP, "fj_eval_potential: %s", hipGetErrorString(e));
+    k_eval<<<(n + 255) / 256, 256, tab_bytes(c), c->stream>>>(ta, n, d, d + cap, d + 2 * cap,
+                                                              dR ? d + 3 * cap : nullptr, dR ? d + 4 * cap : nullptr);
+    HIPCHK(c, hipGetLastError());
+    HIPCHK(c, hipMemcpyAsync(phi, d + 2 * cap, b, hipMemcpyDeviceToHost, c->stream));
+    if (dR) {
+        HIPCHK(c, hipMemcpyAsync(dR, d + 3 * cap, b, hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipMemcpyAsync(dz, d + 4 * cap, b, hipMemcpyDeviceToHost, c->stream));
+    }
+    HIPCHK(c, hipStreamSynchronize(c->stream));
     return 0;
 }
 
@@ -3118,37 +3114,6 @@ int fj_tabulate_delta(fj_ctx *c, double *Egrid, double *Dgrid) {
     return 0;
 }
 
-
-// Diagnostic (not part of include/fjmodels.h): the per-node setup record of
-// the last fj_actions call, NF_COUNT fields x n nodes, plus the flags.
-int fj_debug_nodes(fj_ctx *c, int n, double *out, int *flags) {
-    if (!c || n < 1 || n > c->node_cap || !out) return FJ_E_ARG;
-    for (int f = 0; f < NF_COUNT; f++)
-        HIPCHK(c, hipMemcpy(out + (size_t)f * n, c->d_nodef + (size_t)f * n, n * sizeof(double), hipMemcpyDeviceToHost));
-    if (flags) HIPCHK(c, hipMemcpy(flags, c->d_flag, n * sizeof(int), hipMemcpyDeviceToHost));
-    return 0;
-}
-
-
-// Diagnostic (not part of include/fjmodels.h): integrand samples at psi[],
-// then {QAGS result, ier, QK21 calls, first-rule result, abserr, resabs, resasc}
-// for stored node `node` of the last fj_actions call.
-int fj_debug_integral(fj_ctx *c, long long node, int type, int npsi, const double *psi, double *out) {
-    if (!c || node < 0 || node >= c->last_nodes || npsi < 0 || !out) return FJ_E_ARG;
-    double *d = nullptr;
-    const size_t nws = ws_doubles(QCAP_FULL, 1);
-    const size_t nout = (size_t)npsi + 8;
-    HIPCHK(c, hipMalloc((void **)&d, (npsi + nout + nws) * sizeof(double)));
-    if (npsi) HIPCHK(c, hipMemcpy(d, psi, npsi * sizeof(double), hipMemcpyHostToDevice));
-    NodeBuf nb{c->d_nodef, c->last_nodes, c->d_Jr, c->d_Jz, c->d_flag};
-    k_debug_integral<<<1, 64, tab_bytes(c), c->stream>>>(tab_args(c), nb, node, type, d, npsi, d + npsi + nout,
-                                                        d + npsi);
-    HIPCHK(c, hipGetLastError());
-    HIPCHK(c, hipStreamSynchronize(c->stream));
-    HIPCHK(c, hipMemcpy(out, d + npsi, (npsi + 7) * sizeof(double), hipMemcpyDeviceToHost));
-    (void)hipFree(d);
-    return 0;
-}
 
 // ---------------------------------------------------------------------------
 // Action surrogate

@@ -56,11 +56,13 @@ struct DUturn {   // duturnfn (uvOrb.cpp:74-79)
 struct Uturn {    // uturnfn (uvOrb.cpp:70-73) with dU (61-63)
     const PotView *T;
     const NodeConst *c;
+    int *nphi;        // potential evaluations, or null
     FJ_HD double operator()(double u) const {
         double shu, chu;
         fjm_sinhcosh(u, &shu, &chu);
         const double shu2 = shu * shu;
         const double D = c->Delta;
+        if (nphi) (*nphi)++;
         const double Phi = pot_phi(*T, D * shu * c->sinv, D * chu * c->cosv);
         const double dU = (shu2 + c->sv2) * Phi - c->K2;
         return c->E * (shu2 - c->sh1sq) + c->K1 - dU - c->K3 * (1 / shu2 - c->invsh1sq);
@@ -70,11 +72,13 @@ struct Uturn {    // uturnfn (uvOrb.cpp:70-73) with dU (61-63)
 struct Vturn {    // vturnfn (uvOrb.cpp:49-52) with dV (46-48)
     const PotView *T;
     const NodeConst *c;
+    int *nphi;        // potential evaluations, or null
     FJ_HD double operator()(double v) const {
         double s, cs;
         fjm_sincos(v, &s, &cs);
         const double sv2 = s * s;
         const double D = c->Delta;
+        if (nphi) (*nphi)++;
         const double Phi = pot_phi(*T, D * c->shuX * s, D * c->chuX * cs);
         const double dV = c->KV - (c->shu2 + sv2) * Phi;
         return c->E * sv2 + c->Iv + dV - .5 * c->Lzsq / (D * D * sv2);
@@ -148,10 +152,12 @@ FJ_HD inline JInt make_jint(const PotView *T, const NodeConst &c, double ubar, d
 // Returns 0 (H >= 0: f = 0), 1 (bound; c/ubar/Deltau/vturn valid) or 2
 // (unbound: Jr = Jz = 1e31).  want_u / want_v flag the integrals to run;
 // otherwise Jr / Jz hold their final value (0 at a degenerate turning point).
+// nev counts the potential + gradient evaluations (duturnfn); *nphi, when
+// given, the potential evaluations, those with a gradient included.
 FJ_HD inline int node_setup(const PotView &T, const double *Eg, const double *Dg, int ng, double R, double z,
                             double Phi_h, const double *V, double Dx, bool check_H, NodeConst &c, double &ubar,
                             double &Deltau, double &vturn, int &want_u, int &want_v, double &Jr, double &Jz,
-                            long long &nev, int &brent_it, int *bracket_fail = nullptr) {
+                            long long &nev, int &brent_it, int *bracket_fail = nullptr, int *nphi = nullptr) {
     int flag = 0;
     want_u = 0; want_v = 0; Jr = 0; Jz = 0; ubar = 0; Deltau = 0; vturn = 0;
     const double H = .5 * (V[0] * V[0] + V[1] * V[1] + V[2] * V[2]) + Phi_h;   // DF.cpp:83
@@ -200,6 +206,7 @@ FJ_HD inline int node_setup(const PotView &T, const double *Eg, const double *Dg
     fjm_sinhcosh(umid, &shm, &chm);
         c.sh1sq = shm * shm;
         const double ch1sq = chm * chm;
+        if (nphi) *nphi += 3 + (int)nev;   // Phiu1, PhiX, PhivPIH and the duturnfn calls so far
         const double Phiu1 = pot_phi(T, Delta * shm * c.sinv, Delta * chm * c.cosv);
         fjm_sinhcosh(u, &c.shuX, &c.chuX);
         const double PhiX = pot_phi(T, Delta * c.shuX * c.sinv, Delta * c.chuX * c.cosv);
@@ -218,7 +225,8 @@ FJ_HD inline int node_setup(const PotView &T, const double *Eg, const double *Dg
         c.Iv = .5 * (pv * pv) / Delta2 - c.E * sv2 + .5 * c.Lzsq / (Delta2 * sv2) - dVX;
 
         // GetTurnu(1): uvOrb.cpp:131-201
-        Uturn U{&T, &c};
+        Uturn U{&T, &c, nphi};
+        const long long nev0 = nev;
         double ut_ = 1, ut = U(ut_), dutt = du(ut_), eps = .1;
         double uinner = ut_, uouter = ut_, u1 = ut_, u2 = ut_, fu1 = ut, fu2 = ut, fuinner = ut, fuouter = ut;
         bool noroot = false;
@@ -287,7 +295,8 @@ FJ_HD inline int node_setup(const PotView &T, const double *Eg, const double *Dg
             want_u = 1;
         }
         // GetTurnv(.5*PIH): uvOrb.cpp:203-224
-        Vturn Vf{&T, &c};
+        if (nphi) *nphi += (int)(nev - nev0);   // the duturnfn calls of GetTurnu
+        Vturn Vf{&T, &c, nphi};
         {
             const double vtry = .5 * PIH;
             double vtop = vtry, vbot = vtry, ft, fb;

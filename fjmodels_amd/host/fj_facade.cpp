@@ -157,6 +157,12 @@ fJParams readParam(const char *fName) {
         else if (key == "merge") P.merge = val();
         else if (key == "delta_refresh") P.delta_refresh = (int)val();
         else if (key == "lineprofile") P.lineprofile = (int)val();
+        else if (key == "comp2_potential") {
+            const std::string s = str();
+            if (s == "updated") P.comp2_potential = 0;
+            else if (s == "total") P.comp2_potential = 1;
+            else throw Error(FJ_E_ARG, "comp2_potential must be updated or total");
+        }
         else if (key == "quad") {
             const std::string s = str();
             if (s == "log22") P.quad = FJ_QUAD_LOG22;
@@ -432,6 +438,29 @@ std::vector<double> vir2(Potential *p, double **rhlH, double **vrotlH, double **
                     or_global(sigplH, sigpl)[0], or_global(sigzlH, sigzl)[0], o.data()),
           "fj_virial");
     if (print) {   // libs/Stats.cpp:37-40
+        printf("Mass(%3.0f): %f %f\n", ar[NR - 1], o[0], o[1]);
+        printf("KE, PE, W/K = %f %f %f\n", o[2], o[3], o[4]);
+        printf("Kxx, Wxx, Wxx/Kxx = %f %f %f\n", o[5], o[6], o[7]);
+        printf("Kzz, Wzz, Wzz/Kzz = %f %f %f\n", o[8], o[9], o[10]);
+    }
+    return o;
+}
+
+fj_stats nodeStats() {
+    fj_stats st;
+    check(fj_get_stats(ctx(), &st), "fj_get_stats");
+    return st;
+}
+
+std::vector<double> vir2Total(const std::vector<double> &a, const std::vector<double> &b, bool print) {
+    if (a.size() != 11 || b.size() != 11) throw Error(FJ_E_ARG, "vir2Total: two outputs of vir2");
+    std::vector<double> o(11);
+    o[0] = a[0];                 // r_max^2 dPhi_0/dr of the total potential (the same in both)
+    for (int i : {1, 2, 3, 5, 6, 8, 9}) o[i] = a[i] + b[i];
+    o[4] = o[3] / o[2];
+    o[7] = o[6] / o[5];
+    o[10] = o[9] / o[8];
+    if (print) {
         printf("Mass(%3.0f): %f %f\n", ar[NR - 1], o[0], o[1]);
         printf("KE, PE, W/K = %f %f %f\n", o[2], o[3], o[4]);
         printf("Kxx, Wxx, Wxx/Kxx = %f %f %f\n", o[5], o[6], o[7]);

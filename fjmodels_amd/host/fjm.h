@@ -65,6 +65,12 @@ struct fJParams {
     //        reference tabulates it once, from the guess: fJmodels.cpp:127)
     //   lineprofile 1: write the .lp file of writeOut.cpp:159-186 (the
     //        reference's LINEPROFILE build) after each iteration
+    //   comp2_potential updated|total: the potential component 2's moments
+    //        are computed in.  updated (default, the reference's flow):
+    //        component 1's computeNewPhi installs component 1's new
+    //        potential as the global one (Potential.cpp:182-188), and
+    //        component 2 reads that (fJmodels.cpp:69-73).  total: both
+    //        components read the total potential of the previous iteration.
     int rot_form = 0, rot_form_2 = 0;
     double rot_k = UNSET, rot_k_2 = UNSET;
     int nr = 60, ntheta = 30, npoly = 22, ngrid = 30;
@@ -76,6 +82,7 @@ struct fJParams {
     double merge = 0;
     int delta_refresh = 0;
     int lineprofile = 0;
+    int comp2_potential = 0;   // 0 updated, 1 total
 };
 
 // Number of components found by checkParameterFile (readParam.cpp:125-150).
@@ -178,6 +185,16 @@ void actions(int n, const double *R, const double *z, const double *vR, const do
 std::vector<double> vir2(Potential *p, double **rhlH = nullptr, double **vrotlH = nullptr,
                          double **sigRlH = nullptr, double **sigplH = nullptr, double **sigzlH = nullptr,
                          bool print = true);
+
+// Statistics of the last node phase (fj_get_stats): timings, node counts,
+// Kronrod rules, Brent iterations, potential / gradient evaluations.
+fj_stats nodeStats();
+
+// Virial sums of a two-component model: the additive outputs of vir2 for
+// each component, both evaluated in the total potential (the reference has no
+// two-component virial), added; the ratios formed from the sums; Mass(r_max)
+// of the total potential.  Prints in vir2's format when print is set.
+std::vector<double> vir2Total(const std::vector<double> &c1, const std::vector<double> &c2, bool print = true);
 
 // mergePhi's distance (include/UtilsLeg.h:266-278): max over (n, np) of
 // |new - old| / |new[n][0]|.

@@ -74,6 +74,9 @@ typedef struct {
      *   guard, where the reference would keep walking;
      * nonfinite_nodes: nodes whose f(J) is NaN or infinite (they poison the moment sums). */
     long long unbound_nodes, bracket_failures, nonfinite_nodes;
+    /* node-work counters: potential evaluations Phi(R,z) (those with the gradient included) and
+     * potential + gradient evaluations (Potential::operator(), dR, dz: Potential.cpp:206-276) */
+    long long phi_evals, grad_evals;
 } fj_stats;
 
 /* Context on HIP device `device` (no reference counterpart: the reference

@@ -4,8 +4,12 @@ restatement runs in seconds (FP64, bit for bit):
 * rotating NFW f(J) with the documented (1-k) f + k tanh(chi Jphi/J0) f term
   (df/DF.cpp:116, README.md:80-83; k = 0.5, chi = 1), and the code's
   f (1 + k tanh) form, on an NFW guess potential, tanh-sinh and log rules;
-* two components (fJmodels.cpp:38-81 twoComp): each computeNewPhi reads the
-  total potential, updatePhil (potLeg.cpp:21-29) sums the components.
+* two components through the C ABI in the total-potential flow (the driver's
+  comp2_potential total): both components' computeNewPhi read the total
+  potential of the previous iteration, and updatePhil (potLeg.cpp:21-29) sums
+  them.  The reference's own flow, in which component 2 reads component 1's
+  new potential (pot/Potential.cpp:182-188, fJmodels.cpp:69-73), is checked
+  through the driver in test_gpu_trajectory.py.
 """
 import numpy as np
 import pytest
