@@ -2341,7 +2341,9 @@ int run_nodes(fj_ctx *c, int npts) {
     }
     c->sur_iter++;
     HIPCHK(c, hipEventRecord(ev[0], c->stream));
-    PointArgs pa{c->d_pR, c->d_pz, c->d_pPhi, c->d_pVe, c->d_pVc};
+    // the circular-speed scale Vc feeds only the surrogate's inputs
+    const bool need_vc = c->sur_init;
+    PointArgs pa{c->d_pR, c->d_pz, c->d_pPhi, c->d_pVe, need_vc ? c->d_pVc : nullptr};
     k_points<<<(npts + 255) / 256, 256, lds, c->stream>>>(ta, pa, npts);
     HIPCHK(c, hipGetLastError());
     HIPCHK(c, hipEventRecord(ev[1], c->stream));
@@ -2423,7 +2425,7 @@ int run_nodes(fj_ctx *c, int npts) {
     c->st.bracket_failures = (long long)h.bracket_fail;
     c->st.nonfinite_nodes = (long long)h.nonfinite;
     // k_points: Phi(R,z) and Phi(10 r_max, 10 r_max), plus the gradient at (R,z) for the surrogate's Vc
-    const long long pts_grad = c->d_pVc ? npts : 0;
+    const long long pts_grad = need_vc ? npts : 0;
     c->st.grad_evals = (long long)h.grad + pts_grad;
     c->st.phi_evals = (long long)h.phi + 21 * (long long)h.qk + 2 * (long long)npts + pts_grad;
     if (sur) c->sst.ms_surrogate = t_sur;
