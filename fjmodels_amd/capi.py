@@ -16,6 +16,7 @@ _dp = C.POINTER(C.c_double)
 FJ_QUAD_LOG22, FJ_QUAD_TANHSINH22 = 0, 1
 FJ_POISSON_QUADPACK, FJ_POISSON_BLOCKSCAN = 0, 1
 FJ_PREC_FP64, FJ_PREC_MIXED = 0, 1
+NOALLOC_MAX_POINTS = 4   # FJ_NOALLOC_MAX_POINTS of include/fjmodels.h
 GUESS = {"Hernquist": 0, "Isochrone": 1, "NFW": 2, "NFWext": 3}
 SLOPES = {"Hernquist": (1.667, 5.0), "Isochrone": (0.0, 5.0), "NFW": (1.667, 3.0)}
 
@@ -24,6 +25,7 @@ EXPORTS = [
     "fj_ctx_create", "fj_ctx_destroy", "fj_last_error", "fj_set_grid", "fj_set_quad_rule",
     "fj_set_precision", "fj_set_potential", "fj_get_potential", "fj_set_delta_table",
     "fj_get_delta_table", "fj_set_df", "fj_guess_rhl", "fj_poisson", "fj_moments", "fj_rhofDF",
+    "fj_rhofDF_noalloc",
     "fj_project_legendre", "fj_compute_new_phi", "fj_tabulate_delta", "fj_virial",
     "fj_eval_potential", "fj_actions", "fj_df_points", "fj_sigma_df", "fj_line_profile",
     "fj_velocity_distribution", "fj_get_stats",
@@ -105,6 +107,7 @@ def load():
         "fj_poisson": [v, i, _dp, _dp, _dp, _dp],
         "fj_moments": [v] + [_dp] * 6,
         "fj_rhofDF": [v, i, _dp, _dp, _dp] + [_dp] * 6,
+        "fj_rhofDF_noalloc": [v, i, _dp, _dp, _dp] + [_dp] * 6,
         "fj_project_legendre": [v] + [_dp] * 12,
         "fj_compute_new_phi": [v, i] + [_dp] * 9,
         "fj_tabulate_delta": [v, _dp, _dp],
@@ -248,6 +251,16 @@ class Context:
         out6 = np.zeros((n, 6))
         o = [np.zeros(n) for _ in range(6)]
         self._chk(self.lib.fj_rhofDF(self.h, n, _p(R), _p(z), _p(out6), *[_p(x) for x in o]), "fj_rhofDF")
+        return out6, o
+
+    def rhofDF_noalloc(self, R, z):
+        """fj_rhofDF_noalloc: rhofDF at n <= NOALLOC_MAX_POINTS points without device allocation."""
+        R, z = _a(R), _a(z)
+        n = len(R)
+        out6 = np.zeros((n, 6))
+        o = [np.zeros(n) for _ in range(6)]
+        self._chk(self.lib.fj_rhofDF_noalloc(self.h, n, _p(R), _p(z), _p(out6), *[_p(x) for x in o]),
+                  "fj_rhofDF_noalloc")
         return out6, o
 
     def project(self, rho, vrot, sigR, sigp, sigz, sigRz):

@@ -985,11 +985,11 @@ namespace fj {
 // rlist, elist [cap+1], epsilon table [56], res3la [4], then the ints
 // iord [cap+1], each rounded up to whole chunks.  The float instance of the
 // same cap takes no more bytes than the double one.
-FJ_HD inline long long ws_lane_doubles(int cap) {
+FJ_HD constexpr long long ws_lane_doubles(int cap) {
     constexpr int CH = ws_chunk<double>();
     return 4 * ws_round(cap + 1, CH) + ws_round(56, CH) + ws_round(4, CH);
 }
-inline size_t ws_doubles(int cap, long long nlanes) {
+FJ_HD constexpr size_t ws_doubles(int cap, long long nlanes) {
     return (size_t)(ws_lane_doubles(cap) * nlanes + (ws_round(cap + 1, ws_chunk<int>()) * nlanes + 1) / 2);
 }
 

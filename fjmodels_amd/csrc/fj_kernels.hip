@@ -22,6 +22,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <mutex>
 #include <type_traits>
 #include <vector>
 
@@ -1994,6 +1995,51 @@ __global__ void kb_delta(const ModelDev *md, const PotView *pv, int M, double *s
                  const_cast<double *>(md[m].ta.D), status);
 }
 
+// ---------------------------------------------------------------------------
+// Scratch of fj_rhofDF_noalloc: one statically sized device array holds every
+// buffer the node phase needs beyond the caller's inputs and outputs (points,
+// node records, task lists, counters, QAGS workspaces), so that the call
+// allocates nothing.  Each GPU the library runs on holds its own copy for the
+// life of the process; host code reaches it only through the address
+// hipGetSymbolAddress returns for the current device.
+// ---------------------------------------------------------------------------
+constexpr int NOALLOC_LANES = 8192;        // persistent integrator lanes (32 blocks of 256)
+constexpr int NOALLOC_OVF_LANES = 256;     // lanes of the full-limit overflow pass
+constexpr long long NOALLOC_NODES = (long long)FJ_NOALLOC_MAX_POINTS * NQ;
+
+struct NoallocLayout {
+    size_t R, z, Phi, Ve, out6, mom, nodef, Jr, Jz, flag, list_u, list_v, ovf, cnt, ctr, ws, wsfull, total;
+};
+constexpr size_t na_align(size_t b) { return (b + 255) / 256 * 256; }
+constexpr NoallocLayout noalloc_layout() {
+    NoallocLayout L{};
+    size_t o = 0;
+    const size_t P = FJ_NOALLOC_MAX_POINTS, N = (size_t)NOALLOC_NODES, D = sizeof(double), I = sizeof(int);
+    L.R = o; o = na_align(o + P * D);
+    L.z = o; o = na_align(o + P * D);
+    L.Phi = o; o = na_align(o + P * D);
+    L.Ve = o; o = na_align(o + P * D);
+    L.out6 = o; o = na_align(o + 6 * P * D);
+    L.mom = o; o = na_align(o + 6 * P * D);
+    L.nodef = o; o = na_align(o + (size_t)NF_COUNT * N * D);
+    L.Jr = o; o = na_align(o + N * D);
+    L.Jz = o; o = na_align(o + N * D);
+    L.flag = o; o = na_align(o + N * I);
+    L.list_u = o; o = na_align(o + N * I);
+    L.list_v = o; o = na_align(o + N * I);
+    L.ovf = o; o = na_align(o + 2 * N * I);
+    L.cnt = o; o = na_align(o + 16 * I);
+    L.ctr = o; o = na_align(o + sizeof(Counters));
+    L.ws = o; o = na_align(o + ws_doubles(QCAP, NOALLOC_LANES) * D);
+    L.wsfull = o; o = na_align(o + ws_doubles(QCAP_FULL, NOALLOC_OVF_LANES) * D);
+    L.total = o;
+    return L;
+}
+constexpr NoallocLayout NA = noalloc_layout();
+static_assert(NA.total <= (size_t)64 << 20, "no-allocation scratch above its 64 MiB budget");
+
+static __device__ __align__(256) unsigned char scratch[NA.total];
+
 }  // namespace
 
 // ===========================================================================
@@ -2022,6 +2068,7 @@ struct fj_ctx {
     // point buffers
     int pt_cap = 0;
     double *d_eval = nullptr; int eval_cap = 0;   // fj_eval_potential: R, z, Phi, dR, dz [eval_cap] each
+    unsigned char *noalloc = nullptr;   // this device's copy of the fj_rhofDF_noalloc scratch
     double *d_pR = nullptr, *d_pz = nullptr, *d_pPhi = nullptr, *d_pVe = nullptr, *d_out6 = nullptr, *d_mom = nullptr;
     // grid buffers
     double *d_rhl = nullptr, *d_gl = nullptr, *d_pw = nullptr, *d_pws = nullptr; int *d_status = nullptr;
@@ -2570,6 +2617,22 @@ int fj_ctx_create(int device, fj_ctx **out) {
         delete c;
         return FJ_E_HIP;
     }
+    // address of this device's scratch array (set-up, once per device)
+    static std::mutex mu;
+    static unsigned char *addr[64] = {};
+    {
+        std::lock_guard<std::mutex> lk(mu);
+        if (device < 64 && !addr[device]) {
+            void *ptr = nullptr;
+            if (hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)) != hipSuccess || !ptr) {
+                (void)hipStreamDestroy(c->stream);
+                delete c;
+                return FJ_E_HIP;
+            }
+            addr[device] = static_cast<unsigned char *>(ptr);
+        }
+        c->noalloc = device < 64 ? addr[device] : nullptr;
+    }
     *out = c;
     return 0;
 }
@@ -2764,6 +2827,87 @@ int fj_rhofDF(fj_ctx *c, int n, const double *R, const double *z, double *out6, 
     double *outs[6] = {rho, vrot, sigR, sigp, sigz, sigRz};
     for (int m = 0; m < 6; m++)
         if (outs[m]) HIPCHK(c, hipMemcpy(outs[m], c->d_mom + (size_t)m * n, n * sizeof(double), hipMemcpyDeviceToHost));
+    return 0;
+}
+
+// fj_rhofDF with every device buffer in the static scratch array: the same
+// kernels and launch order as run_nodes (FP64 or mixed precision, no
+// surrogate), with fewer integrator lanes; counters cleared by a kernel.
+int fj_rhofDF_noalloc(fj_ctx *c, int n, const double *R, const double *z, double *out6, double *rho, double *vrot,
+                      double *sigR, double *sigp, double *sigz, double *sigRz) {
+    if (!c || !R || !z) return FJ_E_ARG;
+    if (n < 1 || n > FJ_NOALLOC_MAX_POINTS)
+        return set_err(c, FJ_E_ARG, "fj_rhofDF_noalloc: n = %d points, allowed 1..%d", n, FJ_NOALLOC_MAX_POINTS);
+    if (!c->noalloc) return set_err(c, FJ_E_STATE, "fj_rhofDF_noalloc: no scratch address for device %d", c->device);
+    if (!c->have_pot || !c->have_delta || !c->have_df)
+        return set_err(c, FJ_E_STATE, "potential, Delta table and DF must be set before the node phase");
+    if (c->sur_on) return set_err(c, FJ_E_STATE, "fj_rhofDF_noalloc: switch the action surrogate off first");
+    HIPCHK(c, hipSetDevice(c->device));
+    unsigned char *b = c->noalloc;
+    auto D = [b](size_t off) { return reinterpret_cast<double *>(b + off); };
+    auto I = [b](size_t off) { return reinterpret_cast<int *>(b + off); };
+    unsigned int *cnt = reinterpret_cast<unsigned int *>(b + NA.cnt);
+    Counters *ctr = reinterpret_cast<Counters *>(b + NA.ctr);
+    const bool mixed = c->prec == FJ_PREC_MIXED;
+    const long long nn = (long long)n * NQ;
+    const TabArgs ta = tab_args(c);
+    const size_t lds = tab_bytes(c);
+    HIPCHK(c, hipMemcpyAsync(D(NA.R), R, n * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(D(NA.z), z, n * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    // the array keeps the previous call's contents: clear the counters
+    kb_zero<<<1, 256, 0, c->stream>>>(cnt, 16);
+    kb_zero<<<1, 256, 0, c->stream>>>(reinterpret_cast<unsigned int *>(ctr), (int)(sizeof(Counters) / 4));
+    PointArgs pa{D(NA.R), D(NA.z), D(NA.Phi), D(NA.Ve), nullptr};
+    k_points<<<(n + 255) / 256, 256, lds, c->stream>>>(ta, pa, n);
+    HIPCHK(c, hipGetLastError());
+    NodeBuf nb{D(NA.nodef), nn, D(NA.Jr), D(NA.Jz), I(NA.flag)};
+    SetupArgs sa{D(NA.R), D(NA.z), D(NA.Phi), D(NA.Ve), 0, n, c->quad, nb, cnt + 0, cnt + 1, I(NA.list_u),
+                 I(NA.list_v), ctr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+    k_setup<<<(unsigned)((nn + 255) / 256), 256, lds, c->stream>>>(ta, sa);
+    HIPCHK(c, hipGetLastError());
+    IntegArgs ia{nb, I(NA.list_u), I(NA.list_v), cnt + 0, cnt + 1, cnt + 2, I(NA.ovf), cnt + 3, D(NA.ws),
+                 mixed ? QCAP_MIXED : QCAP, -1, ctr};
+    launch_integrals(c, ta, ia, NOALLOC_LANES, mixed);
+    HIPCHK(c, hipGetLastError());
+    unsigned int novf = 0;
+    HIPCHK(c, hipMemcpyAsync(&novf, cnt + 3, sizeof novf, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    if (novf) {
+        int fl = (int)((novf + 255) / 256 * 256);
+        if (fl > NOALLOC_OVF_LANES) fl = NOALLOC_OVF_LANES;
+        IntegArgs io = ia;
+        io.next = cnt + 4; io.ws = D(NA.wsfull); io.cap = QCAP_FULL; io.ntask_override = (int)novf;
+        io.ovf_cnt = cnt + 5;
+        launch_integrals(c, ta, io, fl, false);
+        HIPCHK(c, hipGetLastError());
+    }
+    MomArgs ma{D(NA.Phi), D(NA.Ve), D(NA.R), 0, n, c->quad, nb, c->df, D(NA.out6), D(NA.mom), D(NA.mom) + n,
+               D(NA.mom) + 2 * n, D(NA.mom) + 3 * n, D(NA.mom) + 4 * n, D(NA.mom) + 5 * n, &ctr->nonfinite};
+    if (mixed) k_moments_f32<<<n, 256, 0, c->stream>>>(ma);
+    else k_moments<<<n, 256, 0, c->stream>>>(ma);
+    HIPCHK(c, hipGetLastError());
+    if (out6) HIPCHK(c, hipMemcpyAsync(out6, D(NA.out6), (size_t)n * 6 * sizeof(double), hipMemcpyDeviceToHost,
+                                       c->stream));
+    double *outs[6] = {rho, vrot, sigR, sigp, sigz, sigRz};
+    for (int m = 0; m < 6; m++)
+        if (outs[m]) HIPCHK(c, hipMemcpyAsync(outs[m], D(NA.mom) + (size_t)m * n, n * sizeof(double),
+                                              hipMemcpyDeviceToHost, c->stream));
+    Counters h;
+    HIPCHK(c, hipMemcpyAsync(&h, ctr, sizeof h, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    memset(&c->st, 0, sizeof c->st);
+    c->st.points = n;
+    c->st.nodes = nn;
+    c->st.bound_nodes = (long long)h.bound;
+    c->st.qk_calls = (long long)h.qk;
+    c->st.brent_iters = (long long)h.brent;
+    c->st.overflow_tasks = novf;
+    c->st.integral_lanes = NOALLOC_LANES;
+    c->st.unbound_nodes = (long long)h.unbound;
+    c->st.bracket_failures = (long long)h.bracket_fail;
+    c->st.nonfinite_nodes = (long long)h.nonfinite;
+    c->st.grad_evals = (long long)h.grad;
+    c->st.phi_evals = (long long)h.phi + 21 * (long long)h.qk + 2 * (long long)n;
     return 0;
 }
 

@@ -128,6 +128,22 @@ int fj_moments(fj_ctx *ctx, double *rho, double *vrot, double *sigR, double *sig
 int fj_rhofDF(fj_ctx *ctx, int n, const double *R, const double *z, double *out6,
               double *rho, double *vrot, double *sigR, double *sigp, double *sigz, double *sigRz);
 
+/* fj_rhofDF without any device allocation.  No counterpart in the reference:
+ * it is for callers that cannot allocate device memory in the call (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * Every buffer beyond the caller's inputs and outputs (points, node records,
+ * task lists, counters, QAGS workspaces) lives in one static device array of
+ * the library, at most 64 MiB, which each GPU the library runs on holds until
+ * the process ends; its address is taken once per device by fj_ctx_create.
+ * n is limited to FJ_NOALLOC_MAX_POINTS (FJ_E_ARG beyond); the context's
+ * potential, Delta table, DF, rule and precision apply, the action surrogate
+ * must be off.  Results are those of fj_rhofDF.  All calls on one GPU share
+ * the array: calls that overlap on one GPU (two contexts or threads at once)
+ * race, so serialise them. */
+#define FJ_NOALLOC_MAX_POINTS 4
+int fj_rhofDF_noalloc(fj_ctx *ctx, int n, const double *R, const double *z, double *out6,
+                      double *rho, double *vrot, double *sigR, double *sigp, double *sigz, double *sigRz);
+
 /* The rho[0][0] fix (potLeg.cpp:68) and get_Ylm (UtilsLeg.h:200-253):
  * grid moments [nr][ntheta] -> Legendre coefficients [nr][npoly]. */
 int fj_project_legendre(fj_ctx *ctx, const double *rho, const double *vrot, const double *sigR,
