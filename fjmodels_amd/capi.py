@@ -15,7 +15,7 @@ _dp = C.POINTER(C.c_double)
 
 FJ_QUAD_LOG22, FJ_QUAD_TANHSINH22 = 0, 1
 FJ_POISSON_QUADPACK, FJ_POISSON_BLOCKSCAN = 0, 1
-FJ_PREC_FP64, FJ_PREC_MIXED = 0, 1
+FJ_PREC_FP64, FJ_PREC_MIXED, FJ_PREC_FAST = 0, 1, 2
 NOALLOC_MAX_POINTS = 4   # FJ_NOALLOC_MAX_POINTS of include/fjmodels.h
 GUESS = {"Hernquist": 0, "Isochrone": 1, "NFW": 2, "NFWext": 3}
 SLOPES = {"Hernquist": (1.667, 5.0), "Isochrone": (0.0, 5.0), "NFW": (1.667, 3.0)}
@@ -23,7 +23,7 @@ SLOPES = {"Hernquist": (1.667, 5.0), "Isochrone": (0.0, 5.0), "NFW": (1.667, 3.0
 # Every symbol include/fjmodels.h declares.
 EXPORTS = [
     "fj_ctx_create", "fj_ctx_destroy", "fj_last_error", "fj_set_grid", "fj_set_quad_rule",
-    "fj_set_precision", "fj_set_potential", "fj_get_potential", "fj_set_delta_table",
+    "fj_set_precision", "fj_set_fast_order", "fj_set_potential", "fj_get_potential", "fj_set_delta_table",
     "fj_get_delta_table", "fj_set_df", "fj_guess_rhl", "fj_poisson", "fj_moments", "fj_rhofDF",
     "fj_rhofDF_noalloc",
     "fj_project_legendre", "fj_compute_new_phi", "fj_tabulate_delta", "fj_virial",
@@ -98,6 +98,7 @@ def load():
         "fj_set_grid": [v, i, _dp, i, i, i],
         "fj_set_quad_rule": [v, i],
         "fj_set_precision": [v, i],
+        "fj_set_fast_order": [v, i],
         "fj_set_potential": [v, _dp, _dp, _dp],
         "fj_get_potential": [v, _dp, _dp, _dp],
         "fj_set_delta_table": [v, _dp, _dp],
@@ -204,6 +205,10 @@ class Context:
 
     def set_precision(self, mode):
         self._chk(self.lib.fj_set_precision(self.h, mode), "fj_set_precision")
+
+    def set_fast_order(self, n):
+        """Gauss-Legendre order of the FJ_PREC_FAST action integrals."""
+        self._chk(self.lib.fj_set_fast_order(self.h, n), "fj_set_fast_order")
 
     def set_potential(self, phil, Pr, Pr2):
         a, b, c = _a(phil), _a(Pr), _a(Pr2)

@@ -1,0 +1,300 @@
+// fj_fast.hpp -- per-node work of the fast node phase (FJ_PREC_FAST).
+//
+// The same Staeckel-fudge actions as uvOrb (uvOrb/uvOrb.cpp:15-262), computed
+// another way, for speed:
+//   * the reference point of the u side is the node itself instead of umid
+//     (Getu0, uvOrb.cpp:109-129): umid only sets the zero of Er and dU, and
+//     cancels exactly from uturnfn (uvOrb.cpp:25-31, 61-73), so the bracket
+//     walk from u = 58 and its Brent solve are not needed;
+//   * the turning points are bracketed by the reference's own walks
+//     (GetTurnu from u = 1, GetTurnv from pi/4: uvOrb.cpp:131-224, which
+//     decide which allowed interval of the u and v momenta is the orbit's)
+//     and, after wrp_GSLroots' endpoint test (|F| < 5e-4), solved by
+//     Anderson-Bjorck regula falsi in FP64 to 1e-9 relative;
+//   * the action integrals Juint / Jvint (uvOrb.cpp:226-262) are fixed-order
+//     Gauss-Legendre rules over psi (the cos-mapping of the reference makes
+//     the integrands vanish smoothly at both turning points).
+// With energies measured from E, the two momentum functions are
+//   F_u(u) = B_u - C / sh^2 u - (sh^2 u + sin^2 v0) (Phi(u, v0) - E)
+//   F_v(v) = B_v - C / sin^2 v - (sh^2 u0 + sin^2 v) (Phi(u0, v) - E)
+// with C = Lz^2 / (2 Delta^2), K = (sh^2 u0 + sin^2 v0) (Phi(R, z) - E),
+// B_u = pu^2 / (2 Delta^2) + C / sh^2 u0 + K, B_v = pv^2 / (2 Delta^2) + C / sin^2 v0 + K,
+// so that F_u(u0) = pu^2 / (2 Delta^2) and F_v(v0) = pv^2 / (2 Delta^2).
+// Jr = sqrt(2) Delta Du / pi int_0^pi sqrt(F_u(ubar - Du cos psi)) sin psi dpsi and
+// Jz = 2 sqrt(2) Delta (pi/2 - vt) / pi int_0^pi/2 sqrt(F_v(pi/2 - (pi/2 - vt) cos psi)) sin psi dpsi.
+// Host/device code: the FP64 parts also compile into the host test build.
+#pragma once
+#include "fj_node.hpp"
+
+namespace fj {
+
+struct FastNode {
+    double Delta, E, C, Bu, Bv;
+    double shu0, chu0, sv0, cv0;   // node's spheroidal coordinates (v0 folded into [0, pi/2])
+    double u0, v0;
+    double uin, uout, vt;          // turning points
+    int ju, jv;                    // 1: the integral is needed (else the action is 0)
+};
+
+// Anderson-Bjorck regula falsi on a bracket [a, b] with fa * fb <= 0; stops
+// when the bracket is below rtol relative.  *nev counts evaluations.
+template <class F>
+FJ_HD inline double fast_root(const F &f, double a, double b, double fa, double fb, double rtol, int *nev) {
+    if (fa == 0) return a;
+    if (fb == 0) return b;
+    for (int it = 0; it < 100; it++) {
+        const double x = b - fb * (b - a) / (fb - fa);
+        const double fx = f(x);
+        (*nev)++;
+        if (fx == 0) return x;
+        if ((fx > 0) != (fb > 0)) { a = b; fa = fb; }
+        else {
+            const double m = 1 - fx / fb;
+            fa *= m > 0 ? m : 0.5;
+        }
+        b = x; fb = fx;
+        if (fabs(b - a) <= rtol * fabs(b)) break;
+    }
+    return b;
+}
+
+struct FastFu {
+    using Tab = PotView;
+    PotView T;
+    const FastNode *n;
+    FJ_HD double operator()(double u) const {
+        double shu, chu;
+        fjm_sinhcosh(u, &shu, &chu);
+        const double sh2 = shu * shu, D = n->Delta;
+        const double Phi = pot_phi(T, D * shu * n->sv0, D * chu * n->cv0);
+        return n->Bu - n->C / sh2 - (sh2 + n->sv0 * n->sv0) * (Phi - n->E);
+    }
+};
+
+// dF_u/du (duturnfn, uvOrb.cpp:74-79, with F_u's reference point)
+struct FastDFu {
+    PotView T;
+    const FastNode *n;
+    FJ_HD double operator()(double u) const {
+        double shu, chu;
+        fjm_sinhcosh(u, &shu, &chu);
+        const double sh2 = shu * shu, D = n->Delta;
+        double dR, dz;
+        const double Phi = pot_phi_grad(T, D * shu * n->sv0, D * chu * n->cv0, &dR, &dz);
+        const double dPhidu = D * (dR * chu * n->sv0 + dz * shu * n->cv0);
+        return 2 * (n->E - Phi) * shu * chu - (sh2 + n->sv0 * n->sv0) * dPhidu + 2 * n->C * chu / (sh2 * shu);
+    }
+};
+
+struct FastFv {
+    PotView T;
+    const FastNode *n;
+    FJ_HD double operator()(double v) const {
+        double s, cs;
+        fjm_sincos(v, &s, &cs);
+        const double s2 = s * s, D = n->Delta;
+        const double Phi = pot_phi(T, D * n->shu0 * s, D * n->chu0 * cs);
+        return n->Bv - n->C / s2 - (n->shu0 * n->shu0 + s2) * (Phi - n->E);
+    }
+};
+
+constexpr double FAST_ROOT_RTOL = 1e-9;
+
+// wrp_GSLroots (include/Utils.h:134-167): an end of the bracket whose |F| is
+// below 5e-4 is taken as the root (low first); otherwise the root is solved
+// for.  The early return decides the reference's turning points of
+// near-circular and near-planar orbits, so it is kept.
+template <class F>
+FJ_HD inline double fast_root_ref(const F &f, double lo, double hi, double flo, double fhi, int *nev) {
+    if (fabs(flo) < 5e-4) return lo;
+    if (fabs(fhi) < 5e-4) return hi;
+    return fast_root(f, lo, hi, flo, fhi, FAST_ROOT_RTOL, nev);
+}
+
+// Brackets of the two u turning points found by GetTurnu's walk.
+struct UBracket {
+    double uinner, u1, u2, uouter;
+    bool noroot;
+};
+
+// GetTurnu(1) (uvOrb.cpp:131-201): the reference's bracket walk from u = 1.
+// The walk only takes signs of F_u (and of dF_u in its outer branches), so
+// Uw may be a lower-precision evaluation of F_u; dU is FP64.
+template <class UF>
+FJ_HD inline UBracket fast_walk_u(const UF &Uw, const FastDFu &dU, int *nev) {
+    UBracket b;
+    double ut_ = 1, ut = Uw(ut_), dutt = dU(ut_), eps = .1;
+    *nev += 2;
+    double uinner = ut_, uouter = ut_, u1 = ut_, u2 = ut_;
+    bool noroot = false;
+    int guard = 0;
+    if (ut < 0) {
+        if (dutt > 0) {
+            uinner = ut_;
+            while (guard++ < 4000) {
+                while (ut < 0 && dutt > 0 && guard++ < 4000) {
+                    uinner = ut_; ut_ *= (1 + eps);
+                    ut = Uw(ut_); dutt = dU(ut_); *nev += 2;
+                }
+                if (ut >= 0) { u1 = ut_; break; }
+                ut_ /= (1 + eps); eps *= .2;
+                if (eps < TINY) { uinner = uouter = ut_; noroot = true; break; }
+                ut_ *= (1 + eps);
+                ut = Uw(ut_); dutt = dU(ut_); *nev += 2;
+            }
+            if (!noroot) {
+                while (ut > 0 && guard++ < 8000) { u2 = ut_; ut_ *= 1.1; ut = Uw(ut_); (*nev)++; }
+                uouter = ut_;
+            }
+        } else {
+            while (guard++ < 4000) {
+                while (ut < 0 && dutt < 0 && guard++ < 4000) {
+                    uouter = ut_; ut_ /= (1 + eps);
+                    ut = Uw(ut_); dutt = dU(ut_); *nev += 2;
+                }
+                if (ut >= 0) { u2 = ut_; break; }
+                ut_ *= (1 + eps); eps *= .2;
+                if (eps < TINY) { uinner = uouter = ut_; noroot = true; break; }
+                ut_ /= (1 + eps);
+                ut = Uw(ut_); dutt = dU(ut_); *nev += 2;
+            }
+            if (!noroot) {
+                while (ut > 0 && guard++ < 8000) { u1 = ut_; ut_ *= .9; ut = Uw(ut_); (*nev)++; }
+                uinner = ut_;
+            }
+        }
+    } else {
+        const double f_at_1 = ut;
+        u1 = u2 = ut_;
+        while (ut > 0 && guard++ < 4000) { u2 = ut_; ut_ *= 1.1; ut = Uw(ut_); (*nev)++; }
+        uouter = ut_;
+        ut = f_at_1;   // the inward walk starts where the outward one stopped (uvOrb.cpp:186-190)
+        while (ut > 0 && guard++ < 8000) { u1 = ut_; ut_ *= .9; ut = Uw(ut_); (*nev)++; }
+        uinner = ut_;
+    }
+    b.uinner = uinner; b.u1 = u1; b.u2 = u2; b.uouter = uouter; b.noroot = noroot;
+    return b;
+}
+
+// Everything of one node before the action integrals: H < 0 test, Delta(E),
+// spheroidal coordinates (Uvpt.cpp:11-24), E, the constants above and the
+// turning points.  Returns 0 (H >= 0), 1 (bound) or 2 (E >= 0: unbound).
+// *nev counts the potential evaluations.  The u walk runs on UW (built from
+// tw and the node; FastFu for FP64), whose bracket is then checked in FP64.
+template <class UW>
+FJ_HD inline int fast_setup(const PotView &T, const typename UW::Tab &tw, const double *Eg, const double *Dg, int ng,
+                            double R, double z, double Phi_h, const double *V, FastNode &n, int *nev) {
+    n.ju = 0; n.jv = 0;
+    const double H = .5 * (V[0] * V[0] + V[1] * V[1] + V[2] * V[2]) + Phi_h;   // DF.cpp:83
+    if (!(H < 0)) return 0;
+    const double Delta = d_deltafn(Eg, Dg, ng, H);
+    const double Delta2 = Delta * Delta, R2 = R * R, z2 = z * z, r2 = R2 + z2;
+    const double shu2 = .5 * (r2 - Delta2 + sqrt((Delta2 - r2) * (Delta2 - r2) + 4 * Delta2 * R2)) / Delta2;
+    const double shu = sqrt(shu2), chu = sqrt(1 + shu2);
+    const double cv2 = .5 * (r2 + Delta2 - sqrt((Delta2 + r2) * (Delta2 + r2) - 4 * Delta2 * z2)) / Delta2;
+    const double cva = sqrt(cv2), sv2 = 1 - cv2, sva = sqrt(sv2 > 0 ? sv2 : 0);
+    const double cv = z < 0 ? -cva : cva;
+    const double pR = V[0], pz = V[2];
+    const double pu = Delta * (pR * chu * sva + pz * shu * cv);
+    const double pv = Delta * (pR * shu * cv - pz * chu * sva);
+    const double Lz = R * V[1];
+    n.Delta = Delta;
+    n.E = .5 * (pR * pR + pz * pz) + .5 * Lz * Lz / R2 + Phi_h;   // uvOrb.cpp:22
+    if (n.E >= 0 || isnan(n.E)) return 2;
+    n.C = .5 * Lz * Lz / Delta2;
+    const double K = (shu2 + sv2) * (Phi_h - n.E);
+    n.Bu = .5 * pu * pu / Delta2 + n.C / shu2 + K;
+    n.Bv = .5 * pv * pv / Delta2 + n.C / sv2 + K;
+    n.shu0 = shu; n.chu0 = chu; n.sv0 = sva; n.cv0 = cva;   // Phi is even in z
+    n.u0 = fjm_asinh(shu);
+    n.v0 = fjm_acos(cva);                                   // in [0, pi/2]
+    const FastFu U{T, &n};
+    const FastDFu dU{T, &n};
+    const UW Uw{tw, &n};
+    const UBracket b = fast_walk_u(Uw, dU, nev);
+    if (!b.noroot) {
+        // FP64 values at the bracket ends (the roots' brackets and wrp_GSLroots'
+        // endpoint test); where the low-precision walk took a sign that FP64
+        // does not confirm, the end moves on by the walk's own step
+        double ui = b.uinner, u1 = b.u1, u2 = b.u2, uo = b.uouter;
+        double fi = U(ui), f1 = U(u1), f2 = u2 == u1 ? f1 : U(u2), fo = U(uo);
+        *nev += 3 + (u2 != u1);
+        int g = 0;
+        while (fi > 0 && g++ < 200) { u1 = ui; f1 = fi; ui *= .9; fi = U(ui); (*nev)++; }
+        while (!(f1 > 0) && u1 < u2 && g++ < 400) { ui = u1; fi = f1; u1 /= .9; f1 = U(u1); (*nev)++; }
+        while (fo > 0 && g++ < 600) { u2 = uo; f2 = fo; uo *= 1.1; fo = U(uo); (*nev)++; }
+        while (!(f2 > 0) && u2 > u1 && g++ < 800) { uo = u2; fo = f2; u2 /= 1.1; f2 = U(u2); (*nev)++; }
+        n.uin = fast_root_ref(U, ui, u1, fi, f1, nev);
+        n.uout = fast_root_ref(U, u2, uo, f2, fo, nev);
+    } else {
+        n.uin = b.uinner; n.uout = b.uouter;
+    }
+    n.ju = !(fabs(n.uout - n.uin) < TINY);
+    // GetTurnv(pi/4) (uvOrb.cpp:203-224)
+    const FastFv Vf{T, &n};
+    n.vt = 0;
+    {
+        const double vtry = .5 * PIH;
+        double vtop = vtry, vbot = vtry, ft, fb;
+        const double f0 = Vf(vtry);
+        (*nev)++;
+        bool done = false;
+        int guard = 0;
+        if (f0 < 0) {
+            fb = f0; ft = f0;
+            while (fb < 0 && guard++ < 4000) {
+                vbot = .5 * (PIH + vbot);
+                if (PIH - vbot < TINY) { n.vt = PIH; done = true; break; }
+                fb = Vf(vbot); (*nev)++;
+            }
+        } else {
+            ft = f0; fb = f0;
+            while (ft > 0 && guard++ < 4000) {
+                vtop *= .5;
+                if (vtop < TINY) { n.vt = 0; done = true; break; }
+                ft = Vf(vtop); (*nev)++;
+            }
+        }
+        if (!done) {
+            if (!(vbot < vtop)) n.vt = fast_root_ref(Vf, vtop, vbot, ft, fb, nev);
+            else n.vt = fast_root_ref(Vf, vbot, vtop, fb, ft, nev);
+        }
+    }
+    n.jv = !(fabs(n.vt - PIH) < TINY);
+    return 1;
+}
+
+// The two action integrals with an n-point Gauss-Legendre rule on [0, 1]
+// (x[k], w[k]): FP64 version (host test build and reference for the FP32
+// kernel).
+FJ_HD inline void fast_actions64(const PotView &T, const FastNode &n, int ngl, const double *x, const double *w,
+                                 double *Jr, double *Jz) {
+    const double rtwo = 1.4142135623730951;
+    *Jr = 0; *Jz = 0;
+    const FastFu fu{T, &n};
+    const FastFv fv{T, &n};
+    if (n.ju) {
+        const double ubar = .5 * (n.uin + n.uout), du = .5 * (n.uout - n.uin);
+        double s = 0;
+        for (int k = 0; k < ngl; k++) {
+            double sp, cp;
+            fjm_sincos(PI * x[k], &sp, &cp);
+            const double F = fu(ubar - du * cp);
+            s += w[k] * sqrt(F > 0 ? F : 0) * sp;
+        }
+        *Jr = rtwo * n.Delta * du * s;
+    }
+    if (n.jv) {
+        const double dv = PIH - n.vt;
+        double s = 0;
+        for (int k = 0; k < ngl; k++) {
+            double sp, cp;
+            fjm_sincos(PIH * x[k], &sp, &cp);
+            const double F = fv(PIH - dv * cp);
+            s += w[k] * sqrt(F > 0 ? F : 0) * sp;
+        }
+        *Jz = rtwo * n.Delta * dv * s;
+    }
+}
+
+}  // namespace fj

@@ -11,6 +11,9 @@
 //               so the expensive evaluations stay in lock step
 //   k_moments   per point: f(J) at every node and the 6 weighted sums
 //               (GaussQuad.cpp:160-180) reduced in a fixed order
+// Fast node phase (FJ_PREC_FAST, fj_fast.hpp): k_fast fuses node setup,
+// Gauss-Legendre action integrals, f(J) and the moment sums (no node
+// records); k_fast_finish adds the slices of each point in a fixed order.
 // Grid phase: k_project (potLeg.cpp:68 + UtilsLeg.h:200-253), k_poisson
 // (Potential.cpp:110-200), k_guess (Potential.cpp:63-80), k_delta
 // (delta/tabulate.cpp + delta/Delta.cpp), k_virial (libs/Stats.cpp).
@@ -1008,6 +1011,127 @@ __device__ inline void moments_point_f32(const MomArgs &ma, int pl, float (*red)
         ma.sigRz[p] = sqrt(fabs(o[5]) / o[0]);
         ma.rho[p] = ma.quad ? 4 * o[0] : o[0];
     }
+}
+
+// ---------------------------------------------------------------------------
+// Fast node phase
+// ---------------------------------------------------------------------------
+constexpr int FAST_GL_MAX = 128;
+__constant__ FastGL32 c_fgl[FAST_GL_MAX];
+
+struct FastArgs {
+    const double *R, *z, *Phi, *Ve;   // [npts]
+    int npts, quad, S, ngl;           // S slices of the velocity nodes per point
+    fj_df_params df;
+    double *part;                     // [npts][S][6] slice sums
+    Counters *ctr;
+};
+
+// Block b: tile b % tiles of 64 grid points, node slice b / tiles.  Lane l of
+// each wave takes point 64 tile + l, so a wave follows one velocity node at
+// 64 neighbouring points (orbits of like shape); the 4 waves take every 4th
+// node of the slice.  Per lane, FP64 sums over its nodes; the 4 waves' sums
+// are added in wave order and stored as the slice's sums.
+#ifndef FJ_FAST_BLOCKS_PER_CU
+#define FJ_FAST_BLOCKS_PER_CU 32
+#endif
+#ifndef FJ_FAST_WALK32
+#define FJ_FAST_WALK32 1   // signs of the u bracket walk from FP32 evaluations, checked in FP64
+#endif
+#ifndef FJ_FAST_WAVES
+#define FJ_FAST_WAVES 4   // waves per SIMD the register allocation of k_fast aims at
+#endif
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(FJ_FAST_WAVES)))
+k_fast(TabArgs ta, FastArgs fa) {
+    extern __shared__ double smem[];
+    const LdsTabs L = stage_tables(ta, smem);
+    float *f32 = reinterpret_cast<float *>(smem + tab_lds_doubles(ta.nr, ta.npoly, ta.ngrid));
+    const PotView32 T32 = stage_tables32(L, f32);
+    double *red = reinterpret_cast<double *>(f32 + ((tab32_lds_floats(ta.nr, ta.npoly) + 1) & ~(size_t)1));
+    const int tiles = (fa.npts + 63) / 64;
+    const int tile = blockIdx.x % tiles, sl = blockIdx.x / tiles;
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int p = tile * 64 + lane;
+    const bool valid = p < fa.npts;
+    const int QS = (NQ + fa.S - 1) / fa.S;
+    const int q0 = sl * QS, q1 = q0 + QS < NQ ? q0 + QS : NQ;
+    double acc[6] = {0, 0, 0, 0, 0, 0};
+    int nbound = 0, nunb = 0, nbad = 0;
+    long long nev = 0;
+    if (valid) {
+        const double R = fa.R[p], z = fa.z[p], Phi_h = fa.Phi[p], Ve = fa.Ve[p];
+        const double jacob = Ve * Ve * Ve;
+        for (int q = q0 + wave; q < q1; q += 4) {
+            double V[3], w;
+            node_velocity(fa.quad, q, Ve, V, w);
+            FastNode fn;
+            int ne = 0;
+            #if FJ_FAST_WALK32
+            const int flag = fast_setup<FastFu32>(L.T, &T32, L.E, L.D, L.ngrid, R, z, Phi_h, V, fn, &ne);
+#else
+            const int flag = fast_setup<FastFu>(L.T, L.T, L.E, L.D, L.ngrid, R, z, Phi_h, V, fn, &ne);
+#endif
+            nev += ne;
+            if (flag == 0) continue;   // H >= 0: f = 0 (DF.cpp:84)
+            double Jr = 1e31, Jz = 1e31;
+            if (flag == 1) {
+                fast_actions32(T32, fn, c_fgl, fa.ngl, &Jr, &Jz);
+                nbound++;
+                nev += (long long)(fn.ju + fn.jv) * fa.ngl;
+            } else {
+                nunb++;
+            }
+            const double DF = d_df(fa.df, Jr, R * V[1], Jz);
+            nbad += !isfinite(DF);
+            const double jd = jacob * DF;
+            acc[0] += w * jd;
+            acc[1] += w * (jd * V[1]);
+            acc[2] += w * (jd * V[0] * V[0]);
+            acc[3] += w * (jd * V[1] * V[1]);
+            acc[4] += w * (jd * V[2] * V[2]);
+            acc[5] += w * (jd * V[0] * V[2]);
+        }
+    }
+    for (int m = 0; m < 6; m++) red[(wave * 6 + m) * 64 + lane] = acc[m];
+    __syncthreads();
+    if (wave == 0 && valid) {
+        double *o = fa.part + ((size_t)p * fa.S + sl) * 6;
+        for (int m = 0; m < 6; m++) {
+            double t = red[m * 64 + lane];
+            for (int w2 = 1; w2 < 4; w2++) t += red[(w2 * 6 + m) * 64 + lane];
+            o[m] = t;
+        }
+    }
+    // counters: one atomic per wave
+    unsigned long long b = nbound, u = nunb, nb = nbad, ev = (unsigned long long)nev;
+    for (int off = 32; off > 0; off >>= 1) {
+        b += __shfl_down(b, off); u += __shfl_down(u, off);
+        nb += __shfl_down(nb, off); ev += __shfl_down(ev, off);
+    }
+    if (lane == 0) {
+        if (b) atomicAdd(&fa.ctr->bound, b);
+        if (u) atomicAdd(&fa.ctr->unbound, u);
+        if (nb) atomicAdd(&fa.ctr->nonfinite, nb);
+        if (ev) atomicAdd(&fa.ctr->phi, ev);
+    }
+}
+
+// Sums of the S slices of each point in slice order, then the moments of
+// rhofDF (Integ.cpp:160-161) as k_moments forms them.
+__global__ void __launch_bounds__(256) k_fast_finish(int npts, int S, int quad, const double *part, double *out6,
+                                                     double *mom) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= npts) return;
+    double o[6] = {0, 0, 0, 0, 0, 0};
+    for (int s = 0; s < S; s++)
+        for (int m = 0; m < 6; m++) o[m] += part[((size_t)p * S + s) * 6 + m];
+    for (int m = 0; m < 6; m++) out6[(size_t)p * 6 + m] = o[m];
+    mom[p] = quad ? 4 * o[0] : o[0];
+    mom[npts + p] = o[1] / o[0];
+    mom[2 * npts + p] = sqrt(o[2] / o[0]);
+    mom[3 * npts + p] = sqrt(o[3] / o[0]);
+    mom[4 * npts + p] = sqrt(o[4] / o[0]);
+    mom[5 * npts + p] = sqrt(fabs(o[5]) / o[0]);
 }
 
 // ---------------------------------------------------------------------------
@@ -2069,6 +2193,10 @@ struct fj_ctx {
     int pt_cap = 0;
     double *d_eval = nullptr; int eval_cap = 0;   // fj_eval_potential: R, z, Phi, dR, dz [eval_cap] each
     unsigned char *noalloc = nullptr;   // this device's copy of the fj_rhofDF_noalloc scratch
+    // fast node phase: slice sums [npts][S][6], Gauss-Legendre order of the action integrals
+    double *d_fpart = nullptr; size_t fpart_cap = 0;
+    int fast_ngl = 64;
+    bool fgl_ready = false;
     double *d_pR = nullptr, *d_pz = nullptr, *d_pPhi = nullptr, *d_pVe = nullptr, *d_out6 = nullptr, *d_mom = nullptr;
     // grid buffers
     double *d_rhl = nullptr, *d_gl = nullptr, *d_pw = nullptr, *d_pws = nullptr; int *d_status = nullptr;
@@ -2365,12 +2493,103 @@ int sur_collect(fj_ctx *c, const NodeBuf &nb, int p0, int np, const int *only, u
     return 0;
 }
 
+// Gauss-Legendre rule on [0, 1] with n nodes (Newton on P_n to full
+// precision), as the FP32 constants of the fast action integrals.
+void fast_gl_table(int n, FastGL32 *t) {
+    for (int i = 0; i < n; i++) {
+        double z = cos(M_PI * (i + 0.75) / (n + 0.5)), pp = 1;
+        for (int it = 0; it < 100; it++) {
+            double p1 = 1, p2 = 0;
+            for (int j = 0; j < n; j++) {
+                const double p3 = p2;
+                p2 = p1;
+                p1 = ((2 * j + 1) * z * p2 - j * p3) / (j + 1);
+            }
+            pp = n * (z * p1 - p2) / (z * z - 1);
+            const double dz = p1 / pp;
+            z -= dz;
+            if (fabs(dz) < 1e-16) break;
+        }
+        const double x = 0.5 * (1 - z), w = 1 / ((1 - z * z) * pp * pp);   // node on [0, 1], weight / 2 * 2
+        t[i].w = (float)w;
+        t[i].cu = (float)cos(M_PI * x); t[i].su = (float)sin(M_PI * x);
+        t[i].cv = (float)cos(0.5 * M_PI * x); t[i].sv = (float)sin(0.5 * M_PI * x);
+    }
+}
+
+// Fast node phase: k_points, k_fast, k_fast_finish; results in d_out6 and d_mom.
+int run_fast(fj_ctx *c, int npts) {
+    if (!c->fgl_ready) {
+        FastGL32 t[FAST_GL_MAX];
+        fast_gl_table(c->fast_ngl, t);
+        HIPCHK(c, hipMemcpyToSymbol(HIP_SYMBOL(c_fgl), t, sizeof(FastGL32) * c->fast_ngl));
+        c->fgl_ready = true;
+    }
+    int r;
+    if (!c->d_cnt) {
+        if ((r = dalloc(c, &c->d_cnt, 16)) || (r = dalloc(c, &c->d_ctr, 1))) return r;
+    }
+    const int tiles = (npts + 63) / 64;
+    hipDeviceProp_t prop;
+    HIPCHK(c, hipGetDeviceProperties(&prop, c->device));
+    // many short blocks (FJ_FAST_BLOCKS_PER_CU per CU) so that the last round of
+    // blocks is a small part of the launch; at least 4 nodes per wave
+    int S = (FJ_FAST_BLOCKS_PER_CU * prop.multiProcessorCount + tiles - 1) / tiles;
+    if (S > NQ / 16) S = NQ / 16;
+    if (S < 1) S = 1;
+    const size_t need = (size_t)npts * S * 6;
+    if (need > c->fpart_cap) {
+        if ((r = dalloc(c, &c->d_fpart, need))) return r;
+        c->fpart_cap = need;
+    }
+    const TabArgs ta = tab_args(c);
+    const size_t lds_pts = tab_bytes(c);
+    const size_t lds = tab_bytes(c) + ((tab32_lds_floats(c->nr, c->npoly) + 1) & ~(size_t)1) * sizeof(float) +
+                       4 * 6 * 64 * sizeof(double);
+    hipEvent_t ev[4];
+    for (int i = 0; i < 4; i++) HIPCHK(c, hipEventCreate(&ev[i]));
+    HIPCHK(c, hipMemsetAsync(c->d_ctr, 0, sizeof(Counters), c->stream));
+    memset(&c->st, 0, sizeof c->st);
+    HIPCHK(c, hipEventRecord(ev[0], c->stream));
+    PointArgs pa{c->d_pR, c->d_pz, c->d_pPhi, c->d_pVe, nullptr};
+    k_points<<<(npts + 255) / 256, 256, lds_pts, c->stream>>>(ta, pa, npts);
+    HIPCHK(c, hipGetLastError());
+    HIPCHK(c, hipEventRecord(ev[1], c->stream));
+    FastArgs fa{c->d_pR, c->d_pz, c->d_pPhi, c->d_pVe, npts, c->quad, S, c->fast_ngl, c->df, c->d_fpart, c->d_ctr};
+    k_fast<<<(unsigned)(tiles * S), 256, lds, c->stream>>>(ta, fa);
+    HIPCHK(c, hipGetLastError());
+    HIPCHK(c, hipEventRecord(ev[2], c->stream));
+    k_fast_finish<<<(npts + 255) / 256, 256, 0, c->stream>>>(npts, S, c->quad, c->d_fpart, c->d_out6, c->d_mom);
+    HIPCHK(c, hipGetLastError());
+    HIPCHK(c, hipEventRecord(ev[3], c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    Counters h;
+    HIPCHK(c, hipMemcpy(&h, c->d_ctr, sizeof h, hipMemcpyDeviceToHost));
+    c->st.ms_points = elapsed(ev[0], ev[1]);
+    c->st.ms_setup = elapsed(ev[1], ev[2]);     // the fused node kernel
+    c->st.ms_moments = elapsed(ev[2], ev[3]);
+    c->st.ms_total = elapsed(ev[0], ev[3]);
+    c->st.points = npts;
+    c->st.nodes = (long long)npts * NQ;
+    c->st.bound_nodes = (long long)h.bound;
+    c->st.unbound_nodes = (long long)h.unbound;
+    c->st.nonfinite_nodes = (long long)h.nonfinite;
+    c->st.phi_evals = (long long)h.phi + 2 * (long long)npts;
+    c->st.setup_lanes = tiles * S * 256;
+    for (int i = 0; i < 4; i++) (void)hipEventDestroy(ev[i]);
+    return 0;
+}
+
 // Node phase for npts points already in d_pR/d_pz; results in d_out6 and d_mom.
 int run_nodes(fj_ctx *c, int npts) {
     if (!c->have_pot || !c->have_delta || !c->have_df)
         return set_err(c, FJ_E_STATE, "potential, Delta table and DF must be set before the node phase");
     const bool mixed = c->prec == FJ_PREC_MIXED;
     const bool sur = c->sur_on && c->sur_init && c->sur_norm;
+    if (c->prec == FJ_PREC_FAST) {
+        if (sur) return set_err(c, FJ_E_STATE, "the action surrogate runs with FJ_PREC_FP64 or FJ_PREC_MIXED");
+        return run_fast(c, npts);
+    }
     const long long chunk = chunk_points(c, npts);
     int r = ensure_nodes(c, chunk * NQ);
     if (r) return r;
@@ -2643,7 +2862,7 @@ int fj_ctx_destroy(fj_ctx *c) {
     void *ptrs[] = {c->d_ar, c->d_phil, c->d_Pr, c->d_Pr2, c->d_E, c->d_D, c->d_ci, c->d_si, c->d_poly, c->d_lut,
                     c->d_rk, c->d_nodef, c->d_Jr, c->d_Jz, c->d_flag, c->d_list_u, c->d_list_v, c->d_ovf, c->d_cnt,
                     c->d_ctr, c->d_ws, c->d_wsfull, c->d_pR, c->d_pz, c->d_pPhi, c->d_pVe, c->d_out6,
-                    c->d_mom, c->d_rhl, c->d_gl, c->d_pw, c->d_pws, c->d_status, c->d_eval};
+                    c->d_mom, c->d_rhl, c->d_gl, c->d_pw, c->d_pws, c->d_status, c->d_eval, c->d_fpart};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     void *sp[] = {c->d_pVc, c->d_X, c->d_T, c->d_bt, c->d_bw, c->d_pos, c->d_loss, c->d_sx16, c->d_sy, c->d_judge_J,
@@ -2721,8 +2940,17 @@ int fj_set_quad_rule(fj_ctx *c, int preset) {
 
 int fj_set_precision(fj_ctx *c, int mode) {
     if (!c) return FJ_E_ARG;
-    if (mode != FJ_PREC_FP64 && mode != FJ_PREC_MIXED) return set_err(c, FJ_E_ARG, "unknown precision %d", mode);
+    if (mode != FJ_PREC_FP64 && mode != FJ_PREC_MIXED && mode != FJ_PREC_FAST)
+        return set_err(c, FJ_E_ARG, "unknown precision %d", mode);
     c->prec = mode;
+    return 0;
+}
+
+int fj_set_fast_order(fj_ctx *c, int n) {
+    if (!c) return FJ_E_ARG;
+    if (n < 4 || n > FAST_GL_MAX) return set_err(c, FJ_E_ARG, "Gauss-Legendre order %d outside 4..%d", n, FAST_GL_MAX);
+    c->fast_ngl = n;
+    c->fgl_ready = false;
     return 0;
 }
 
@@ -2842,6 +3070,7 @@ int fj_rhofDF_noalloc(fj_ctx *c, int n, const double *R, const double *z, double
     if (!c->have_pot || !c->have_delta || !c->have_df)
         return set_err(c, FJ_E_STATE, "potential, Delta table and DF must be set before the node phase");
     if (c->sur_on) return set_err(c, FJ_E_STATE, "fj_rhofDF_noalloc: switch the action surrogate off first");
+    if (c->prec == FJ_PREC_FAST) return set_err(c, FJ_E_STATE, "fj_rhofDF_noalloc: FJ_PREC_FP64 or FJ_PREC_MIXED only");
     HIPCHK(c, hipSetDevice(c->device));
     unsigned char *b = c->noalloc;
     auto D = [b](size_t off) { return reinterpret_cast<double *>(b + off); };

@@ -13,6 +13,7 @@
 
 #include "fj_device.hpp"
 #include "fj_node.hpp"
+#include "fj_fast.hpp"
 
 namespace fj {
 
@@ -41,59 +42,59 @@ __device__ __forceinline__ float pot_phi32(const PotView32 &T, float R, float z,
     const float c = z / rs;
     const int nr = T.nr, np_ = T.npoly;
     const float *ar = T.ar;
+    const float c2 = c * c;
+    // the coefficients are formed and summed in one pass (no per-lane array)
+    float phi, pm2 = 1.f, pm1 = 1.5f * c2 - .5f;
     float r = rs;
-    float coef[MAXPOLY];
     if (r >= ar[nr - 1]) {
         // Keplerian tail: phil[nr-1][l] (ar[nr-1]/r)^(l+1), l = 2np
         const float q = ar[nr - 1] / r, q2 = q * q;
-        coef[0] = fmaf(T.phil[(nr - 1) * np_], q, -shift) + T.phil0lo[nr - 1] * q;
+        const float *last = T.phil + (nr - 1) * np_;
+        phi = fmaf(last[0], q, -shift) + T.phil0lo[nr - 1] * q;
         float qp = q * q2;
         for (int np = 1; np < np_; np++) {
-            coef[np] = T.phil[(nr - 1) * np_ + np] * qp;
+            float p;
+            if (np == 1) p = pm1;
+            else { p = (T.la[np] * pm2 + pm1 * (c2 - T.lb[np])) * T.lg[np]; pm2 = pm1; pm1 = p; }
+            phi += (last[np] * qp) * p;
             qp *= q2;
         }
-    } else {
-        if (r < ar[0]) r = ar[0];
-        const float t = (__log2f(r) - T.lg0) * T.lsc;
-        int bot = T.lut[t > 0.f ? (t < (float)(NLUT - 1) ? (int)t : NLUT - 1) : 0];
-        while (bot < nr - 2 && r >= ar[bot + 1]) bot++;
-        while (bot > 0 && !(r >= ar[bot])) bot--;
-        const int top = bot + 1;
-        const float db = r - ar[bot];
-        const float f1 = db / (ar[top] - ar[bot]);
-        const bool quad = top < 10;
-        float f2 = 0, f3 = 0;
-        int thr = 0;
-        if (quad) {
-            thr = (f1 < 0.5f && bot > 0) ? bot - 1 : top + 1;
-            f2 = (r - ar[top]) * db / ((ar[thr] - ar[top]) * (ar[thr] - ar[bot]));
-            f3 = (ar[thr] - ar[bot]) / (ar[top] - ar[bot]);
-        }
-        {
-            // monopole minus shift: b - shift + f1 (t - b) + f2 (h - b - f3 (t - b))
-            const float *lo = T.phil0lo;
-            const float b = T.phil[bot * np_];
-            const float bc = (b - shift) + lo[bot];
-            const float tb = (T.phil[top * np_] - b) + (lo[top] - lo[bot]);
-            float v = bc + f1 * tb;
-            if (quad) v += f2 * (((T.phil[thr * np_] - b) + (lo[thr] - lo[bot])) - f3 * tb);
-            coef[0] = v;
-        }
-        for (int np = 1; np < np_; np++) {
-            const float t = T.phil[top * np_ + np], b = T.phil[bot * np_ + np];
-            float v = b + f1 * (t - b);
-            if (quad) v += f2 * (T.phil[thr * np_ + np] - b - f3 * (t - b));
-            coef[np] = v;
-        }
+        return phi;
     }
-    const float c2 = c * c;
-    float phi = coef[0];
-    float pm2 = 1.f, pm1 = 1.5f * c2 - .5f;
+    if (r < ar[0]) r = ar[0];
+    const float t = (__log2f(r) - T.lg0) * T.lsc;
+    int bot = T.lut[t > 0.f ? (t < (float)(NLUT - 1) ? (int)t : NLUT - 1) : 0];
+    while (bot < nr - 2 && r >= ar[bot + 1]) bot++;
+    while (bot > 0 && !(r >= ar[bot])) bot--;
+    const int top = bot + 1;
+    const float db = r - ar[bot];
+    const float f1 = db / (ar[top] - ar[bot]);
+    const bool quad = top < 10;
+    float f2 = 0, f3 = 0;
+    int thr = 0;
+    if (quad) {
+        thr = (f1 < 0.5f && bot > 0) ? bot - 1 : top + 1;
+        f2 = (r - ar[top]) * db / ((ar[thr] - ar[top]) * (ar[thr] - ar[bot]));
+        f3 = (ar[thr] - ar[bot]) / (ar[top] - ar[bot]);
+    }
+    const float *pb = T.phil + bot * np_, *pt = T.phil + top * np_, *ph = T.phil + thr * np_;
+    {
+        // monopole minus shift: b - shift + f1 (t - b) + f2 (h - b - f3 (t - b))
+        const float *lo = T.phil0lo;
+        const float b = pb[0];
+        const float bc = (b - shift) + lo[bot];
+        const float tb = (pt[0] - b) + (lo[top] - lo[bot]);
+        phi = bc + f1 * tb;
+        if (quad) phi += f2 * (((ph[0] - b) + (lo[thr] - lo[bot])) - f3 * tb);
+    }
     for (int np = 1; np < np_; np++) {
+        const float tv = pt[np], b = pb[np];
+        float v = b + f1 * (tv - b);
+        if (quad) v += f2 * (ph[np] - b - f3 * (tv - b));
         float p;
         if (np == 1) p = pm1;
         else { p = (T.la[np] * pm2 + pm1 * (c2 - T.lb[np])) * T.lg[np]; pm2 = pm1; pm1 = p; }
-        phi += coef[np] * p;
+        phi += v * p;
     }
     return phi;
 }
@@ -158,6 +159,75 @@ __device__ __forceinline__ float df32(const DF32 &d, float Jr, float Jphi, float
     const float t = tanhf(d.chi * Jphi * d.invJ0);
     if (d.rot_form == 1) return (1.f - d.k) * even + d.k * t * even;
     return even + d.k * t * even;
+}
+
+// Fast mode (fj_fast.hpp): the two action integrals as Gauss-Legendre rules
+// in FP32.  Per rule node k: weight w_k of the rule on [0, 1] and the fixed
+// cos / sin of psi_k = pi x_k (Ju) and psi_k = pi x_k / 2 (Jv).  Energies are
+// measured from c = fl(E): Phi - c comes from the float-pair monopole of
+// pot_phi32, and E - c is carried separately.
+struct FastGL32 {
+    float w, cu, su, cv, sv;
+};
+
+// F_u of fj_fast.hpp in FP32, for the signs of the u bracket walk.
+struct FastFu32 {
+    using Tab = const PotView32 *;
+    const PotView32 *T;
+    const FastNode *n;
+    __device__ double operator()(double u) const {
+        float sh, ch;
+        sinhcosh32((float)u, &sh, &ch);
+        const float sh2 = sh * sh;
+        const float c = (float)n->E, dE = (float)(n->E - (double)c);
+        const float D = (float)n->Delta, sv0 = (float)n->sv0, cv0 = (float)n->cv0;
+        const float Pc = pot_phi32(*T, D * sh * sv0, D * ch * cv0, c) - dE;   // Phi - E
+        return (double)((float)n->Bu - (float)n->C / sh2 - (sh2 + sv0 * sv0) * Pc);
+    }
+};
+
+__device__ __forceinline__ void fast_actions32(const PotView32 &T, const FastNode &n, const FastGL32 *gl, int ngl,
+                                               double *Jr, double *Jz) {
+    const double rtwo = 1.4142135623730951;
+    const float c = (float)n.E;
+    const float dE = (float)(n.E - (double)c);
+    const float D = (float)n.Delta, sv0 = (float)n.sv0, cv0 = (float)n.cv0, C = (float)n.C;
+    const float sv0sq = sv0 * sv0;
+    *Jr = 0;
+    *Jz = 0;
+    if (n.ju) {
+        const double du = .5 * (n.uout - n.uin);
+        const float ub = (float)(.5 * (n.uin + n.uout)), duf = (float)du, Bu = (float)n.Bu;
+        float s = 0.f;
+#pragma unroll 2
+        for (int k = 0; k < ngl; k++) {
+            const float u = ub - duf * gl[k].cu;
+            float sh, ch;
+            sinhcosh32(u, &sh, &ch);
+            const float sh2 = sh * sh;
+            const float Pc = pot_phi32(T, D * sh * sv0, D * ch * cv0, c) - dE;   // Phi - E
+            const float F = Bu - C / sh2 - (sh2 + sv0sq) * Pc;
+            s += gl[k].w * (sqrtf(fmaxf(0.f, F)) * gl[k].su);
+        }
+        *Jr = rtwo * n.Delta * du * (double)s;
+    }
+    if (n.jv) {
+        const double dv = PIH - n.vt;
+        const float dvf = (float)dv, Bv = (float)n.Bv;
+        const float sh0 = (float)n.shu0, ch0 = (float)n.chu0, sh0sq = sh0 * sh0;
+        float s = 0.f;
+#pragma unroll 2
+        for (int k = 0; k < ngl; k++) {
+            const float v = 1.5707963f - dvf * gl[k].cv;
+            float sn, cs;
+            sincosf(v, &sn, &cs);
+            const float s2 = sn * sn;
+            const float Pc = pot_phi32(T, D * sh0 * sn, D * ch0 * cs, c) - dE;
+            const float F = Bv - C / s2 - (sh0sq + s2) * Pc;
+            s += gl[k].w * (sqrtf(fmaxf(0.f, F)) * gl[k].sv);
+        }
+        *Jz = rtwo * n.Delta * dv * (double)s;
+    }
 }
 
 // Compensated (Kahan) FP32 accumulator.

@@ -37,7 +37,7 @@ struct NodeConst {
 };
 
 struct DUturn {   // duturnfn (uvOrb.cpp:74-79)
-    const PotView *T;
+    PotView T;        // by value: the table pointers stay visible as LDS addresses
     const NodeConst *c;
     long long *nev;
     FJ_HD double operator()(double u) const {
@@ -46,7 +46,7 @@ struct DUturn {   // duturnfn (uvOrb.cpp:74-79)
         const double shu2 = shu * shu;
         const double D = c->Delta;
         double dR, dz;
-        const double Phi0 = pot_phi_grad(*T, D * shu * c->sinv, D * chu * c->cosv, &dR, &dz);
+        const double Phi0 = pot_phi_grad(T, D * shu * c->sinv, D * chu * c->cosv, &dR, &dz);
         (*nev)++;
         const double dPhidu = D * (dR * chu * c->sv + dz * shu * c->cv);
         return 2 * (c->E - Phi0) * shu * chu - (shu2 + c->sv2) * dPhidu + c->Lzsq * chu / (D * D * shu2 * shu);
@@ -54,7 +54,7 @@ struct DUturn {   // duturnfn (uvOrb.cpp:74-79)
 };
 
 struct Uturn {    // uturnfn (uvOrb.cpp:70-73) with dU (61-63)
-    const PotView *T;
+    PotView T;        // by value: the table pointers stay visible as LDS addresses
     const NodeConst *c;
     int *nphi;        // potential evaluations, or null
     FJ_HD double operator()(double u) const {
@@ -63,14 +63,14 @@ struct Uturn {    // uturnfn (uvOrb.cpp:70-73) with dU (61-63)
         const double shu2 = shu * shu;
         const double D = c->Delta;
         if (nphi) (*nphi)++;
-        const double Phi = pot_phi(*T, D * shu * c->sinv, D * chu * c->cosv);
+        const double Phi = pot_phi(T, D * shu * c->sinv, D * chu * c->cosv);
         const double dU = (shu2 + c->sv2) * Phi - c->K2;
         return c->E * (shu2 - c->sh1sq) + c->K1 - dU - c->K3 * (1 / shu2 - c->invsh1sq);
     }
 };
 
 struct Vturn {    // vturnfn (uvOrb.cpp:49-52) with dV (46-48)
-    const PotView *T;
+    PotView T;        // by value: the table pointers stay visible as LDS addresses
     const NodeConst *c;
     int *nphi;        // potential evaluations, or null
     FJ_HD double operator()(double v) const {
@@ -79,7 +79,7 @@ struct Vturn {    // vturnfn (uvOrb.cpp:49-52) with dV (46-48)
         const double sv2 = s * s;
         const double D = c->Delta;
         if (nphi) (*nphi)++;
-        const double Phi = pot_phi(*T, D * c->shuX * s, D * c->chuX * cs);
+        const double Phi = pot_phi(T, D * c->shuX * s, D * c->chuX * cs);
         const double dV = c->KV - (c->shu2 + sv2) * Phi;
         return c->E * sv2 + c->Iv + dV - .5 * c->Lzsq / (D * D * sv2);
     }
@@ -184,7 +184,7 @@ FJ_HD inline int node_setup(const PotView &T, const double *Eg, const double *Dg
         flag = 1;
         c.sv = sv; c.cv = cv; c.sv2 = sv2; c.sinv = sv; c.cosv = fjm_cos(v); c.shu2 = shu2;
         // Getu0(us = 58): uvOrb.cpp:109-129
-        DUturn du{&T, &c, &nev};
+        DUturn du{T, &c, &nev};
         double utry = 58, ui = 58, uo = 58, dfui = 0, dfuo = 0;
         double dut = du(utry);
         int guard = 0;
@@ -225,7 +225,7 @@ FJ_HD inline int node_setup(const PotView &T, const double *Eg, const double *Dg
         c.Iv = .5 * (pv * pv) / Delta2 - c.E * sv2 + .5 * c.Lzsq / (Delta2 * sv2) - dVX;
 
         // GetTurnu(1): uvOrb.cpp:131-201
-        Uturn U{&T, &c, nphi};
+        Uturn U{T, &c, nphi};
         const long long nev0 = nev;
         double ut_ = 1, ut = U(ut_), dutt = du(ut_), eps = .1;
         double uinner = ut_, uouter = ut_, u1 = ut_, u2 = ut_, fu1 = ut, fu2 = ut, fuinner = ut, fuouter = ut;
@@ -296,7 +296,7 @@ FJ_HD inline int node_setup(const PotView &T, const double *Eg, const double *Dg
         }
         // GetTurnv(.5*PIH): uvOrb.cpp:203-224
         if (nphi) *nphi += (int)(nev - nev0);   // the duturnfn calls of GetTurnu
-        Vturn Vf{&T, &c, nphi};
+        Vturn Vf{T, &c, nphi};
         {
             const double vtry = .5 * PIH;
             double vtop = vtry, vbot = vtry, ft, fb;

@@ -48,7 +48,18 @@ enum { FJ_POISSON_QUADPACK = 0, FJ_POISSON_BLOCKSCAN = 1 };
  * and DF (tasks that reach the FP32 interval cap rerun in FP64); FP64 Delta,
  * spheroidal coordinates, turning-point roots and the Poisson solve.
  * Results are not bitwise those of FJ_PREC_FP64. */
-enum { FJ_PREC_FP64 = 0, FJ_PREC_MIXED = 1 };
+enum { FJ_PREC_FP64 = 0, FJ_PREC_MIXED = 1, FJ_PREC_FAST = 2 };
+/* FJ_PREC_FAST: the same Staeckel-fudge actions computed for speed, fused
+ * with f(J) and the moment sums in one kernel (no node records): the u-side
+ * reference point at the node (Getu0's umid cancels from uturnfn,
+ * uvOrb.cpp:25-31,70-73), the reference's turning-point bracket walks
+ * (uvOrb.cpp:131-224) with wrp_GSLroots' endpoint test and FP64 regula falsi,
+ * and the action integrals (uvOrb.cpp:226-262) as FP32 Gauss-Legendre rules
+ * of fj_set_fast_order points (64 by default) instead of QAGS.  The action
+ * surrogate does not run in this mode. */
+
+/* Gauss-Legendre order of the FJ_PREC_FAST action integrals (4..128). */
+int fj_set_fast_order(fj_ctx *ctx, int n);
 
 /* DF parameters: df/DF.cpp:18-20,36-63 (alpha[2], dphi_h, dz_h, dphi_g, dz_g)
  * and pot/models.cpp:13 (mass, J0, chi).  rot_form 0 is the code's

@@ -4,6 +4,7 @@
 #include <vector>
 
 #include "fj_node.hpp"
+#include "fj_fast.hpp"
 #include "fj_math.h"
 
 using namespace fj;
@@ -257,6 +258,70 @@ int fjh_actions(int nr, const double *ar, int npoly, const double *phil, const d
         Jz[i] = jz;
         if (nqk) { nqk[2 * i] = q1; nqk[2 * i + 1] = q2; }
     }
+    return 0;
+}
+
+// Fast-mode actions (fj_fast.hpp) at n phase-space points with an ngl-point
+// Gauss-Legendre rule (x, w on [0, 1]); nev[i]: potential evaluations of the
+// turning-point search of point i; tp (optional, 3n): uin, uout, vturn.
+int fjh_fast_actions(int nr, const double *ar, int npoly, const double *phil, const double *Pr, int ng,
+                     const double *Eg, const double *Dg, int n, const double *R, const double *z, const double *Phi,
+                     const double *vR, const double *vphi, const double *vz, int ngl, const double *x,
+                     const double *w, double *Jr, double *Jz, int *nev, double *tp) {
+    HostPot H(nr, ar, npoly, phil, Pr, npoly);
+    const PotView &T = H.T;
+    for (int i = 0; i < n; i++) {
+        const double V[3] = {vR[i], vphi[i], vz[i]};
+        FastNode fn;
+        int ne = 0;
+        const int flag = fast_setup<FastFu>(T, T, Eg, Dg, ng, R[i], z[i], Phi[i], V, fn, &ne);
+        double jr = 0, jz = 0;
+        if (flag == 2) { jr = 1e31; jz = 1e31; }
+        else if (flag == 1) fast_actions64(T, fn, ngl, x, w, &jr, &jz);
+        Jr[i] = jr; Jz[i] = jz;
+        if (nev) nev[i] = ne;
+        if (tp) { tp[3 * i] = fn.uin; tp[3 * i + 1] = fn.uout; tp[3 * i + 2] = fn.vt; }
+    }
+    return 0;
+}
+
+// rhofDF's six sums at one point with the fast-mode actions (FP64 integrands,
+// ngl-point Gauss-Legendre rule), summed in node order.
+int fjh_fast_rhofDF(int nr, const double *ar, int npoly, const double *phil, const double *Pr, int ng,
+                    const double *Eg, const double *Dg, const double *xa, const double *wa, const double *xp,
+                    const double *wp, double R, double z, double A, double B, double dphi_h, double dz_h,
+                    double dphi_g, double dz_g, double mass, double J0, double chi, double k, int rot_form, int ngl,
+                    const double *gx, const double *gw, double *out6) {
+    HostPot H(nr, ar, npoly, phil, Pr, npoly);
+    const PotView &T = H.T;
+    const double far = 10 * ar[nr - 1];
+    const double Phi_h = pot_phi(T, R, z);
+    const double Ve = sqrt(-2 * (Phi_h - pot_phi(T, far, far)));
+    const double jacob = Ve * Ve * Ve;
+    double s[6] = {0, 0, 0, 0, 0, 0};
+    for (int node = 0; node < 22 * 22 * 22; node++) {
+        const int i = node / 484, j = (node / 22) % 22, kk = node % 22;
+        const double V[3] = {1e-6 + Ve * xa[i], 1e-6 + Ve * xp[j], 1e-6 + Ve * xa[kk]};
+        const double wt = wa[i] * wp[j] * wa[kk];
+        FastNode fn;
+        int ne = 0;
+        const int flag = fast_setup<FastFu>(T, T, Eg, Dg, ng, R, z, Phi_h, V, fn, &ne);
+        if (flag == 0) continue;
+        double jr = 1e31, jz = 1e31;
+        if (flag == 1) fast_actions64(T, fn, ngl, gx, gw, &jr, &jz);
+        const double Jphi = R * V[1];
+        const double hh = jr + dphi_h * fabs(Jphi) + dz_h * jz, gg = jr + dphi_g * fabs(Jphi) + dz_g * jz;
+        double DF = mass / fjm_powi(J0, 3) * fj_max(0., fjm_pow(1. + J0 / hh, A) / (fjm_pow(1. + gg / J0, B))) /
+                    fjm_powi(TPI, 3);
+        if (chi != 0.) {
+            const double t = fjm_tanh(chi * Jphi / J0);
+            DF = rot_form == 1 ? (1. - k) * DF + k * t * DF : DF + k * t * DF;
+        }
+        const double jd = jacob * DF;
+        s[0] += wt * jd; s[1] += wt * (jd * V[1]); s[2] += wt * (jd * V[0] * V[0]);
+        s[3] += wt * (jd * V[1] * V[1]); s[4] += wt * (jd * V[2] * V[2]); s[5] += wt * (jd * V[0] * V[2]);
+    }
+    for (int m = 0; m < 6; m++) out6[m] = s[m];
     return 0;
 }
 

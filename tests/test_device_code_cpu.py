@@ -228,3 +228,39 @@ def test_qags_float_instance(H):
                     assert err <= 100 * tol, (f, a, b, epsrel, ier.value, r.value, ref)
     print(stats)
     assert sum(1 for s in stats if s[0] == 0) >= len(stats) // 2
+
+
+def test_fast_mode_actions_near_reference(H, models):
+    """The fast mode's actions (fj_fast.hpp: the reference's walks from the
+    node's own constants, regula falsi roots, 64-point Gauss-Legendre
+    integrals), host build with FP64 integrands, against the oracle's
+    (Getu0 + Brent + QAGS): the same turning-point brackets and endpoint
+    returns, so the differences are quadrature-level (the QAGS tolerance is
+    1e-3 relative)."""
+    H.fjh_fast_actions.argtypes = [C.c_int, dp, C.c_int, dp, dp, C.c_int, dp, dp, C.c_int] + [dp] * 6 + \
+        [C.c_int, dp, dp, dp, dp, C.POINTER(C.c_int), dp]
+    p = models["hq40"]
+    phil, Pr, E, D = tabs(p)
+    gx, gw = np.polynomial.legendre.leggauss(64)
+    gx, gw = 0.5 * (gx + 1), 0.5 * gw
+    er, ez = [], []
+    for R, z in [(0.011, 0.004), (0.866, 0.5), (8.0, 3.0), (60.0, 40.0)]:
+        Phi, V, Dl = point_nodes(p, R, z)
+        V = V[::23].copy()
+        n = len(V)
+        Fr, Fz = np.zeros(n), np.zeros(n)
+        H.fjh_fast_actions(p.nr, P(p.ar), p.npoly, P(phil), P(Pr), p.ngrid, P(E), P(D), n, P(np.full(n, R)),
+                           P(np.full(n, z)), P(np.full(n, Phi)), P(V[:, 0]), P(V[:, 1]), P(V[:, 2]), 64, P(gx),
+                           P(gw), P(Fr), P(Fz), None, None)
+        for i in range(n):
+            Jr, Jz = p.actions(p.deltafn(0.5 * np.sum(V[i] ** 2) + Phi), R, z, V[i, 0], V[i, 1], V[i, 2], Phi)
+            if Jr > 1e30:
+                assert Fr[i] > 1e30
+                continue
+            s = Jr + Jz + abs(R * V[i, 1])
+            er.append(abs(Fr[i] - Jr) / s)
+            ez.append(abs(Fz[i] - Jz) / s)
+    er, ez = np.array(er), np.array(ez)
+    print("Jr: median %.2e max %.2e; Jz: median %.2e max %.2e" % (np.median(er), er.max(), np.median(ez), ez.max()))
+    assert np.median(er) < 1e-4 and np.median(ez) < 1e-5
+    assert er.max() < 5e-3 and ez.max() < 2e-3
