@@ -38,6 +38,11 @@ def f_qk21(npoly):
     return 21 * (11 * npoly + 25) + 190
 
 
+def log(msg):
+    """One progress line on stderr (the JSON line alone goes to stdout)."""
+    print("bench: " + msg, file=sys.stderr, flush=True)
+
+
 def line_tasks_bytes(stats):
     """Algorithmic bytes of k_integrals: per action integral the 12 node
     constants read and the action written (doubles)."""
@@ -45,39 +50,66 @@ def line_tasks_bytes(stats):
     return float(np.mean([s["tasks"] for s in stats])) * 13 * 8
 
 
-def cpu_baseline(npoly, ntheta, nr, ar, tables, delta, sample_every=8, threads=None):
-    import numpy as np
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(npoly, ntheta, nr, ar, tables, delta, threads=None):
+    """The oracle (the C restatement of the reference algorithm, built -O3 for
+    x86-64-v3, libfjoracle_o3.so) timed on this host's cores on a bounded
+    sample of the same iteration: rhofDF on every 8th grid point of the
+    40 x 20 grid, grid-parallel (OpenMP dynamic over points, the stronger CPU
+    baseline), and on every 32nd point reference-structured (serial over
+    points, one OpenMP team over the 22^3 nodes of each, as
+    integ/GaussQuad.cpp:160-180 and pot/potLeg.cpp:54-58 run it)."""
+    import ctypes as C
     from oracle import oracle as O
     threads = threads or min(16, os.cpu_count() or 1)
-    p = O.Pot(ar, npoly, ntheta)
-    p.set_tables(*tables)
-    p.set_delta(*delta)
+    lib = C.CDLL(os.path.join(ROOT, "oracle", "libfjoracle_o3.so"))
+    dp = C.POINTER(C.c_double)
+    lib.fjo_pot_new.restype = C.c_void_p
+    lib.fjo_pot_new.argtypes = [C.c_int, dp, C.c_int, C.c_int, C.c_int]
+    lib.fjo_pot_set_tables.argtypes = [C.c_void_p, dp, dp, dp]
+    lib.fjo_pot_set_delta.argtypes = [C.c_void_p, dp, dp]
+    lib.fjo_time_grid.restype = C.c_double
+    lib.fjo_time_grid.argtypes = [C.c_void_p, C.POINTER(O.DFParams), C.c_int, C.c_int, C.c_int, C.POINTER(C.c_int)]
+    lib.fjo_pot_free.argtypes = [C.c_void_p]
+    a = O.arr(ar)
+    h = lib.fjo_pot_new(nr, O.ptr(a), npoly, ntheta, len(delta[0]))
+    t = [O.arr(x) for x in tables]
+    lib.fjo_pot_set_tables(h, *[O.ptr(x) for x in t])
+    dl = [O.arr(x) for x in delta]
+    lib.fjo_pot_set_delta(h, *[O.ptr(x) for x in dl])
     df = O.df_params("Hernquist")
-    ci, _ = O.gauleg(ntheta)
-    si = np.sqrt(1 - ci * ci)
-    idx = list(range(0, nr * ntheta, sample_every))
-    # rhofDF at the sampled grid points, parallel over points
-    sub = O.Pot(ar, npoly, ntheta)   # same tables; points evaluated one by one in threads
-    sub.set_tables(*tables)
-    sub.set_delta(*delta)
-    import concurrent.futures as cf
-    pts = [(ar[i // ntheta] * si[i % ntheta], ar[i // ntheta] * ci[i % ntheta]) for i in idx]
-    t0 = time.perf_counter()
-    with cf.ThreadPoolExecutor(max_workers=threads) as ex:
-        list(ex.map(lambda rz: p.rhofDF(df, rz[0], rz[1]), pts))
-    dt = time.perf_counter() - t0
-    return {"value": len(pts) / dt, "unit": "grid-pt DF moments/s", "cores": threads, "kind": "port",
-            "sample": f"{len(pts)} of the {nr * ntheta} grid points (every {sample_every}th), one rhofDF each, "
-                      f"oracle C restatement, {threads} threads, {dt:.1f} s"}
+    out = {}
+    for variant, name, stride in ((0, "grid_parallel", 8), (1, "reference_structured", 32)):
+        n = C.c_int(0)
+        dt = lib.fjo_time_grid(h, C.byref(df), threads, variant, stride, C.byref(n))
+        out[name] = {"value": n.value / dt, "points": n.value, "seconds": dt}
+    lib.fjo_pot_free(h)
+    best = out["grid_parallel"]
+    return {"value": best["value"], "unit": "grid-pt DF moments/s", "cores": threads, "kind": "port",
+            "cpu": cpu_model(), "cpus_on_host": os.cpu_count(),
+            "sample": "rhofDF at %d of the %d grid points (every 8th) of iteration 0, grid-parallel OpenMP "
+                      "(dynamic over points), %d threads, %.1f s; oracle C restatement built -O3 x86-64-v3"
+                      % (best["points"], nr * ntheta, threads, best["seconds"]),
+            "variants": out}
 
 
 def mixed_leg(ctx, model, prev, last, steps, pts, npoly):
     """Mixed precision (FP32 action integrands, Kronrod rules, QAGS
     bookkeeping and DF; FP64 roots, Delta and Poisson) with the exact
     block-scan Poisson solve.  Accuracy: one iteration from the same
-    potential as the last FP64 iteration, every coefficient's difference
-    scaled by the monopole at its radius; speed: `steps` iterations
-    continuing from there.  Roofline: the counted QK21 flops of all rules of
+    potential as the last FP64 iteration, with the same (QUADPACK) Poisson
+    solve, every coefficient's difference scaled by the monopole at its
+    radius; speed: `steps` iterations continuing from there with the
+    block-scan Poisson solve.  Roofline: the counted QK21 flops of all rules of
     the iteration over the FP32 integrator's launch time, against the FP32
     vector peak; the counters do not split off the rules of the FP64
     overflow pass (0.9 % of the tasks), so this bounds the FP32 pass from
@@ -88,10 +120,10 @@ def mixed_leg(ctx, model, prev, last, steps, pts, npoly):
     ref = last[0]
     ctx.set_potential(*prev)
     model.total = prev
-    model.mode = fj.FJ_POISSON_BLOCKSCAN
     ctx.set_precision(fj.FJ_PREC_MIXED)
-    got = model.iterate()[0]
+    got = model.iterate()[0]            # same (QUADPACK) Poisson solve as the FP64 reference iteration
     err = {k: float(np.max(np.abs(got[k] - ref[k]) / np.abs(ref[k][:, :1]))) for k in ("rhl", "phil", "sigRl", "sigzl")}
+    model.mode = fj.FJ_POISSON_BLOCKSCAN
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     st = []
@@ -106,8 +138,9 @@ def mixed_leg(ctx, model, prev, last, steps, pts, npoly):
     t_int = float(np.mean([x["ms_integrals"] for x in st])) * 1e-3
     ach = qk * f_qk21(npoly) / t_int / 1e12
     return {"value": pts / dt, "ms_per_step": 1e3 * dt, "poisson": "blockscan",
-            "err_vs_fp64": err,
-            "roofline": {"bound": "mfma", "pipe": "VALU FP32", "kernel": "k_integrals_f32",
+            "err_vs_fp64": err, "err_note": "one iteration from the same potential as the last FP64 one, both with "
+                                            "the QUADPACK Poisson solve; |diff| / monopole at the radius",
+            "roofline": {"bound": "valu", "pipe": "VALU FP32", "kernel": "k_integrals_f32",
                          "achieved": ach, "peak": PEAK_FP32, "unit": "TFLOP/s", "frac": ach / PEAK_FP32,
                          "ms_per_launch": t_int * 1e3},
             "phases_ms": {k: float(np.mean([x[k] for x in st])) for k in
@@ -115,14 +148,53 @@ def mixed_leg(ctx, model, prev, last, steps, pts, npoly):
             "counters": {k: int(np.mean([x[k] for x in st])) for k in ("qk_calls", "overflow_tasks", "integral_lanes")}}
 
 
+def fast_leg(ctx, model, prev, last, steps, pts):
+    """Fast node phase (FJ_PREC_FAST: one fused kernel per iteration; the
+    Staeckel fudge with the reference's turning-point walks, FP64 roots and
+    64-point FP32 Gauss-Legendre action integrals instead of QAGS).
+    Accuracy as in the mixed leg (one iteration from the same potential as
+    the last FP64 one, QUADPACK Poisson on both sides); speed: `steps`
+    iterations continuing from there, QUADPACK Poisson.  Roofline: the
+    potential / integrand evaluations of k_fast (FP64 and FP32 mixed), at the
+    operation count of one evaluation, against the FP64 vector peak."""
+    import numpy as np
+    import torch
+    import fjmodels_amd as fj
+    ref = last[0]
+    ctx.set_potential(*prev)
+    model.total = prev
+    ctx.set_precision(fj.FJ_PREC_FAST)
+    got = model.iterate()[0]
+    err = {k: float(np.max(np.abs(got[k] - ref[k]) / np.abs(ref[k][:, :1])))
+           for k in ("rhl", "phil", "sigRl", "sigpl", "sigzl")}
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    st = []
+    for _ in range(steps):
+        model.iterate()
+        st.append(ctx.stats())
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / steps
+    ctx.set_precision(fj.FJ_PREC_FP64)
+    t_k = float(np.mean([x["ms_setup"] for x in st])) * 1e-3
+    evals = float(np.mean([x["phi_evals"] for x in st]))
+    return {"value": pts / dt, "ms_per_step": 1e3 * dt, "poisson": "quadpack",
+            "err_vs_fp64": err, "err_note": "one iteration from the same potential as the last FP64 one, both with "
+                                            "the QUADPACK Poisson solve; |diff| / monopole at the radius",
+            "k_fast": {"ms_per_launch": 1e3 * t_k, "potential_evaluations": evals,
+                       "evaluations_per_s": evals / t_k},
+            "phases_ms": {k: float(np.mean([x[k] for x in st])) for k in ("ms_points", "ms_setup", "ms_moments")}}
+
+
 def surrogate_leg(ctx, model, guess_tables, tol=1e-4, max_iter=30, keep=0.15, epochs=4, judge=0.05,
                   max_sur=8, sur_floor=4e-3, gate=0.05, collect_from=3, serve_from=5):
-    """The Hernquist model converged three times from the guess potential to
+    """The Hernquist model converged four times from the guess potential to
     the same tolerance: successive density coefficients differ by less than
     `tol` of the monopole (the driver's convergence test on rho_l).  All
-    three use the QUADPACK Poisson solve.
+    four use the QUADPACK Poisson solve.
 
     plain: FP64 Staeckel actions every iteration.
+    fast: the fast node phase (FJ_PREC_FAST) every iteration.
     mixed: the same with the mixed-precision node phase.
     surrogate (with the mixed node phase for its Staeckel iterations):
     Staeckel iterations, keeping from iteration `collect_from` on a random
@@ -149,8 +221,8 @@ def surrogate_leg(ctx, model, guess_tables, tol=1e-4, max_iter=30, keep=0.15, ep
     def drho(a, b):
         return float(np.max(np.abs(a["rhl"] - b["rhl"]) / np.abs(a["rhl"][:, :1])))
 
-    def run(sur, mixed):
-        ctx.set_precision(fj.FJ_PREC_MIXED if mixed else fj.FJ_PREC_FP64)
+    def run(sur, prec):
+        ctx.set_precision(prec)
         ctx.set_potential(*guess_tables)
         model.total = [x.copy() for x in guess_tables]
         model.philP = [tuple(x.copy() for x in guess_tables)]
@@ -212,19 +284,28 @@ def surrogate_leg(ctx, model, guess_tables, tol=1e-4, max_iter=30, keep=0.15, ep
         ctx.set_precision(fj.FJ_PREC_FP64)
         return dt, out, vir, st, hist, "".join(kinds), t_train, {k: float(np.median(v)) for k, v in t_kind.items()}
 
-    t_p, out_p, vir_p, _, h_p, k_p, _, i_p = run(False, False)
-    t_m, out_m, vir_m, _, h_m, k_m, _, i_m = run(False, True)
-    t_s, out_s, vir_s, st, h_s, k_s, t_tr, i_s = run(True, True)
-    return {"tol_rho_l": tol, "ms_plain": 1e3 * t_p, "ms_mixed": 1e3 * t_m, "ms_surrogate": 1e3 * t_s,
-            "speedup": t_p / t_s, "speedup_mixed_only": t_p / t_m,
-            "iterations_plain": len(k_p), "iterations_mixed": len(k_m), "iterations_surrogate_run": k_s,
+    t_p, out_p, vir_p, _, h_p, k_p, _, i_p = run(False, fj.FJ_PREC_FP64)
+    log("converged plain FP64: %d iterations, %.2f s" % (len(k_p), t_p))
+    t_f, out_f, vir_f, _, h_f, k_f, _, i_f = run(False, fj.FJ_PREC_FAST)
+    log("converged fast: %d iterations, %.2f s" % (len(k_f), t_f))
+    t_m, out_m, vir_m, _, h_m, k_m, _, i_m = run(False, fj.FJ_PREC_MIXED)
+    log("converged mixed: %d iterations, %.2f s" % (len(k_m), t_m))
+    t_s, out_s, vir_s, st, h_s, k_s, t_tr, i_s = run(True, fj.FJ_PREC_MIXED)
+    log("converged surrogate: %s, %.2f s" % (k_s, t_s))
+    return {"tol_rho_l": tol, "ms_plain": 1e3 * t_p, "ms_fast": 1e3 * t_f, "ms_mixed": 1e3 * t_m,
+            "ms_surrogate": 1e3 * t_s,
+            "speedup_fast": t_p / t_f, "speedup": t_p / t_s, "speedup_mixed_only": t_p / t_m,
+            "iterations_plain": len(k_p), "iterations_fast": len(k_f), "iterations_mixed": len(k_m),
+            "iterations_surrogate_run": k_s,
             "ms_training": 1e3 * t_tr,
-            "ms_median_iteration": {"plain": i_p["P"], "mixed": i_m["P"],
+            "ms_median_iteration": {"plain": i_p["P"], "fast": i_f["P"], "mixed": i_m["P"],
                                     "surrogate_run_staeckel": i_s["P"], "surrogate_run_served": i_s.get("S")},
+            "rho_l_diff_converged_fast": drho(out_f, out_p),
             "rho_l_diff_converged": drho(out_s, out_p), "rho_l_diff_converged_mixed": drho(out_m, out_p),
-            "virial_WK_plain": float(vir_p[4]), "virial_WK_mixed": float(vir_m[4]),
+            "virial_WK_plain": float(vir_p[4]), "virial_WK_fast": float(vir_f[4]), "virial_WK_mixed": float(vir_m[4]),
             "virial_WK_surrogate": float(vir_s[4]),
             "d_rho_l_plain": [None if x is None else float("%.3g" % x) for x in h_p],
+            "d_rho_l_fast": [None if x is None else float("%.3g" % x) for x in h_f],
             "d_rho_l_surrogate_run": [None if x is None else float("%.3g" % x) for x in h_s],
             "judge_last_served_iteration": {k: st[k] for k in ("judge_werr_r_med", "judge_werr_z_med",
                                                              "judge_err_r_med", "judge_err_r_p95", "judge_err_z_med",
@@ -335,6 +416,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision / block-scan leg")
+    ap.add_argument("--no-fast", action="store_true", help="skip the fast node phase leg")
     ap.add_argument("--no-surrogate", action="store_true", help="skip the plain vs surrogate convergence leg")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-models leg")
     ap.add_argument("--sweep", type=int, default=32,
@@ -389,7 +471,7 @@ def main():
         "config": {"workload": "single-component Hernquist DPL f(J), 40 log radii [0.01,100] x 20 GL cos(theta), "
                                "l<=8, 22^3 velocity nodes/point, guess potential + Delta table, SCF iterations",
                    "nr": nr, "ntheta": ntheta, "npoly": npoly, "nodes_per_point": 10648},
-        "roofline": {"bound": "mfma", "pipe": "VALU FP64 (MI355X FP64 vector and matrix peaks are equal)",
+        "roofline": {"bound": "valu", "pipe": "VALU FP64 (vector FP64 peak)",
                      "kernel": "k_integrals", "achieved": achieved, "peak": PEAK_FP64, "unit": "TFLOP/s",
                      "frac": achieved / PEAK_FP64, "traffic": K_INTEGRALS_TRAFFIC_BYTES,
                      "traffic_note": "HBM bytes per launch from rocprofv3 FETCH_SIZE/WRITE_SIZE (calibrated); "
@@ -402,16 +484,25 @@ def main():
                      ("bound_nodes", "tasks", "qk_calls", "overflow_tasks", "brent_iters", "integral_lanes",
                       "unbound_nodes", "bracket_failures", "nonfinite_nodes")},
     }
+    log("FP64 leg: %.2f ms per iteration" % ms)
+    if not args.no_fast:
+        line["fast"] = fast_leg(ctx, model, prev, last, args.steps, pts)
+        log("fast leg: %.2f ms per iteration" % line["fast"]["ms_per_step"])
     if not args.no_mixed:
         line["mixed"] = mixed_leg(ctx, model, prev, last, args.steps, pts, npoly)
+        log("mixed leg: %.2f ms per iteration" % line["mixed"]["ms_per_step"])
     if not args.no_surrogate:
         line["surrogate"] = surrogate_leg(ctx, model, guess_tables)
+        log("convergence leg done")
     if not args.no_batch:
         line["batch"] = batch_leg(ctx)
+        log("batch leg done")
     if args.sweep > 0:
         line["sweep"] = sweep_leg(ctx, args.sweep)
+        log("sweep leg done")
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(npoly, ntheta, nr, ar, guess_tables, delta)
+        log("CPU baseline done")
     if args.copy_calibration:
         a = torch.ones(1 << 27, dtype=torch.float64, device="cuda")   # 1 GiB
         b = torch.empty_like(a)

@@ -4,8 +4,8 @@
 // is fused: one radial bracket search and one pass over the Legendre
 // coefficients give Phi, dPhi/dR and dPhi/dz together, with the derivative
 // recurrences run alongside instead of re-running legend() per order.
-// Root finding (include/Utils.h:161-194) and the adaptive quadratures
-// (include/Utils.h:200-212, pot/Potential.cpp:161-206) are written as
+// Root finding (include/Utils.h:134-167) and the adaptive quadratures
+// (include/Utils.h:173-185, pot/Potential.cpp:161-206) are written as
 // steppers so that a kernel can drive many of them in lock step.
 #pragma once
 #if defined(__HIPCC__)
@@ -26,7 +26,7 @@
 
 namespace fj {
 
-// include/Utils.h:49-60
+// include/Utils.h:22-33
 constexpr double EPSABSROOTS = 1e-8;
 constexpr double EPSRELROOTS = 1e-6;
 constexpr double EPSABSINTEG = 1e-5;
@@ -306,7 +306,7 @@ FJ_HD __forceinline__ double pot_phi_grad(const PotView &T, double R, double z, 
 
 // ---------------------------------------------------------------------------
 // Brent root finder in GSL iterate form (Brent 1973) driven as
-// wrp_GSLroots (Utils.h:161-194), with F(low), F(high) supplied.
+// wrp_GSLroots (Utils.h:134-167), with F(low), F(high) supplied.
 // ---------------------------------------------------------------------------
 struct Brent {
     double a, b, c, d, e, fa, fb, fc;

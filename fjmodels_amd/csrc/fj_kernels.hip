@@ -761,7 +761,7 @@ struct IntegArgs {
 };
 
 
-// k_integrals: persistent QAGS over the Ju and Jv task lists (Utils.h:200-212,
+// k_integrals: persistent QAGS over the Ju and Jv task lists (Utils.h:173-185,
 // uvOrb.cpp:226-262).  Each loop step every busy lane applies one QK21 rule to
 // its own current interval; lanes that finish fetch the next task.
 #ifndef FJ_INTEG_WAVES
@@ -1165,7 +1165,7 @@ __global__ void k_project(int nr, int ng, int np_, const double *ar, const doubl
     rhL[idx] = a; vrotL[idx] = b; sigRL[idx] = c; sigpL[idx] = d; sigzL[idx] = e; sigRzL[idx] = f;
 }
 
-// Guess densities (pot/models.cpp:85-110)
+// Guess densities (pot/models.cpp:40-68)
 __device__ inline double d_rho_guess(int model, double mass, double r0, double q, double q2, double R, double z) {
     const double m = sqrt(R * R + z * z / q2);
     switch (model) {
@@ -2765,7 +2765,7 @@ int run_poisson(fj_ctx *c, int mode, const double *d_rhl, double *d_phil, double
     c->st.ms_poisson = elapsed(a, b);
     (void)hipEventDestroy(a);
     (void)hipEventDestroy(b);
-    if (status) return set_err(c, FJ_E_NUMERIC, "NaN in the external Poisson integral (Potential.cpp:195)");
+    if (status) return set_err(c, FJ_E_NUMERIC, "NaN in the external Poisson integral (Potential.cpp:144-146)");
     return 0;
 }
 

@@ -1,6 +1,6 @@
 // fj_mixed.hpp -- FP32 pieces of the mixed-precision node phase.
 //
-// In the mixed mode the action integrands (uvOrb.cpp:226-262: Juint, Jvint
+// In the mixed mode the action integrands (Juint, Jvint: uvOrb.cpp:53-56, 80-83,
 // with the potential of Potential.cpp:206-219), their Kronrod rules and the
 // QAGS bookkeeping (QUADPACK's single-precision qk21 and qagse: the float
 // instance of the stepper in fj_device.hpp) and the DF (DF.cpp:65-119) run in

@@ -103,7 +103,7 @@ struct JInt {
         double F;
         const double D = k[0], E = k[1];
         if (type == 0) {
-            // Juint (uvOrb.cpp:226-231): u = ubar - Deltau cos(psi)
+            // Juint (uvOrb.cpp:80-83, integrated by GetJu 226-245): u = ubar - Deltau cos(psi)
             const double u = k[2] - k[3] * cp;
             double shu, chu;
             fjm_sinhcosh(u, &shu, &chu);
@@ -112,7 +112,7 @@ struct JInt {
             const double dU = (shu2 + k[6]) * Phi - k[9];
             F = E * (shu2 - k[7]) + k[8] - dU - k[10] * (1 / shu2 - k[11]);
         } else {
-            // Jvint (uvOrb.cpp:245-250): v = pi/2 - (pi/2 - vturn) cos(psi)
+            // Jvint (uvOrb.cpp:53-56, integrated by GetJv 247-262): v = pi/2 - (pi/2 - vturn) cos(psi)
             const double v = PIH - (PIH - k[2]) * cp;
             double s, cs;
             fjm_sincos(v, &s, &cs);

@@ -102,7 +102,7 @@ void free_mat(double **m) {
 }
 
 // ---------------------------------------------------------------------------
-// readParam (io/readParam.cpp:30-104): "key value" lines; lines starting
+// readParam (io/readParam.cpp:31-111): "key value" lines; lines starting
 // with '#', ' ', '\t' or empty are skipped; unknown keys are ignored.
 // ---------------------------------------------------------------------------
 fJParams readParam(const char *fName) {

@@ -206,10 +206,10 @@ double potentialDistance(double **philOld, double **philNew);
 // ---------------------------------------------------------------------------
 // SigmaDF (Integ.cpp:168-172) from the global rhl.
 double SigmaDF(double R, double z);
-// line_profile (Integ.cpp:188-197); the second form takes n velocities.
+// line_profile (Integ.cpp:177-190); the second form takes n velocities.
 double line_profile(double R, double z, double vx, Potential *p);
 void line_profile(double R, double z, int n, const double *vx, double *out);
-// P_vp, P_vz, P_vr (Integ.cpp:202-236).
+// P_vp, P_vz, P_vr (Integ.cpp:191-220).
 double P_vp(double R, double z, double vphi, Potential *p);
 double P_vz(double R, double z, double vz, Potential *p);
 double P_vr(double R, double z, double vr, Potential *p);

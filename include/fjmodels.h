@@ -27,7 +27,7 @@ enum {
     FJ_E_HIP = 2,        /* HIP runtime error                             */
     FJ_E_STATE = 3,      /* call made before the data it needs was set    */
     FJ_E_NUMERIC = 4,    /* NaN in the Poisson external integral
-                            (pot/Potential.cpp:195-197 exits with 1)      */
+                            (pot/Potential.cpp:144-146 exits with 1)      */
     FJ_E_DELTA = 5       /* Delta tabulation hit a reference exit/failure
                             path (delta/Delta.cpp:121,145-151)            */
 };
@@ -36,7 +36,7 @@ enum {
 enum { FJ_QUAD_LOG22 = 0,        /* xpt11/xw11 on all axes: Int3D_111111_vec */
        FJ_QUAD_TANHSINH22 = 1 }; /* xpt01 x xpt11 x xpt01: Int3D_011101_vec  */
 
-/* Guess densities (pot/models.cpp:85-110, Potential::selectGuessRho). */
+/* Guess densities (pot/models.cpp:40-68, Potential::selectGuessRho). */
 enum { FJ_GUESS_HERNQUIST = 0, FJ_GUESS_ISOCHRONE = 1, FJ_GUESS_NFW = 2, FJ_GUESS_NFWEXT = 3 };
 
 /* Poisson radial integrals: QUADPACK QAGS/QAGIU as the reference
@@ -202,9 +202,9 @@ int fj_df_points(fj_ctx *ctx, int n, const double *R, const double *z, const dou
  *   fj_sigma_df   SigmaDF (Integ.cpp:168-172, SigmaInteg 33-43): 2 Int1D_01
  *                 of ev_dens (UtilsLeg.h:323-334, from rhl [nr][npoly]) along
  *                 r from R to ar[nr-1], at n points (R, z);
- *   fj_line_profile  line_profile (Integ.cpp:188-197, lineProfile 64-92) at n
+ *   fj_line_profile  line_profile (Integ.cpp:177-190, lineProfile 65-95) at n
  *                 line-of-sight velocities vx: Int3D_111111 / SigmaDF;
- *   fj_velocity_distribution  P_vp, P_vz, P_vr (Integ.cpp:202-236, Pvphi /
+ *   fj_velocity_distribution  P_vp, P_vz, P_vr (Integ.cpp:191-220, Pvphi /
  *                 Pvz / Pvr 97-138; which = 0, 1, 2) at n values v of the
  *                 fixed component. */
 int fj_sigma_df(fj_ctx *ctx, const double *rhl, int n, const double *R, const double *z, double *sigma);

@@ -26,7 +26,7 @@
 #include <omp.h>
 #endif
 
-/* include/Utils.h:49-60 */
+/* include/Utils.h:22-33 */
 #define EPSABSROOTS 1e-8
 #define EPSRELROOTS 1e-6
 #define EPSABSODE 1e-10
@@ -339,7 +339,7 @@ double fjo_dz(const fjo_pot *p, double R, double z) {
 
 /* ------------------------------------------------------------------------ */
 /* Brent root finder: GSL roots/brent.c iterate form of Brent (1973),         */
-/* driven by the reference wrapper include/Utils.h:161-194.                   */
+/* driven by the reference wrapper include/Utils.h:134-167.                   */
 /* ------------------------------------------------------------------------ */
 typedef struct { double a, b, c, d, e, fa, fb, fc; } brent_state;
 
@@ -520,8 +520,16 @@ static void qk_rule(int n, const double *xgk, const double *wg, const double *wg
     *abserr = rescale_error(err, rabs, rasc);
 }
 
+/* Interval recorder of fjo_orbit_state: the [a, b] of every QK21 rule applied. */
+static _Thread_local double *g_rec;
+static _Thread_local int g_rec_n, g_rec_cap;
+
 void fjo_qk21(fjo_fn f, void *prm, double a, double b, double *r, double *e, double *ra, double *rs) {
     g_cnt.qk21_calls++;
+    if (g_rec) {
+        if (g_rec_n < g_rec_cap) { g_rec[2 * g_rec_n] = a; g_rec[2 * g_rec_n + 1] = b; }
+        g_rec_n++;
+    }
     qk_rule(11, xgk21, wg10, wgk21, f, prm, a, b, r, e, ra, rs);
 }
 void fjo_qk15(fjo_fn f, void *prm, double a, double b, double *r, double *e, double *ra, double *rs) {
@@ -968,7 +976,7 @@ int fjo_evolve_apply(fjo_ode_fn fn, void *prm, int dim, double eps_abs, double e
 }
 
 /* ------------------------------------------------------------------------ */
-/* Guess densities: pot/models.cpp:85-110                                    */
+/* Guess densities: pot/models.cpp:40-68                                    */
 /* ------------------------------------------------------------------------ */
 typedef struct { double mass, r0, q, q2; } guess_par;
 static double rho_guess(int model, const guess_par *g, double R, double z) {
@@ -1453,8 +1461,13 @@ static double GetTurnv(uvorb_t *o, double vtry) {   /* uvOrb.cpp:203-224 */
 }
 
 /* uvOrb.cpp:15-36 + Ju 226-245 + Jv 247-262 */
-int fjo_actions(const fjo_pot *p, double Delta, double R, double z, double Phi0,
-                double vR, double vphi, double vz, double *Jr, double *Jz) {
+/* state != NULL: also returns the orbit state the integrands read (15 doubles:
+   Delta E Er ch1sq sh1sq Phiu1 Lzsq u v sv2 shu2 Iv ubar Deltau vturn) and in
+   qk[0], qk[1] the first QK21 rule of the Ju and Jv integrals (NAN where that
+   action is zero without an integral). */
+static int actions_impl(const fjo_pot *p, double Delta, double R, double z, double Phi0,
+                        double vR, double vphi, double vz, double *Jr, double *Jz,
+                        double *state, double *qk, double *iv, int cap, int *nrule) {
     uvorb_t o;
     memset(&o, 0, sizeof o);
     o.pot = p;
@@ -1483,6 +1496,7 @@ int fjo_actions(const fjo_pot *p, double Delta, double R, double z, double Phi0,
            .5 * o.Lzsq / (Delta2 * o.ch1sq) * (1 / shu2 - 1 / o.sh1sq) + dU(&o, u) / o.ch1sq;
     o.Iv = .5 * (pv * pv) / Delta2 - o.E * sv2 + .5 * o.Lzsq / (Delta2 * sv2) - dV(&o, v);
     const double rtwo = sqrt(2);
+    if (iv) { g_rec = iv; g_rec_n = 0; g_rec_cap = cap; nrule[0] = nrule[1] = 0; }
     /* GetJu */
     GetTurnu(&o, 1);
     if (fabs(o.uouter - o.uinner) < TINY) *Jr = 0;
@@ -1493,6 +1507,7 @@ int fjo_actions(const fjo_pot *p, double Delta, double R, double z, double Phi0,
         fjo_qags(Juint, &o, 0, PI, EPSABSINTEG, EPSRELINTEG, WRKSP, &res, &err, NULL, NULL);
         *Jr = rtwo * Delta * o.Deltau * res / PI;
     }
+    if (iv) nrule[0] = g_rec_n;
     /* GetJv */
     o.vturn = GetTurnv(&o, .5 * PIH);
     if (fabs(o.vturn - PIH) < TINY) *Jz = 0;
@@ -1501,7 +1516,29 @@ int fjo_actions(const fjo_pot *p, double Delta, double R, double z, double Phi0,
         fjo_qags(Jvint, &o, 0, PIH, EPSABSINTEG, EPSRELINTEG, WRKSP, &res, &err, NULL, NULL);
         *Jz = 2 * rtwo * Delta * (PIH - o.vturn) * res / PI;
     }
+    if (iv) { nrule[1] = g_rec_n - nrule[0]; g_rec = NULL; }
+    if (state) {
+        const double st[15] = {o.Delta, o.E, o.Er, o.ch1sq, o.sh1sq, o.Phiu1, o.Lzsq, o.u, o.v,
+                               o.sv2, o.shu2, o.Iv, o.ubar, o.Deltau, o.vturn};
+        memcpy(state, st, sizeof st);
+        double e, ra, rs;
+        qk[0] = qk[1] = NAN;
+        if (!(fabs(o.uouter - o.uinner) < TINY)) qk_rule(11, xgk21, wg10, wgk21, Juint, &o, 0, PI, &qk[0], &e, &ra, &rs);
+        if (!(fabs(o.vturn - PIH) < TINY)) qk_rule(11, xgk21, wg10, wgk21, Jvint, &o, 0, PIH, &qk[1], &e, &ra, &rs);
+    }
     return 0;
+}
+
+int fjo_actions(const fjo_pot *p, double Delta, double R, double z, double Phi0,
+                double vR, double vphi, double vz, double *Jr, double *Jz) {
+    return actions_impl(p, Delta, R, z, Phi0, vR, vphi, vz, Jr, Jz, NULL, NULL, NULL, 0, NULL);
+}
+
+int fjo_orbit_state(const fjo_pot *p, double Delta, double R, double z, double Phi0,
+                    double vR, double vphi, double vz, double *state, double *qk,
+                    double *iv, int cap, int *nrule) {
+    double Jr, Jz;
+    return actions_impl(p, Delta, R, z, Phi0, vR, vphi, vz, &Jr, &Jz, state, qk, iv, cap, nrule);
 }
 
 /* df/DF.cpp:65-76 */
@@ -1567,6 +1604,74 @@ double fjo_rhofDF(const fjo_pot *p, const fjo_df *d, double R, double z, double 
     *sigR = sqrt(out[2] / out[0]); *sigp = sqrt(out[3] / out[0]); *sigz = sqrt(out[4] / out[0]);
     *sigRz = sqrt(fabs(out[5]) / out[0]);
     return p->quad ? 4 * out[0] : out[0];
+}
+
+/* rhofDF with the reference's own parallel structure (GaussQuad.cpp:160-180):
+ * one OpenMP team over the 22^3 nodes of the point, collapse(3), static
+ * schedule, reduction(+) on the six sums (their order left to OpenMP).
+ * For the CPU timing baseline only; the sums are not fixed-order. */
+static double rhofDF_teams(const fjo_pot *p, const fjo_df *d, double R, double z, int nthreads) {
+    double Phi_h = fjo_phi(p, R, z);
+    double far = 10 * p->ar[p->nr - 1];
+    double Ve = sqrt(-2 * (Phi_h - fjo_phi(p, far, far)));
+    double x[3] = {R, z, Phi_h};
+    const double *xa = p->quad ? xpt01 : xpt11, *wa = p->quad ? xw01 : xw11;
+    double s0 = 0, s1 = 0, s2 = 0, s3 = 0, s4 = 0, s5 = 0;
+#ifdef _OPENMP
+#pragma omp parallel for collapse(3) schedule(static) num_threads(nthreads) reduction(+ : s0, s1, s2, s3, s4, s5)
+#endif
+    for (int i = 0; i < 22; i++)
+        for (int j = 0; j < 22; j++)
+            for (int k = 0; k < 22; k++) {
+                double V[3] = {1e-6 + Ve * xa[i], 1e-6 + Ve * xpt11[j], 1e-6 + Ve * xa[k]};
+                double jd = Ve * Ve * Ve * fjo_df_eval(p, d, x, V);
+                double w = wa[i] * xw11[j] * wa[k];
+                s0 += w * jd; s1 += w * jd * V[1]; s2 += w * jd * V[0] * V[0];
+                s3 += w * jd * V[1] * V[1]; s4 += w * jd * V[2] * V[2]; s5 += w * jd * V[0] * V[2];
+            }
+    (void)nthreads;
+    return s0 + s1 + s2 + s3 + s4 + s5;
+}
+
+/* CPU timing baseline of the grid loop of computeRhl (potLeg.cpp:54-58) on
+ * every stride-th grid point: variant 0 grid-parallel (one point per thread,
+ * dynamic schedule), variant 1 reference-structured (serial loop over the
+ * points, one OpenMP team over the nodes of each).  Returns the wall time in
+ * seconds; *npts the number of points. */
+double fjo_time_grid(const fjo_pot *p, const fjo_df *d, int nthreads, int variant, int stride, int *npts) {
+    const int NR = p->nr, NG = p->ngauss;
+    double *ci = (double *)malloc(NG * sizeof(double)), *wi = (double *)malloc(NG * sizeof(double));
+    fjo_gauleg(NG, 0, 1, ci, wi);
+    const int n = (NR * NG + stride - 1) / stride;
+    double *res = (double *)malloc(n * sizeof(double));
+#ifdef _OPENMP
+    if (nthreads <= 0) nthreads = omp_get_max_threads();
+    const double t0 = omp_get_wtime();
+#endif
+    if (variant == 0) {
+#ifdef _OPENMP
+#pragma omp parallel for schedule(dynamic, 1) num_threads(nthreads)
+#endif
+        for (int k = 0; k < n; k++) {
+            const int idx = k * stride, nr = idx / NG, ng = idx % NG;
+            double out[6], a, b, c, e, f;
+            const double sg = sqrt(1 - ci[ng] * ci[ng]);
+            res[k] = fjo_rhofDF(p, d, p->ar[nr] * sg, p->ar[nr] * ci[ng], out, &a, &b, &c, &e, &f);
+        }
+    } else {
+        for (int k = 0; k < n; k++) {
+            const int idx = k * stride, nr = idx / NG, ng = idx % NG;
+            const double sg = sqrt(1 - ci[ng] * ci[ng]);
+            res[k] = rhofDF_teams(p, d, p->ar[nr] * sg, p->ar[nr] * ci[ng], nthreads);
+        }
+    }
+    double dt = 0;
+#ifdef _OPENMP
+    dt = omp_get_wtime() - t0;
+#endif
+    *npts = n;
+    free(ci); free(wi); free(res);
+    return dt;
 }
 
 /* pot/potLeg.cpp:35-58: the grid loop, without the rho[0][0] fix. */
@@ -1733,7 +1838,7 @@ static void phi_ve(const fjo_pot *p, double R, double z, double *Phi, double *Ve
     *Ve = sqrt(-2 * (*Phi - fjo_phi(p, far, far)));
 }
 
-/* line_profile (Integ.cpp:188-197): Int3D_111111 (GaussQuad.cpp:124-134) of
+/* line_profile (Integ.cpp:177-190): Int3D_111111 (GaussQuad.cpp:124-134) of
  * lineProfile (Integ.cpp:64-92) in serial (i, j, k) order, over SigmaDF. */
 double fjo_line_profile(const fjo_pot *p, const fjo_df *d, const double *rhl, double R, double z, double vx) {
     double Phi, Ve;
@@ -1750,7 +1855,7 @@ double fjo_line_profile(const fjo_pot *p, const fjo_df *d, const double *rhl, do
     return sum / Sigma;
 }
 
-/* P_vp / P_vz / P_vr (Integ.cpp:202-236): Int2D_0101 / Int2D_0111
+/* P_vp / P_vz / P_vr (Integ.cpp:191-220): Int2D_0101 / Int2D_0111
  * (GaussQuad.cpp:52-71) of Pvphi / Pvz / Pvr (Integ.cpp:97-138), serial
  * (i, j) order; which = 0, 1, 2. */
 double fjo_vdist(const fjo_pot *p, const fjo_df *d, int which, double R, double z, double v) {

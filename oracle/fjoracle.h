@@ -87,6 +87,14 @@ double fjo_best_delta(const fjo_pot *p, double E, double Lz, double R0, int *sta
 /* Stackel-fudge actions at one phase-space point; returns 0 if bound. */
 int fjo_actions(const fjo_pot *p, double Delta, double R, double z, double Phi0,
                 double vR, double vphi, double vz, double *Jr, double *Jz);
+/* fjo_actions plus the orbit state of the integrands (15 doubles: Delta E Er
+   ch1sq sh1sq Phiu1 Lzsq u v sv2 shu2 Iv ubar Deltau vturn) and the first QK21
+   rule of the Ju and Jv integrals in qk[0..1] (NAN where no integral is taken),
+   and the [a, b] of every QK21 rule QAGS applied, Ju's first (nrule[0] of them)
+   then Jv's (nrule[1]), in iv[2 * cap]; returns 1 for an unbound node. */
+int fjo_orbit_state(const fjo_pot *p, double Delta, double R, double z, double Phi0,
+                    double vR, double vphi, double vz, double *state, double *qk,
+                    double *iv, int cap, int *nrule);
 double fjo_df_eval(const fjo_pot *p, const fjo_df *df, const double *x, const double *v);
 double fjo_df_hg(const fjo_df *df, double Jr, double Jphi, double Jz);
 /* Observables (integ/Integ.cpp:33-43, 63-138, 165-236): projected density
@@ -107,6 +115,10 @@ int fjo_compute_new_phi(fjo_pot *p, const fjo_df *df, int nthreads,
                         double *rhl, double *vrotl, double *sigRl, double *sigpl,
                         double *sigzl, double *sigRzl,
                         double *philP, double *PrP, double *Pr2P);
+/* CPU timing baseline: rhofDF on every stride-th grid point, variant 0
+ * grid-parallel (dynamic over points), 1 reference-structured (serial over
+ * points, an OpenMP team over each point's nodes); returns wall seconds. */
+double fjo_time_grid(const fjo_pot *p, const fjo_df *df, int nthreads, int variant, int stride, int *npts);
 /* Virial diagnostic (libs/Stats.cpp:12-41); out = {mass_r2Pr, mass_sum, KE, PE,
  * W/K, KRR, WRR, WRR/KRR, Kzz, Wzz, Wzz/Kzz}. */
 void fjo_vir2(const fjo_pot *p, const double *rhl, const double *vrotl, const double *sigRl,

@@ -1,6 +1,6 @@
 """C++ facade (fjmodels_amd/host): parameter files, output names and the
 models/*.out format, on the CPU.  The reference's behaviour these follow:
-io/readParam.cpp:30-150 (keys, skipped lines, component count),
+io/readParam.cpp:31-144 (keys, skipped lines, component count),
 io/writeOut.cpp:22-104 (composeName with 2-decimal fields and the 5-char model
 prefix; header lines, ar, nine [NR][NPOLY] matrices with a trailing space,
 ostream's default 6 significant digits)."""

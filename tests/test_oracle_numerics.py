@@ -9,7 +9,7 @@ term, a wrong sign or index) fails it:
   * QAGS vs scipy.integrate.quad, which wraps QUADPACK dqagse: same result,
     same number of subintervals and evaluations;
   * QAGIU vs scipy's dqagie (different transform arithmetic, same algorithm);
-  * Brent (GSL iterate form, Utils.h:161-194) vs scipy.optimize.brentq and the
+  * Brent (GSL iterate form, Utils.h:134-167) vs scipy.optimize.brentq and the
     wrapper's |F| < 5e-4 early return;
   * RK8PD coefficients: row sums, quadrature order conditions, and 8th-order
     convergence on a linear oscillator.

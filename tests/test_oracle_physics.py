@@ -3,7 +3,7 @@
   * potential evaluation (Potential.cpp:206-276): tables linear in r make the
     radial interpolation exact, so Phi, dPhi/dR, dPhi/dz must equal the
     analytic sum of (a_k + b_k r) P_2k(cos theta) and its derivatives;
-  * guess densities (models.cpp:85-110): textbook Hernquist, isochrone, NFW;
+  * guess densities (models.cpp:40-68): textbook Hernquist, isochrone, NFW;
   * Poisson (Potential.cpp:110-200): QAGS vs the exact integral of the
     piecewise-linear rho_l, and the Hernquist monopole -M/(r+a) on a fine grid;
   * Staeckel-fudge actions (uvOrb.cpp): spherical isochrone with a tiny focal
