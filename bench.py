@@ -9,10 +9,14 @@ Delta table from the guess potential.  A step is one SCF iteration
 new potential.  Inputs are device resident; the timed loop runs K iterations.
 
 Prints one JSON line.  The roofline entry is for the dominant kernel
-(k_integrals, the persistent QAGS over the action integrals); its algorithmic
-flops per launch = (number of QK21 rules applied) x F_QK21(npoly), with
-F_QK21 = 21 * (11 npoly + 25) + 190 (operation count of one integrand
-evaluation and of the rule's sums; transcendentals count 1).
+(k_integrals, the persistent QAGS over the action integrals): its flops per
+launch are the QK21 rules the launch applied (counted by the kernel) times
+the operations of one rule, counted by an op-counting build of the reference
+integrands and rule (oracle/opcount.cpp, run over every bound node of
+iteration 0 of this workload by oracle/count_ops.py: 2,755 per rule in the
+reference's form, transcendentals counting 1); its traffic is the HBM bytes
+per launch of the committed rocprofv3 FETCH_SIZE / WRITE_SIZE runs.  Both
+figures are read from the k_integrals profile summary under profiles/.
 The CPU baseline times the oracle (the C restatement of the reference
 algorithm) with OpenMP on a bounded sample of the same grid points.
 """
@@ -27,20 +31,25 @@ sys.path.insert(0, ROOT)
 
 PEAK_FP64 = 78.6   # TFLOP/s, MI355X
 PEAK_FP32 = 157.3  # TFLOP/s, MI355X vector FP32
-# HBM bytes per k_integrals launch of this workload (timed launches of the default
-# schedule): rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE runs, scaled by what the
-# counters read for a 1 GiB device copy (fetch x 2.0, write x 1.0); the runs and
-# their commands are in profiles/k_integrals_hernquist_40x20_fp64.md.
-K_INTEGRALS_TRAFFIC_BYTES = 185.1e9
+K_INTEGRALS_PROFILE = os.path.join(ROOT, "profiles", "k_integrals_hernquist_40x20_fp64.md")
 
 
-def f_qk21(npoly):
-    return 21 * (11 * npoly + 25) + 190
-
-
-def log(msg):
-    """One progress line on stderr (the JSON line alone goes to stdout)."""
-    print("bench: " + msg, file=sys.stderr, flush=True)
+def profile_figures(path=K_INTEGRALS_PROFILE):
+    """The figures a profile summary states as `key: value` lines: the
+    counted operations per QK21 rule (oracle/count_ops.py, op-counting build
+    of the action integrals on this workload) and the HBM bytes per launch
+    (rocprofv3 FETCH_SIZE / WRITE_SIZE runs, calibrated on a device copy)."""
+    out = {}
+    try:
+        for line in open(path):
+            k, sep, v = line.partition(":")
+            if sep and k.strip() in ("ops_per_rule_reference_form", "ops_per_rule_hoisted",
+                                     "traffic_bytes_per_launch", "traffic_commit"):
+                v = v.strip()
+                out[k.strip()] = v if k.strip() == "traffic_commit" else float(v)
+    except OSError:
+        pass
+    return out
 
 
 def line_tasks_bytes(stats):
@@ -70,7 +79,7 @@ def cpu_baseline(npoly, ntheta, nr, ar, tables, delta, threads=None):
     integ/GaussQuad.cpp:160-180 and pot/potLeg.cpp:54-58 run it)."""
     import ctypes as C
     from oracle import oracle as O
-    threads = threads or min(16, os.cpu_count() or 1)
+    threads = threads or len(os.sched_getaffinity(0))   # every core this process may run on
     lib = C.CDLL(os.path.join(ROOT, "oracle", "libfjoracle_o3.so"))
     dp = C.POINTER(C.c_double)
     lib.fjo_pot_new.restype = C.c_void_p
@@ -136,7 +145,7 @@ def mixed_leg(ctx, model, prev, last, steps, pts, npoly):
     model.mode = fj.FJ_POISSON_QUADPACK
     qk = float(np.mean([x["qk_calls"] for x in st]))
     t_int = float(np.mean([x["ms_integrals"] for x in st])) * 1e-3
-    ach = qk * f_qk21(npoly) / t_int / 1e12
+    ach = qk * profile_figures().get("ops_per_rule_reference_form", float("nan")) / t_int / 1e12
     return {"value": pts / dt, "ms_per_step": 1e3 * dt, "poisson": "blockscan",
             "err_vs_fp64": err, "err_note": "one iteration from the same potential as the last FP64 one, both with "
                                             "the QUADPACK Poisson solve; |diff| / monopole at the radius",
@@ -460,8 +469,10 @@ def main():
 
     qk = np.mean([s["qk_calls"] for s in stats])
     t_int = np.mean([s["ms_integrals"] for s in stats]) * 1e-3
-    flops = qk * f_qk21(npoly)
-    achieved = flops / t_int / 1e12
+    fig = profile_figures()
+    ops_rule = fig.get("ops_per_rule_reference_form")
+    flops = qk * ops_rule if ops_rule else None
+    achieved = flops / t_int / 1e12 if flops else None
     line = {
         "metric": "grid-pt DF moments/s per SCF iter, plain FP64",
         "value": value, "unit": "grid-pt DF moments/s", "n_gpus": 1,
@@ -473,10 +484,18 @@ def main():
                    "nr": nr, "ntheta": ntheta, "npoly": npoly, "nodes_per_point": 10648},
         "roofline": {"bound": "valu", "pipe": "VALU FP64 (vector FP64 peak)",
                      "kernel": "k_integrals", "achieved": achieved, "peak": PEAK_FP64, "unit": "TFLOP/s",
-                     "frac": achieved / PEAK_FP64, "traffic": K_INTEGRALS_TRAFFIC_BYTES,
-                     "traffic_note": "HBM bytes per launch from rocprofv3 FETCH_SIZE/WRITE_SIZE (calibrated); "
-                                     "algorithmic bytes (node records) %.3g" % (line_tasks_bytes(stats)),
-                     "flops_per_launch": flops, "ms_per_launch": t_int * 1e3},
+                     "frac": achieved / PEAK_FP64 if achieved else None,
+                     "traffic": fig.get("traffic_bytes_per_launch"),
+                     "traffic_note": "HBM bytes per launch from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE runs "
+                                     "(calibrated on a device copy; profiles/%s, build %s); algorithmic bytes "
+                                     "(node records) %.3g" % (os.path.basename(K_INTEGRALS_PROFILE),
+                                                              fig.get("traffic_commit", "?"),
+                                                              line_tasks_bytes(stats)),
+                     "flops_per_launch": flops, "ms_per_launch": t_int * 1e3,
+                     "ops_per_rule": ops_rule,
+                     "ops_note": "operations per QK21 rule from the op-counting build of the reference integrands "
+                                 "(oracle/opcount.cpp over iteration 0 of this workload); with the per-orbit "
+                                 "constants hoisted: %s" % fig.get("ops_per_rule_hoisted")},
         "phases_ms": {k: float(np.mean([s[k] for s in stats])) for k in
                       ("ms_points", "ms_setup", "ms_integrals", "ms_overflow", "ms_moments", "ms_project",
                        "ms_poisson")},

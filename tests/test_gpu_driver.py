@@ -75,12 +75,15 @@ def run_driver(pf, outdir, *extra):
     return r.stdout
 
 
-def test_driver_resume_reproduces_uninterrupted_run(tmp_path):
+@pytest.mark.parametrize("refresh", [0, 1])
+def test_driver_resume_reproduces_uninterrupted_run(tmp_path, refresh):
     """--resume from the .f64 sidecar of iteration 0, numbered from 1, gives
     the bits of iteration 1 of the uninterrupted run (guess potential and
-    Delta table rebuilt as at the start, potential tables from the file)."""
+    Delta table rebuilt as at the start, potential tables from the file;
+    with delta_refresh 1 the Delta table is tabulated again from the loaded
+    potential, as the uninterrupted run does after writing it)."""
     pf = tmp_path / "param.txt"
-    pf.write_text(PARAM)
+    pf.write_text(PARAM + ("delta_refresh 1\n" if refresh else ""))
     a, b = tmp_path / "a", tmp_path / "b"
     a.mkdir()
     b.mkdir()
