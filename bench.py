@@ -52,6 +52,11 @@ def profile_figures(path=K_INTEGRALS_PROFILE):
     return out
 
 
+def log(msg):
+    """One progress line on stderr (the JSON line alone goes to stdout)."""
+    print("bench: " + msg, file=sys.stderr, flush=True)
+
+
 def line_tasks_bytes(stats):
     """Algorithmic bytes of k_integrals: per action integral the 12 node
     constants read and the action written (doubles)."""
