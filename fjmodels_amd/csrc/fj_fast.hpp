@@ -102,14 +102,28 @@ constexpr double FAST_ROOT_RTOL = 1e-9;
 
 // wrp_GSLroots (include/Utils.h:134-167): an end of the bracket whose |F| is
 // below 5e-4 is taken as the root (low first); otherwise the root is solved
-// for.  The early return decides the reference's turning points of
-// near-circular and near-planar orbits, so it is kept.
+// for, on f, to rtol relative.  The early return decides the reference's
+// turning points of near-circular and near-planar orbits, so it is kept;
+// flo and fhi are FP64 values.
 template <class F>
-FJ_HD inline double fast_root_ref(const F &f, double lo, double hi, double flo, double fhi, int *nev) {
+FJ_HD inline double fast_root_ref(const F &f, double lo, double hi, double flo, double fhi, int *nev,
+                                  double rtol = FAST_ROOT_RTOL) {
     if (fabs(flo) < 5e-4) return lo;
     if (fabs(fhi) < 5e-4) return hi;
-    return fast_root(f, lo, hi, flo, fhi, FAST_ROOT_RTOL, nev);
+    return fast_root(f, lo, hi, flo, fhi, rtol, nev);
 }
+
+// The evaluations of fast_setup: all in FP64 (host test build).  The device
+// kernel's policy (fj_mixed.hpp) walks and solves on FP32 evaluations and
+// checks each bracket's ends in FP64.
+struct FastPolicy64 {
+    using Tab = PotView;
+    using U = FastFu;
+    using V = FastFv;
+    using DU = FastDFu;
+    static constexpr double rtol = FAST_ROOT_RTOL;
+    FJ_HD static DU make_du(const PotView &T, const Tab &, const FastNode *n) { return DU{T, n}; }
+};
 
 // Brackets of the two u turning points found by GetTurnu's walk.
 struct UBracket {
@@ -119,12 +133,13 @@ struct UBracket {
 
 // GetTurnu(1) (uvOrb.cpp:131-201): the reference's bracket walk from u = 1.
 // The walk only takes signs of F_u (and of dF_u in its outer branches), so
-// Uw may be a lower-precision evaluation of F_u; dU is FP64.
-template <class UF>
-FJ_HD inline UBracket fast_walk_u(const UF &Uw, const FastDFu &dU, int *nev) {
+// Uw and dU may be lower-precision evaluations.
+template <class UF, class DUF>
+FJ_HD inline UBracket fast_walk_u(const UF &Uw, const DUF &dU, int *nev) {
     UBracket b;
-    double ut_ = 1, ut = Uw(ut_), dutt = dU(ut_), eps = .1;
-    *nev += 2;
+    double ut_ = 1, ut = Uw(ut_), dutt = 0, eps = .1;
+    (*nev)++;
+    if (ut < 0) { dutt = dU(ut_); (*nev)++; }   // read only on this side
     double uinner = ut_, uouter = ut_, u1 = ut_, u2 = ut_;
     bool noroot = false;
     int guard = 0;
@@ -179,10 +194,11 @@ FJ_HD inline UBracket fast_walk_u(const UF &Uw, const FastDFu &dU, int *nev) {
 // Everything of one node before the action integrals: H < 0 test, Delta(E),
 // spheroidal coordinates (Uvpt.cpp:11-24), E, the constants above and the
 // turning points.  Returns 0 (H >= 0), 1 (bound) or 2 (E >= 0: unbound).
-// *nev counts the potential evaluations.  The u walk runs on UW (built from
-// tw and the node; FastFu for FP64), whose bracket is then checked in FP64.
-template <class UW>
-FJ_HD inline int fast_setup(const PotView &T, const typename UW::Tab &tw, const double *Eg, const double *Dg, int ng,
+// *nev counts the potential evaluations.  The u and v walks and the root
+// solves run on P::U and P::V (built from tw and the node); the ends of each
+// bracket are checked in FP64, and the endpoint test takes the FP64 values.
+template <class P>
+FJ_HD inline int fast_setup(const PotView &T, const typename P::Tab &tw, const double *Eg, const double *Dg, int ng,
                             double R, double z, double Phi_h, const double *V, FastNode &n, int *nev) {
     n.ju = 0; n.jv = 0;
     const double H = .5 * (V[0] * V[0] + V[1] * V[1] + V[2] * V[2]) + Phi_h;   // DF.cpp:83
@@ -209,8 +225,8 @@ FJ_HD inline int fast_setup(const PotView &T, const typename UW::Tab &tw, const 
     n.u0 = fjm_asinh(shu);
     n.v0 = fjm_acos(cva);                                   // in [0, pi/2]
     const FastFu U{T, &n};
-    const FastDFu dU{T, &n};
-    const UW Uw{tw, &n};
+    const typename P::DU dU = P::make_du(T, tw, &n);
+    const typename P::U Uw{tw, &n};
     const UBracket b = fast_walk_u(Uw, dU, nev);
     if (!b.noroot) {
         // FP64 values at the bracket ends (the roots' brackets and wrp_GSLroots'
@@ -224,40 +240,64 @@ FJ_HD inline int fast_setup(const PotView &T, const typename UW::Tab &tw, const 
         while (!(f1 > 0) && u1 < u2 && g++ < 400) { ui = u1; fi = f1; u1 /= .9; f1 = U(u1); (*nev)++; }
         while (fo > 0 && g++ < 600) { u2 = uo; f2 = fo; uo *= 1.1; fo = U(uo); (*nev)++; }
         while (!(f2 > 0) && u2 > u1 && g++ < 800) { uo = u2; fo = f2; u2 /= 1.1; f2 = U(u2); (*nev)++; }
-        n.uin = fast_root_ref(U, ui, u1, fi, f1, nev);
-        n.uout = fast_root_ref(U, u2, uo, f2, fo, nev);
+        n.uin = fast_root_ref(Uw, ui, u1, fi, f1, nev, P::rtol);
+        n.uout = fast_root_ref(Uw, u2, uo, f2, fo, nev, P::rtol);
     } else {
         n.uin = b.uinner; n.uout = b.uouter;
     }
     n.ju = !(fabs(n.uout - n.uin) < TINY);
-    // GetTurnv(pi/4) (uvOrb.cpp:203-224)
+    // GetTurnv(pi/4) (uvOrb.cpp:203-224): halve towards pi/2 (F < 0 at pi/4)
+    // or towards 0 (F >= 0) until F changes sign.  The first value, which
+    // picks the direction, is FP64; the walk runs on Vw, then the two ends of
+    // its last step are checked in FP64 and moved by the walk's own step
+    // where the FP64 sign differs.
     const FastFv Vf{T, &n};
+    const typename P::V Vw{tw, &n};
     n.vt = 0;
     {
         const double vtry = .5 * PIH;
-        double vtop = vtry, vbot = vtry, ft, fb;
         const double f0 = Vf(vtry);
         (*nev)++;
+        const bool up = f0 < 0;                 // walk vbot towards pi/2
+        // a: last point with f0's sign, b: first point past it
+        double a = vtry, b = vtry, fb = f0;
         bool done = false;
         int guard = 0;
-        if (f0 < 0) {
-            fb = f0; ft = f0;
-            while (fb < 0 && guard++ < 4000) {
-                vbot = .5 * (PIH + vbot);
-                if (PIH - vbot < TINY) { n.vt = PIH; done = true; break; }
-                fb = Vf(vbot); (*nev)++;
-            }
-        } else {
-            ft = f0; fb = f0;
-            while (ft > 0 && guard++ < 4000) {
-                vtop *= .5;
-                if (vtop < TINY) { n.vt = 0; done = true; break; }
-                ft = Vf(vtop); (*nev)++;
-            }
+        while ((up ? fb < 0 : fb > 0) && guard++ < 4000) {
+            a = b;
+            b = up ? .5 * (PIH + b) : .5 * b;
+            if (up ? PIH - b < TINY : b < TINY) { n.vt = up ? PIH : 0; done = true; break; }
+            fb = Vw(b); (*nev)++;
         }
         if (!done) {
-            if (!(vbot < vtop)) n.vt = fast_root_ref(Vf, vtop, vbot, ft, fb, nev);
-            else n.vt = fast_root_ref(Vf, vbot, vtop, fb, ft, nev);
+            double fa = a == vtry ? f0 : Vf(a), fb64 = Vf(b);
+            *nev += 1 + (a != vtry);
+            int g = 0;
+            // FP64 says a is already past the sign change: step back
+            while ((up ? !(fa < 0) : !(fa > 0)) && a != vtry && g++ < 200) {
+                b = a; fb64 = fa;
+                a = up ? 2 * a - PIH : 2 * a;
+                if (up ? a < vtry : a > vtry) a = vtry;
+                fa = a == vtry ? f0 : Vf(a); (*nev)++;
+            }
+            // FP64 says b has not reached it: step on
+            while ((up ? fb64 < 0 : fb64 > 0) && g++ < 400) {
+                a = b; fa = fb64;
+                b = up ? .5 * (PIH + b) : .5 * b;
+                if (up ? PIH - b < TINY : b < TINY) { n.vt = up ? PIH : 0; done = true; break; }
+                fb64 = Vf(b); (*nev)++;
+            }
+            if (!done) {
+                // wrp_GSLroots on the reference's bracket, [vtry, b] or [b, vtry]:
+                // its endpoint test (low end first), then the root, solved
+                // within the walk's last step [a, b]
+                const double lo = up ? vtry : b, hi = up ? b : vtry;
+                const double flo = up ? f0 : fb64, fhi = up ? fb64 : f0;
+                if (fabs(flo) < 5e-4) n.vt = lo;
+                else if (fabs(fhi) < 5e-4) n.vt = hi;
+                else if (up) n.vt = fast_root(Vw, a, b, fa, fb64, P::rtol, nev);
+                else n.vt = fast_root(Vw, b, a, fb64, fa, P::rtol, nev);
+            }
         }
     }
     n.jv = !(fabs(n.vt - PIH) < TINY);

@@ -124,16 +124,20 @@ __device__ inline LdsTabs stage_tables(const TabArgs &a, double *smem) {
 
 // FP32 copies of the radial grid, Phi_l table and even-Legendre step
 // constants for the mixed-precision integrand, after the FP64 tables.
-__host__ __device__ inline size_t tab32_lds_floats(int nr, int npoly) {
-    return (size_t)2 * nr + (size_t)nr * npoly + 3 * MAXPOLY;
+__host__ __device__ inline size_t tab32_lds_floats(int nr, int npoly, bool with_pr = false) {
+    return (size_t)2 * nr + (size_t)nr * npoly * (with_pr ? 2 : 1) + 3 * MAXPOLY;
 }
 
-__device__ inline PotView32 stage_tables32(const LdsTabs &L, float *f) {
+// with_pr: also the Phi_l' table (the fast phase's FP32 gradient).
+__device__ inline PotView32 stage_tables32(const LdsTabs &L, float *f, bool with_pr = false) {
     const int nr = L.T.nr, np = L.T.npoly;
     float *ar = f, *phil = ar + nr, *la = phil + nr * np, *lb = la + MAXPOLY, *lg = lb + MAXPOLY;
     float *lo = lg + MAXPOLY;
+    float *pr = with_pr ? lo + nr : nullptr;
     for (int i = threadIdx.x; i < nr; i += blockDim.x) ar[i] = (float)L.T.ar[i];
     for (int i = threadIdx.x; i < nr * np; i += blockDim.x) phil[i] = (float)L.T.phil[i];
+    if (with_pr)
+        for (int i = threadIdx.x; i < nr * np; i += blockDim.x) pr[i] = (float)L.T.Pr[i];
     for (int i = threadIdx.x; i < nr; i += blockDim.x) {
         const double p0 = L.T.phil[i * np];
         lo[i] = (float)(p0 - (double)(float)p0);
@@ -146,7 +150,7 @@ __device__ inline PotView32 stage_tables32(const LdsTabs &L, float *f) {
     }
     __syncthreads();
     PotView32 T;
-    T.nr = nr; T.npoly = np; T.ar = ar; T.phil = phil; T.phil0lo = lo; T.la = la; T.lb = lb; T.lg = lg;
+    T.nr = nr; T.npoly = np; T.ar = ar; T.phil = phil; T.phil0lo = lo; T.la = la; T.lb = lb; T.lg = lg; T.Pr = pr;
     T.lut = L.T.lut; T.lg0 = L.T.lg0; T.lsc = L.T.lsc;
     return T;
 }
@@ -1036,7 +1040,7 @@ struct FastArgs {
 #define FJ_FAST_BLOCKS_PER_CU 32
 #endif
 #ifndef FJ_FAST_WALK32
-#define FJ_FAST_WALK32 1   // signs of the u bracket walk from FP32 evaluations, checked in FP64
+#define FJ_FAST_WALK32 1   // walks and root solves on FP32 evaluations, bracket ends checked in FP64
 #endif
 #ifndef FJ_FAST_WAVES
 #define FJ_FAST_WAVES 4   // waves per SIMD the register allocation of k_fast aims at
@@ -1046,8 +1050,8 @@ k_fast(TabArgs ta, FastArgs fa) {
     extern __shared__ double smem[];
     const LdsTabs L = stage_tables(ta, smem);
     float *f32 = reinterpret_cast<float *>(smem + tab_lds_doubles(ta.nr, ta.npoly, ta.ngrid));
-    const PotView32 T32 = stage_tables32(L, f32);
-    double *red = reinterpret_cast<double *>(f32 + ((tab32_lds_floats(ta.nr, ta.npoly) + 1) & ~(size_t)1));
+    const PotView32 T32 = stage_tables32(L, f32, true);
+    double *red = reinterpret_cast<double *>(f32 + ((tab32_lds_floats(ta.nr, ta.npoly, true) + 1) & ~(size_t)1));
     const int tiles = (fa.npts + 63) / 64;
     const int tile = blockIdx.x % tiles, sl = blockIdx.x / tiles;
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -1066,10 +1070,10 @@ k_fast(TabArgs ta, FastArgs fa) {
             node_velocity(fa.quad, q, Ve, V, w);
             FastNode fn;
             int ne = 0;
-            #if FJ_FAST_WALK32
-            const int flag = fast_setup<FastFu32>(L.T, &T32, L.E, L.D, L.ngrid, R, z, Phi_h, V, fn, &ne);
+#if FJ_FAST_WALK32
+            const int flag = fast_setup<FastPolicy32>(L.T, &T32, L.E, L.D, L.ngrid, R, z, Phi_h, V, fn, &ne);
 #else
-            const int flag = fast_setup<FastFu>(L.T, L.T, L.E, L.D, L.ngrid, R, z, Phi_h, V, fn, &ne);
+            const int flag = fast_setup<FastPolicy64>(L.T, L.T, L.E, L.D, L.ngrid, R, z, Phi_h, V, fn, &ne);
 #endif
             nev += ne;
             if (flag == 0) continue;   // H >= 0: f = 0 (DF.cpp:84)
@@ -1116,15 +1120,26 @@ k_fast(TabArgs ta, FastArgs fa) {
     }
 }
 
-// Sums of the S slices of each point in slice order, then the moments of
-// rhofDF (Integ.cpp:160-161) as k_moments forms them.
+// Sums of the S slices of each point, then the moments of rhofDF
+// (Integ.cpp:160-161) as k_moments forms them.  One block per point: lane t
+// adds slices t, t + 256, ... in order, then a pairwise tree over the lanes
+// (a fixed order).
 __global__ void __launch_bounds__(256) k_fast_finish(int npts, int S, int quad, const double *part, double *out6,
                                                      double *mom) {
-    const int p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= npts) return;
+    __shared__ double red[6][256];
+    const int p = blockIdx.x, t = threadIdx.x;
     double o[6] = {0, 0, 0, 0, 0, 0};
-    for (int s = 0; s < S; s++)
+    for (int s = t; s < S; s += 256)
         for (int m = 0; m < 6; m++) o[m] += part[((size_t)p * S + s) * 6 + m];
+    for (int m = 0; m < 6; m++) red[m][t] = o[m];
+    __syncthreads();
+    for (int st = 128; st > 0; st >>= 1) {
+        if (t < st)
+            for (int m = 0; m < 6; m++) red[m][t] += red[m][t + st];
+        __syncthreads();
+    }
+    if (t != 0) return;
+    for (int m = 0; m < 6; m++) o[m] = red[m][0];
     for (int m = 0; m < 6; m++) out6[(size_t)p * 6 + m] = o[m];
     mom[p] = quad ? 4 * o[0] : o[0];
     mom[npts + p] = o[1] / o[0];
@@ -2197,6 +2212,7 @@ struct fj_ctx {
     double *d_fpart = nullptr; size_t fpart_cap = 0;
     int fast_ngl = 64;
     bool fgl_ready = false;
+    int fast_cus = 0;                 // compute units of the device (fast phase's launch shape)
     double *d_pR = nullptr, *d_pz = nullptr, *d_pPhi = nullptr, *d_pVe = nullptr, *d_out6 = nullptr, *d_mom = nullptr;
     // grid buffers
     double *d_rhl = nullptr, *d_gl = nullptr, *d_pw = nullptr, *d_pws = nullptr; int *d_status = nullptr;
@@ -2530,11 +2546,14 @@ int run_fast(fj_ctx *c, int npts) {
         if ((r = dalloc(c, &c->d_cnt, 16)) || (r = dalloc(c, &c->d_ctr, 1))) return r;
     }
     const int tiles = (npts + 63) / 64;
-    hipDeviceProp_t prop;
-    HIPCHK(c, hipGetDeviceProperties(&prop, c->device));
+    if (!c->fast_cus) {
+        hipDeviceProp_t prop;
+        HIPCHK(c, hipGetDeviceProperties(&prop, c->device));
+        c->fast_cus = prop.multiProcessorCount;
+    }
     // many short blocks (FJ_FAST_BLOCKS_PER_CU per CU) so that the last round of
     // blocks is a small part of the launch; at least 4 nodes per wave
-    int S = (FJ_FAST_BLOCKS_PER_CU * prop.multiProcessorCount + tiles - 1) / tiles;
+    int S = (FJ_FAST_BLOCKS_PER_CU * c->fast_cus + tiles - 1) / tiles;
     if (S > NQ / 16) S = NQ / 16;
     if (S < 1) S = 1;
     const size_t need = (size_t)npts * S * 6;
@@ -2544,7 +2563,7 @@ int run_fast(fj_ctx *c, int npts) {
     }
     const TabArgs ta = tab_args(c);
     const size_t lds_pts = tab_bytes(c);
-    const size_t lds = tab_bytes(c) + ((tab32_lds_floats(c->nr, c->npoly) + 1) & ~(size_t)1) * sizeof(float) +
+    const size_t lds = tab_bytes(c) + ((tab32_lds_floats(c->nr, c->npoly, true) + 1) & ~(size_t)1) * sizeof(float) +
                        4 * 6 * 64 * sizeof(double);
     hipEvent_t ev[4];
     for (int i = 0; i < 4; i++) HIPCHK(c, hipEventCreate(&ev[i]));
@@ -2559,7 +2578,7 @@ int run_fast(fj_ctx *c, int npts) {
     k_fast<<<(unsigned)(tiles * S), 256, lds, c->stream>>>(ta, fa);
     HIPCHK(c, hipGetLastError());
     HIPCHK(c, hipEventRecord(ev[2], c->stream));
-    k_fast_finish<<<(npts + 255) / 256, 256, 0, c->stream>>>(npts, S, c->quad, c->d_fpart, c->d_out6, c->d_mom);
+    k_fast_finish<<<npts, 256, 0, c->stream>>>(npts, S, c->quad, c->d_fpart, c->d_out6, c->d_mom);
     HIPCHK(c, hipGetLastError());
     HIPCHK(c, hipEventRecord(ev[3], c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));

@@ -30,6 +30,7 @@ struct PotView32 {
     const float *la, *lb, *lg;
     const int *lut;    // bracket table of PotView (fj_device.hpp)
     float lg0, lsc;
+    const float *Pr;   // Phi_l' table (staged for the fast phase only, else null)
 };
 
 // Phi(R, z) - shift.  The shift enters through the monopole, whose
@@ -96,6 +97,84 @@ __device__ __forceinline__ float pot_phi32(const PotView32 &T, float R, float z,
         else { p = (T.la[np] * pm2 + pm1 * (c2 - T.lb[np])) * T.lg[np]; pm2 = pm1; pm1 = p; }
         phi += v * p;
     }
+    return phi;
+}
+
+// Phi - shift and dPhi/dR, dPhi/dz in FP32, for the signs the fast phase's
+// u walk takes of dF_u/du.  As Potential::dR / dz (Potential.cpp:221-276)
+// the radial derivative comes from the Phi_l' table (three-table intpo2,
+// UtilsLeg.h:127-158; beyond r_max the derivative of the Keplerian tail) and
+// the angular one from the Legendre derivatives, here by the recurrences
+// l P_l = (2l - 1) c P_l-1 - (l - 1) P_l-2 and P'_l = P'_l-2 + (2l - 1) P_l-1;
+// s and c from the clamped radius, as there.
+__device__ __forceinline__ float pot_grad32(const PotView32 &T, float R, float z, float shift, float *dR,
+                                            float *dz) {
+    const int nr = T.nr, np_ = T.npoly;
+    const float *ar = T.ar;
+    const float rs = sqrtf(R * R + z * z);
+    const bool kep = rs >= ar[nr - 1];
+    const float r = rs < ar[0] ? ar[0] : rs;
+    const float c = z / r, s = R / r;
+    float phi, g1, dthc = 0.f;
+    int bot = 0, top = 0, thr = 0;
+    float f1 = 0, f2 = 0, f3 = 0, q = 0, q2 = 0, qp = 0;
+    bool quad = false;
+    if (kep) {
+        q = ar[nr - 1] / r; q2 = q * q;
+        const float *last = T.phil + (nr - 1) * np_;
+        phi = fmaf(last[0], q, -shift) + T.phil0lo[nr - 1] * q;
+        g1 = -last[0] * q / r;
+        qp = q * q2;
+    } else {
+        const float t = (__log2f(r) - T.lg0) * T.lsc;
+        bot = T.lut[t > 0.f ? (t < (float)(NLUT - 1) ? (int)t : NLUT - 1) : 0];
+        while (bot < nr - 2 && r >= ar[bot + 1]) bot++;
+        while (bot > 0 && !(r >= ar[bot])) bot--;
+        top = bot + 1;
+        const float db = r - ar[bot];
+        f1 = db / (ar[top] - ar[bot]);
+        quad = top < 10;
+        if (quad) {
+            thr = (f1 < 0.5f && bot > 0) ? bot - 1 : top + 1;
+            f2 = (r - ar[top]) * db / ((ar[thr] - ar[top]) * (ar[thr] - ar[bot]));
+            f3 = (ar[thr] - ar[bot]) / (ar[top] - ar[bot]);
+        }
+        const float *lo = T.phil0lo;
+        const float b = T.phil[bot * np_], tb = (T.phil[top * np_] - b) + (lo[top] - lo[bot]);
+        phi = ((b - shift) + lo[bot]) + f1 * tb;
+        if (quad) phi += f2 * (((T.phil[thr * np_] - b) + (lo[thr] - lo[bot])) - f3 * tb);
+        const float pb = T.Pr[bot * np_], pt = T.Pr[top * np_];
+        g1 = pb + f1 * (pt - pb);
+        if (quad) g1 += f2 * (T.Pr[thr * np_] - pb - f3 * (pt - pb));
+    }
+    float Pm2 = 0.f, Pm1 = 1.f, dPm2 = 0.f, dPm1 = 0.f;   // P_l-2, P_l-1 and their derivatives
+    for (int np = 1; np < np_; np++) {
+        for (int i = 2 * np - 1; i <= 2 * np; i++) {
+            const float P = ((float)(2 * i - 1) * c * Pm1 - (float)(i - 1) * Pm2) / (float)i;
+            const float dP = dPm2 + (float)(2 * i - 1) * Pm1;
+            Pm2 = Pm1; Pm1 = P; dPm2 = dPm1; dPm1 = dP;
+        }
+        float v, d;
+        if (kep) {
+            const float l = T.phil[(nr - 1) * np_ + np];
+            v = l * qp;
+            d = -(float)(2 * np + 1) * v / r;
+            qp *= q2;
+        } else {
+            const float tv = T.phil[top * np_ + np], b = T.phil[bot * np_ + np];
+            v = b + f1 * (tv - b);
+            if (quad) v += f2 * (T.phil[thr * np_ + np] - b - f3 * (tv - b));
+            const float pt = T.Pr[top * np_ + np], pb = T.Pr[bot * np_ + np];
+            d = pb + f1 * (pt - pb);
+            if (quad) d += f2 * (T.Pr[thr * np_ + np] - pb - f3 * (pt - pb));
+        }
+        phi += v * Pm1;
+        g1 += d * Pm1;
+        dthc += v * dPm1;
+    }
+    const float dth = -s * dthc;                          // dPhi/dtheta
+    *dR = dth * (c / r) + g1 * s;
+    *dz = dth * (-s / r) + g1 * c;
     return phi;
 }
 
@@ -184,6 +263,51 @@ struct FastFu32 {
         const float Pc = pot_phi32(*T, D * sh * sv0, D * ch * cv0, c) - dE;   // Phi - E
         return (double)((float)n->Bu - (float)n->C / sh2 - (sh2 + sv0 * sv0) * Pc);
     }
+};
+
+// dF_u/du of fj_fast.hpp (FastDFu) in FP32, for the signs of the u walk.
+struct FastDFu32 {
+    const PotView32 *T;
+    const FastNode *n;
+    __device__ double operator()(double u) const {
+        float sh, ch;
+        sinhcosh32((float)u, &sh, &ch);
+        const float sh2 = sh * sh;
+        const float c = (float)n->E, dE = (float)(n->E - (double)c);
+        const float D = (float)n->Delta, sv0 = (float)n->sv0, cv0 = (float)n->cv0;
+        float dR, dz;
+        const float Pc = pot_grad32(*T, D * sh * sv0, D * ch * cv0, c, &dR, &dz) - dE;   // Phi - E
+        const float dPhidu = D * (dR * ch * sv0 + dz * sh * cv0);
+        return (double)(-2.f * Pc * sh * ch - (sh2 + sv0 * sv0) * dPhidu + 2.f * (float)n->C * ch / (sh2 * sh));
+    }
+};
+
+// F_v of fj_fast.hpp in FP32.
+struct FastFv32 {
+    const PotView32 *T;
+    const FastNode *n;
+    __device__ double operator()(double v) const {
+        float sn, cs;
+        sincosf((float)v, &sn, &cs);
+        const float s2 = sn * sn;
+        const float c = (float)n->E, dE = (float)(n->E - (double)c);
+        const float D = (float)n->Delta, sh0 = (float)n->shu0, ch0 = (float)n->chu0;
+        const float Pc = pot_phi32(*T, D * sh0 * sn, D * ch0 * cs, c) - dE;   // Phi - E
+        return (double)((float)n->Bv - (float)n->C / s2 - (sh0 * sh0 + s2) * Pc);
+    }
+};
+
+// fast_setup's evaluations in the kernel: the u and v walks (with the signs
+// of dF_u/du) and the root solves on FP32 evaluations (turning points to
+// 2e-6 relative, below what the FP32 action rules resolve), the bracket
+// checks and the endpoint test in FP64.
+struct FastPolicy32 {
+    using Tab = const PotView32 *;
+    using U = FastFu32;
+    using V = FastFv32;
+    using DU = FastDFu32;
+    static constexpr double rtol = 2e-6;
+    __device__ static DU make_du(const PotView &, Tab tw, const FastNode *n) { return DU{tw, n}; }
 };
 
 __device__ __forceinline__ void fast_actions32(const PotView32 &T, const FastNode &n, const FastGL32 *gl, int ngl,

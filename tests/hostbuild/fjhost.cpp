@@ -274,7 +274,7 @@ int fjh_fast_actions(int nr, const double *ar, int npoly, const double *phil, co
         const double V[3] = {vR[i], vphi[i], vz[i]};
         FastNode fn;
         int ne = 0;
-        const int flag = fast_setup<FastFu>(T, T, Eg, Dg, ng, R[i], z[i], Phi[i], V, fn, &ne);
+        const int flag = fast_setup<FastPolicy64>(T, T, Eg, Dg, ng, R[i], z[i], Phi[i], V, fn, &ne);
         double jr = 0, jz = 0;
         if (flag == 2) { jr = 1e31; jz = 1e31; }
         else if (flag == 1) fast_actions64(T, fn, ngl, x, w, &jr, &jz);
@@ -305,7 +305,7 @@ int fjh_fast_rhofDF(int nr, const double *ar, int npoly, const double *phil, con
         const double wt = wa[i] * wp[j] * wa[kk];
         FastNode fn;
         int ne = 0;
-        const int flag = fast_setup<FastFu>(T, T, Eg, Dg, ng, R, z, Phi_h, V, fn, &ne);
+        const int flag = fast_setup<FastPolicy64>(T, T, Eg, Dg, ng, R, z, Phi_h, V, fn, &ne);
         if (flag == 0) continue;
         double jr = 1e31, jz = 1e31;
         if (flag == 1) fast_actions64(T, fn, ngl, gx, gw, &jr, &jz);
