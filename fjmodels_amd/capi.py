@@ -32,7 +32,7 @@ EXPORTS = [
     "fj_surrogate_init", "fj_surrogate_collect", "fj_surrogate_train", "fj_surrogate_mode",
     "fj_surrogate_get_stats", "fj_surrogate_params", "fj_surrogate_forward", "fj_surrogate_gradient",
     "fj_batch_create", "fj_batch_destroy", "fj_batch_setup", "fj_batch_autotune", "fj_batch_iterate",
-    "fj_batch_get", "fj_batch_get_stats",
+    "fj_batch_get", "fj_batch_set", "fj_batch_get_stats",
 ]
 
 MLP_NPARAM = 5 * 128 + 128 + 2 * (128 * 128 + 128) + 128 * 2 + 2   # 34,050
@@ -135,6 +135,7 @@ def load():
         "fj_batch_autotune": [v],
         "fj_batch_iterate": [v, i],
         "fj_batch_get": [v, i] + [_dp] * 10,
+        "fj_batch_set": [v, i] + [_dp] * 5,
         "fj_batch_get_stats": [v, C.POINTER(BatchStats)],
     }
     for name, args in sig.items():
@@ -437,6 +438,21 @@ class Batch:
         out = dict(zip(["rhl", "sigRl", "sigpl", "sigzl", "vrotl", "phil", "Pr", "Pr2"], t))
         out["Egrid"], out["Dgrid"] = E, D
         return out
+
+    def set(self, m, phil=None, Pr=None, Pr2=None, Egrid=None, Dgrid=None):
+        """Installs model m's potential tables and / or Delta table."""
+        nr, npoly = self.shapes[m]
+        arrs = []
+        for x, n in ((phil, nr * npoly), (Pr, nr * npoly), (Pr2, nr * npoly), (Egrid, 30), (Dgrid, 30)):
+            if x is None:
+                arrs.append(None)
+                continue
+            a = _a(x)
+            if a.size != n:
+                raise ValueError("fj_batch_set: table of %d values where %d are needed" % (a.size, n))
+            arrs.append(a)
+        self.ctx._chk(self.lib.fj_batch_set(self.h, m, *[None if a is None else _p(a) for a in arrs]),
+                      "fj_batch_set")
 
     def stats(self):
         s = BatchStats()

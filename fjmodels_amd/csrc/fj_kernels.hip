@@ -4135,6 +4135,21 @@ int fj_batch_get(fj_batch *b, int m, double *rhl, double *sigRl, double *sigpl, 
     return 0;
 }
 
+int fj_batch_set(fj_batch *b, int m, const double *phil, const double *Pr, const double *Pr2, const double *Egrid,
+                 const double *Dgrid) {
+    if (!b || m < 0 || m >= b->M) return FJ_E_ARG;
+    fj_ctx *c = b->c;
+    const ModelDev &d = b->hmd[m];
+    const size_t n = (size_t)d.nr * d.npoly * sizeof(double);
+    // the model's own device tables (TabArgs holds them read-only for the kernels)
+    double *dst[5] = {const_cast<double *>(d.ta.phil), const_cast<double *>(d.ta.Pr), d.Pr2,
+                      const_cast<double *>(d.ta.E), const_cast<double *>(d.ta.D)};
+    const double *src[5] = {phil, Pr, Pr2, Egrid, Dgrid};
+    for (int k = 0; k < 5; k++)
+        if (src[k]) HIPCHK(c, hipMemcpy(dst[k], src[k], k < 3 ? n : 30 * sizeof(double), hipMemcpyHostToDevice));
+    return 0;
+}
+
 int fj_batch_get_stats(const fj_batch *b, fj_batch_stats *out) {
     if (!b || !out) return FJ_E_ARG;
     *out = b->st;

@@ -303,6 +303,12 @@ int fj_batch_iterate(fj_batch *b, int niter);
 /* Model m's state: any pointer may be NULL; tables [nr][npoly], Egrid/Dgrid [30]. */
 int fj_batch_get(fj_batch *b, int m, double *rhl, double *sigRl, double *sigpl, double *sigzl, double *vrotl,
                  double *phil, double *Pr, double *Pr2, double *Egrid, double *Dgrid);
+/* Installs model m's potential tables [nr][npoly] and Delta table [30] (any
+ * pointer may be NULL: that table is kept), as updatePhil (potLeg.cpp:21-29)
+ * and tabulateDelta (delta/tabulate.cpp:13-44) leave them for the next
+ * computeNewPhi: resuming a model, or starting it from given tables. */
+int fj_batch_set(fj_batch *b, int m, const double *phil, const double *Pr, const double *Pr2, const double *Egrid,
+                 const double *Dgrid);
 int fj_batch_get_stats(const fj_batch *b, fj_batch_stats *out);
 
 #ifdef __cplusplus
