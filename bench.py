@@ -200,6 +200,98 @@ def fast_leg(ctx, model, prev, last, steps, pts):
             "phases_ms": {k: float(np.mean([x[k] for x in st])) for k in ("ms_points", "ms_setup", "ms_moments")}}
 
 
+def sync():
+    """torch.cuda.synchronize() when torch drives the device in this process
+    (the library's own calls return when their results are on the host)."""
+    import torch
+    if torch.cuda.is_initialized():
+        torch.cuda.synchronize()
+
+
+def drho(a, b):
+    """Largest difference of two rho_l tables, scaled by the monopole at each radius."""
+    import numpy as np
+    return float(np.max(np.abs(a["rhl"] - b["rhl"]) / np.abs(a["rhl"][:, :1])))
+
+
+def converge(ctx, model, guess_tables, prec, tol=1e-4, max_iter=30, sur=None):
+    """One SCF run of `model` from the guess potential to `tol` (successive
+    rho_l differ by less than tol of the monopole, the driver's test), with
+    the node phase precision `prec`; `sur`: the surrogate schedule (keys
+    keep, epochs, judge, gate, max_sur, sur_floor, collect_from, serve_from,
+    capacity, batch, lr, seed), whose Staeckel iterations run at `prec`.
+    Returns (seconds, last output, virial, surrogate stats, rho_l change per
+    iteration, iteration kinds, training seconds, median ms per kind)."""
+    import numpy as np
+    import fjmodels_amd as fj
+    ctx.set_precision(prec)
+    ctx.set_potential(*guess_tables)
+    model.total = [x.copy() for x in guess_tables]
+    model.philP = [tuple(x.copy() for x in guess_tables)]
+    sync()
+    t0 = time.perf_counter()
+    hist, kinds, t_train, t_kind = [], [], 0.0, {}
+    judged = []                  # per served iteration: decision and the judge's weighted medians
+    prev, out = None, None
+    phase = "collect" if sur else "plain"
+    n_sur = 0
+    for it in range(max_iter):
+        if sur and it == sur["collect_from"]:
+            ctx.surrogate_init(seed=sur.get("seed", 7), capacity=sur.get("capacity", 1 << 20))
+            ctx.surrogate_collect(sur["keep"])
+        if sur and phase == "collect" and it == sur["serve_from"]:
+            ctx.surrogate_collect(1.0)            # from now on: the judged nodes
+            tt = time.perf_counter()
+            ctx.surrogate_train(sur["epochs"], batch=sur.get("batch", 32768), lr=sur.get("lr", 3e-3))
+            t_train += time.perf_counter() - tt
+            ctx.surrogate_mode(True, judge=sur["judge"], jz_min=1e-3)
+            phase = "sur"
+        if phase == "sur":
+            saved = ([x.copy() for x in model.total], [tuple(x.copy() for x in p) for p in model.philP])
+        ti = time.perf_counter()
+        out = model.iterate()[0]
+        ti = 1e3 * (time.perf_counter() - ti)
+        t_kind.setdefault("S" if phase == "sur" else "P", []).append(ti)
+        if phase == "sur":
+            js = ctx.surrogate_stats()
+            rejected = max(js["judge_werr_r_med"], js["judge_werr_z_med"]) > sur["gate"]
+            judged.append({"iteration": it, "kind": "R" if rejected else "S",
+                           "werr_r_med": js["judge_werr_r_med"], "werr_z_med": js["judge_werr_z_med"]})
+            if rejected:
+                kinds.append("R")                 # rejected by the judge
+                ctx.surrogate_mode(False)
+                ctx.surrogate_collect(0.0)
+                model.total, model.philP = saved
+                ctx.set_potential(*model.total)
+                phase = "plain"
+                continue
+        kinds.append("S" if phase == "sur" else "P")
+        d = drho(out, prev) if prev is not None else None
+        hist.append(d)
+        prev = out
+        if phase == "sur":
+            n_sur += 1
+            if (d is not None and d < sur["sur_floor"]) or n_sur >= sur["max_sur"]:
+                ctx.surrogate_mode(False)
+                ctx.surrogate_collect(0.0)
+                phase = "plain"
+            else:
+                tt = time.perf_counter()
+                ctx.surrogate_train(1, batch=sur.get("batch", 32768), lr=sur.get("lr", 3e-3) / 3)
+                t_train += time.perf_counter() - tt
+        elif phase == "plain" and d is not None and d < tol:
+            break
+    sync()
+    dt = time.perf_counter() - t0
+    vir = ctx.virial(out["rhl"], out["vrotl"], out["sigRl"], out["sigpl"], out["sigzl"])
+    st = dict(ctx.surrogate_stats(), judge_history=judged) if sur else None
+    if sur:
+        ctx.surrogate_mode(False)
+        ctx.surrogate_collect(0.0)
+    ctx.set_precision(fj.FJ_PREC_FP64)
+    return dt, out, vir, st, hist, "".join(kinds), t_train, {k: float(np.median(v)) for k, v in t_kind.items()}
+
+
 def surrogate_leg(ctx, model, guess_tables, tol=1e-4, max_iter=30, keep=0.15, epochs=4, judge=0.05,
                   max_sur=8, sur_floor=4e-3, gate=0.05, collect_from=3, serve_from=5):
     """The Hernquist model converged four times from the guess potential to
@@ -228,75 +320,12 @@ def surrogate_leg(ctx, model, guess_tables, tol=1e-4, max_iter=30, keep=0.15, ep
     median wall time of one Staeckel and one served iteration (the judge's
     rejected ones included), the iteration counts, the differences of the converged rho_l from the plain
     run's (scaled by the monopole) and the virial ratios."""
-    import numpy as np
-    import torch
     import fjmodels_amd as fj
+    sur = dict(keep=keep, epochs=epochs, judge=judge, gate=gate, max_sur=max_sur, sur_floor=sur_floor,
+               collect_from=collect_from, serve_from=serve_from)
 
-    def drho(a, b):
-        return float(np.max(np.abs(a["rhl"] - b["rhl"]) / np.abs(a["rhl"][:, :1])))
-
-    def run(sur, prec):
-        ctx.set_precision(prec)
-        ctx.set_potential(*guess_tables)
-        model.total = [x.copy() for x in guess_tables]
-        model.philP = [tuple(x.copy() for x in guess_tables)]
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        hist, kinds, t_train, t_kind = [], [], 0.0, {}
-        prev, out = None, None
-        phase = "collect" if sur else "plain"
-        n_sur = 0
-        for it in range(max_iter):
-            if sur and it == collect_from:
-                ctx.surrogate_init(seed=7, capacity=1 << 20)
-                ctx.surrogate_collect(keep)
-            if phase == "collect" and it == serve_from:
-                ctx.surrogate_collect(1.0)            # from now on: the judged nodes
-                tt = time.perf_counter()
-                ctx.surrogate_train(epochs, batch=32768, lr=3e-3)
-                t_train += time.perf_counter() - tt
-                ctx.surrogate_mode(True, judge=judge, jz_min=1e-3)
-                phase = "sur"
-            if phase == "sur":
-                saved = ([x.copy() for x in model.total], [tuple(x.copy() for x in p) for p in model.philP])
-            ti = time.perf_counter()
-            out = model.iterate()[0]
-            ti = 1e3 * (time.perf_counter() - ti)
-            t_kind.setdefault("S" if phase == "sur" else "P", []).append(ti)
-            if phase == "sur":
-                js = ctx.surrogate_stats()
-                if max(js["judge_werr_r_med"], js["judge_werr_z_med"]) > gate:
-                    kinds.append("R")                 # rejected by the judge
-                    ctx.surrogate_mode(False)
-                    ctx.surrogate_collect(0.0)
-                    model.total, model.philP = saved
-                    ctx.set_potential(*model.total)
-                    phase = "plain"
-                    continue
-            kinds.append("S" if phase == "sur" else "P")
-            d = drho(out, prev) if prev is not None else None
-            hist.append(d)
-            prev = out
-            if phase == "sur":
-                n_sur += 1
-                if (d is not None and d < sur_floor) or n_sur >= max_sur:
-                    ctx.surrogate_mode(False)
-                    ctx.surrogate_collect(0.0)
-                    phase = "plain"
-                else:
-                    tt = time.perf_counter()
-                    ctx.surrogate_train(1, batch=32768, lr=1e-3)
-                    t_train += time.perf_counter() - tt
-            elif phase == "plain" and d is not None and d < tol:
-                break
-        torch.cuda.synchronize()
-        dt = time.perf_counter() - t0
-        vir = ctx.virial(out["rhl"], out["vrotl"], out["sigRl"], out["sigpl"], out["sigzl"])
-        st = ctx.surrogate_stats() if sur else None
-        if sur:
-            ctx.surrogate_mode(False)
-        ctx.set_precision(fj.FJ_PREC_FP64)
-        return dt, out, vir, st, hist, "".join(kinds), t_train, {k: float(np.median(v)) for k, v in t_kind.items()}
+    def run(with_sur, prec):
+        return converge(ctx, model, guess_tables, prec, tol, max_iter, sur if with_sur else None)
 
     t_p, out_p, vir_p, _, h_p, k_p, _, i_p = run(False, fj.FJ_PREC_FP64)
     log("converged plain FP64: %d iterations, %.2f s" % (len(k_p), t_p))
@@ -331,6 +360,66 @@ def surrogate_leg(ctx, model, guess_tables, tol=1e-4, max_iter=30, keep=0.15, ep
                     "training: %d epochs "
                     "before iteration %d, then one pass per served iteration" % (100 * judge, 100 * gate, epochs,
                                                                                   serve_from)}
+
+
+def cfg5_leg(tol=1e-4, max_iter=30, keep=0.065, epochs=16, judge=0.05, gate=0.05):
+    """Learned-surrogate configuration: the rotating NFW model, 96 log radii
+    in [0.01, 100] x 32 cos(theta) nodes, l <= 12 (npoly 7), NFW DPL f(J)
+    with chi = 1 and the README rotation form (k = 0.5), NFW guess potential
+    and its Delta table, converged from the guess potential to `tol` three
+    ways, all with the block-scan Poisson solve (with the reference's
+    QUADPACK integrals, epsrel 1e-3, successive rho_l of this model keep
+    differing by 2-10 % of the monopole for 30 iterations: the solve's
+    adaptive decisions change from one potential to the next): plain FP64;
+    the fast node phase; and
+    the surrogate schedule: Staeckel (mixed precision) iterations 0-2,
+    keeping a random `keep` of the bound nodes of iterations 1 and 2 as
+    training samples (about 4 M), training `epochs` passes before iteration
+    3, serving iterations 3.. with the judge recomputing a random `judge`
+    fraction of nodes; a served iteration whose density-weighted median
+    action error exceeds `gate` is discarded and the run continues with
+    Staeckel iterations, as do the last iterations to `tol`.  Reports times,
+    iteration kinds, rho_l and Phi_l at convergence against the plain run
+    (scaled by the monopole), the virial ratios and the judge's figures."""
+    import numpy as np
+    import fjmodels_amd as fj
+    ctx = fj.Context(0)
+    ar = fj.log_grid(96, 0.01, 100.0)
+    comp = {"model": "NFW", "mass": 1.0, "r0": 1.0, "q": 1.0, "df": fj.df_params("NFW", chi=1.0, rot_form=1)}
+    model = fj.Model(ctx, ar, 32, 7, [comp], poisson_mode=fj.FJ_POISSON_BLOCKSCAN)
+    model.setup()
+    guess = [x.copy() for x in model.total]
+    sur = dict(keep=keep, epochs=epochs, judge=judge, gate=gate, max_sur=8, sur_floor=4e-3, collect_from=1,
+               serve_from=3, capacity=1 << 22)
+
+    def dphi(a, b):
+        return float(np.max(np.abs(a["phil"] - b["phil"]) / np.abs(a["phil"][:, :1])))
+
+    t_p, out_p, vir_p, _, h_p, k_p, _, i_p = converge(ctx, model, guess, fj.FJ_PREC_FP64, tol, max_iter)
+    log("cfg5 plain FP64: %d iterations, %.2f s" % (len(k_p), t_p))
+    t_f, out_f, vir_f, _, h_f, k_f, _, i_f = converge(ctx, model, guess, fj.FJ_PREC_FAST, tol, max_iter)
+    log("cfg5 fast: %d iterations, %.2f s" % (len(k_f), t_f))
+    t_s, out_s, vir_s, st, h_s, k_s, t_tr, i_s = converge(ctx, model, guess, fj.FJ_PREC_MIXED, tol, max_iter, sur)
+    log("cfg5 surrogate: %s, %.2f s" % (k_s, t_s))
+    ctx.close()
+    return {"workload": "NFW DPL f(J), chi 1, k 0.5 (README form), 96 log radii [0.01,100] x 32 GL cos(theta), "
+                        "l<=12, 22^3 nodes/point, guess potential + Delta table, block-scan Poisson",
+            "tol_rho_l": tol, "ms_plain": 1e3 * t_p, "ms_fast": 1e3 * t_f, "ms_surrogate": 1e3 * t_s,
+            "speedup_fast": t_p / t_f, "speedup_surrogate": t_p / t_s,
+            "iterations": {"plain": k_p, "fast": k_f, "surrogate": k_s},
+            "ms_median_iteration": {"plain": i_p["P"], "fast": i_f["P"], "surrogate_run_staeckel": i_s["P"],
+                                    "surrogate_run_served": i_s.get("S")},
+            "ms_training": 1e3 * t_tr, "training_samples": st["n_train"],
+            "rho_l_diff_fast": drho(out_f, out_p), "phi_l_diff_fast": dphi(out_f, out_p),
+            "rho_l_diff_surrogate": drho(out_s, out_p), "phi_l_diff_surrogate": dphi(out_s, out_p),
+            "virial_WK": {"plain": float(vir_p[4]), "fast": float(vir_f[4]), "surrogate": float(vir_s[4])},
+            "judge_last_served_iteration": {k: st[k] for k in ("judge_werr_r_med", "judge_werr_z_med",
+                                                             "judge_err_r_med", "judge_err_r_p95",
+                                                             "judge_err_z_med", "judge_err_z_p95", "nodes_sur",
+                                                             "fallback", "judged")},
+            "judge_history": st["judge_history"],
+            "gate": gate,
+            "note": "iteration kinds: P Staeckel, S served by the network, R served and rejected by the judge"}
 
 
 def batch_leg(ctx, n_models=8, iters=2, seed=5):
@@ -433,6 +522,7 @@ def main():
     ap.add_argument("--no-fast", action="store_true", help="skip the fast node phase leg")
     ap.add_argument("--no-surrogate", action="store_true", help="skip the plain vs surrogate convergence leg")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-models leg")
+    ap.add_argument("--no-cfg5", action="store_true", help="skip the NFW surrogate configuration leg")
     ap.add_argument("--sweep", type=int, default=32,
                     help="models of the seeded 256-model sweep draw to run (0: skip the sweep leg)")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -518,6 +608,9 @@ def main():
     if not args.no_surrogate:
         line["surrogate"] = surrogate_leg(ctx, model, guess_tables)
         log("convergence leg done")
+    if not args.no_cfg5:
+        line["cfg5"] = cfg5_leg()
+        log("cfg5 leg done")
     if not args.no_batch:
         line["batch"] = batch_leg(ctx)
         log("batch leg done")
