@@ -202,9 +202,11 @@ def fast_leg(ctx, model, prev, last, steps, pts):
 
 def sync():
     """torch.cuda.synchronize() when torch drives the device in this process
-    (the library's own calls return when their results are on the host)."""
-    import torch
-    if torch.cuda.is_initialized():
+    (the library's own calls return when their results are on the host);
+    torch is not imported here, so that a caller that has not loaded it
+    keeps a single HIP runtime in the process."""
+    torch = sys.modules.get("torch")
+    if torch is not None and torch.cuda.is_initialized():
         torch.cuda.synchronize()
 
 

@@ -14,9 +14,11 @@ pytestmark = pytest.mark.gpu
 
 
 def free_device_memory(lib):
+    """hipMemGetInfo of the HIP runtime the library itself is bound to (looked
+    up through the library's handle, which searches its dependencies), not of
+    another copy of the runtime that the process may also have loaded."""
     free, total = C.c_size_t(), C.c_size_t()
-    hip = C.CDLL("libamdhip64.so")
-    assert hip.hipMemGetInfo(C.byref(free), C.byref(total)) == 0
+    assert lib.hipMemGetInfo(C.byref(free), C.byref(total)) == 0
     return free.value
 
 
