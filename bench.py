@@ -77,9 +77,9 @@ def cpu_model():
 def cpu_baseline(npoly, ntheta, nr, ar, tables, delta, threads=None):
     """The oracle (the C restatement of the reference algorithm, built -O3 for
     x86-64-v3, libfjoracle_o3.so) timed on this host's cores on a bounded
-    sample of the same iteration: rhofDF on every 8th grid point of the
+    sample of the same iteration: rhofDF on every 4th grid point of the
     40 x 20 grid, grid-parallel (OpenMP dynamic over points, the stronger CPU
-    baseline), and on every 32nd point reference-structured (serial over
+    baseline), and on every 16th point reference-structured (serial over
     points, one OpenMP team over the 22^3 nodes of each, as
     integ/GaussQuad.cpp:160-180 and pot/potLeg.cpp:54-58 run it)."""
     import ctypes as C
@@ -102,7 +102,7 @@ def cpu_baseline(npoly, ntheta, nr, ar, tables, delta, threads=None):
     lib.fjo_pot_set_delta(h, *[O.ptr(x) for x in dl])
     df = O.df_params("Hernquist")
     out = {}
-    for variant, name, stride in ((0, "grid_parallel", 8), (1, "reference_structured", 32)):
+    for variant, name, stride in ((0, "grid_parallel", 4), (1, "reference_structured", 16)):
         n = C.c_int(0)
         dt = lib.fjo_time_grid(h, C.byref(df), threads, variant, stride, C.byref(n))
         out[name] = {"value": n.value / dt, "points": n.value, "seconds": dt}
@@ -110,7 +110,7 @@ def cpu_baseline(npoly, ntheta, nr, ar, tables, delta, threads=None):
     best = out["grid_parallel"]
     return {"value": best["value"], "unit": "grid-pt DF moments/s", "cores": threads, "kind": "port",
             "cpu": cpu_model(), "cpus_on_host": os.cpu_count(),
-            "sample": "rhofDF at %d of the %d grid points (every 8th) of iteration 0, grid-parallel OpenMP "
+            "sample": "rhofDF at %d of the %d grid points (every 4th) of iteration 0, grid-parallel OpenMP "
                       "(dynamic over points), %d threads, %.1f s; oracle C restatement built -O3 x86-64-v3"
                       % (best["points"], nr * ntheta, threads, best["seconds"]),
             "variants": out}
