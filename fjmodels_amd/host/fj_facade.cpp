@@ -177,14 +177,15 @@ fJParams readParam(const char *fName) {
             const std::string s = str();
             if (s == "fp64") P.precision = FJ_PREC_FP64;
             else if (s == "mixed") P.precision = FJ_PREC_MIXED;
-            else throw Error(FJ_E_ARG, "precision must be fp64 or mixed");
+            else if (s == "fast") P.precision = FJ_PREC_FAST;
+            else throw Error(FJ_E_ARG, "precision must be fp64, mixed or fast");
         }
     }
     checkParameterFile(&P);
     return P;
 }
 
-// checkParameterFile (readParam.cpp:110-150)
+// checkParameterFile (readParam.cpp:114-144)
 void checkParameterFile(const fJParams *f) {
     const bool one = (f->modName != "null" || (f->alpha != UNSET && f->beta != UNSET)) && f->chi != UNSET &&
                      f->mass != UNSET && f->r0 != UNSET && f->q != UNSET && f->dphi_h_in != UNSET &&

@@ -57,8 +57,9 @@ struct fJParams {
     //   tol: stop when the potential changes by less than tol
     //        (mergePhi's distance, include/UtilsLeg.h:266-278); 0 = run itermax
     //   poisson quadpack|blockscan
-    //   precision fp64|mixed: node phase in FP64 (default) or with FP32
-    //        action integrands and DF (fj_set_precision)
+    //   precision fp64|mixed|fast: node phase in FP64 (default), with FP32
+    //        action integrands and DF, or the fast node phase with
+    //        Gauss-Legendre action integrals (fj_set_precision)
     //   merge f: after each computeNewPhi, Phi <- (1+f) Phi - f Phi_old
     //        (mergePhi, include/UtilsLeg.h:266-278; one component); 0 = off
     //   delta_refresh 1: tabulate Delta again from each new potential (the
