@@ -170,3 +170,28 @@ def test_driver_line_profiles(tmp_path):
         for col, k in ((1, 0), (3, nr * 2 // 3), (5, nr * 5 // 6)):
             lp = p.line_profile(dfo, ref["rhl"], p.ar[k], p.ar[0], 3 * v[i])
             assert abs(rows[i][col] - lp) <= 5e-6 * abs(lp), (i, col, rows[i][col], lp)
+
+
+def test_driver_fast_precision_near_fp64(tmp_path):
+    """precision fast: the driver's fast node phase (the reference's turning
+    point walks, FP32 roots, 64-point Gauss-Legendre action integrals) on the
+    benchmark's radial grid (40 log radii in [0.01, 100], 4 cos(theta)
+    nodes) gives each iteration's coefficients within 2e-4 of the FP64 run's
+    monopole at the radius.  The rule's error grows with the radial cell
+    width (the potential is interpolated piecewise in r, so the integrands
+    have kinks): on 12 radii over the same decades it is 6e-3."""
+    param = PARAM.replace("nr 12", "nr 40").replace("rmin 0.05", "rmin 0.01").replace("rmax 50", "rmax 100")
+    runs = {}
+    for prec in ("fp64", "fast"):
+        d = tmp_path / prec
+        d.mkdir()
+        pf = tmp_path / f"param_{prec}.txt"
+        pf.write_text(param + f"precision {prec}\n")
+        run_driver(pf, d)
+        runs[prec] = [read_f64(d / f"e2e_{it}.out.f64", 40, 3)[1] for it in range(2)]
+    for it in range(2):
+        a, b = runs["fp64"][it], runs["fast"][it]
+        for n in ("rhl", "sigRl", "sigzl", "phil"):
+            err = np.max(np.abs(b[n] - a[n]) / np.abs(a[n][:, :1]))
+            print("iteration", it, n, "fast vs FP64:", err)
+            assert err < 2e-4, (it, n, err)
