@@ -514,6 +514,27 @@ def sweep_leg(ctx, n_models=32, seed=0, iters=2):
     return st
 
 
+def sweep_fast_leg(ctx, n_models=256, seed=0, iters=2):
+    """The model sweep under the fast node phase: the first n_models of the
+    seeded draw (all 256 by default) in one batch, since fast-phase batches
+    hold no node records; every phase one launch over all models (the node
+    phase one fused kernel), block-scan Poisson, graph replays after the
+    first iteration."""
+    import fjmodels_amd as fj
+    from fjmodels_amd import sweep as S
+    models = fj.draw_models(256, seed=seed)[:n_models]
+    ctx.set_precision(fj.FJ_PREC_FAST)
+    try:
+        sw = fj.Sweep(ctx, models)
+        _, st = sw.run(iters)
+    finally:
+        ctx.set_precision(fj.FJ_PREC_FP64)
+    st["unit"] = "grid-pt DF moments/s"
+    st["precision"] = "fast node phase (reference walks, FP32 roots, 64-point Gauss-Legendre actions)"
+    st["poisson"] = "blockscan"
+    return st
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -527,6 +548,8 @@ def main():
     ap.add_argument("--no-cfg5", action="store_true", help="skip the NFW surrogate configuration leg")
     ap.add_argument("--sweep", type=int, default=32,
                     help="models of the seeded 256-model sweep draw to run (0: skip the sweep leg)")
+    ap.add_argument("--sweep-fast", type=int, default=256,
+                    help="models of the sweep draw to run in one batch under the fast node phase (0: skip)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run a 1 GiB device-to-device copy (reference for FETCH_SIZE/WRITE_SIZE counters)")
     args = ap.parse_args()
@@ -619,6 +642,9 @@ def main():
     if args.sweep > 0:
         line["sweep"] = sweep_leg(ctx, args.sweep)
         log("sweep leg done")
+    if args.sweep_fast > 0:
+        line["sweep_fast"] = sweep_fast_leg(ctx, args.sweep_fast)
+        log("fast sweep leg done")
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(npoly, ntheta, nr, ar, guess_tables, delta)
         log("CPU baseline done")

@@ -180,6 +180,7 @@ class Context:
             raise FJError(f"fj_ctx_create failed ({rc}): no HIP device {device}?")
         self.h = h
         self.nr = self.ntheta = self.npoly = self.ngrid = 0
+        self.precision = FJ_PREC_FP64
 
     def close(self):
         if self.h:
@@ -206,6 +207,7 @@ class Context:
 
     def set_precision(self, mode):
         self._chk(self.lib.fj_set_precision(self.h, mode), "fj_set_precision")
+        self.precision = mode
 
     def set_fast_order(self, n):
         """Gauss-Legendre order of the FJ_PREC_FAST action integrals."""

@@ -1045,18 +1045,15 @@ struct FastArgs {
 #ifndef FJ_FAST_WAVES
 #define FJ_FAST_WAVES 4   // waves per SIMD the register allocation of k_fast aims at
 #endif
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(FJ_FAST_WAVES)))
-k_fast(TabArgs ta, FastArgs fa) {
-    extern __shared__ double smem[];
-    const LdsTabs L = stage_tables(ta, smem);
-    float *f32 = reinterpret_cast<float *>(smem + tab_lds_doubles(ta.nr, ta.npoly, ta.ngrid));
-    const PotView32 T32 = stage_tables32(L, f32, true);
-    double *red = reinterpret_cast<double *>(f32 + ((tab32_lds_floats(ta.nr, ta.npoly, true) + 1) & ~(size_t)1));
-    const int tiles = (fa.npts + 63) / 64;
-    const int tile = blockIdx.x % tiles, sl = blockIdx.x / tiles;
+// The work of one k_fast block: tile `tile` (64 points) of the fa.npts points
+// that start at global point pbase, node slice sl of fa.S; tables in L / T32,
+// red: 4 x 6 x 64 doubles of LDS.
+__device__ __forceinline__ void fast_block(const LdsTabs &L, const PotView32 &T32, double *red, const FastArgs &fa,
+                                           int tile, int sl, int pbase) {
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int p = tile * 64 + lane;
-    const bool valid = p < fa.npts;
+    const int pl = tile * 64 + lane;
+    const bool valid = pl < fa.npts;
+    const int p = pbase + pl;
     const int QS = (NQ + fa.S - 1) / fa.S;
     const int q0 = sl * QS, q1 = q0 + QS < NQ ? q0 + QS : NQ;
     double acc[6] = {0, 0, 0, 0, 0, 0};
@@ -1120,14 +1117,40 @@ k_fast(TabArgs ta, FastArgs fa) {
     }
 }
 
+// LDS of a fast-phase block: FP64 tables, FP32 tables with Phi_l', the
+// per-wave sums.
+__host__ __device__ inline size_t fast_lds_bytes(int nr, int npoly, int ngrid) {
+    return tab_lds_doubles(nr, npoly, ngrid) * sizeof(double) +
+           ((tab32_lds_floats(nr, npoly, true) + 1) & ~(size_t)1) * sizeof(float) + 4 * 6 * 64 * sizeof(double);
+}
+
+__device__ __forceinline__ double *fast_stage(const TabArgs &ta, double *smem, LdsTabs &L, PotView32 &T32) {
+    L = stage_tables(ta, smem);
+    float *f32 = reinterpret_cast<float *>(smem + tab_lds_doubles(ta.nr, ta.npoly, ta.ngrid));
+    T32 = stage_tables32(L, f32, true);
+    return reinterpret_cast<double *>(f32 + ((tab32_lds_floats(ta.nr, ta.npoly, true) + 1) & ~(size_t)1));
+}
+
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(FJ_FAST_WAVES)))
+k_fast(TabArgs ta, FastArgs fa) {
+    extern __shared__ double smem[];
+    LdsTabs L;
+    PotView32 T32;
+    double *red = fast_stage(ta, smem, L, T32);
+    const int tiles = (fa.npts + 63) / 64;
+    fast_block(L, T32, red, fa, blockIdx.x % tiles, blockIdx.x / tiles, 0);
+}
+
 // Sums of the S slices of each point, then the moments of rhofDF
 // (Integ.cpp:160-161) as k_moments forms them.  One block per point: lane t
 // adds slices t, t + 256, ... in order, then a pairwise tree over the lanes
 // (a fixed order).
+// quadp (batches): each point's velocity rule; else quad for all.
 __global__ void __launch_bounds__(256) k_fast_finish(int npts, int S, int quad, const double *part, double *out6,
-                                                     double *mom) {
+                                                     double *mom, const int *quadp = nullptr) {
     __shared__ double red[6][256];
     const int p = blockIdx.x, t = threadIdx.x;
+    if (quadp) quad = quadp[p];
     double o[6] = {0, 0, 0, 0, 0, 0};
     for (int s = t; s < S; s += 256)
         for (int m = 0; m < 6; m++) o[m] += part[((size_t)p * S + s) * 6 + m];
@@ -2061,6 +2084,24 @@ __global__ void __launch_bounds__(256) kb_moments(const ModelDev *md, const int 
     else moments_point(ma, p, red);
 }
 
+// Fast node phase over all models of a batch: block b takes global tile
+// b % ntiles (model by the tile prefix tstart) and node slice b / ntiles.
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(FJ_FAST_WAVES)))
+kb_fast(const ModelDev *md, const int *tstart, int M, int ntiles, FastArgs base) {
+    extern __shared__ double smem[];
+    const int t = blockIdx.x % ntiles, sl = blockIdx.x / ntiles;
+    const int m = find_seg(tstart, M, t);
+    const ModelDev &d = md[m];
+    LdsTabs L;
+    PotView32 T32;
+    double *red = fast_stage(d.ta, smem, L, T32);
+    FastArgs fa = base;
+    fa.npts = d.np;
+    fa.quad = d.quad;
+    fa.df = d.df;
+    fast_block(L, T32, red, fa, t - tstart[m], sl, d.po);
+}
+
 // rho[0][0] fix + get_Ylm for every model; entries [rstart[m], rstart[m+1]).
 __global__ void kb_project(const ModelDev *md, const int *rstart, int M, const double *mom, int npts_total) {
     const int idx = blockIdx.x * blockDim.x + threadIdx.x;
@@ -2534,14 +2575,20 @@ void fast_gl_table(int n, FastGL32 *t) {
 }
 
 // Fast node phase: k_points, k_fast, k_fast_finish; results in d_out6 and d_mom.
-int run_fast(fj_ctx *c, int npts) {
+// The Gauss-Legendre table of the fast phase's action rules, on the device.
+int fast_gl_ensure(fj_ctx *c) {
     if (!c->fgl_ready) {
         FastGL32 t[FAST_GL_MAX];
         fast_gl_table(c->fast_ngl, t);
         HIPCHK(c, hipMemcpyToSymbol(HIP_SYMBOL(c_fgl), t, sizeof(FastGL32) * c->fast_ngl));
         c->fgl_ready = true;
     }
+    return 0;
+}
+
+int run_fast(fj_ctx *c, int npts) {
     int r;
+    if ((r = fast_gl_ensure(c))) return r;
     if (!c->d_cnt) {
         if ((r = dalloc(c, &c->d_cnt, 16)) || (r = dalloc(c, &c->d_ctr, 1))) return r;
     }
@@ -2563,8 +2610,7 @@ int run_fast(fj_ctx *c, int npts) {
     }
     const TabArgs ta = tab_args(c);
     const size_t lds_pts = tab_bytes(c);
-    const size_t lds = tab_bytes(c) + ((tab32_lds_floats(c->nr, c->npoly, true) + 1) & ~(size_t)1) * sizeof(float) +
-                       4 * 6 * 64 * sizeof(double);
+    const size_t lds = fast_lds_bytes(c->nr, c->npoly, c->ngrid);
     hipEvent_t ev[4];
     for (int i = 0; i < 4; i++) HIPCHK(c, hipEventCreate(&ev[i]));
     HIPCHK(c, hipMemsetAsync(c->d_ctr, 0, sizeof(Counters), c->stream));
@@ -3755,6 +3801,13 @@ struct fj_batch {
     int per_cu64 = 1, per_cu32 = 1;                         // their resident blocks per CU
     int setup_bs = 256, integ_blocks = 0, integ_max_blocks = 0, ovf_blocks = 64;
     bool have_setup = false;
+    // fast node phase (a batch created under FJ_PREC_FAST): no node records;
+    // tiles of 64 points per model (prefix tstart), S node slices, slice sums
+    bool fast = false;
+    int *d_tstart = nullptr, *d_pquad = nullptr;
+    int tiles = 0, fast_S = 1, fast_ngl = 0;
+    double *d_fpart = nullptr;
+    size_t lds_fast = 0;
     hipGraph_t graph = nullptr;
     hipGraphExec_t gexec = nullptr;
     fj_batch_stats st{};
@@ -3787,6 +3840,19 @@ int batch_iteration(fj_batch *b) {
     hipStream_t s = c->stream;
     const int M = b->M;
     const bool mixed = c->prec == FJ_PREC_MIXED;
+    if (b->fast) {
+        kb_zero64<<<1, 64, 0, s>>>(reinterpret_cast<unsigned long long *>(b->d_ctr), (int)(sizeof(Counters) / 8));
+        PointArgs pa{b->d_R, b->d_z, b->d_Phi, b->d_Ve, b->d_Vc};
+        kb_points<<<(b->npts + 255) / 256, 256, 0, s>>>(b->d_md, b->d_pmodel, pa, b->npts);
+        FastArgs fa{b->d_R, b->d_z, b->d_Phi, b->d_Ve, 0, 0, b->fast_S, c->fast_ngl, fj_df_params{}, b->d_fpart,
+                    b->d_ctr};
+        kb_fast<<<(unsigned)(b->tiles * b->fast_S), 256, b->lds_fast, s>>>(b->d_md, b->d_tstart, M, b->tiles, fa);
+        k_fast_finish<<<b->npts, 256, 0, s>>>(b->npts, b->fast_S, 0, b->d_fpart, b->d_out6, b->d_mom, b->d_pquad);
+        kb_project<<<(b->rows + 63) / 64, 64, 0, s>>>(b->d_md, b->d_rstart, M, b->d_mom, b->npts);
+        kb_poisson<<<dim3(b->npoly_max, M), PSCAN_THREADS, 0, s>>>(b->d_md);
+        HIPCHK(c, hipGetLastError());
+        return 0;
+    }
     kb_zero<<<(8 * M + 8 + 255) / 256, 256, 0, s>>>(b->d_cnt, 8 * M + 8);
     kb_zero64<<<1, 64, 0, s>>>(reinterpret_cast<unsigned long long *>(b->d_ctr), (int)(sizeof(Counters) / 8));
     PointArgs pa{b->d_R, b->d_z, b->d_Phi, b->d_Ve, b->d_Vc};
@@ -3823,6 +3889,82 @@ int batch_iteration(fj_batch *b) {
     return 0;
 }
 
+// Grid points of every model (R = r sin theta, z = r cos theta at the Gauss-
+// Legendre cos theta nodes), the model table, point -> model map, row
+// prefixes and the Legendre constants, copied to the device (the fast-phase
+// batch; fj_batch_create does the same inline).
+int batch_points(fj_batch *b, const fj_model_spec *spec, const std::vector<int> &pmodel,
+                 const std::vector<int> &rstart) {
+    fj_ctx *c = b->c;
+    const int M = b->M;
+    std::vector<double> R(b->npts), z(b->npts);
+    for (int m = 0; m < M; m++) {
+        const ModelDev &d = b->hmd[m];
+        std::vector<double> hc(d.ntheta), hw(d.ntheta);
+        host_gauleg(d.ntheta, 0, 1, hc.data(), hw.data());
+        for (int n = 0; n < d.nr; n++)
+            for (int g = 0; g < d.ntheta; g++) {
+                R[d.po + n * d.ntheta + g] = spec[m].ar[n] * sqrt(1 - hc[g] * hc[g]);
+                z[d.po + n * d.ntheta + g] = spec[m].ar[n] * hc[g];
+            }
+    }
+    HIPCHK(c, hipMemcpy(b->d_md, b->hmd.data(), M * sizeof(ModelDev), hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(b->d_pmodel, pmodel.data(), pmodel.size() * sizeof(int), hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(b->d_rstart, rstart.data(), (M + 1) * sizeof(int), hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(b->d_R, R.data(), R.size() * sizeof(double), hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(b->d_z, z.data(), z.size() * sizeof(double), hipMemcpyHostToDevice));
+    const LegCoef lc = make_legcoef(MAXPOLY);
+    HIPCHK(c, hipMemcpyToSymbol(HIP_SYMBOL(c_legcoef), &lc, sizeof lc));
+    return 0;
+}
+
+// The rest of fj_batch_create for a fast-phase batch: no node records, no
+// integrator workspaces; tiles of 64 points per model and the slice sums.
+int batch_create_fast(fj_batch *b, const fj_model_spec *spec, const std::vector<int> &pmodel,
+                      std::vector<int> &rstart, size_t lds_max, fj_batch **out) {
+    fj_ctx *c = b->c;
+    const int M = b->M;
+    int r;
+    rstart.push_back(b->rows);
+    b->lds_tab = lds_max;
+    std::vector<int> tstart(M), pquad(b->npts);
+    int acc = 0;
+    for (int m = 0; m < M; m++) {
+        tstart[m] = acc;
+        acc += (b->hmd[m].np + 63) / 64;
+        for (int i = 0; i < b->hmd[m].np; i++) pquad[b->hmd[m].po + i] = b->hmd[m].quad;
+    }
+    b->tiles = acc;
+    hipDeviceProp_t prop;
+    HIPCHK(c, hipGetDeviceProperties(&prop, c->device));
+    b->cus = prop.multiProcessorCount;
+    int S = (FJ_FAST_BLOCKS_PER_CU * b->cus + b->tiles - 1) / b->tiles;
+    if (S > NQ / 16) S = NQ / 16;
+    if (S < 1) S = 1;
+    b->fast_S = S;
+    if ((r = balloc(b, &b->d_md, M)) || (r = balloc(b, &b->d_pv, M)) || (r = balloc(b, &b->d_pmodel, b->npts)) ||
+        (r = balloc(b, &b->d_rstart, M + 1)) || (r = balloc(b, &b->d_R, b->npts)) || (r = balloc(b, &b->d_z, b->npts)) ||
+        (r = balloc(b, &b->d_Phi, b->npts)) || (r = balloc(b, &b->d_Ve, b->npts)) || (r = balloc(b, &b->d_Vc, b->npts)) ||
+        (r = balloc(b, &b->d_out6, (size_t)6 * b->npts)) || (r = balloc(b, &b->d_mom, (size_t)6 * b->npts)) ||
+        (r = balloc(b, &b->d_cnt, (size_t)8 * M + 8)) || (r = balloc(b, &b->d_ctr, 1)) ||
+        (r = balloc(b, &b->d_backup, (size_t)3 * b->rows)) || (r = balloc(b, &b->d_tstart, M)) ||
+        (r = balloc(b, &b->d_pquad, b->npts)) || (r = balloc(b, &b->d_fpart, (size_t)b->npts * S * 6))) {
+        fj_batch_destroy(b);
+        return r;
+    }
+    if ((r = batch_points(b, spec, pmodel, rstart))) { fj_batch_destroy(b); return r; }
+    HIPCHK(c, hipMemcpy(b->d_tstart, tstart.data(), M * sizeof(int), hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(b->d_pquad, pquad.data(), b->npts * sizeof(int), hipMemcpyHostToDevice));
+    b->st.nodes = b->nodes;
+    b->st.points = b->npts;
+    // each model's table view (the Delta fit of fj_batch_setup reads them)
+    kb_potviews<<<(M + 63) / 64, 64, 0, c->stream>>>(b->d_md, M, b->d_pv);
+    HIPCHK(c, hipGetLastError());
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    *out = b;
+    return 0;
+}
+
 // Installed tables of every model <-> backup (autotuning restores them).
 int batch_save(fj_batch *b, bool restore) {
     size_t off = 0;
@@ -3851,6 +3993,7 @@ int fj_batch_create(fj_ctx *c, int M, const fj_model_spec *spec, fj_batch **out)
     fj_batch *b = new fj_batch();
     b->c = c;
     b->M = M;
+    b->fast = c->prec == FJ_PREC_FAST;
     int r = 0;
     const int NG = 30;
     std::vector<int> pmodel, bstart[2], rstart;
@@ -3921,9 +4064,12 @@ int fj_batch_create(fj_ctx *c, int M, const fj_model_spec *spec, fj_batch **out)
         const size_t l32 = lt + (tab32_lds_floats(nr, np_) + 20 * 256) * sizeof(float);
         b->lds_integ = l64 > b->lds_integ ? l64 : b->lds_integ;
         b->lds_integ32 = l32 > b->lds_integ32 ? l32 : b->lds_integ32;
+        const size_t lf = fast_lds_bytes(nr, np_, NG);
+        b->lds_fast = lf > b->lds_fast ? lf : b->lds_fast;
     }
     if (r) { fj_batch_destroy(b); return r; }
     b->nodes = (long long)b->npts * NQ;
+    if (b->fast) return batch_create_fast(b, spec, pmodel, rstart, lds_max, out);
     if (b->nodes > (64ll << 20)) {
         const long long n = b->nodes;
         fj_batch_destroy(b);
@@ -4054,6 +4200,10 @@ int fj_batch_autotune(fj_batch *b) {
     if (!b) return FJ_E_ARG;
     if (!b->have_setup) return set_err(b->c, FJ_E_STATE, "fj_batch_setup first");
     fj_ctx *c = b->c;
+    if (b->fast) {   // one launch shape: nothing to tune
+        b->st.ms_tune = 0;
+        return 0;
+    }
     int r;
     hipEvent_t e0, e1, t0, t1;
     HIPCHK(c, hipEventCreate(&e0)); HIPCHK(c, hipEventCreate(&e1));
@@ -4096,6 +4246,15 @@ int fj_batch_iterate(fj_batch *b, int niter) {
     if (!b->have_setup) return set_err(b->c, FJ_E_STATE, "fj_batch_setup first");
     fj_ctx *c = b->c;
     int r;
+    if (b->fast != (c->prec == FJ_PREC_FAST))
+        return set_err(c, FJ_E_STATE, b->fast ? "batch created under FJ_PREC_FAST: iterate it under FJ_PREC_FAST"
+                                              : "batch created under FJ_PREC_FP64 / FJ_PREC_MIXED: "
+                                                "FJ_PREC_FAST needs a batch created under it");
+    if (b->fast) {
+        if ((r = fast_gl_ensure(c))) return r;
+        if (b->gexec && b->fast_ngl != c->fast_ngl) batch_drop_graph(b);   // the order is in the graph
+        b->fast_ngl = c->fast_ngl;
+    }
     if (!b->gexec) {
         HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
         r = batch_iteration(b);

@@ -12,11 +12,13 @@ import time
 
 import numpy as np
 
-from .capi import Batch, df_params
+from .capi import Batch, df_params, FJ_PREC_FAST
 from .scf import log_grid
 
 NODES_PER_POINT = 22 ** 3
 BATCH_NODE_LIMIT = 64 << 20
+# fast-phase batches keep per point only the grid arrays and a few slice sums
+FAST_BATCH_NODE_LIMIT = 1 << 40
 KINDS = ("Hernquist", "Isochrone", "NFW")
 
 
@@ -73,8 +75,14 @@ class Sweep:
     first replay includes the capture), setup time (guess potentials,
     Poisson, Delta tables) apart."""
 
-    def __init__(self, ctx, models, node_limit=BATCH_NODE_LIMIT, autotune=False):
+    def __init__(self, ctx, models, node_limit=None, autotune=False):
+        """node_limit: velocity nodes per batch; by default BATCH_NODE_LIMIT,
+        or FAST_BATCH_NODE_LIMIT under FJ_PREC_FAST, whose batches hold no
+        node records (the whole 256-model draw is one batch)."""
         self.ctx, self.models, self.autotune = ctx, models, autotune
+        if node_limit is None:
+            fast = getattr(ctx, "precision", None) == FJ_PREC_FAST
+            node_limit = FAST_BATCH_NODE_LIMIT if fast else BATCH_NODE_LIMIT
         self.groups = partition(models, node_limit)
 
     def run(self, niter=2, progress=None):

@@ -42,3 +42,33 @@ def test_sweep_matches_single_models(ctx):
         ref = single(ctx, m, 2)
         for n in NAMES + ["Egrid", "Dgrid"]:
             assert np.array_equal(got[k][n], ref[n]), (k, n, np.max(np.abs(got[k][n] - ref[n])))
+
+
+def test_fast_sweep_one_batch_matches_single_models(ctx):
+    """Under FJ_PREC_FAST the sweep packs every model into one batch (no node
+    records), one fast-phase launch over all models.  The per-node work is
+    the single-context fast phase's; the node slices differ (their number
+    follows the launch's tile count), so each point's FP64 sums are added in
+    another grouping: every table within 1e-11 of its monopole after two
+    iterations, the Delta tables bit for bit."""
+    import fjmodels_amd as fj
+    from fjmodels_amd import sweep as S
+    draw = fj.draw_models(24, seed=0)
+    order = sorted(range(len(draw)), key=lambda k: S.model_nodes(draw[k]))[:6]
+    ms = [draw[k] for k in sorted(order)]
+    ctx.set_precision(fj.FJ_PREC_FAST)
+    try:
+        sw = fj.Sweep(ctx, ms)
+        assert len(sw.groups) == 1
+        got, st = sw.run(2)
+        print("fast sweep", st)
+        for k, m in enumerate(ms):
+            ref = single(ctx, m, 2)
+            for n in ("Egrid", "Dgrid"):
+                assert np.array_equal(got[k][n], ref[n]), (k, n)
+            for n in NAMES:
+                scale = np.abs(ref[n if n != "vrotl" else "sigpl"][:, :1])
+                err = np.max(np.abs(got[k][n] - ref[n]) / scale)
+                assert err < 1e-11, (k, n, err)
+    finally:
+        ctx.set_precision(fj.FJ_PREC_FP64)
