@@ -23,7 +23,7 @@ SLOPES = {"Hernquist": (1.667, 5.0), "Isochrone": (0.0, 5.0), "NFW": (1.667, 3.0
 # Every symbol include/fjmodels.h declares.
 EXPORTS = [
     "fj_ctx_create", "fj_ctx_destroy", "fj_last_error", "fj_set_grid", "fj_set_quad_rule",
-    "fj_set_precision", "fj_set_fast_order", "fj_set_potential", "fj_get_potential", "fj_set_delta_table",
+    "fj_set_precision", "fj_set_fast_order", "fj_set_stream", "fj_set_potential", "fj_get_potential", "fj_set_delta_table",
     "fj_get_delta_table", "fj_set_df", "fj_guess_rhl", "fj_poisson", "fj_moments", "fj_rhofDF",
     "fj_rhofDF_noalloc",
     "fj_project_legendre", "fj_compute_new_phi", "fj_tabulate_delta", "fj_virial",
@@ -98,6 +98,7 @@ def load():
         "fj_set_grid": [v, i, _dp, i, i, i],
         "fj_set_quad_rule": [v, i],
         "fj_set_precision": [v, i],
+        "fj_set_stream": [v, v],
         "fj_set_fast_order": [v, i],
         "fj_set_potential": [v, _dp, _dp, _dp],
         "fj_get_potential": [v, _dp, _dp, _dp],
@@ -208,6 +209,12 @@ class Context:
     def set_precision(self, mode):
         self._chk(self.lib.fj_set_precision(self.h, mode), "fj_set_precision")
         self.precision = mode
+
+    def set_stream(self, stream):
+        """Order this context's calls on a caller's hipStream_t (an integer
+        handle of the HIP runtime the library is bound to), or on its own
+        stream (None)."""
+        self._chk(self.lib.fj_set_stream(self.h, C.c_void_p(stream) if stream else None), "fj_set_stream")
 
     def set_fast_order(self, n):
         """Gauss-Legendre order of the FJ_PREC_FAST action integrals."""

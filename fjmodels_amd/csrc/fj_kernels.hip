@@ -2227,7 +2227,8 @@ static __device__ __align__(256) unsigned char scratch[NA.total];
 // ===========================================================================
 struct fj_ctx {
     int device = 0;
-    hipStream_t stream = nullptr;
+    hipStream_t stream = nullptr;       // the stream every call is ordered on
+    hipStream_t own_stream = nullptr;   // the context's own (stream unless fj_set_stream gave another)
     int nr = 0, ntheta = 0, npoly = 0, ngrid = 0, quad = 0, prec = 0;
     bool have_pot = false, have_delta = false, have_df = false;
     std::vector<double> ar, ci, wi, si, poly;
@@ -2901,6 +2902,7 @@ int fj_ctx_create(int device, fj_ctx **out) {
         delete c;
         return FJ_E_HIP;
     }
+    c->own_stream = c->stream;
     // address of this device's scratch array (set-up, once per device)
     static std::mutex mu;
     static unsigned char *addr[64] = {};
@@ -2909,7 +2911,7 @@ int fj_ctx_create(int device, fj_ctx **out) {
         if (device < 64 && !addr[device]) {
             void *ptr = nullptr;
             if (hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)) != hipSuccess || !ptr) {
-                (void)hipStreamDestroy(c->stream);
+                (void)hipStreamDestroy(c->own_stream);
                 delete c;
                 return FJ_E_HIP;
             }
@@ -2918,6 +2920,14 @@ int fj_ctx_create(int device, fj_ctx **out) {
         c->noalloc = device < 64 ? addr[device] : nullptr;
     }
     *out = c;
+    return 0;
+}
+
+int fj_set_stream(fj_ctx *c, void *stream) {
+    if (!c) return FJ_E_ARG;
+    HIPCHK(c, hipSetDevice(c->device));
+    HIPCHK(c, hipStreamSynchronize(c->stream));   // work already queued stays ordered before the switch
+    c->stream = stream ? static_cast<hipStream_t>(stream) : c->own_stream;
     return 0;
 }
 
@@ -2937,7 +2947,7 @@ int fj_ctx_destroy(fj_ctx *c) {
                   c->mlp.h32[2], c->mlp.sk.part, c->d_wpk, c->d_wbias};
     for (void *p : sp)
         if (p) (void)hipFree(p);
-    if (c->stream) (void)hipStreamDestroy(c->stream);
+    if (c->own_stream) (void)hipStreamDestroy(c->own_stream);
     delete c;
     return 0;
 }

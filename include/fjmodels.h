@@ -94,6 +94,14 @@ typedef struct {
  * keeps this state in globals, include/Grid.h:19-24). */
 int fj_ctx_create(int device, fj_ctx **out);
 int fj_ctx_destroy(fj_ctx *ctx);
+/* The stream the context's calls are ordered on: a hipStream_t of the caller
+ * (of the HIP runtime this library is bound to, on the context's device), or
+ * NULL for the context's own stream.  Work already queued is finished first;
+ * the caller keeps ownership of its stream and must keep it alive while the
+ * context uses it.  No counterpart in the reference, which has no device
+ * streams: a context on a caller's stream takes the place of the per-run
+ * globals of include/Grid.h inside a caller's own stream-ordered work. */
+int fj_set_stream(fj_ctx *ctx, void *stream);
 const char *fj_last_error(const fj_ctx *ctx);
 
 /* Grid sizes and radii: replaces the compile-time NR/NGAUSS/NPOLY/NGRID of
