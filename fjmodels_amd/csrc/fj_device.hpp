@@ -458,16 +458,19 @@ FJ_HD __forceinline__ T qk_rescale(T err, T resabs, T resasc) {
 
 // Storage for the 2(NK-1) off-centre function values of one rule.  FvLocal
 // keeps them in the caller's frame; FvStrided points into memory shared by a
-// group of lanes (LDS on the device), element k at p[k * ld].
+// block of FV_LANES lanes (LDS on the device), element k at p[k * FV_LANES].
+// The stride is a compile-time constant so that every element's address is
+// the lane's base plus an immediate offset: with a run-time stride the
+// compiler precomputed the 20 addresses per lane and spilled them.
 template <int N>
 struct FvLocal {
     double v[N];
     FJ_HD double &operator[](int k) { return v[k]; }
 };
+constexpr int FV_LANES = 256;   // block size of every kernel that stores rule values in LDS
 struct FvStrided {
     double *p;
-    int ld;
-    FJ_HD double &operator[](int k) const { return p[k * ld]; }
+    FJ_HD double &operator[](int k) const { return p[k * FV_LANES]; }
 };
 
 // Kronrod rule with NK abscissae >= 0 (11: QK21, 8: QK15).  The integrand is
@@ -520,11 +523,10 @@ FJ_HD __forceinline__ QKOut qk_rule(F &f, double a, double b, FV &&fv) {
 
 // FP32 pieces may contract to fused multiply-adds (the FP64 code may not).
 #pragma clang fp contract(fast)
-// The 2(NK-1) off-centre values of one FP32 rule, element k at p[k * ld] (LDS).
+// The 2(NK-1) off-centre values of one FP32 rule, element k at p[k * FV_LANES] (LDS).
 struct FvStrided32 {
     float *p;
-    int ld;
-    FJ_HD float &operator[](int k) const { return p[k * ld]; }
+    FJ_HD float &operator[](int k) const { return p[k * FV_LANES]; }
 };
 
 // qk_rule in FP32 (QUADPACK's single-precision qk21 / qk15): the same

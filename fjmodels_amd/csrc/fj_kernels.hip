@@ -849,7 +849,7 @@ k_integrals(TabArgs ta, IntegArgs ia) {
     const LdsTabs L = stage_tables(ta, smem);
     const long long lane_g = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     const QagsWork w = lane_work(ia.ws, ia.cap, lane_g, (long long)gridDim.x * blockDim.x);
-    const FvStrided fv{smem + tab_lds_doubles(ta.nr, ta.npoly, ta.ngrid) + threadIdx.x, (int)blockDim.x};
+    const FvStrided fv{smem + tab_lds_doubles(ta.nr, ta.npoly, ta.ngrid) + threadIdx.x};
     JInt f;
     f.T = &L.T;
     integrals_loop(ia, f, w, fv);
@@ -870,7 +870,7 @@ k_integrals_f32(TabArgs ta, IntegArgs ia) {
     const long long lane_g = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     const QagsWorkT<float> w =
         lane_work(reinterpret_cast<float *>(ia.ws), ia.cap, lane_g, (long long)gridDim.x * blockDim.x);
-    const FvStrided32 fv{f32 + tab32_lds_floats(ta.nr, ta.npoly) + threadIdx.x, (int)blockDim.x};
+    const FvStrided32 fv{f32 + tab32_lds_floats(ta.nr, ta.npoly) + threadIdx.x};
     JInt32 f;
     f.T = &T32;
     integrals_loop(ia, f, w, fv);
@@ -2024,7 +2024,7 @@ kb_integrals(const ModelDev *md, const int *block_model, int M, IntegArgs base, 
         const LdsTabs L = stage_tables(d.ta, smem);
         const IntegArgs ia = batch_integ_args<JInt>(base, d, cnt, m);
         const QagsWork w = lane_work(ia.ws, ia.cap, lane_g, (long long)gridDim.x * blockDim.x);
-        const FvStrided fv{smem + tab_lds_doubles(d.ta.nr, d.ta.npoly, d.ta.ngrid) + threadIdx.x, (int)blockDim.x};
+        const FvStrided fv{smem + tab_lds_doubles(d.ta.nr, d.ta.npoly, d.ta.ngrid) + threadIdx.x};
         JInt f;
         f.T = &L.T;
         integrals_loop(ia, f, w, fv);
@@ -2049,7 +2049,7 @@ kb_integrals_f32(const ModelDev *md, const int *block_model, int M, IntegArgs ba
         const IntegArgs ia = batch_integ_args<JInt32>(base, d, cnt, m);
         const QagsWorkT<float> w =
             lane_work(reinterpret_cast<float *>(ia.ws), ia.cap, lane_g, (long long)gridDim.x * blockDim.x);
-        const FvStrided32 fv{f32 + tab32_lds_floats(d.ta.nr, d.ta.npoly) + threadIdx.x, (int)blockDim.x};
+        const FvStrided32 fv{f32 + tab32_lds_floats(d.ta.nr, d.ta.npoly) + threadIdx.x};
         JInt32 f;
         f.T = &T32;
         integrals_loop(ia, f, w, fv);
@@ -2066,7 +2066,7 @@ __global__ void __launch_bounds__(256) kb_integrals_ovf(IntegArgs ia) {
     extern __shared__ double smem[];
     const long long lane_g = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     const QagsWork w = lane_work(ia.ws, ia.cap, lane_g, (long long)gridDim.x * blockDim.x);
-    const FvStrided fv{smem + threadIdx.x, (int)blockDim.x};
+    const FvStrided fv{smem + threadIdx.x};
     JInt f;
     f.T = ia.pv;
     integrals_loop<true>(ia, f, w, fv);
