@@ -40,7 +40,7 @@ struct DUturn {   // duturnfn (uvOrb.cpp:74-79)
     PotView T;        // by value: the table pointers stay visible as LDS addresses
     const NodeConst *c;
     long long *nev;
-    FJ_HD double operator()(double u) const {
+    FJ_HD __forceinline__ double operator()(double u) const {
         double shu, chu;
         fjm_sinhcosh(u, &shu, &chu);
         const double shu2 = shu * shu;
@@ -57,7 +57,7 @@ struct Uturn {    // uturnfn (uvOrb.cpp:70-73) with dU (61-63)
     PotView T;        // by value: the table pointers stay visible as LDS addresses
     const NodeConst *c;
     int *nphi;        // potential evaluations, or null
-    FJ_HD double operator()(double u) const {
+    FJ_HD __forceinline__ double operator()(double u) const {
         double shu, chu;
         fjm_sinhcosh(u, &shu, &chu);
         const double shu2 = shu * shu;
@@ -73,7 +73,7 @@ struct Vturn {    // vturnfn (uvOrb.cpp:49-52) with dV (46-48)
     PotView T;        // by value: the table pointers stay visible as LDS addresses
     const NodeConst *c;
     int *nphi;        // potential evaluations, or null
-    FJ_HD double operator()(double v) const {
+    FJ_HD __forceinline__ double operator()(double v) const {
         double s, cs;
         fjm_sincos(v, &s, &cs);
         const double sv2 = s * s;
