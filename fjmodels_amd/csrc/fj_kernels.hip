@@ -1563,10 +1563,10 @@ __global__ void k_vir_sum(int nr, int ng, const double *ar, const double *Pr, in
 // include/Utils.h:217-263.  One lane per energy; RK8PD (Prince & Dormand
 // 1981) with the standard y-control step rule; orbit samples in global scratch.
 // ---------------------------------------------------------------------------
-__constant__ double c_rk_c[13] = {0.0, 1.0 / 18.0, 1.0 / 12.0, 1.0 / 8.0, 5.0 / 16.0, 3.0 / 8.0, 59.0 / 400.0,
+constexpr double c_rk_c[13] = {0.0, 1.0 / 18.0, 1.0 / 12.0, 1.0 / 8.0, 5.0 / 16.0, 3.0 / 8.0, 59.0 / 400.0,
                                   93.0 / 200.0, 5490023248.0 / 9719169821.0, 13.0 / 20.0,
                                   1201146811.0 / 1299019798.0, 1.0, 1.0};
-__constant__ double c_rk_a[13][12] = {
+constexpr double c_rk_a[13][12] = {
     {0},
     {1.0 / 18.0},
     {1.0 / 48.0, 1.0 / 16.0},
@@ -1591,11 +1591,11 @@ __constant__ double c_rk_a[13][12] = {
     {403863854.0 / 491063109.0, 0.0, 0.0, -5068492393.0 / 434740067.0, -411421997.0 / 543043805.0,
      652783627.0 / 914296604.0, 11173962825.0 / 925320556.0, -13158990841.0 / 6184727034.0,
      3936647629.0 / 1978049680.0, -160528059.0 / 685178525.0, 248638103.0 / 1413531060.0, 0.0}};
-__constant__ double c_rk_b8[13] = {14005451.0 / 335480064.0, 0.0, 0.0, 0.0, 0.0, -59238493.0 / 1068277825.0,
+constexpr double c_rk_b8[13] = {14005451.0 / 335480064.0, 0.0, 0.0, 0.0, 0.0, -59238493.0 / 1068277825.0,
                                    181606767.0 / 758867731.0, 561292985.0 / 797845732.0,
                                    -1041891430.0 / 1371343529.0, 760417239.0 / 1151165299.0,
                                    118820643.0 / 751138087.0, -528747749.0 / 2220607170.0, 1.0 / 4.0};
-__constant__ double c_rk_b7[13] = {13451932.0 / 455176623.0, 0.0, 0.0, 0.0, 0.0, -808719846.0 / 976000145.0,
+constexpr double c_rk_b7[13] = {13451932.0 / 455176623.0, 0.0, 0.0, 0.0, 0.0, -808719846.0 / 976000145.0,
                                    1757004468.0 / 5645159321.0, 656045339.0 / 265891186.0,
                                    -3867574721.0 / 1518517206.0, 465885868.0 / 322736535.0,
                                    53011238.0 / 667516719.0, 2.0 / 45.0, 0.0};
@@ -1633,17 +1633,24 @@ __device__ inline void dl_evolve(const DeltaLane &d, double &t, double t1, doubl
         int final_step;
         if ((dt >= 0.0 && h0 > dt) || (dt < 0.0 && h0 < dt)) { h0 = dt; final_step = 1; }
         else final_step = 0;
+        // stages unrolled over compile-time coefficients: k stays in registers
+        // and the zero coefficients drop out (the sums skip them, as written)
+#pragma unroll
         for (int s = 1; s < 13; s++) {
+#pragma unroll
             for (int i = 0; i < 4; i++) {
                 double acc = 0;
+#pragma unroll
                 for (int j = 0; j < s; j++)
                     if (c_rk_a[s][j] != 0.0) acc += c_rk_a[s][j] * k[j][i];
                 yt[i] = y0[i] + h0 * acc;
             }
             dl_eqs(d, yt, k[s]);
         }
+#pragma unroll
         for (int i = 0; i < 4; i++) {
             double s8 = 0, s7 = 0;
+#pragma unroll
             for (int j = 0; j < 13; j++) {
                 if (c_rk_b8[j] != 0.0) s8 += c_rk_b8[j] * k[j][i];
                 if (c_rk_b7[j] != 0.0) s7 += c_rk_b7[j] * k[j][i];
@@ -1741,7 +1748,7 @@ struct DLds2dD2 {  // Delta.cpp:210-234 with getv 189-208
 
 struct DLRcE {   // Utils.h:243-246
     const PotView *T; double E;
-    __device__ double operator()(double R) const {
+    __device__ __forceinline__ double operator()(double R) const {
         double dR, dz;
         const double phi = pot_phi_grad(*T, R, 0, &dR, &dz);
         (void)phi;
@@ -1750,7 +1757,7 @@ struct DLRcE {   // Utils.h:243-246
 };
 struct DLRc {    // Utils.h:218-221
     const PotView *T; double Lzsq;
-    __device__ double operator()(double R) const { return Lzsq / fjm_powi(R, 3) - dl_dR(*T, R, 0); }
+    __device__ __forceinline__ double operator()(double R) const { return Lzsq / fjm_powi(R, 3) - dl_dR(*T, R, 0); }
 };
 
 // Utils.h:222-241 GetRc
@@ -1796,7 +1803,7 @@ __device__ inline void delta_energy(const PotView &T, int NG, const double *prep
 // One energy per wave (block of 64, lane 0 works): the energies' nested
 // Brent / RK8PD loops take different trip counts, and lanes of one wave
 // would wait for each other at every loop level.
-__global__ void k_delta(TabArgs ta, const double *prep, double *scratch, double *Eout, double *Dout, int *status) {
+__global__ void __launch_bounds__(64) k_delta(TabArgs ta, const double *prep, double *scratch, double *Eout, double *Dout, int *status) {
     extern __shared__ double smem[];
     const LdsTabs L = stage_tables(ta, smem);
     const int nE = blockIdx.x;
@@ -2166,7 +2173,7 @@ __global__ void kb_delta_prep(const ModelDev *md, const PotView *pv, int M) {
 }
 
 // One (model, energy) per wave, as k_delta.
-__global__ void kb_delta(const ModelDev *md, const PotView *pv, int M, double *scratch, int *status) {
+__global__ void __launch_bounds__(64) kb_delta(const ModelDev *md, const PotView *pv, int M, double *scratch, int *status) {
     const int i = blockIdx.x;
     const int NG = md[0].ta.ngrid;
     if (threadIdx.x != 0 || i >= M * NG) return;
