@@ -1603,6 +1603,7 @@ constexpr double c_rk_b7[13] = {13451932.0 / 455176623.0, 0.0, 0.0, 0.0, 0.0, -8
 struct DeltaLane {
     const PotView *T;
     double E, Lz, Rc, R0, D2, sqDR, Ri, zi, sv, cv, v0;
+    double Lzp2;       // pow(Lz, 2.0) of the equations of motion, formed once per energy
     double *Rp, *zp;   // [200] orbit samples (scratch)
     int N, status;
 };
@@ -1619,7 +1620,7 @@ __device__ inline void dl_eqs(const DeltaLane &d, const double *y, double *f) { 
     pot_phi_grad(*d.T, y[0], y[1], &dR, &dz);
     f[0] = y[2];
     f[1] = y[3];
-    f[2] = -dR + fjm_pow(d.Lz, 2.0) / fjm_pow(y[0], 3.0);
+    f[2] = -dR + d.Lzp2 / fjm_pow(y[0], 3.0);
     f[3] = -dz;
 }
 
@@ -1819,6 +1820,7 @@ __device__ inline void delta_energy(const PotView &T, int NG, const double *prep
     const double E = Ec1 + .5 * (t * t);
     DeltaLane d;
     d.T = &T; d.E = E; d.Lz = .25 * Lz1; d.status = 0;
+    d.Lzp2 = fjm_pow(d.Lz, 2.0);
     d.Rp = scratch; d.zp = d.Rp + 200;
     d.D2 = 0; d.sqDR = 0; d.Ri = 0; d.zi = 0; d.sv = 0; d.cv = 0; d.v0 = 0; d.N = 0;
     // Delta ctor: Rc = RcE(E, R0) (Utils.h:247-263)
