@@ -33,14 +33,30 @@ struct PotView32 {
     const float *Pr;   // Phi_l' table (staged for the fast phase only, else null)
 };
 
+// FP32 division and square root.  Fast = false: IEEE, correctly rounded (the
+// mixed path).  Fast = true: the hardware reciprocal and square root
+// (v_rcp_f32, v_sqrt_f32; about 1 ulp), for the fast node phase, whose error
+// is the fixed-order action rule's (1e-5 relative), not FP32 rounding.
+template <bool Fast>
+__device__ __forceinline__ float fdiv32(float a, float b) {
+    if constexpr (Fast) return a * __builtin_amdgcn_rcpf(b);
+    else return a / b;
+}
+template <bool Fast>
+__device__ __forceinline__ float fsqrt32(float x) {
+    if constexpr (Fast) return __builtin_amdgcn_sqrtf(x);
+    else return sqrtf(x);
+}
+
 // Phi(R, z) - shift.  The shift enters through the monopole, whose
 // interpolation runs on differences of the float-pair table, so that at the
 // bottom of a deep potential (shift close to Phi, E - Phi << |Phi|) the
 // result carries no cancellation.
+template <bool Fast = false>
 __device__ __forceinline__ float pot_phi32(const PotView32 &T, float R, float z, float shift) {
     const float r2 = R * R + z * z;
-    const float rs = sqrtf(r2);
-    const float c = z / rs;
+    const float rs = fsqrt32<Fast>(r2);
+    const float c = fdiv32<Fast>(z, rs);
     const int nr = T.nr, np_ = T.npoly;
     const float *ar = T.ar;
     const float c2 = c * c;
@@ -49,7 +65,7 @@ __device__ __forceinline__ float pot_phi32(const PotView32 &T, float R, float z,
     float r = rs;
     if (r >= ar[nr - 1]) {
         // Keplerian tail: phil[nr-1][l] (ar[nr-1]/r)^(l+1), l = 2np
-        const float q = ar[nr - 1] / r, q2 = q * q;
+        const float q = fdiv32<Fast>(ar[nr - 1], r), q2 = q * q;
         const float *last = T.phil + (nr - 1) * np_;
         phi = fmaf(last[0], q, -shift) + T.phil0lo[nr - 1] * q;
         float qp = q * q2;
@@ -69,14 +85,14 @@ __device__ __forceinline__ float pot_phi32(const PotView32 &T, float R, float z,
     while (bot > 0 && !(r >= ar[bot])) bot--;
     const int top = bot + 1;
     const float db = r - ar[bot];
-    const float f1 = db / (ar[top] - ar[bot]);
+    const float f1 = fdiv32<Fast>(db, ar[top] - ar[bot]);
     const bool quad = top < 10;
     float f2 = 0, f3 = 0;
     int thr = 0;
     if (quad) {
         thr = (f1 < 0.5f && bot > 0) ? bot - 1 : top + 1;
-        f2 = (r - ar[top]) * db / ((ar[thr] - ar[top]) * (ar[thr] - ar[bot]));
-        f3 = (ar[thr] - ar[bot]) / (ar[top] - ar[bot]);
+        f2 = fdiv32<Fast>((r - ar[top]) * db, (ar[thr] - ar[top]) * (ar[thr] - ar[bot]));
+        f3 = fdiv32<Fast>(ar[thr] - ar[bot], ar[top] - ar[bot]);
     }
     const float *pb = T.phil + bot * np_, *pt = T.phil + top * np_, *ph = T.phil + thr * np_;
     {
@@ -107,23 +123,25 @@ __device__ __forceinline__ float pot_phi32(const PotView32 &T, float R, float z,
 // the angular one from the Legendre derivatives, here by the recurrences
 // l P_l = (2l - 1) c P_l-1 - (l - 1) P_l-2 and P'_l = P'_l-2 + (2l - 1) P_l-1;
 // s and c from the clamped radius, as there.
+template <bool Fast = false>
 __device__ __forceinline__ float pot_grad32(const PotView32 &T, float R, float z, float shift, float *dR,
                                             float *dz) {
     const int nr = T.nr, np_ = T.npoly;
     const float *ar = T.ar;
-    const float rs = sqrtf(R * R + z * z);
+    const float rs = fsqrt32<Fast>(R * R + z * z);
     const bool kep = rs >= ar[nr - 1];
     const float r = rs < ar[0] ? ar[0] : rs;
-    const float c = z / r, s = R / r;
+    const float ir = fdiv32<Fast>(1.f, r);
+    const float c = Fast ? z * ir : z / r, s = Fast ? R * ir : R / r;
     float phi, g1, dthc = 0.f;
     int bot = 0, top = 0, thr = 0;
     float f1 = 0, f2 = 0, f3 = 0, q = 0, q2 = 0, qp = 0;
     bool quad = false;
     if (kep) {
-        q = ar[nr - 1] / r; q2 = q * q;
+        q = Fast ? ar[nr - 1] * ir : ar[nr - 1] / r; q2 = q * q;
         const float *last = T.phil + (nr - 1) * np_;
         phi = fmaf(last[0], q, -shift) + T.phil0lo[nr - 1] * q;
-        g1 = -last[0] * q / r;
+        g1 = Fast ? -last[0] * q * ir : -last[0] * q / r;
         qp = q * q2;
     } else {
         const float t = (__log2f(r) - T.lg0) * T.lsc;
@@ -132,12 +150,12 @@ __device__ __forceinline__ float pot_grad32(const PotView32 &T, float R, float z
         while (bot > 0 && !(r >= ar[bot])) bot--;
         top = bot + 1;
         const float db = r - ar[bot];
-        f1 = db / (ar[top] - ar[bot]);
+        f1 = fdiv32<Fast>(db, ar[top] - ar[bot]);
         quad = top < 10;
         if (quad) {
             thr = (f1 < 0.5f && bot > 0) ? bot - 1 : top + 1;
-            f2 = (r - ar[top]) * db / ((ar[thr] - ar[top]) * (ar[thr] - ar[bot]));
-            f3 = (ar[thr] - ar[bot]) / (ar[top] - ar[bot]);
+            f2 = fdiv32<Fast>((r - ar[top]) * db, (ar[thr] - ar[top]) * (ar[thr] - ar[bot]));
+            f3 = fdiv32<Fast>(ar[thr] - ar[bot], ar[top] - ar[bot]);
         }
         const float *lo = T.phil0lo;
         const float b = T.phil[bot * np_], tb = (T.phil[top * np_] - b) + (lo[top] - lo[bot]);
@@ -150,7 +168,8 @@ __device__ __forceinline__ float pot_grad32(const PotView32 &T, float R, float z
     float Pm2 = 0.f, Pm1 = 1.f, dPm2 = 0.f, dPm1 = 0.f;   // P_l-2, P_l-1 and their derivatives
     for (int np = 1; np < np_; np++) {
         for (int i = 2 * np - 1; i <= 2 * np; i++) {
-            const float P = ((float)(2 * i - 1) * c * Pm1 - (float)(i - 1) * Pm2) / (float)i;
+            const float P = Fast ? ((float)(2 * i - 1) * c * Pm1 - (float)(i - 1) * Pm2) * __builtin_amdgcn_rcpf((float)i)
+                                 : ((float)(2 * i - 1) * c * Pm1 - (float)(i - 1) * Pm2) / (float)i;
             const float dP = dPm2 + (float)(2 * i - 1) * Pm1;
             Pm2 = Pm1; Pm1 = P; dPm2 = dPm1; dPm1 = dP;
         }
@@ -158,7 +177,7 @@ __device__ __forceinline__ float pot_grad32(const PotView32 &T, float R, float z
         if (kep) {
             const float l = T.phil[(nr - 1) * np_ + np];
             v = l * qp;
-            d = -(float)(2 * np + 1) * v / r;
+            d = Fast ? -(float)(2 * np + 1) * v * ir : -(float)(2 * np + 1) * v / r;
             qp *= q2;
         } else {
             const float tv = T.phil[top * np_ + np], b = T.phil[bot * np_ + np];
@@ -173,16 +192,23 @@ __device__ __forceinline__ float pot_grad32(const PotView32 &T, float R, float z
         dthc += v * dPm1;
     }
     const float dth = -s * dthc;                          // dPhi/dtheta
-    *dR = dth * (c / r) + g1 * s;
-    *dz = dth * (-s / r) + g1 * c;
+    *dR = dth * (Fast ? c * ir : c / r) + g1 * s;
+    *dz = dth * (Fast ? -s * ir : -s / r) + g1 * c;
     return phi;
 }
 
 // sinh and cosh from one expm1 (accurate for small |u| too).
+template <bool Fast = false>
 __device__ __forceinline__ void sinhcosh32(float u, float *s, float *c) {
     const float em = expm1f(u), ep = em + 1.f;
-    *s = 0.5f * em * (em + 2.f) / ep;
-    *c = 0.5f * (ep + 1.f / ep);
+    if constexpr (Fast) {
+        const float ie = __builtin_amdgcn_rcpf(ep);
+        *s = 0.5f * em * (em + 2.f) * ie;
+        *c = 0.5f * (ep + ie);
+    } else {
+        *s = 0.5f * em * (em + 2.f) / ep;
+        *c = 0.5f * (ep + 1.f / ep);
+    }
 }
 
 // Juint / Jvint in FP32 with JInt's packing of the node constants, with the
@@ -256,12 +282,12 @@ struct FastFu32 {
     const FastNode *n;
     __device__ double operator()(double u) const {
         float sh, ch;
-        sinhcosh32((float)u, &sh, &ch);
+        sinhcosh32<true>((float)u, &sh, &ch);
         const float sh2 = sh * sh;
         const float c = (float)n->E, dE = (float)(n->E - (double)c);
         const float D = (float)n->Delta, sv0 = (float)n->sv0, cv0 = (float)n->cv0;
-        const float Pc = pot_phi32(*T, D * sh * sv0, D * ch * cv0, c) - dE;   // Phi - E
-        return (double)((float)n->Bu - (float)n->C / sh2 - (sh2 + sv0 * sv0) * Pc);
+        const float Pc = pot_phi32<true>(*T, D * sh * sv0, D * ch * cv0, c) - dE;   // Phi - E
+        return (double)((float)n->Bu - (float)n->C * __builtin_amdgcn_rcpf(sh2) - (sh2 + sv0 * sv0) * Pc);
     }
 };
 
@@ -271,14 +297,15 @@ struct FastDFu32 {
     const FastNode *n;
     __device__ double operator()(double u) const {
         float sh, ch;
-        sinhcosh32((float)u, &sh, &ch);
+        sinhcosh32<true>((float)u, &sh, &ch);
         const float sh2 = sh * sh;
         const float c = (float)n->E, dE = (float)(n->E - (double)c);
         const float D = (float)n->Delta, sv0 = (float)n->sv0, cv0 = (float)n->cv0;
         float dR, dz;
-        const float Pc = pot_grad32(*T, D * sh * sv0, D * ch * cv0, c, &dR, &dz) - dE;   // Phi - E
+        const float Pc = pot_grad32<true>(*T, D * sh * sv0, D * ch * cv0, c, &dR, &dz) - dE;   // Phi - E
         const float dPhidu = D * (dR * ch * sv0 + dz * sh * cv0);
-        return (double)(-2.f * Pc * sh * ch - (sh2 + sv0 * sv0) * dPhidu + 2.f * (float)n->C * ch / (sh2 * sh));
+        return (double)(-2.f * Pc * sh * ch - (sh2 + sv0 * sv0) * dPhidu +
+                        2.f * (float)n->C * ch * __builtin_amdgcn_rcpf(sh2 * sh));
     }
 };
 
@@ -292,8 +319,8 @@ struct FastFv32 {
         const float s2 = sn * sn;
         const float c = (float)n->E, dE = (float)(n->E - (double)c);
         const float D = (float)n->Delta, sh0 = (float)n->shu0, ch0 = (float)n->chu0;
-        const float Pc = pot_phi32(*T, D * sh0 * sn, D * ch0 * cs, c) - dE;   // Phi - E
-        return (double)((float)n->Bv - (float)n->C / s2 - (sh0 * sh0 + s2) * Pc);
+        const float Pc = pot_phi32<true>(*T, D * sh0 * sn, D * ch0 * cs, c) - dE;   // Phi - E
+        return (double)((float)n->Bv - (float)n->C * __builtin_amdgcn_rcpf(s2) - (sh0 * sh0 + s2) * Pc);
     }
 };
 
@@ -327,11 +354,11 @@ __device__ __forceinline__ void fast_actions32(const PotView32 &T, const FastNod
         for (int k = 0; k < ngl; k++) {
             const float u = ub - duf * gl[k].cu;
             float sh, ch;
-            sinhcosh32(u, &sh, &ch);
+            sinhcosh32<true>(u, &sh, &ch);
             const float sh2 = sh * sh;
-            const float Pc = pot_phi32(T, D * sh * sv0, D * ch * cv0, c) - dE;   // Phi - E
-            const float F = Bu - C / sh2 - (sh2 + sv0sq) * Pc;
-            s += gl[k].w * (sqrtf(fmaxf(0.f, F)) * gl[k].su);
+            const float Pc = pot_phi32<true>(T, D * sh * sv0, D * ch * cv0, c) - dE;   // Phi - E
+            const float F = Bu - C * __builtin_amdgcn_rcpf(sh2) - (sh2 + sv0sq) * Pc;
+            s += gl[k].w * (fsqrt32<true>(fmaxf(0.f, F)) * gl[k].su);
         }
         *Jr = rtwo * n.Delta * du * (double)s;
     }
@@ -346,9 +373,9 @@ __device__ __forceinline__ void fast_actions32(const PotView32 &T, const FastNod
             float sn, cs;
             sincosf(v, &sn, &cs);
             const float s2 = sn * sn;
-            const float Pc = pot_phi32(T, D * sh0 * sn, D * ch0 * cs, c) - dE;
-            const float F = Bv - C / s2 - (sh0sq + s2) * Pc;
-            s += gl[k].w * (sqrtf(fmaxf(0.f, F)) * gl[k].sv);
+            const float Pc = pot_phi32<true>(T, D * sh0 * sn, D * ch0 * cs, c) - dE;
+            const float F = Bv - C * __builtin_amdgcn_rcpf(s2) - (sh0sq + s2) * Pc;
+            s += gl[k].w * (fsqrt32<true>(fmaxf(0.f, F)) * gl[k].sv);
         }
         *Jz = rtwo * n.Delta * dv * (double)s;
     }
