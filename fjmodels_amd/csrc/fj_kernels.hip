@@ -1021,11 +1021,12 @@ __device__ inline void moments_point_f32(const MomArgs &ma, int pl, float (*red)
 // Fast node phase
 // ---------------------------------------------------------------------------
 constexpr int FAST_GL_MAX = 128;
-__constant__ FastGL32 c_fgl[FAST_GL_MAX];
+__constant__ FastGL32 c_fgl[FAST_GL_MAX];    // J_r rule (ngl points)
+__constant__ FastGL32 c_fglv[FAST_GL_MAX];   // J_z rule (fast_v_order(ngl) points)
 
 struct FastArgs {
     const double *R, *z, *Phi, *Ve;   // [npts]
-    int npts, quad, S, ngl;           // S slices of the velocity nodes per point
+    int npts, quad, S, ngl, nglv;     // S slices of the velocity nodes per point; the two rules' orders
     fj_df_params df;
     double *part;                     // [npts][S][6] slice sums
     Counters *ctr;
@@ -1076,9 +1077,9 @@ __device__ __forceinline__ void fast_block(const LdsTabs &L, const PotView32 &T3
             if (flag == 0) continue;   // H >= 0: f = 0 (DF.cpp:84)
             double Jr = 1e31, Jz = 1e31;
             if (flag == 1) {
-                fast_actions32(T32, fn, c_fgl, fa.ngl, &Jr, &Jz);
+                fast_actions32(T32, fn, c_fgl, fa.ngl, c_fglv, fa.nglv, &Jr, &Jz);
                 nbound++;
-                nev += (long long)(fn.ju + fn.jv) * fa.ngl;
+                nev += (long long)fn.ju * fa.ngl + (long long)fn.jv * fa.nglv;
             } else {
                 nunb++;
             }
@@ -2591,6 +2592,9 @@ int fast_gl_ensure(fj_ctx *c) {
         FastGL32 t[FAST_GL_MAX];
         fast_gl_table(c->fast_ngl, t);
         HIPCHK(c, hipMemcpyToSymbol(HIP_SYMBOL(c_fgl), t, sizeof(FastGL32) * c->fast_ngl));
+        const int nv = fast_v_order(c->fast_ngl);
+        fast_gl_table(nv, t);
+        HIPCHK(c, hipMemcpyToSymbol(HIP_SYMBOL(c_fglv), t, sizeof(FastGL32) * nv));
         c->fgl_ready = true;
     }
     return 0;
@@ -2630,7 +2634,8 @@ int run_fast(fj_ctx *c, int npts) {
     k_points<<<(npts + 255) / 256, 256, lds_pts, c->stream>>>(ta, pa, npts);
     HIPCHK(c, hipGetLastError());
     HIPCHK(c, hipEventRecord(ev[1], c->stream));
-    FastArgs fa{c->d_pR, c->d_pz, c->d_pPhi, c->d_pVe, npts, c->quad, S, c->fast_ngl, c->df, c->d_fpart, c->d_ctr};
+    FastArgs fa{c->d_pR, c->d_pz, c->d_pPhi, c->d_pVe, npts, c->quad, S, c->fast_ngl, fast_v_order(c->fast_ngl),
+                c->df, c->d_fpart, c->d_ctr};
     k_fast<<<(unsigned)(tiles * S), 256, lds, c->stream>>>(ta, fa);
     HIPCHK(c, hipGetLastError());
     HIPCHK(c, hipEventRecord(ev[2], c->stream));
@@ -3863,7 +3868,8 @@ int batch_iteration(fj_batch *b) {
         kb_zero64<<<1, 64, 0, s>>>(reinterpret_cast<unsigned long long *>(b->d_ctr), (int)(sizeof(Counters) / 8));
         PointArgs pa{b->d_R, b->d_z, b->d_Phi, b->d_Ve, b->d_Vc};
         kb_points<<<(b->npts + 255) / 256, 256, 0, s>>>(b->d_md, b->d_pmodel, pa, b->npts);
-        FastArgs fa{b->d_R, b->d_z, b->d_Phi, b->d_Ve, 0, 0, b->fast_S, c->fast_ngl, fj_df_params{}, b->d_fpart,
+        FastArgs fa{b->d_R, b->d_z, b->d_Phi, b->d_Ve, 0, 0, b->fast_S, c->fast_ngl, fast_v_order(c->fast_ngl),
+                    fj_df_params{}, b->d_fpart,
                     b->d_ctr};
         kb_fast<<<(unsigned)(b->tiles * b->fast_S), 256, b->lds_fast, s>>>(b->d_md, b->d_tstart, M, b->tiles, fa);
         k_fast_finish<<<b->npts, 256, 0, s>>>(b->npts, b->fast_S, 0, b->d_fpart, b->d_out6, b->d_mom, b->d_pquad);

@@ -268,7 +268,9 @@ __device__ __forceinline__ float df32(const DF32 &d, float Jr, float Jphi, float
 
 // Fast mode (fj_fast.hpp): the two action integrals as Gauss-Legendre rules
 // in FP32.  Per rule node k: weight w_k of the rule on [0, 1] and the fixed
-// cos / sin of psi_k = pi x_k (Ju) and psi_k = pi x_k / 2 (Jv).  Energies are
+// cos / sin of psi_k = pi x_k (Ju) and psi_k = pi x_k / 2 (Jv); the Ju rule
+// reads cu, su of its table, the Jv rule (of fast_v_order points) cv, sv of
+// its own.  Energies are
 // measured from c = fl(E): Phi - c comes from the float-pair monopole of
 // pot_phi32, and E - c is carried separately.
 struct FastGL32 {
@@ -338,7 +340,7 @@ struct FastPolicy32 {
 };
 
 __device__ __forceinline__ void fast_actions32(const PotView32 &T, const FastNode &n, const FastGL32 *gl, int ngl,
-                                               double *Jr, double *Jz) {
+                                               const FastGL32 *glv, int nglv, double *Jr, double *Jz) {
     const double rtwo = 1.4142135623730951;
     const float c = (float)n.E;
     const float dE = (float)(n.E - (double)c);
@@ -368,14 +370,14 @@ __device__ __forceinline__ void fast_actions32(const PotView32 &T, const FastNod
         const float sh0 = (float)n.shu0, ch0 = (float)n.chu0, sh0sq = sh0 * sh0;
         float s = 0.f;
 #pragma unroll 2
-        for (int k = 0; k < ngl; k++) {
-            const float v = 1.5707963f - dvf * gl[k].cv;
+        for (int k = 0; k < nglv; k++) {
+            const float v = 1.5707963f - dvf * glv[k].cv;
             float sn, cs;
             sincosf(v, &sn, &cs);
             const float s2 = sn * sn;
             const float Pc = pot_phi32<true>(T, D * sh0 * sn, D * ch0 * cs, c) - dE;
             const float F = Bv - C * __builtin_amdgcn_rcpf(s2) - (sh0sq + s2) * Pc;
-            s += gl[k].w * (fsqrt32<true>(fmaxf(0.f, F)) * gl[k].sv);
+            s += glv[k].w * (fsqrt32<true>(fmaxf(0.f, F)) * glv[k].sv);
         }
         *Jz = rtwo * n.Delta * dv * (double)s;
     }

@@ -55,10 +55,12 @@ enum { FJ_PREC_FP64 = 0, FJ_PREC_MIXED = 1, FJ_PREC_FAST = 2 };
  * uvOrb.cpp:25-31,70-73), the reference's turning-point bracket walks
  * (uvOrb.cpp:131-224) with wrp_GSLroots' endpoint test and FP64 regula falsi,
  * and the action integrals (uvOrb.cpp:226-262) as FP32 Gauss-Legendre rules
- * of fj_set_fast_order points (64 by default) instead of QAGS.  The action
- * surrogate does not run in this mode. */
+ * instead of QAGS: fj_set_fast_order points (64 by default) for J_r, half
+ * as many (at least 4) for the smoother J_z integrand.  The action surrogate
+ * does not run in this mode. */
 
-/* Gauss-Legendre order of the FJ_PREC_FAST action integrals (4..128). */
+/* Gauss-Legendre order of the FJ_PREC_FAST J_r integral (4..128); the J_z
+ * rule takes half of it, at least 4 points. */
 int fj_set_fast_order(fj_ctx *ctx, int n);
 
 /* DF parameters: df/DF.cpp:18-20,36-63 (alpha[2], dphi_h, dz_h, dphi_g, dz_g)

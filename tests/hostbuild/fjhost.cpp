@@ -267,7 +267,8 @@ int fjh_actions(int nr, const double *ar, int npoly, const double *phil, const d
 int fjh_fast_actions(int nr, const double *ar, int npoly, const double *phil, const double *Pr, int ng,
                      const double *Eg, const double *Dg, int n, const double *R, const double *z, const double *Phi,
                      const double *vR, const double *vphi, const double *vz, int ngl, const double *x,
-                     const double *w, double *Jr, double *Jz, int *nev, double *tp) {
+                     const double *w, int nglv, const double *xv, const double *wv, double *Jr, double *Jz, int *nev,
+                     double *tp) {
     HostPot H(nr, ar, npoly, phil, Pr, npoly);
     const PotView &T = H.T;
     for (int i = 0; i < n; i++) {
@@ -277,7 +278,7 @@ int fjh_fast_actions(int nr, const double *ar, int npoly, const double *phil, co
         const int flag = fast_setup<FastPolicy64>(T, T, Eg, Dg, ng, R[i], z[i], Phi[i], V, fn, &ne);
         double jr = 0, jz = 0;
         if (flag == 2) { jr = 1e31; jz = 1e31; }
-        else if (flag == 1) fast_actions64(T, fn, ngl, x, w, &jr, &jz);
+        else if (flag == 1) fast_actions64(T, fn, ngl, x, w, nglv, xv, wv, &jr, &jz);
         Jr[i] = jr; Jz[i] = jz;
         if (nev) nev[i] = ne;
         if (tp) { tp[3 * i] = fn.uin; tp[3 * i + 1] = fn.uout; tp[3 * i + 2] = fn.vt; }
@@ -291,7 +292,8 @@ int fjh_fast_rhofDF(int nr, const double *ar, int npoly, const double *phil, con
                     const double *Eg, const double *Dg, const double *xa, const double *wa, const double *xp,
                     const double *wp, double R, double z, double A, double B, double dphi_h, double dz_h,
                     double dphi_g, double dz_g, double mass, double J0, double chi, double k, int rot_form, int ngl,
-                    const double *gx, const double *gw, double *out6) {
+                    const double *gx, const double *gw, int nglv, const double *gxv, const double *gwv,
+                    double *out6) {
     HostPot H(nr, ar, npoly, phil, Pr, npoly);
     const PotView &T = H.T;
     const double far = 10 * ar[nr - 1];
@@ -308,7 +310,7 @@ int fjh_fast_rhofDF(int nr, const double *ar, int npoly, const double *phil, con
         const int flag = fast_setup<FastPolicy64>(T, T, Eg, Dg, ng, R, z, Phi_h, V, fn, &ne);
         if (flag == 0) continue;
         double jr = 1e31, jz = 1e31;
-        if (flag == 1) fast_actions64(T, fn, ngl, gx, gw, &jr, &jz);
+        if (flag == 1) fast_actions64(T, fn, ngl, gx, gw, nglv, gxv, gwv, &jr, &jz);
         const double Jphi = R * V[1];
         const double hh = jr + dphi_h * fabs(Jphi) + dz_h * jz, gg = jr + dphi_g * fabs(Jphi) + dz_g * jz;
         double DF = mass / fjm_powi(J0, 3) * fj_max(0., fjm_pow(1. + J0 / hh, A) / (fjm_pow(1. + gg / J0, B))) /

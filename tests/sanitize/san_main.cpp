@@ -18,7 +18,8 @@ int fjh_fast_rhofDF(int nr, const double *ar, int npoly, const double *phil, con
                     const double *Eg, const double *Dg, const double *xa, const double *wa, const double *xp,
                     const double *wp, double R, double z, double A, double B, double dphi_h, double dz_h,
                     double dphi_g, double dz_g, double mass, double J0, double chi, double k, int rot_form, int ngl,
-                    const double *glx, const double *glw, double *out6);
+                    const double *glx, const double *glw, int nglv, const double *glxv, const double *glwv,
+                    double *out6);
 }
 
 static fjo_pot *g_pot;
@@ -60,8 +61,9 @@ int main() {
     // host build of the device node code at two points, exact and fast paths
     double x11[22], w11[22], x01[22], w01[22];
     fjo_quad_tables(x11, w11, x01, w01);
-    std::vector<double> gx(64), gw(64);
+    std::vector<double> gx(64), gw(64), gxv(32), gwv(32);
     fjo_gauleg(64, 0.0, 1.0, gx.data(), gw.data());
+    fjo_gauleg(32, 0.0, 1.0, gxv.data(), gwv.data());
     const double pts[2][2] = {{0.5, 0.2}, {3.0, 1.0}};
     for (const auto &q : pts) {
         double a[6], b[6], o[6], vr, s1, s2, s3, s4;
@@ -70,7 +72,7 @@ int main() {
                    df.rot_form, a);
         fjh_fast_rhofDF(nr, ar.data(), npoly, phil.data(), Pr.data(), ngrid, E.data(), D.data(), x11, w11, x11, w11,
                         q[0], q[1], df.A, df.B, df.dphi_h, df.dz_h, df.dphi_g, df.dz_g, df.mass, df.J0, df.chi, df.k,
-                        df.rot_form, 64, gx.data(), gw.data(), b);
+                        df.rot_form, 64, gx.data(), gw.data(), 32, gxv.data(), gwv.data(), b);
         fjo_rhofDF(p, &df, q[0], q[1], o, &vr, &s1, &s2, &s3, &s4);
         for (int m = 0; m < 6; m++)
             if (a[m] != o[m]) return fail("host build differs from the oracle");

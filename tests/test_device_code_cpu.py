@@ -238,11 +238,13 @@ def test_fast_mode_actions_near_reference(H, models):
     returns, so the differences are quadrature-level (the QAGS tolerance is
     1e-3 relative)."""
     H.fjh_fast_actions.argtypes = [C.c_int, dp, C.c_int, dp, dp, C.c_int, dp, dp, C.c_int] + [dp] * 6 + \
-        [C.c_int, dp, dp, dp, dp, C.POINTER(C.c_int), dp]
+        [C.c_int, dp, dp, C.c_int, dp, dp, dp, dp, C.POINTER(C.c_int), dp]
     p = models["hq40"]
     phil, Pr, E, D = tabs(p)
     gx, gw = np.polynomial.legendre.leggauss(64)
     gx, gw = 0.5 * (gx + 1), 0.5 * gw
+    vx, vw = np.polynomial.legendre.leggauss(32)   # the J_z rule: half the order
+    vx, vw = 0.5 * (vx + 1), 0.5 * vw
     er, ez = [], []
     for R, z in [(0.011, 0.004), (0.866, 0.5), (8.0, 3.0), (60.0, 40.0)]:
         Phi, V, Dl = point_nodes(p, R, z)
@@ -251,7 +253,7 @@ def test_fast_mode_actions_near_reference(H, models):
         Fr, Fz = np.zeros(n), np.zeros(n)
         H.fjh_fast_actions(p.nr, P(p.ar), p.npoly, P(phil), P(Pr), p.ngrid, P(E), P(D), n, P(np.full(n, R)),
                            P(np.full(n, z)), P(np.full(n, Phi)), P(V[:, 0]), P(V[:, 1]), P(V[:, 2]), 64, P(gx),
-                           P(gw), P(Fr), P(Fz), None, None)
+                           P(gw), 32, P(vx), P(vw), P(Fr), P(Fz), None, None)
         for i in range(n):
             Jr, Jz = p.actions(p.deltafn(0.5 * np.sum(V[i] ** 2) + Phi), R, z, V[i, 0], V[i, 1], V[i, 2], Phi)
             if Jr > 1e30:

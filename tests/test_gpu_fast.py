@@ -62,18 +62,22 @@ def host_fast(p, R, z, ngl=64):
     subprocess.run(["make", "-s", "-C", HB], check=True)
     lib = C.CDLL(os.path.join(HB, "libfjhost.so"))
     lib.fjh_fast_rhofDF.argtypes = [C.c_int, dp, C.c_int, dp, dp, C.c_int, dp, dp, dp, dp, dp, dp, C.c_double,
-                                    C.c_double] + [C.c_double] * 9 + [C.c_double, C.c_int, C.c_int, dp, dp, dp]
+                                    C.c_double] + [C.c_double] * 9 + [C.c_double, C.c_int, C.c_int, dp, dp,
+                                                                      C.c_int, dp, dp, dp]
     phil, Pr, _ = p.tables()
     E, D = p.delta()
     gx, gw = np.polynomial.legendre.leggauss(ngl)
     gx, gw = 0.5 * (gx + 1), 0.5 * gw
+    nv = max(4, ngl // 2)   # the J_z rule of the fast phase: half the order
+    vx, vw = np.polynomial.legendre.leggauss(nv)
+    vx, vw = 0.5 * (vx + 1), 0.5 * vw
     d = O.df_params("Hernquist")
     out = np.zeros((len(R), 6))
     for i in range(len(R)):
         o = np.zeros(6)
         lib.fjh_fast_rhofDF(p.nr, P(p.ar), p.npoly, P(phil), P(Pr), p.ngrid, P(E), P(D), P(X11), P(W11), P(X11),
                             P(W11), R[i], z[i], d.A, d.B, d.dphi_h, d.dz_h, d.dphi_g, d.dz_g, d.mass, d.J0, d.chi,
-                            d.k, d.rot_form, ngl, P(gx), P(gw), P(o))
+                            d.k, d.rot_form, ngl, P(gx), P(gw), nv, P(vx), P(vw), P(o))
         out[i] = o
     return out
 
