@@ -624,10 +624,7 @@ struct QagsWorkT {
     FJ_HD Lanes<T> alist() const { return list(0); }
     FJ_HD Lanes<T> blist() const { return list(1); }
     FJ_HD Lanes<T> rlist() const { return list(2); }
-    // eord[i] = the error of interval iord[i]: dqagse's elist kept in the
-    // order of iord, so that dqpsrt compares against eord[i] instead of
-    // elist[iord[i]] (independent loads instead of a chain of dependent ones)
-    FJ_HD Lanes<T> eord() const { return list(3); }
+    FJ_HD Lanes<T> elist() const { return list(3); }
     FJ_HD Lanes<T> rlist2() const { return at(4 * l1()); }
     FJ_HD Lanes<T> res3la() const { return at(4 * l1() + ws_round(56, CH)); }
     FJ_HD Lanes<int> iord() const {
@@ -694,66 +691,51 @@ FJ_HD __forceinline__ double qags_solve(F &f, double a, double b, double epsabs,
 
 namespace fj {
 
-// QUADPACK dqpsrt (1-based lists) on the error list kept in sorted order:
-// every elist[iord[k]] of dqpsrt is eord[k], and each move of an index in
-// iord moves its error in eord with it.  On entry position nrmax holds
-// maxerr, whose error has just changed to emax (dqpsrt's errmax =
-// elist(maxerr)), and the new interval last has error emin (elist(last)).
-// The comparisons are dqpsrt's, on the same values.
+// QUADPACK dqpsrt (1-based lists).
 template <class T>
-FJ_HD __forceinline__ void qpsrt(int limit, int last, int &maxerr, T &ermax, T emax, T emin, const Lanes<T> &eord,
+FJ_HD __forceinline__ void qpsrt(int limit, int last, int &maxerr, T &ermax, const Lanes<T> &elist,
                                       const Lanes<int> &iord, int &nrmax) {
     int i, j, k, ido, ibeg, jbnd, isucc, jupbn;
-    T esucc;
-    const T errmax = emax, errmin = emin;
-    if (last <= 2) {
-        // last == 2 (maxerr == 1): elist(1) = emax, elist(2) = emin
-        iord[1] = 1; iord[2] = 2;
-        eord[1] = errmax; eord[2] = errmin;
-        goto L90;
-    }
+    T errmax, errmin;
+    if (last <= 2) { iord[1] = 1; iord[2] = 2; goto L90; }
+    errmax = elist[maxerr];
     if (nrmax != 1) {
         ido = nrmax - 1;
         for (i = 1; i <= ido; i++) {
             isucc = iord[nrmax - 1];
-            esucc = eord[nrmax - 1];
-            if (errmax <= esucc) break;
+            if (errmax <= elist[isucc]) break;
             iord[nrmax] = isucc;
-            eord[nrmax] = esucc;
             nrmax--;
         }
     }
     jupbn = last;
     if (last > (limit / 2 + 2)) jupbn = limit + 3 - last;
+    errmin = elist[last];
     jbnd = jupbn - 1;
     ibeg = nrmax + 1;
     if (ibeg <= jbnd) {
         for (i = ibeg; i <= jbnd; i++) {
             isucc = iord[i];
-            esucc = eord[i];
-            if (errmax >= esucc) goto L60;
+            if (errmax >= elist[isucc]) goto L60;
             iord[i - 1] = isucc;
-            eord[i - 1] = esucc;
         }
     }
-    iord[jbnd] = maxerr; eord[jbnd] = errmax;
-    iord[jupbn] = last; eord[jupbn] = errmin;
+    iord[jbnd] = maxerr;
+    iord[jupbn] = last;
     goto L90;
 L60:
-    iord[i - 1] = maxerr; eord[i - 1] = errmax;
+    iord[i - 1] = maxerr;
     k = jbnd;
     for (j = i; j <= jbnd; j++) {
         isucc = iord[k];
-        esucc = eord[k];
-        if (errmin < esucc) { iord[k + 1] = last; eord[k + 1] = errmin; goto L90; }
+        if (errmin < elist[isucc]) { iord[k + 1] = last; goto L90; }
         iord[k + 1] = isucc;
-        eord[k + 1] = esucc;
         k--;
     }
-    iord[i] = last; eord[i] = errmin;
+    iord[i] = last;
 L90:
     maxerr = iord[nrmax];
-    ermax = eord[nrmax];
+    ermax = elist[maxerr];
 }
 
 // QUADPACK dqelg (epsilon algorithm), epstab and res3la 1-based.
@@ -886,7 +868,7 @@ FJ_HD __forceinline__ void qags_consume(QagsT<T> &q, const QagsWorkT<T> &w, int 
         q.errbnd = fmax(q.epsabs, q.epsrel * q.dres);
         q.last = 1;
         w.alist()[1] = q.a0; w.blist()[1] = q.b0;
-        w.rlist()[1] = q.result; w.eord()[1] = q.abserr; w.iord()[1] = 1;
+        w.rlist()[1] = q.result; w.elist()[1] = q.abserr; w.iord()[1] = 1;
         if (q.abserr <= T(1.0e+02) * epmach * q.defabs && q.abserr > q.errbnd) q.ier = 2;
         if (q.limit == 1) q.ier = 1;
         if (q.ier != 0 || (q.abserr <= q.errbnd && q.abserr != resabs) || q.abserr == 0.0) { q.state = 3; return; }
@@ -936,17 +918,18 @@ FJ_HD __forceinline__ void qags_consume(QagsT<T> &q, const QagsWorkT<T> &w, int 
         w.blist()[last] = b1;
         w.rlist()[q.maxerr] = area2;
         w.rlist()[last] = area1;
+        w.elist()[q.maxerr] = error2;
+        w.elist()[last] = error1;
     } else {
         w.alist()[last] = a2;
         w.blist()[q.maxerr] = b1;
         w.blist()[last] = b2;
+        w.elist()[q.maxerr] = error1;
+        w.elist()[last] = error2;
     }
     {
-        // elist(maxerr), elist(last) of dqagse: the larger half's error stays
-        // at index maxerr
-        const T emax = error2 > error1 ? error2 : error1, emin = error2 > error1 ? error1 : error2;
         int maxerr = q.maxerr, nrmax = q.nrmax;
-        qpsrt(limit, last, maxerr, q.errmax, emax, emin, w.eord(), w.iord(), nrmax);
+        qpsrt(limit, last, maxerr, q.errmax, w.elist(), w.iord(), nrmax);
         q.maxerr = maxerr; q.nrmax = nrmax;
     }
     if (q.errsum <= q.errbnd) { qags_finish(q, w, 1); return; }
@@ -974,7 +957,7 @@ FJ_HD __forceinline__ void qags_consume(QagsT<T> &q, const QagsWorkT<T> &w, int 
                 if (last > (2 + limit / 2)) jupbnd = limit + 3 - last;
                 for (int k = id; k <= jupbnd; k++) {
                     q.maxerr = w.iord()[q.nrmax];
-                    q.errmax = w.eord()[q.nrmax];
+                    q.errmax = w.elist()[q.maxerr];
                     if (fabs(w.blist()[q.maxerr] - w.alist()[q.maxerr]) > q.small) { next = true; break; }
                     q.nrmax++;
                 }
@@ -1002,7 +985,7 @@ FJ_HD __forceinline__ void qags_consume(QagsT<T> &q, const QagsWorkT<T> &w, int 
             if (q.numrl2 == 1) q.noext = 1;
             if (q.ier == 5) { qags_finish(q, w, 0); return; }
             q.maxerr = w.iord()[1];
-            q.errmax = w.eord()[1];
+            q.errmax = w.elist()[q.maxerr];
             q.nrmax = 1;
             q.extrap = 0;
             q.small = q.small * T(0.5);
@@ -1021,7 +1004,7 @@ namespace fj {
 
 // QAGS workspace of nlanes lanes, interleaved by lane (element by element
 // or by chunks, FJ_WS_CHUNK_BYTES): per lane the doubles alist, blist,
-// rlist, eord [cap+1], epsilon table [56], res3la [4], then the ints
+// rlist, elist [cap+1], epsilon table [56], res3la [4], then the ints
 // iord [cap+1], each rounded up to whole chunks.  The float instance of the
 // same cap takes no more bytes than the double one.
 FJ_HD constexpr long long ws_lane_doubles(int cap) {
