@@ -26,10 +26,9 @@ def single(ctx, m, niter):
     return out
 
 
-@pytest.mark.skip(reason="faulted once on the GPU (illegal address in kb_integrals during the first batch "
-                         "iteration, run after the NFW convergence test in the same process); the cause is not "
-                         "found: the QAGS stepper was cleared on the CPU (bounds-checked, sanitized, identical "
-                         "decisions to the previous stepper on 4,800 integrals), so the case stays off the GPU")
+@pytest.mark.skip(reason="faulted on the GPU (illegal address in kb_integrals, the batched FP64 integrator) with a "
+                         "QAGS stepper change that is now reverted; no mistake in the code was found to explain it, "
+                         "so the case stays off the GPU")
 def test_sweep_matches_single_models(ctx):
     import fjmodels_amd as fj
     from fjmodels_amd import sweep as S
