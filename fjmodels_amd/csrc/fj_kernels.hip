@@ -29,6 +29,8 @@
 #include <type_traits>
 #include <vector>
 
+#include <hipcub/hipcub.hpp>
+
 #include "../../include/fjmodels.h"
 #include "fj_device.hpp"
 #include "fj_observe.h"
@@ -363,6 +365,41 @@ k_setup(TabArgs ta, SetupArgs sa) {
     if (!sa.sel && !sa.vR) tid = setup_lane_node(tid, sa.npts);
 #endif
     setup_node(L, sa, tid);
+}
+
+// Sort key of k_setup's lane order from the records of the previous node
+// phase: for a bound node, its turning points u_in and u_out on the scale of
+// GetTurnu's steps (ln 1.1) and its v turning point as the halvings of
+// GetTurnv from pi/4; 0 for H >= 0, 1 for E >= 0.  Launched in the order of
+// the sorted keys, a wave takes nodes whose bracket walks and Brent solves
+// run alike.  Any key gives the same results: the records stay node-indexed,
+// so only which lane computes a node changes.  Writes the identity into perm
+// and the node count into *nsel.
+constexpr int SETUP_KEY_BITS = 25;
+__global__ void __launch_bounds__(256) k_setup_key(NodeBuf nb, long long nn, unsigned int *key, int *perm,
+                                                   unsigned int *nsel) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i == 0) *nsel = (unsigned int)nn;
+    if (i >= nn) return;
+    perm[i] = (int)i;
+    const int flag = nb.flag[i];
+    unsigned int k = flag == 2 ? 1u : 0u;
+    if (flag == 1) {
+        const double ub = nb.f[NF_UBAR * nb.cap + i], du = nb.f[NF_DELTAU * nb.cap + i];
+        const double vt = nb.f[NF_VTURN * nb.cap + i];
+        // ln u / ln 1.1 + 256, clamped to [0, 511]
+        const auto ubin = [](double u) -> unsigned int {
+            const float t = u > 0 ? __logf((float)u) * 10.49205f + 256.f : 0.f;
+            return t > 0.f ? (t < 511.f ? (unsigned int)t : 511u) : 0u;
+        };
+        // halvings from pi/4 towards pi/2 (32 + level) or towards 0 (level), level in [0, 31]
+        const bool up = vt > 0.25 * PI;
+        const double d = up ? PIH - vt : vt;
+        const float lv = d > 0 ? __log2f((float)(0.25 * PI / d)) : 31.f;
+        const unsigned int lvl = lv > 0.f ? (lv < 31.f ? (unsigned int)lv : 31u) : 0u;
+        k = (1u << 24) + (ubin(ub + du) << 15) + (ubin(ub - du) << 6) + (up ? 32u : 0u) + lvl;
+    }
+    key[i] = k;
 }
 
 // df_hg (DF.cpp:65-76) and the rotation term (DF.cpp:114-119)
@@ -2252,10 +2289,15 @@ struct fj_ctx {
     long long node_cap = 0;
     double *d_nodef = nullptr, *d_Jr = nullptr, *d_Jz = nullptr;
     int *d_flag = nullptr, *d_list_u = nullptr, *d_list_v = nullptr, *d_ovf = nullptr;
-    unsigned int *d_cnt = nullptr;   // [0] u, [1] v, [2] next, [3] ovf, [4] next2
+    unsigned int *d_cnt = nullptr;   // [0] u, [1] v, [2] next, [3] ovf, [4] next2, [6..8] surrogate, [10] k_setup nodes
     Counters *d_ctr = nullptr;
     double *d_ws = nullptr; long long ws_lanes = 0; int ws_cap = 0;
     double *d_wsfull = nullptr; long long wsfull_lanes = 0;
+    // lane order of k_setup (k_setup_key): keys and node permutation, in and
+    // out of the radix sort [2][order_cap] each, the sort's scratch
+    unsigned int *d_okey = nullptr; int *d_operm = nullptr; long long order_cap = 0;
+    unsigned char *d_otemp = nullptr; size_t otemp_bytes = 0;
+    long long rec_nodes = -1;   // nodes of the point-major records the last node phase left (-1: none)
     // point buffers
     int pt_cap = 0;
     double *d_eval = nullptr; int eval_cap = 0;   // fj_eval_potential: R, z, Phi, dR, dz [eval_cap] each
@@ -2417,6 +2459,7 @@ int ensure_nodes(fj_ctx *c, long long nodes) {
             (r = dalloc(c, &c->d_ovf, (size_t)2 * nodes)))
             return r;
         c->node_cap = nodes;
+        c->rec_nodes = -1;
     }
     if (!c->d_cnt) {
         if ((r = dalloc(c, &c->d_cnt, 16)) || (r = dalloc(c, &c->d_ctr, 1))) return r;
@@ -2660,6 +2703,36 @@ int run_fast(fj_ctx *c, int npts) {
     return 0;
 }
 
+// k_setup's lane order: nodes sorted by k_setup_key over the records of the
+// previous node phase (radix sort on the stream); sets sa->sel / sa->nsel.
+int setup_order(fj_ctx *c, const NodeBuf &nb, long long nn, SetupArgs *sa) {
+    int r = 0;
+    if (nn > c->order_cap) {
+        if ((r = dalloc(c, &c->d_okey, (size_t)2 * nn)) || (r = dalloc(c, &c->d_operm, (size_t)2 * nn))) return r;
+        c->order_cap = nn;
+    }
+    unsigned int *kin = c->d_okey, *kout = c->d_okey + nn;
+    int *pin = c->d_operm, *pout = c->d_operm + nn;
+    size_t need = 0;
+    HIPCHK(c, hipcub::DeviceRadixSort::SortPairs(nullptr, need, kin, kout, pin, pout, (int)nn, 0, SETUP_KEY_BITS,
+                                                 c->stream));
+    if (need > c->otemp_bytes) {
+        if ((r = dalloc(c, &c->d_otemp, need))) return r;
+        c->otemp_bytes = need;
+    }
+    k_setup_key<<<(unsigned)((nn + 255) / 256), 256, 0, c->stream>>>(nb, nn, kin, pin, c->d_cnt + 10);
+    HIPCHK(c, hipGetLastError());
+    HIPCHK(c, hipcub::DeviceRadixSort::SortPairs(c->d_otemp, need, kin, kout, pin, pout, (int)nn, 0, SETUP_KEY_BITS,
+                                                 c->stream));
+    sa->sel = pout;
+    sa->nsel = c->d_cnt + 10;
+    return 0;
+}
+
+#ifndef FJ_SETUP_SORT
+#define FJ_SETUP_SORT 1   // k_setup in the order of the previous node phase's turning points
+#endif
+
 // Node phase for npts points already in d_pR/d_pz; results in d_out6 and d_mom.
 int run_nodes(fj_ctx *c, int npts) {
     if (!c->have_pot || !c->have_delta || !c->have_df)
@@ -2723,6 +2796,9 @@ int run_nodes(fj_ctx *c, int npts) {
             sa.nsel = c->d_cnt + 6;
         }
         HIPCHK(c, hipEventRecord(ev[2], c->stream));
+        // one chunk whose records the previous node phase left: lanes in the order of its turning points
+        if (FJ_SETUP_SORT && !sur && np == npts && c->rec_nodes == nn && (r = setup_order(c, nb, nn, &sa)))
+            return r;
         const long long nsetup = sur ? (long long)nsel : nn;
         if (nsetup > 0) k_setup<<<(unsigned)((nsetup + 255) / 256), 256, lds, c->stream>>>(ta, sa);
         HIPCHK(c, hipGetLastError());
@@ -2757,6 +2833,8 @@ int run_nodes(fj_ctx *c, int npts) {
     }
     HIPCHK(c, hipEventRecord(ev[7], c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
+    // the records hold every node's turning points when one chunk ran k_setup on all of them
+    c->rec_nodes = (!sur && chunk >= npts) ? (long long)npts * NQ : -1;
     Counters h;
     HIPCHK(c, hipMemcpy(&h, c->d_ctr, sizeof h, hipMemcpyDeviceToHost));
     c->st.ms_points = elapsed(ev[0], ev[1]);
@@ -2867,6 +2945,7 @@ int run_actions(fj_ctx *c, int n, const double *vR, const double *vphi, const do
     k_points<<<(n + 255) / 256, 256, lds, c->stream>>>(ta, pa, n);
     NodeBuf nb{c->d_nodef, n, c->d_Jr, c->d_Jz, c->d_flag};
     c->last_nodes = n;
+    c->rec_nodes = -1;
     HIPCHK(c, hipMemsetAsync(c->d_cnt, 0, 16 * sizeof(unsigned int), c->stream));
     HIPCHK(c, hipMemsetAsync(c->d_ctr, 0, sizeof(Counters), c->stream));
     SetupArgs sa{c->d_pR, c->d_pz, c->d_pPhi, c->d_pVe, 0, n, c->quad, nb, c->d_cnt + 0, c->d_cnt + 1,
@@ -2951,7 +3030,8 @@ int fj_ctx_destroy(fj_ctx *c) {
     void *ptrs[] = {c->d_ar, c->d_phil, c->d_Pr, c->d_Pr2, c->d_E, c->d_D, c->d_ci, c->d_si, c->d_poly, c->d_lut,
                     c->d_rk, c->d_nodef, c->d_Jr, c->d_Jz, c->d_flag, c->d_list_u, c->d_list_v, c->d_ovf, c->d_cnt,
                     c->d_ctr, c->d_ws, c->d_wsfull, c->d_pR, c->d_pz, c->d_pPhi, c->d_pVe, c->d_out6,
-                    c->d_mom, c->d_rhl, c->d_gl, c->d_pw, c->d_pws, c->d_status, c->d_eval, c->d_fpart};
+                    c->d_mom, c->d_rhl, c->d_gl, c->d_pw, c->d_pws, c->d_status, c->d_eval, c->d_fpart,
+                    c->d_okey, c->d_operm, c->d_otemp};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     void *sp[] = {c->d_pVc, c->d_X, c->d_T, c->d_bt, c->d_bw, c->d_pos, c->d_loss, c->d_sx16, c->d_sy, c->d_judge_J,
