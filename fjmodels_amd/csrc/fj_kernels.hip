@@ -376,30 +376,33 @@ k_setup(TabArgs ta, SetupArgs sa) {
 // so only which lane computes a node changes.  Writes the identity into perm
 // and the node count into *nsel.
 constexpr int SETUP_KEY_BITS = 25;
+__device__ inline unsigned int node_order_key(int flag, double uin, double uout, double vt) {
+    if (flag != 1) return flag == 2 ? 1u : 0u;
+    // ln u / ln 1.1 + 256, clamped to [0, 511]
+    const auto ubin = [](double u) -> unsigned int {
+        const float t = u > 0 ? __logf((float)u) * 10.49205f + 256.f : 0.f;
+        return t > 0.f ? (t < 511.f ? (unsigned int)t : 511u) : 0u;
+    };
+    // halvings from pi/4 towards pi/2 (32 + level) or towards 0 (level), level in [0, 31]
+    const bool up = vt > 0.25 * PI;
+    const double d = up ? PIH - vt : vt;
+    const float lv = d > 0 ? __log2f((float)(0.25 * PI / d)) : 31.f;
+    const unsigned int lvl = lv > 0.f ? (lv < 31.f ? (unsigned int)lv : 31u) : 0u;
+    return (1u << 24) + (ubin(uout) << 15) + (ubin(uin) << 6) + (up ? 32u : 0u) + lvl;
+}
 __global__ void __launch_bounds__(256) k_setup_key(NodeBuf nb, long long nn, unsigned int *key, int *perm,
                                                    unsigned int *nsel) {
     const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i == 0) *nsel = (unsigned int)nn;
+    if (i == 0 && nsel) *nsel = (unsigned int)nn;
     if (i >= nn) return;
     perm[i] = (int)i;
+    if (!key) return;
     const int flag = nb.flag[i];
-    unsigned int k = flag == 2 ? 1u : 0u;
+    double ub = 0, du = 0, vt = 0;
     if (flag == 1) {
-        const double ub = nb.f[NF_UBAR * nb.cap + i], du = nb.f[NF_DELTAU * nb.cap + i];
-        const double vt = nb.f[NF_VTURN * nb.cap + i];
-        // ln u / ln 1.1 + 256, clamped to [0, 511]
-        const auto ubin = [](double u) -> unsigned int {
-            const float t = u > 0 ? __logf((float)u) * 10.49205f + 256.f : 0.f;
-            return t > 0.f ? (t < 511.f ? (unsigned int)t : 511u) : 0u;
-        };
-        // halvings from pi/4 towards pi/2 (32 + level) or towards 0 (level), level in [0, 31]
-        const bool up = vt > 0.25 * PI;
-        const double d = up ? PIH - vt : vt;
-        const float lv = d > 0 ? __log2f((float)(0.25 * PI / d)) : 31.f;
-        const unsigned int lvl = lv > 0.f ? (lv < 31.f ? (unsigned int)lv : 31u) : 0u;
-        k = (1u << 24) + (ubin(ub + du) << 15) + (ubin(ub - du) << 6) + (up ? 32u : 0u) + lvl;
+        ub = nb.f[NF_UBAR * nb.cap + i]; du = nb.f[NF_DELTAU * nb.cap + i]; vt = nb.f[NF_VTURN * nb.cap + i];
     }
-    key[i] = k;
+    key[i] = node_order_key(flag, ub - du, ub + du, vt);
 }
 
 // df_hg (DF.cpp:65-76) and the rotation term (DF.cpp:114-119)
@@ -1177,6 +1180,63 @@ k_fast(TabArgs ta, FastArgs fa) {
     double *red = fast_stage(ta, smem, L, T32);
     const int tiles = (fa.npts + 63) / 64;
     fast_block(L, T32, red, fa, blockIdx.x % tiles, blockIdx.x / tiles, 0);
+}
+
+// Fast node phase with one lane per velocity node, in the order sel (or the
+// tiled order of k_setup): node setup and the FP32 action rules of node
+// gid, its flag and actions into the node-indexed records (k_moments adds
+// the DF), and its sort key for the next pass's order.  The records do not
+// depend on the lane order.
+struct FastNodeArgs {
+    const double *R, *z, *Phi, *Ve;   // [npts]
+    int npts, quad, ngl, nglv;
+    NodeBuf nb;
+    const int *sel;                   // [npts * NQ] lane order, or null
+    unsigned int *key;                // [npts * NQ] keys of this pass
+    Counters *ctr;
+};
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(FJ_FAST_WAVES)))
+k_fast_nodes(TabArgs ta, FastNodeArgs fa) {
+    extern __shared__ double smem[];
+    LdsTabs L;
+    PotView32 T32;
+    (void)fast_stage(ta, smem, L, T32);
+    const long long nn = (long long)fa.npts * NQ;
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const bool valid = t < nn;
+    int flag = 0;
+    long long nev = 0;
+    if (valid) {
+        const long long gid = fa.sel ? (long long)fa.sel[t] : setup_lane_node(t, fa.npts);
+        const int p = (int)(gid / NQ), q = (int)(gid % NQ);
+        const double R = fa.R[p], z = fa.z[p], Phi_h = fa.Phi[p], Ve = fa.Ve[p];
+        double V[3], w;
+        node_velocity(fa.quad, q, Ve, V, w);
+        FastNode fn;
+        int ne = 0;
+        flag = fast_setup<FastPolicy32>(L.T, &T32, L.E, L.D, L.ngrid, R, z, Phi_h, V, fn, &ne);
+        nev = ne;
+        double Jr = 0, Jz = 0;
+        if (flag == 1) {
+            fast_actions32(T32, fn, c_fgl, fa.ngl, c_fglv, fa.nglv, &Jr, &Jz);
+            nev += (long long)fn.ju * fa.ngl + (long long)fn.jv * fa.nglv;
+        } else if (flag == 2) {
+            Jr = 1e31; Jz = 1e31;
+        }
+        fa.nb.flag[gid] = flag;
+        fa.nb.Jr[gid] = Jr;
+        fa.nb.Jz[gid] = Jz;
+        fa.key[gid] = flag == 1 ? node_order_key(1, fn.uin, fn.uout, fn.vt) : node_order_key(flag, 0, 0, 0);
+    }
+    unsigned long long b = valid && flag == 1, u = valid && flag == 2, ev = (unsigned long long)nev;
+    for (int off = 32; off > 0; off >>= 1) {
+        b += __shfl_down(b, off); u += __shfl_down(u, off); ev += __shfl_down(ev, off);
+    }
+    if (__lane_id() == 0) {
+        if (b) atomicAdd(&fa.ctr->bound, b);
+        if (u) atomicAdd(&fa.ctr->unbound, u);
+        if (ev) atomicAdd(&fa.ctr->phi, ev);
+    }
 }
 
 // Sums of the S slices of each point, then the moments of rhofDF
@@ -2298,6 +2358,9 @@ struct fj_ctx {
     unsigned int *d_okey = nullptr; int *d_operm = nullptr; long long order_cap = 0;
     unsigned char *d_otemp = nullptr; size_t otemp_bytes = 0;
     long long rec_nodes = -1;   // nodes of the point-major records the last node phase left (-1: none)
+    // fast node phase, one lane per node: flag, actions and next-pass sort key per node
+    int *d_fflag = nullptr; double *d_fJr = nullptr, *d_fJz = nullptr; unsigned int *d_fkey = nullptr;
+    long long fnode_cap = 0, fast_keys = -1;   // fast_keys: nodes whose keys the last fast pass wrote
     // point buffers
     int pt_cap = 0;
     double *d_eval = nullptr; int eval_cap = 0;   // fj_eval_potential: R, z, Phi, dR, dz [eval_cap] each
@@ -2643,12 +2706,24 @@ int fast_gl_ensure(fj_ctx *c) {
     return 0;
 }
 
+// Fast node phases up to this many nodes keep node-indexed records (20 B per
+// node, plus the sort's 16 B); larger ones sum per lane (k_fast).
+constexpr long long FAST_NODE_RECORDS_MAX = 64ll << 20;
+
+#ifndef FJ_FAST_SORT
+#define FJ_FAST_SORT 1   // k_fast_nodes in the order of the previous fast pass's turning points
+#endif
+int run_fast_nodes(fj_ctx *c, int npts);
+int ensure_order(fj_ctx *c, long long nn);
+int sort_order(fj_ctx *c, long long nn, const unsigned int *kin);
+
 int run_fast(fj_ctx *c, int npts) {
     int r;
     if ((r = fast_gl_ensure(c))) return r;
     if (!c->d_cnt) {
         if ((r = dalloc(c, &c->d_cnt, 16)) || (r = dalloc(c, &c->d_ctr, 1))) return r;
     }
+    if ((long long)npts * NQ <= FAST_NODE_RECORDS_MAX) return run_fast_nodes(c, npts);
     const int tiles = (npts + 63) / 64;
     if (!c->fast_cus) {
         hipDeviceProp_t prop;
@@ -2703,28 +2778,107 @@ int run_fast(fj_ctx *c, int npts) {
     return 0;
 }
 
+// Fast node phase with node-indexed records (k_fast_nodes, then k_moments):
+// lanes in the order of the previous fast pass's turning points when it left
+// keys for these nodes, else in k_setup's tiled order.
+int run_fast_nodes(fj_ctx *c, int npts) {
+    int r;
+    const long long nn = (long long)npts * NQ;
+    if (nn > c->fnode_cap) {
+        if ((r = dalloc(c, &c->d_fflag, (size_t)nn)) || (r = dalloc(c, &c->d_fJr, (size_t)nn)) ||
+            (r = dalloc(c, &c->d_fJz, (size_t)nn)) || (r = dalloc(c, &c->d_fkey, (size_t)nn)))
+            return r;
+        c->fnode_cap = nn;
+        c->fast_keys = -1;
+    }
+    if ((r = ensure_order(c, nn))) return r;
+    const TabArgs ta = tab_args(c);
+    const size_t lds_pts = tab_bytes(c);
+    const size_t lds = fast_lds_bytes(c->nr, c->npoly, c->ngrid);
+    hipEvent_t ev[4];
+    for (int i = 0; i < 4; i++) HIPCHK(c, hipEventCreate(&ev[i]));
+    HIPCHK(c, hipMemsetAsync(c->d_ctr, 0, sizeof(Counters), c->stream));
+    memset(&c->st, 0, sizeof c->st);
+    HIPCHK(c, hipEventRecord(ev[0], c->stream));
+    PointArgs pa{c->d_pR, c->d_pz, c->d_pPhi, c->d_pVe, nullptr};
+    k_points<<<(npts + 255) / 256, 256, lds_pts, c->stream>>>(ta, pa, npts);
+    HIPCHK(c, hipGetLastError());
+    HIPCHK(c, hipEventRecord(ev[1], c->stream));
+    const int *sel = nullptr;
+    if (FJ_FAST_SORT && c->fast_keys == nn) {
+        NodeBuf none{nullptr, nn, nullptr, nullptr, nullptr};
+        k_setup_key<<<(unsigned)((nn + 255) / 256), 256, 0, c->stream>>>(none, nn, nullptr, c->d_operm, nullptr);
+        HIPCHK(c, hipGetLastError());
+        if ((r = sort_order(c, nn, c->d_fkey))) return r;
+        sel = c->d_operm + nn;
+    }
+    NodeBuf nb{nullptr, nn, c->d_fJr, c->d_fJz, c->d_fflag};
+    FastNodeArgs fa{c->d_pR, c->d_pz, c->d_pPhi, c->d_pVe, npts, c->quad, c->fast_ngl, fast_v_order(c->fast_ngl),
+                    nb, sel, c->d_fkey, c->d_ctr};
+    k_fast_nodes<<<(unsigned)((nn + 255) / 256), 256, lds, c->stream>>>(ta, fa);
+    HIPCHK(c, hipGetLastError());
+    HIPCHK(c, hipEventRecord(ev[2], c->stream));
+    MomArgs ma{c->d_pPhi, c->d_pVe, c->d_pR, 0, npts, c->quad, nb, c->df, c->d_out6,
+               c->d_mom, c->d_mom + npts, c->d_mom + 2 * npts, c->d_mom + 3 * npts, c->d_mom + 4 * npts,
+               c->d_mom + 5 * npts, &c->d_ctr->nonfinite};
+    k_moments<<<npts, 256, 0, c->stream>>>(ma);
+    HIPCHK(c, hipGetLastError());
+    HIPCHK(c, hipEventRecord(ev[3], c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    c->fast_keys = nn;
+    Counters h;
+    HIPCHK(c, hipMemcpy(&h, c->d_ctr, sizeof h, hipMemcpyDeviceToHost));
+    c->st.ms_points = elapsed(ev[0], ev[1]);
+    c->st.ms_setup = elapsed(ev[1], ev[2]);     // the node kernel (and the lane order's sort)
+    c->st.ms_moments = elapsed(ev[2], ev[3]);
+    c->st.ms_total = elapsed(ev[0], ev[3]);
+    c->st.points = npts;
+    c->st.nodes = nn;
+    c->st.bound_nodes = (long long)h.bound;
+    c->st.unbound_nodes = (long long)h.unbound;
+    c->st.nonfinite_nodes = (long long)h.nonfinite;
+    c->st.phi_evals = (long long)h.phi + 2 * (long long)npts;
+    c->st.setup_lanes = (nn + 255) / 256 * 256;
+    for (int i = 0; i < 4; i++) (void)hipEventDestroy(ev[i]);
+    return 0;
+}
+
 // k_setup's lane order: nodes sorted by k_setup_key over the records of the
 // previous node phase (radix sort on the stream); sets sa->sel / sa->nsel.
-int setup_order(fj_ctx *c, const NodeBuf &nb, long long nn, SetupArgs *sa) {
+// Sorts the keys kin[0..nn) with the node permutation k_setup_key wrote into
+// d_operm[0..nn); the sorted node order lands in d_operm[nn..2nn).
+int sort_order(fj_ctx *c, long long nn, const unsigned int *kin) {
     int r = 0;
-    if (nn > c->order_cap) {
-        if ((r = dalloc(c, &c->d_okey, (size_t)2 * nn)) || (r = dalloc(c, &c->d_operm, (size_t)2 * nn))) return r;
-        c->order_cap = nn;
-    }
-    unsigned int *kin = c->d_okey, *kout = c->d_okey + nn;
-    int *pin = c->d_operm, *pout = c->d_operm + nn;
     size_t need = 0;
+    unsigned int *kout = c->d_okey + nn;
+    int *pin = c->d_operm, *pout = c->d_operm + nn;
     HIPCHK(c, hipcub::DeviceRadixSort::SortPairs(nullptr, need, kin, kout, pin, pout, (int)nn, 0, SETUP_KEY_BITS,
                                                  c->stream));
     if (need > c->otemp_bytes) {
         if ((r = dalloc(c, &c->d_otemp, need))) return r;
         c->otemp_bytes = need;
     }
-    k_setup_key<<<(unsigned)((nn + 255) / 256), 256, 0, c->stream>>>(nb, nn, kin, pin, c->d_cnt + 10);
-    HIPCHK(c, hipGetLastError());
     HIPCHK(c, hipcub::DeviceRadixSort::SortPairs(c->d_otemp, need, kin, kout, pin, pout, (int)nn, 0, SETUP_KEY_BITS,
                                                  c->stream));
-    sa->sel = pout;
+    return 0;
+}
+
+int ensure_order(fj_ctx *c, long long nn) {
+    int r = 0;
+    if (nn > c->order_cap) {
+        if ((r = dalloc(c, &c->d_okey, (size_t)2 * nn)) || (r = dalloc(c, &c->d_operm, (size_t)2 * nn))) return r;
+        c->order_cap = nn;
+    }
+    return 0;
+}
+
+int setup_order(fj_ctx *c, const NodeBuf &nb, long long nn, SetupArgs *sa) {
+    int r = 0;
+    if ((r = ensure_order(c, nn))) return r;
+    k_setup_key<<<(unsigned)((nn + 255) / 256), 256, 0, c->stream>>>(nb, nn, c->d_okey, c->d_operm, c->d_cnt + 10);
+    HIPCHK(c, hipGetLastError());
+    if ((r = sort_order(c, nn, c->d_okey))) return r;
+    sa->sel = c->d_operm + nn;
     sa->nsel = c->d_cnt + 10;
     return 0;
 }
@@ -3031,7 +3185,7 @@ int fj_ctx_destroy(fj_ctx *c) {
                     c->d_rk, c->d_nodef, c->d_Jr, c->d_Jz, c->d_flag, c->d_list_u, c->d_list_v, c->d_ovf, c->d_cnt,
                     c->d_ctr, c->d_ws, c->d_wsfull, c->d_pR, c->d_pz, c->d_pPhi, c->d_pVe, c->d_out6,
                     c->d_mom, c->d_rhl, c->d_gl, c->d_pw, c->d_pws, c->d_status, c->d_eval, c->d_fpart,
-                    c->d_okey, c->d_operm, c->d_otemp};
+                    c->d_okey, c->d_operm, c->d_otemp, c->d_fflag, c->d_fJr, c->d_fJz, c->d_fkey};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     void *sp[] = {c->d_pVc, c->d_X, c->d_T, c->d_bt, c->d_bw, c->d_pos, c->d_loss, c->d_sx16, c->d_sy, c->d_judge_J,
