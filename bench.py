@@ -163,14 +163,16 @@ def mixed_leg(ctx, model, prev, last, steps, pts, npoly):
 
 
 def fast_leg(ctx, model, prev, last, steps, pts):
-    """Fast node phase (FJ_PREC_FAST: one fused kernel per iteration; the
-    Staeckel fudge with the reference's turning-point walks, FP64 roots and
-    64-point FP32 Gauss-Legendre action integrals instead of QAGS).
-    Accuracy as in the mixed leg (one iteration from the same potential as
-    the last FP64 one, QUADPACK Poisson on both sides); speed: `steps`
-    iterations continuing from there, QUADPACK Poisson.  Roofline: the
-    potential / integrand evaluations of k_fast (FP64 and FP32 mixed), at the
-    operation count of one evaluation, against the FP64 vector peak."""
+    """Fast node phase (FJ_PREC_FAST: one lane per velocity node in
+    k_fast_nodes, then k_moments; the Staeckel fudge with the reference's
+    turning-point walks, FP32 roots and FP32 Gauss-Legendre action integrals
+    of 64 / 32 points instead of QAGS; from the second pass on the lanes
+    follow the previous pass's turning points).  Accuracy as in the mixed
+    leg (one iteration from the same potential as the last FP64 one,
+    QUADPACK Poisson on both sides); speed: `steps` iterations continuing
+    from there, QUADPACK Poisson.  k_fast_nodes: its HIP-event time (with
+    the lane order's sort) and the potential / integrand evaluations it
+    counts."""
     import numpy as np
     import torch
     import fjmodels_amd as fj
@@ -195,8 +197,8 @@ def fast_leg(ctx, model, prev, last, steps, pts):
     return {"value": pts / dt, "ms_per_step": 1e3 * dt, "poisson": "quadpack",
             "err_vs_fp64": err, "err_note": "one iteration from the same potential as the last FP64 one, both with "
                                             "the QUADPACK Poisson solve; |diff| / monopole at the radius",
-            "k_fast": {"ms_per_launch": 1e3 * t_k, "potential_evaluations": evals,
-                       "evaluations_per_s": evals / t_k},
+            "k_fast_nodes": {"ms_per_launch": 1e3 * t_k, "potential_evaluations": evals,
+                             "evaluations_per_s": evals / t_k},
             "phases_ms": {k: float(np.mean([x[k] for x in st])) for k in ("ms_points", "ms_setup", "ms_moments")}}
 
 

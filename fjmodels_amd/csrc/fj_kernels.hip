@@ -2045,9 +2045,11 @@ __global__ void __launch_bounds__(256) kb_points(const ModelDev *md, const int *
 }
 
 // Node setup of every model; model m owns blocks [bstart[m], bstart[m+1]).
+// perm: the batch's lane order (each model's nodes permuted within the
+// model's own range of node indices).
 template <int BS>
 __global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(FJ_SETUP_WAVES))) kb_setup(const ModelDev *md, const int *bstart, int M, SetupArgs base,
-                                               unsigned int *cnt) {
+                                               unsigned int *cnt, const int *perm) {
     extern __shared__ double smem[];
     const int m = find_seg(bstart, M, blockIdx.x);
     const ModelDev &d = md[m];
@@ -2062,7 +2064,23 @@ __global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(FJ_SETU
     sa.cnt_u = cnt + 8 * m;
     sa.cnt_v = cnt + 8 * m + 1;
     const long long local = (long long)(blockIdx.x - bstart[m]) * BS + threadIdx.x;
-    setup_node(L, sa, local < (long long)d.np * NQ ? no + local : (long long)sa.npts * NQ);
+    setup_node(L, sa, local < (long long)d.np * NQ ? (long long)perm[no + local] : (long long)sa.npts * NQ);
+}
+
+// Sort keys of the batch's lane order: node_order_key of the previous
+// iteration's records, the model index above it so that every model's nodes
+// keep their own range of positions.  Writes the identity into perm.
+__global__ void __launch_bounds__(256) kb_order_key(NodeBuf nb, long long nn, const int *pmodel,
+                                                    unsigned long long *key, int *perm) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nn) return;
+    perm[i] = (int)i;
+    const int flag = nb.flag[i];
+    double ub = 0, du = 0, vt = 0;
+    if (flag == 1) {
+        ub = nb.f[NF_UBAR * nb.cap + i]; du = nb.f[NF_DELTAU * nb.cap + i]; vt = nb.f[NF_VTURN * nb.cap + i];
+    }
+    key[i] = ((unsigned long long)pmodel[i / NQ] << SETUP_KEY_BITS) | node_order_key(flag, ub - du, ub + du, vt);
 }
 
 // Blocks of the persistent integrator per model: one for every model with
@@ -4068,6 +4086,14 @@ struct fj_batch {
     size_t lds_fast = 0;
     hipGraph_t graph = nullptr;
     hipGraphExec_t gexec = nullptr;
+    // lane order of kb_setup (FP64 / mixed): keys and permutation, in and out of
+    // the radix sort [2][nodes] each; kb_setup reads the sorted permutation
+    unsigned long long *d_bkey = nullptr;
+    int *d_bperm = nullptr;
+    unsigned char *d_btemp = nullptr;
+    size_t btemp_bytes = 0;
+    bool rec_ok = false;   // the records hold an iteration's turning points
+    int order_bits = SETUP_KEY_BITS;
     fj_batch_stats st{};
 };
 
@@ -4090,6 +4116,22 @@ void batch_drop_graph(fj_batch *b) {
     if (b->graph) (void)hipGraphDestroy(b->graph);
     b->gexec = nullptr;
     b->graph = nullptr;
+}
+
+// kb_setup's lane order for the next iteration: every model's nodes sorted
+// by the turning points the previous iteration left (on the stream, outside
+// any captured graph).  Nothing to do before the first iteration.
+int batch_order(fj_batch *b) {
+    fj_ctx *c = b->c;
+    if (b->fast || !FJ_SETUP_SORT || !b->rec_ok) return 0;
+    const long long nn = b->nodes;
+    NodeBuf nb{b->d_nodef, nn, b->d_Jr, b->d_Jz, b->d_flag};
+    kb_order_key<<<(unsigned)((nn + 255) / 256), 256, 0, c->stream>>>(nb, nn, b->d_pmodel, b->d_bkey, b->d_bperm);
+    HIPCHK(c, hipGetLastError());
+    size_t bytes = b->btemp_bytes;
+    HIPCHK(c, hipcub::DeviceRadixSort::SortPairs(b->d_btemp, bytes, b->d_bkey, b->d_bkey + nn, b->d_bperm,
+                                                 b->d_bperm + nn, (int)nn, 0, b->order_bits, c->stream));
+    return 0;
 }
 
 // One SCF iteration of every model, stream-ordered, no host synchronisation.
@@ -4119,10 +4161,11 @@ int batch_iteration(fj_batch *b) {
     NodeBuf nb{b->d_nodef, b->nodes, b->d_Jr, b->d_Jz, b->d_flag};
     SetupArgs sa{b->d_R, b->d_z, b->d_Phi, b->d_Ve, 0, 0, 0, nb, nullptr, nullptr, b->d_list_u, b->d_list_v,
                  b->d_ctr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+    const int *perm = b->d_bperm + b->nodes;
     if (b->setup_bs == 128)
-        kb_setup<128><<<b->nblk[0], 128, b->lds_tab, s>>>(b->d_md, b->d_bstart[0], M, sa, b->d_cnt);
+        kb_setup<128><<<b->nblk[0], 128, b->lds_tab, s>>>(b->d_md, b->d_bstart[0], M, sa, b->d_cnt, perm);
     else
-        kb_setup<256><<<b->nblk[1], 256, b->lds_tab, s>>>(b->d_md, b->d_bstart[1], M, sa, b->d_cnt);
+        kb_setup<256><<<b->nblk[1], 256, b->lds_tab, s>>>(b->d_md, b->d_bstart[1], M, sa, b->d_cnt, perm);
     // integrator blocks: the chosen count, at most what is resident at this precision
     int ib = b->cus * (mixed ? b->per_cu32 : b->per_cu64);
     if (b->integ_blocks < ib) ib = b->integ_blocks;
@@ -4367,9 +4410,25 @@ int fj_batch_create(fj_ctx *c, int M, const fj_model_spec *spec, fj_batch **out)
         (r = balloc(b, &b->d_Jz, (size_t)nn)) || (r = balloc(b, &b->d_flag, (size_t)nn)) ||
         (r = balloc(b, &b->d_list_u, (size_t)nn)) || (r = balloc(b, &b->d_list_v, (size_t)nn)) ||
         (r = balloc(b, &b->d_ovf, (size_t)2 * nn)) || (r = balloc(b, &b->d_cnt, (size_t)8 * M + 8)) ||
-        (r = balloc(b, &b->d_ctr, 1)) || (r = balloc(b, &b->d_backup, (size_t)3 * b->rows))) {
+        (r = balloc(b, &b->d_ctr, 1)) || (r = balloc(b, &b->d_backup, (size_t)3 * b->rows)) ||
+        (r = balloc(b, &b->d_bkey, (size_t)2 * nn)) || (r = balloc(b, &b->d_bperm, (size_t)2 * nn))) {
         fj_batch_destroy(b);
         return r;
+    }
+    {
+        // the sort's scratch; until an iteration has left records, kb_setup takes the identity order
+        int mb = 0;
+        while ((1 << mb) < M) mb++;
+        b->order_bits = SETUP_KEY_BITS + mb;
+        HIPCHK(c, hipcub::DeviceRadixSort::SortPairs(nullptr, b->btemp_bytes, b->d_bkey, b->d_bkey + nn, b->d_bperm,
+                                                     b->d_bperm + nn, (int)nn, 0, b->order_bits, c->stream));
+        if ((r = balloc(b, &b->d_btemp, b->btemp_bytes))) {
+            fj_batch_destroy(b);
+            return r;
+        }
+        NodeBuf none{nullptr, nn, nullptr, nullptr, nullptr};
+        k_setup_key<<<(unsigned)((nn + 255) / 256), 256, 0, c->stream>>>(none, nn, nullptr, b->d_bperm + nn, nullptr);
+        HIPCHK(c, hipGetLastError());
     }
     HIPCHK(c, hipMemcpy(b->d_md, b->hmd.data(), M * sizeof(ModelDev), hipMemcpyHostToDevice));
     HIPCHK(c, hipMemcpy(b->d_pmodel, pmodel.data(), pmodel.size() * sizeof(int), hipMemcpyHostToDevice));
@@ -4481,7 +4540,9 @@ int fj_batch_autotune(fj_batch *b) {
             b->integ_blocks = ib;
             if ((r = batch_save(b, true))) return r;
             HIPCHK(c, hipEventRecord(e0, c->stream));
+            if ((r = batch_order(b))) return r;
             if ((r = batch_iteration(b))) return r;
+            b->rec_ok = true;
             HIPCHK(c, hipEventRecord(e1, c->stream));
             HIPCHK(c, hipStreamSynchronize(c->stream));
             const float ms = elapsed(e0, e1);
@@ -4530,8 +4591,10 @@ int fj_batch_iterate(fj_batch *b, int niter) {
     HIPCHK(c, hipEventCreate(&e1));
     for (int it = 0; it < niter; it++) {
         HIPCHK(c, hipEventRecord(e0, c->stream));
+        if ((r = batch_order(b))) return r;
         HIPCHK(c, hipGraphLaunch(b->gexec, c->stream));
         HIPCHK(c, hipEventRecord(e1, c->stream));
+        if (!b->fast) b->rec_ok = true;
     }
     HIPCHK(c, hipStreamSynchronize(c->stream));
     if (niter) b->st.ms_last_iter = elapsed(e0, e1);
