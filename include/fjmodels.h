@@ -49,15 +49,16 @@ enum { FJ_POISSON_QUADPACK = 0, FJ_POISSON_BLOCKSCAN = 1 };
  * spheroidal coordinates, turning-point roots and the Poisson solve.
  * Results are not bitwise those of FJ_PREC_FP64. */
 enum { FJ_PREC_FP64 = 0, FJ_PREC_MIXED = 1, FJ_PREC_FAST = 2 };
-/* FJ_PREC_FAST: the same Staeckel-fudge actions computed for speed, fused
- * with f(J) and the moment sums in one kernel (no node records): the u-side
- * reference point at the node (Getu0's umid cancels from uturnfn,
- * uvOrb.cpp:25-31,70-73), the reference's turning-point bracket walks
- * (uvOrb.cpp:131-224) with wrp_GSLroots' endpoint test and FP64 regula falsi,
- * and the action integrals (uvOrb.cpp:226-262) as FP32 Gauss-Legendre rules
- * instead of QAGS: fj_set_fast_order points (64 by default) for J_r, half
- * as many (at least 4) for the smoother J_z integrand.  The action surrogate
- * does not run in this mode. */
+/* FJ_PREC_FAST: the same Staeckel-fudge actions computed for speed, one
+ * lane per velocity node (24 B of records per node; runs above 64 M nodes
+ * sum per lane instead): the u-side reference point at the node (Getu0's
+ * umid cancels from uturnfn, uvOrb.cpp:25-31,70-73), the reference's
+ * turning-point bracket walks (uvOrb.cpp:131-224) on FP32 evaluations with
+ * the bracket ends and wrp_GSLroots' endpoint test in FP64, regula falsi
+ * roots, and the action integrals (uvOrb.cpp:226-262) as FP32
+ * Gauss-Legendre rules instead of QAGS: fj_set_fast_order points (64 by
+ * default) for J_r, half as many (at least 4) for the smoother J_z
+ * integrand.  The action surrogate does not run in this mode. */
 
 /* Gauss-Legendre order of the FJ_PREC_FAST J_r integral (4..128); the J_z
  * rule takes half of it, at least 4 points. */
