@@ -1097,7 +1097,10 @@ __device__ __forceinline__ void fast_block(const LdsTabs &L, const PotView32 &T3
     const int p = pbase + pl;
     const int QS = (NQ + fa.S - 1) / fa.S;
     const int q0 = sl * QS, q1 = q0 + QS < NQ ? q0 + QS : NQ;
-    double acc[6] = {0, 0, 0, 0, 0, 0};
+    // the lane's six sums live in its LDS slots of red (registers across the
+    // node loop would spill)
+    double *acc = red + wave * 6 * 64 + lane;
+    for (int m = 0; m < 6; m++) acc[m * 64] = 0;
     int nbound = 0, nunb = 0, nbad = 0;
     long long nev = 0;
     if (valid) {
@@ -1127,14 +1130,13 @@ __device__ __forceinline__ void fast_block(const LdsTabs &L, const PotView32 &T3
             nbad += !isfinite(DF);
             const double jd = jacob * DF;
             acc[0] += w * jd;
-            acc[1] += w * (jd * V[1]);
-            acc[2] += w * (jd * V[0] * V[0]);
-            acc[3] += w * (jd * V[1] * V[1]);
-            acc[4] += w * (jd * V[2] * V[2]);
-            acc[5] += w * (jd * V[0] * V[2]);
+            acc[64] += w * (jd * V[1]);
+            acc[128] += w * (jd * V[0] * V[0]);
+            acc[192] += w * (jd * V[1] * V[1]);
+            acc[256] += w * (jd * V[2] * V[2]);
+            acc[320] += w * (jd * V[0] * V[2]);
         }
     }
-    for (int m = 0; m < 6; m++) red[(wave * 6 + m) * 64 + lane] = acc[m];
     __syncthreads();
     if (wave == 0 && valid) {
         double *o = fa.part + ((size_t)p * fa.S + sl) * 6;
