@@ -1245,12 +1245,10 @@ k_fast_nodes(TabArgs ta, FastNodeArgs fa) {
 // (Integ.cpp:160-161) as k_moments forms them.  One block per point: lane t
 // adds slices t, t + 256, ... in order, then a pairwise tree over the lanes
 // (a fixed order).
-// quadp (batches): each point's velocity rule; else quad for all.
 __global__ void __launch_bounds__(256) k_fast_finish(int npts, int S, int quad, const double *part, double *out6,
-                                                     double *mom, const int *quadp = nullptr) {
+                                                     double *mom) {
     __shared__ double red[6][256];
     const int p = blockIdx.x, t = threadIdx.x;
-    if (quadp) quad = quadp[p];
     double o[6] = {0, 0, 0, 0, 0, 0};
     for (int s = t; s < S; s += 256)
         for (int m = 0; m < 6; m++) o[m] += part[((size_t)p * S + s) * 6 + m];
@@ -2209,24 +2207,6 @@ __global__ void __launch_bounds__(256) kb_moments(const ModelDev *md, const int 
     ma.quad = d.quad;
     if (mixed) moments_point_f32(ma, p, reinterpret_cast<float (*)[2][256]>(&red[0][0]));
     else moments_point(ma, p, red);
-}
-
-// Fast node phase over all models of a batch: block b takes global tile
-// b % ntiles (model by the tile prefix tstart) and node slice b / ntiles.
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(FJ_FAST_WAVES)))
-kb_fast(const ModelDev *md, const int *tstart, int M, int ntiles, FastArgs base) {
-    extern __shared__ double smem[];
-    const int t = blockIdx.x % ntiles, sl = blockIdx.x / ntiles;
-    const int m = find_seg(tstart, M, t);
-    const ModelDev &d = md[m];
-    LdsTabs L;
-    PotView32 T32;
-    double *red = fast_stage(d.ta, smem, L, T32);
-    FastArgs fa = base;
-    fa.npts = d.np;
-    fa.quad = d.quad;
-    fa.df = d.df;
-    fast_block(L, T32, red, fa, t - tstart[m], sl, d.po);
 }
 
 // rho[0][0] fix + get_Ylm for every model; entries [rstart[m], rstart[m+1]).
@@ -4079,12 +4059,19 @@ struct fj_batch {
     int per_cu64 = 1, per_cu32 = 1;                         // their resident blocks per CU
     int setup_bs = 256, integ_blocks = 0, integ_max_blocks = 0, ovf_blocks = 64;
     bool have_setup = false;
-    // fast node phase (a batch created under FJ_PREC_FAST): no node records;
-    // tiles of 64 points per model (prefix tstart), S node slices, slice sums
+    // fast node phase (a batch created under FJ_PREC_FAST): the models one
+    // after another through k_fast_nodes and k_moments, with one model's node
+    // records at a time (fmax_nodes: the largest model's nodes) and every
+    // node's sort key for the next iteration's lane order
     bool fast = false;
-    int *d_tstart = nullptr, *d_pquad = nullptr;
-    int tiles = 0, fast_S = 1, fast_ngl = 0;
-    double *d_fpart = nullptr;
+    int fast_ngl = 0;
+    long long fmax_nodes = 0;
+    int *d_bfflag = nullptr, *d_bfperm = nullptr;   // perm: [2][fmax_nodes], in and out of the sort
+    double *d_bfJr = nullptr, *d_bfJz = nullptr;
+    unsigned int *d_bfkey = nullptr, *d_bfkout = nullptr;   // key: [nodes] (all models), kout: [fmax_nodes]
+    unsigned char *d_bftemp = nullptr;
+    size_t bftemp_bytes = 0;
+    bool fkeys_ok = false;
     size_t lds_fast = 0;
     hipGraph_t graph = nullptr;
     hipGraphExec_t gexec = nullptr;
@@ -4146,11 +4133,31 @@ int batch_iteration(fj_batch *b) {
         kb_zero64<<<1, 64, 0, s>>>(reinterpret_cast<unsigned long long *>(b->d_ctr), (int)(sizeof(Counters) / 8));
         PointArgs pa{b->d_R, b->d_z, b->d_Phi, b->d_Ve, b->d_Vc};
         kb_points<<<(b->npts + 255) / 256, 256, 0, s>>>(b->d_md, b->d_pmodel, pa, b->npts);
-        FastArgs fa{b->d_R, b->d_z, b->d_Phi, b->d_Ve, 0, 0, b->fast_S, c->fast_ngl, fast_v_order(c->fast_ngl),
-                    fj_df_params{}, b->d_fpart,
-                    b->d_ctr};
-        kb_fast<<<(unsigned)(b->tiles * b->fast_S), 256, b->lds_fast, s>>>(b->d_md, b->d_tstart, M, b->tiles, fa);
-        k_fast_finish<<<b->npts, 256, 0, s>>>(b->npts, b->fast_S, 0, b->d_fpart, b->d_out6, b->d_mom, b->d_pquad);
+        // each model: its nodes in the order of the previous iteration's turning points (model-local
+        // indices), the node kernel, then its points' moments; one model's records at a time
+        const long long fm = b->fmax_nodes;
+        for (int m = 0; m < M; m++) {
+            const ModelDev &d = b->hmd[m];
+            const long long n = (long long)d.np * NQ, off = (long long)d.po * NQ;
+            const int *sel = nullptr;
+            if (FJ_FAST_SORT && b->fkeys_ok) {
+                NodeBuf none{nullptr, n, nullptr, nullptr, nullptr};
+                k_setup_key<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(none, n, nullptr, b->d_bfperm, nullptr);
+                size_t bytes = b->bftemp_bytes;
+                HIPCHK(c, hipcub::DeviceRadixSort::SortPairs(b->d_bftemp, bytes, b->d_bfkey + off, b->d_bfkout,
+                                                             b->d_bfperm, b->d_bfperm + fm, (int)n, 0,
+                                                             SETUP_KEY_BITS, s));
+                sel = b->d_bfperm + fm;
+            }
+            NodeBuf nb{nullptr, n, b->d_bfJr, b->d_bfJz, b->d_bfflag};
+            FastNodeArgs fa{b->d_R + d.po, b->d_z + d.po, b->d_Phi + d.po, b->d_Ve + d.po, d.np, d.quad,
+                            c->fast_ngl, fast_v_order(c->fast_ngl), nb, sel, b->d_bfkey + off, b->d_ctr};
+            k_fast_nodes<<<(unsigned)((n + 255) / 256), 256, b->lds_fast, s>>>(d.ta, fa);
+            MomArgs ma{b->d_Phi, b->d_Ve, b->d_R, d.po, d.np, d.quad, nb, d.df, b->d_out6,
+                       b->d_mom, b->d_mom + b->npts, b->d_mom + 2 * b->npts, b->d_mom + 3 * b->npts,
+                       b->d_mom + 4 * b->npts, b->d_mom + 5 * b->npts, &b->d_ctr->nonfinite};
+            k_moments<<<d.np, 256, 0, s>>>(ma);
+        }
         kb_project<<<(b->rows + 63) / 64, 64, 0, s>>>(b->d_md, b->d_rstart, M, b->d_mom, b->npts);
         kb_poisson<<<dim3(b->npoly_max, M), PSCAN_THREADS, 0, s>>>(b->d_md);
         HIPCHK(c, hipGetLastError());
@@ -4231,34 +4238,29 @@ int batch_create_fast(fj_batch *b, const fj_model_spec *spec, const std::vector<
     int r;
     rstart.push_back(b->rows);
     b->lds_tab = lds_max;
-    std::vector<int> tstart(M), pquad(b->npts);
-    int acc = 0;
+    long long fm = 0;
     for (int m = 0; m < M; m++) {
-        tstart[m] = acc;
-        acc += (b->hmd[m].np + 63) / 64;
-        for (int i = 0; i < b->hmd[m].np; i++) pquad[b->hmd[m].po + i] = b->hmd[m].quad;
+        const long long n = (long long)b->hmd[m].np * NQ;
+        if (n > fm) fm = n;
     }
-    b->tiles = acc;
-    hipDeviceProp_t prop;
-    HIPCHK(c, hipGetDeviceProperties(&prop, c->device));
-    b->cus = prop.multiProcessorCount;
-    int S = (FJ_FAST_BLOCKS_PER_CU * b->cus + b->tiles - 1) / b->tiles;
-    if (S > NQ / 16) S = NQ / 16;
-    if (S < 1) S = 1;
-    b->fast_S = S;
+    b->fmax_nodes = fm;
     if ((r = balloc(b, &b->d_md, M)) || (r = balloc(b, &b->d_pv, M)) || (r = balloc(b, &b->d_pmodel, b->npts)) ||
         (r = balloc(b, &b->d_rstart, M + 1)) || (r = balloc(b, &b->d_R, b->npts)) || (r = balloc(b, &b->d_z, b->npts)) ||
         (r = balloc(b, &b->d_Phi, b->npts)) || (r = balloc(b, &b->d_Ve, b->npts)) || (r = balloc(b, &b->d_Vc, b->npts)) ||
         (r = balloc(b, &b->d_out6, (size_t)6 * b->npts)) || (r = balloc(b, &b->d_mom, (size_t)6 * b->npts)) ||
         (r = balloc(b, &b->d_cnt, (size_t)8 * M + 8)) || (r = balloc(b, &b->d_ctr, 1)) ||
-        (r = balloc(b, &b->d_backup, (size_t)3 * b->rows)) || (r = balloc(b, &b->d_tstart, M)) ||
-        (r = balloc(b, &b->d_pquad, b->npts)) || (r = balloc(b, &b->d_fpart, (size_t)b->npts * S * 6))) {
+        (r = balloc(b, &b->d_backup, (size_t)3 * b->rows)) || (r = balloc(b, &b->d_bfflag, (size_t)fm)) ||
+        (r = balloc(b, &b->d_bfJr, (size_t)fm)) || (r = balloc(b, &b->d_bfJz, (size_t)fm)) ||
+        (r = balloc(b, &b->d_bfkout, (size_t)fm)) || (r = balloc(b, &b->d_bfperm, (size_t)2 * fm)) ||
+        (r = balloc(b, &b->d_bfkey, (size_t)b->nodes))) {
         fj_batch_destroy(b);
         return r;
     }
+    // the sort's scratch for the largest model (the others need no more)
+    HIPCHK(c, hipcub::DeviceRadixSort::SortPairs(nullptr, b->bftemp_bytes, b->d_bfkey, b->d_bfkout, b->d_bfperm,
+                                                 b->d_bfperm + fm, (int)fm, 0, SETUP_KEY_BITS, c->stream));
+    if ((r = balloc(b, &b->d_bftemp, b->bftemp_bytes))) { fj_batch_destroy(b); return r; }
     if ((r = batch_points(b, spec, pmodel, rstart))) { fj_batch_destroy(b); return r; }
-    HIPCHK(c, hipMemcpy(b->d_tstart, tstart.data(), M * sizeof(int), hipMemcpyHostToDevice));
-    HIPCHK(c, hipMemcpy(b->d_pquad, pquad.data(), b->npts * sizeof(int), hipMemcpyHostToDevice));
     b->st.nodes = b->nodes;
     b->st.points = b->npts;
     // each model's table view (the Delta fit of fj_batch_setup reads them)
@@ -4573,9 +4575,24 @@ int fj_batch_iterate(fj_batch *b, int niter) {
                                               : "batch created under FJ_PREC_FP64 / FJ_PREC_MIXED: "
                                                 "FJ_PREC_FAST needs a batch created under it");
     if (b->fast) {
+        // launched directly, without a graph: the lane order's sort runs between the models' launches
         if ((r = fast_gl_ensure(c))) return r;
-        if (b->gexec && b->fast_ngl != c->fast_ngl) batch_drop_graph(b);   // the order is in the graph
         b->fast_ngl = c->fast_ngl;
+        b->st.graph = 0;
+        hipEvent_t e0, e1;
+        HIPCHK(c, hipEventCreate(&e0));
+        HIPCHK(c, hipEventCreate(&e1));
+        for (int it = 0; it < niter; it++) {
+            HIPCHK(c, hipEventRecord(e0, c->stream));
+            if ((r = batch_iteration(b))) return r;
+            HIPCHK(c, hipEventRecord(e1, c->stream));
+            b->fkeys_ok = true;
+        }
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        if (niter) b->st.ms_last_iter = elapsed(e0, e1);
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        return 0;
     }
     if (!b->gexec) {
         HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
@@ -4596,7 +4613,7 @@ int fj_batch_iterate(fj_batch *b, int niter) {
         if ((r = batch_order(b))) return r;
         HIPCHK(c, hipGraphLaunch(b->gexec, c->stream));
         HIPCHK(c, hipEventRecord(e1, c->stream));
-        if (!b->fast) b->rec_ok = true;
+        b->rec_ok = true;
     }
     HIPCHK(c, hipStreamSynchronize(c->stream));
     if (niter) b->st.ms_last_iter = elapsed(e0, e1);
