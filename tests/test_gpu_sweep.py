@@ -48,18 +48,15 @@ def test_sweep_matches_single_models(ctx):
 
 
 def test_fast_sweep_one_batch_matches_single_models(ctx):
-    """Under FJ_PREC_FAST the sweep packs every model into one batch (no node
-    records), one fast-phase launch over all models.  The per-node work is
-    the single-context fast phase's code, but FP32 contraction into fused
-    multiply-adds is free, so the two kernels' per-node values may differ in
-    the last FP32 bit at a few nodes; and the batch sums each point's nodes
-    per lane over node slices where the single context sums them per point
-    (k_moments), so the FP64 sums are added in another grouping.  Every
-    table lies within 1e-11 of its monopole after two iterations (about
-    3e-13 measured), except Phi_l'' (Pr2), where the Poisson solve's second
-    derivative amplifies those differences: within 2e-10 (1.6e-12 and
-    4.9e-11 measured with the exact and the hardware FP32 division of the
-    fast phase).  The Delta tables agree bit for bit."""
+    """Under FJ_PREC_FAST the sweep packs every model into one batch, which
+    runs its models one after another through the single-context fast
+    phase's kernels (k_fast_nodes, then k_moments), with one model's node
+    records at a time.  The batch's own Delta fit, projection and block-scan
+    Poisson launches serve all models.  Every table lies within 1e-11 of its
+    monopole after two iterations, except Phi_l'' (Pr2), where the Poisson
+    solve's second derivative amplifies last-bit differences: within 2e-10
+    (the bound of the earlier slice kernel, whose sums were grouped
+    otherwise: 4.9e-11 measured).  The Delta tables agree bit for bit."""
     import fjmodels_amd as fj
     from fjmodels_amd import sweep as S
     draw = fj.draw_models(24, seed=0)
