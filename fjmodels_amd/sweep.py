@@ -7,6 +7,8 @@ Here the models are packed, in draw order, into as few batches as the
 per-batch node limit of fj_batch_create allows (64 M velocity nodes, i.e.
 about 6,300 grid points of 22^3 nodes), and each batch advances all its
 models with one launch per iteration phase, replayed from a HIP graph.
+Under FJ_PREC_FAST all models go into one batch, which runs them one after
+another through the fast node phase's kernels.
 """
 import time
 
@@ -17,7 +19,8 @@ from .scf import log_grid
 
 NODES_PER_POINT = 22 ** 3
 BATCH_NODE_LIMIT = 64 << 20
-# fast-phase batches keep per point only the grid arrays and a few slice sums
+# fast-phase batches keep one model's node records at a time and a 4-byte
+# sort key per node: no node limit of their own
 FAST_BATCH_NODE_LIMIT = 1 << 40
 KINDS = ("Hernquist", "Isochrone", "NFW")
 
@@ -71,9 +74,10 @@ class Sweep:
     """Runs `models` through the batched path, batch after batch, on ctx's
     device with ctx's precision.  run(niter) returns each model's tables
     after niter SCF iterations (Batch.get) and the timings: iteration time
-    summed over the batches (graph replays of iterations 2..niter, as the
-    first replay includes the capture), setup time (guess potentials,
-    Poisson, Delta tables) apart."""
+    summed over the batches (iterations 2..niter: the first includes the
+    graph capture, or under FJ_PREC_FAST the first pass's lane order before
+    any turning points are known), setup time (guess potentials, Poisson,
+    Delta tables) apart."""
 
     def __init__(self, ctx, models, node_limit=None, autotune=False):
         """node_limit: velocity nodes per batch; by default BATCH_NODE_LIMIT,

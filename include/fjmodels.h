@@ -284,9 +284,10 @@ int fj_surrogate_gradient(fj_ctx *ctx, long long n, const float *x, const float 
  * The arithmetic follows the context's precision (fj_set_precision) and
  * velocity rule of each model; the Poisson step is the block scan.  A batch
  * holds at most 64 M velocity nodes, except one created under FJ_PREC_FAST:
- * it keeps no node records (any number of models) and is iterated under
- * FJ_PREC_FAST only (FJ_E_STATE otherwise), as a QAGS batch is under
- * FJ_PREC_FP64 / FJ_PREC_MIXED only.
+ * it keeps one model's node records at a time plus a 4-byte sort key per
+ * node (any number of models), runs its models one after another without a
+ * HIP graph, and is iterated under FJ_PREC_FAST only (FJ_E_STATE otherwise),
+ * as a QAGS batch is under FJ_PREC_FP64 / FJ_PREC_MIXED only.
  * ------------------------------------------------------------------------ */
 typedef struct fj_batch fj_batch;
 
