@@ -4244,6 +4244,11 @@ int batch_create_fast(fj_batch *b, const fj_model_spec *spec, const std::vector<
         if (n > fm) fm = n;
     }
     b->fmax_nodes = fm;
+    if (fm > 0x7fffffffll) {
+        const long long n = fm;
+        fj_batch_destroy(b);
+        return set_err(c, FJ_E_ARG, "fast batch: a model of %lld nodes; at most %lld per model", n, 0x7fffffffll);
+    }
     if ((r = balloc(b, &b->d_md, M)) || (r = balloc(b, &b->d_pv, M)) || (r = balloc(b, &b->d_pmodel, b->npts)) ||
         (r = balloc(b, &b->d_rstart, M + 1)) || (r = balloc(b, &b->d_R, b->npts)) || (r = balloc(b, &b->d_z, b->npts)) ||
         (r = balloc(b, &b->d_Phi, b->npts)) || (r = balloc(b, &b->d_Ve, b->npts)) || (r = balloc(b, &b->d_Vc, b->npts)) ||
