@@ -123,3 +123,31 @@ def test_fast_grid_phase_against_fp64(fctx):
         err = np.max(np.abs(got[k] - ref[k]) / np.abs(ref[k][:, :1]))
         print(k, "max |diff| / monopole", err)
         assert err < 3e-4, k
+
+
+def test_lane_order_keeps_results(ctx):
+    """From the second node phase on, the node kernels take their nodes in
+    the order of the previous phase's turning points (k_setup, and
+    k_fast_nodes under FJ_PREC_FAST).  Each node's arithmetic is its own and
+    the moments are summed per point in node order, so two phases from the
+    same installed potential, the first on a grid of a new size (the tiled
+    order: no turning points of these nodes yet) and the second in the
+    sorted order, give the same tables bit for bit, in FP64 and in the fast
+    phase."""
+    import fjmodels_amd as fj
+    p = O.guess_model(26, 0.01, 100.0, 5, 9)[0]
+    ctx.set_grid(p.ar, p.ngauss, p.npoly, p.ngrid)
+    ctx.set_quad(fj.FJ_QUAD_LOG22)
+    ctx.set_potential(*p.tables())
+    ctx.set_delta(*p.delta())
+    ctx.set_df(fj.df_params("Hernquist"))
+    try:
+        for prec in (fj.FJ_PREC_FP64, fj.FJ_PREC_FAST):
+            ctx.set_precision(prec)
+            first = ctx.compute_new_phi(fj.FJ_POISSON_BLOCKSCAN)
+            second = ctx.compute_new_phi(fj.FJ_POISSON_BLOCKSCAN)
+            print("precision", prec, "setup ms", ctx.stats()["ms_setup"])
+            for k in first:
+                assert np.array_equal(first[k], second[k]), (prec, k)
+    finally:
+        ctx.set_precision(fj.FJ_PREC_FP64)
