@@ -367,14 +367,13 @@ k_setup(TabArgs ta, SetupArgs sa) {
     setup_node(L, sa, tid);
 }
 
-// Sort key of k_setup's lane order from the records of the previous node
-// phase: for a bound node, its turning points u_in and u_out on the scale of
-// GetTurnu's steps (ln 1.1) and its v turning point as the halvings of
-// GetTurnv from pi/4; 0 for H >= 0, 1 for E >= 0.  Launched in the order of
-// the sorted keys, a wave takes nodes whose bracket walks and Brent solves
-// run alike.  Any key gives the same results: the records stay node-indexed,
-// so only which lane computes a node changes.  Writes the identity into perm
-// and the node count into *nsel.
+// Sort key of a node for the next node phase's lane order: for a bound node
+// (flag 1) its turning points u_in and u_out on the scale of GetTurnu's steps
+// (ln 1.1) and its v turning point as the halvings of GetTurnv from pi/4; 0
+// for H >= 0, 1 for E >= 0.  Taken in the order of the sorted keys, a wave
+// runs nodes whose bracket walks and root solves take like steps.  Any key
+// gives the same results: the records stay node-indexed, so only which lane
+// computes a node changes.
 constexpr int SETUP_KEY_BITS = 25;
 __device__ inline unsigned int node_order_key(int flag, double uin, double uout, double vt) {
     if (flag != 1) return flag == 2 ? 1u : 0u;
@@ -390,6 +389,8 @@ __device__ inline unsigned int node_order_key(int flag, double uin, double uout,
     const unsigned int lvl = lv > 0.f ? (lv < 31.f ? (unsigned int)lv : 31u) : 0u;
     return (1u << 24) + (ubin(uout) << 15) + (ubin(uin) << 6) + (up ? 32u : 0u) + lvl;
 }
+// The keys of the nn nodes of the records nb (none if key is null) and the
+// identity permutation to sort with them; the node count into *nsel if set.
 __global__ void __launch_bounds__(256) k_setup_key(NodeBuf nb, long long nn, unsigned int *key, int *perm,
                                                    unsigned int *nsel) {
     const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -1097,8 +1098,8 @@ __device__ __forceinline__ void fast_block(const LdsTabs &L, const PotView32 &T3
     const int p = pbase + pl;
     const int QS = (NQ + fa.S - 1) / fa.S;
     const int q0 = sl * QS, q1 = q0 + QS < NQ ? q0 + QS : NQ;
-    // the lane's six sums live in its LDS slots of red (registers across the
-    // node loop would spill)
+    // the lane's six sums live in its LDS slots of red: fewer registers held
+    // across the node loop
     double *acc = red + wave * 6 * 64 + lane;
     for (int m = 0; m < 6; m++) acc[m * 64] = 0;
     int nbound = 0, nunb = 0, nbad = 0;
@@ -2706,8 +2707,9 @@ int fast_gl_ensure(fj_ctx *c) {
     return 0;
 }
 
-// Fast node phases up to this many nodes keep node-indexed records (20 B per
-// node, plus the sort's 16 B); larger ones sum per lane (k_fast).
+// Fast node phases up to this many nodes keep node-indexed records (24 B per
+// node with the sort key, plus the sort's 16 B); larger ones sum per lane
+// (k_fast).
 constexpr long long FAST_NODE_RECORDS_MAX = 64ll << 20;
 
 #ifndef FJ_FAST_SORT
@@ -2795,7 +2797,11 @@ int run_fast_nodes(fj_ctx *c, int npts) {
     const TabArgs ta = tab_args(c);
     const size_t lds_pts = tab_bytes(c);
     const size_t lds = fast_lds_bytes(c->nr, c->npoly, c->ngrid);
-    hipEvent_t ev[4];
+    struct Events {   // destroyed on every return
+        hipEvent_t e[4] = {nullptr, nullptr, nullptr, nullptr};
+        ~Events() { for (hipEvent_t x : e) if (x) (void)hipEventDestroy(x); }
+    } evs;
+    hipEvent_t *ev = evs.e;
     for (int i = 0; i < 4; i++) HIPCHK(c, hipEventCreate(&ev[i]));
     HIPCHK(c, hipMemsetAsync(c->d_ctr, 0, sizeof(Counters), c->stream));
     memset(&c->st, 0, sizeof c->st);
@@ -2839,7 +2845,6 @@ int run_fast_nodes(fj_ctx *c, int npts) {
     c->st.nonfinite_nodes = (long long)h.nonfinite;
     c->st.phi_evals = (long long)h.phi + 2 * (long long)npts;
     c->st.setup_lanes = (nn + 255) / 256 * 256;
-    for (int i = 0; i < 4; i++) (void)hipEventDestroy(ev[i]);
     return 0;
 }
 
