@@ -406,6 +406,23 @@ __global__ void __launch_bounds__(256) k_setup_key(NodeBuf nb, long long nn, uns
     key[i] = node_order_key(flag, ub - du, ub + du, vt);
 }
 
+// Sort keys of a task list of the FP32 integrator: the intervals the task's
+// QAGS took in the previous node phase, most first (0xFFFE - count), so that
+// a wave's lanes take tasks of like cost and the costliest start first;
+// entries past the list's length (*cnt) sort last.
+constexpr int TASK_KEY_BITS = 16;
+__global__ void __launch_bounds__(256) k_task_key(const int *list, const unsigned int *cnt, long long n, int type,
+                                                  const int *cost, unsigned int *key) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    unsigned int k = 0xFFFFu;
+    if (i < (long long)*cnt) {
+        const int q = cost[2ll * list[i] + type];
+        k = 0xFFFEu - (unsigned int)(q < 0 ? 0 : q > 0xFFFE ? 0xFFFE : q);
+    }
+    key[i] = k;
+}
+
 // df_hg (DF.cpp:65-76) and the rotation term (DF.cpp:114-119)
 __device__ inline double d_df(const fj_df_params &d, double Jr, double Jphi, double Jz) {
     const double h = Jr + d.dphi_h * fabs(Jphi) + d.dz_h * Jz;
@@ -803,6 +820,7 @@ struct IntegArgs {
     const unsigned int *ntask_dev;   // if set: tasks are ovf_list[0 .. *ntask_dev) (count on the device)
     const PotView *pv;               // if set: task tables pv[pmodel[node / NQ]] (batched overflow pass)
     const int *pmodel;
+    int *cost;                       // FP32 stepper, if set: [2 node + type] the intervals each finished task took
 };
 
 
@@ -869,6 +887,9 @@ __device__ __forceinline__ void integrals_loop(const IntegArgs &ia, FI &f, const
             nqk++;
             qags_consume(q, w, ia.cap, r);
             if (q.state == 3) {
+                if constexpr (std::is_same<R, float>::value) {
+                    if (ia.cost) ia.cost[2 * node + f.type] = (int)q.last;
+                }
                 if (q.overflow) {
                     const unsigned int slot = atomicAdd(ia.ovf_cnt, 1u);
                     ia.ovf_list[slot] = (int)(node * 2 + f.type);
@@ -2359,6 +2380,7 @@ struct fj_ctx {
     unsigned int *d_okey = nullptr; int *d_operm = nullptr; long long order_cap = 0;
     unsigned char *d_otemp = nullptr; size_t otemp_bytes = 0;
     long long rec_nodes = -1;   // nodes of the point-major records the last node phase left (-1: none)
+    int *d_tcost = nullptr;     // [2 node + type] intervals of each FP32 task's QAGS in the last node phase
     // fast node phase, one lane per node: flag, actions and next-pass sort key per node
     int *d_fflag = nullptr; double *d_fJr = nullptr, *d_fJz = nullptr; unsigned int *d_fkey = nullptr;
     long long fnode_cap = 0, fast_keys = -1;   // fast_keys: nodes whose keys the last fast pass wrote
@@ -2520,7 +2542,7 @@ int ensure_nodes(fj_ctx *c, long long nodes) {
         if ((r = dalloc(c, &c->d_nodef, (size_t)nodes * NF_COUNT)) || (r = dalloc(c, &c->d_Jr, (size_t)nodes)) ||
             (r = dalloc(c, &c->d_Jz, (size_t)nodes)) || (r = dalloc(c, &c->d_flag, (size_t)nodes)) ||
             (r = dalloc(c, &c->d_list_u, (size_t)nodes)) || (r = dalloc(c, &c->d_list_v, (size_t)nodes)) ||
-            (r = dalloc(c, &c->d_ovf, (size_t)2 * nodes)))
+            (r = dalloc(c, &c->d_ovf, (size_t)2 * nodes)) || (r = dalloc(c, &c->d_tcost, (size_t)2 * nodes)))
             return r;
         c->node_cap = nodes;
         c->rec_nodes = -1;
@@ -2559,8 +2581,13 @@ void launch_integrals(fj_ctx *c, const TabArgs &ta, const IntegArgs &ia, int lan
 
 // Staeckel actions of the tasks k_setup listed (counters d_cnt[0..1]): the
 // persistent integrator, then the FP64 full-limit pass for overflowed tasks.
+int ensure_order(fj_ctx *c, long long nn);
+
+// order (mixed precision): the task lists sorted by the intervals each task
+// took in the previous node phase, whose records are for these nodes.  (In
+// FP64 the lists keep k_setup's order: sorting them was slower there.)
 int run_integrals(fj_ctx *c, const TabArgs &ta, const NodeBuf &nb, bool mixed, float *t_int, float *t_ovf,
-                  unsigned int *novf_out) {
+                  unsigned int *novf_out, bool order = false) {
     hipEvent_t e0, e1, e2;
     HIPCHK(c, hipEventCreate(&e0));
     HIPCHK(c, hipEventCreate(&e1));
@@ -2568,7 +2595,31 @@ int run_integrals(fj_ctx *c, const TabArgs &ta, const NodeBuf &nb, bool mixed, f
     const int lanes = integ_lanes(c, mixed);
     IntegArgs ia{nb, c->d_list_u, c->d_list_v, c->d_cnt + 0, c->d_cnt + 1, c->d_cnt + 2,
                  c->d_ovf, c->d_cnt + 3, c->d_ws, mixed ? QCAP_MIXED : QCAP, -1, c->d_ctr};
+    if (mixed) ia.cost = c->d_tcost;
     HIPCHK(c, hipEventRecord(e0, c->stream));
+    if (mixed && order) {
+        int r;
+        const long long nn = nb.cap;
+        if ((r = ensure_order(c, nn))) return r;
+        int *sorted[2] = {c->d_operm, c->d_operm + nn};
+        const int *lists[2] = {c->d_list_u, c->d_list_v};
+        for (int type = 0; type < 2; type++) {
+            k_task_key<<<(unsigned)((nn + 255) / 256), 256, 0, c->stream>>>(lists[type], c->d_cnt + type, nn, type,
+                                                                             c->d_tcost, c->d_okey);
+            HIPCHK(c, hipGetLastError());
+            size_t need = 0;
+            HIPCHK(c, hipcub::DeviceRadixSort::SortPairs(nullptr, need, c->d_okey, c->d_okey + nn, lists[type],
+                                                         sorted[type], (int)nn, 0, TASK_KEY_BITS, c->stream));
+            if (need > c->otemp_bytes) {
+                if ((r = dalloc(c, &c->d_otemp, need))) return r;
+                c->otemp_bytes = need;
+            }
+            HIPCHK(c, hipcub::DeviceRadixSort::SortPairs(c->d_otemp, need, c->d_okey, c->d_okey + nn, lists[type],
+                                                         sorted[type], (int)nn, 0, TASK_KEY_BITS, c->stream));
+        }
+        ia.list_u = sorted[0];
+        ia.list_v = sorted[1];
+    }
     launch_integrals(c, ta, ia, lanes, mixed);
     HIPCHK(c, hipGetLastError());
     HIPCHK(c, hipEventRecord(e1, c->stream));
@@ -2956,14 +3007,15 @@ int run_nodes(fj_ctx *c, int npts) {
         }
         HIPCHK(c, hipEventRecord(ev[2], c->stream));
         // one chunk whose records the previous node phase left: lanes in the order of its turning points
-        if (FJ_SETUP_SORT && !sur && np == npts && c->rec_nodes == nn && (r = setup_order(c, nb, nn, &sa)))
-            return r;
+        // (and, in mixed precision, tasks in the order of its interval counts)
+        const bool order = FJ_SETUP_SORT && !sur && np == npts && c->rec_nodes == nn;
+        if (order && (r = setup_order(c, nb, nn, &sa))) return r;
         const long long nsetup = sur ? (long long)nsel : nn;
         if (nsetup > 0) k_setup<<<(unsigned)((nsetup + 255) / 256), 256, lds, c->stream>>>(ta, sa);
         HIPCHK(c, hipGetLastError());
         HIPCHK(c, hipEventRecord(ev[3], c->stream));
         unsigned int novf = 0;
-        if ((r = run_integrals(c, ta, nb, mixed, &t_int, &t_ovf, &novf))) return r;
+        if ((r = run_integrals(c, ta, nb, mixed, &t_int, &t_ovf, &novf, order))) return r;
         HIPCHK(c, hipEventRecord(ev[5], c->stream));
         MomArgs ma{c->d_pPhi, c->d_pVe, c->d_pR, (int)p0, np, c->quad, nb, c->df, c->d_out6,
                    c->d_mom, c->d_mom + npts, c->d_mom + 2 * npts, c->d_mom + 3 * npts, c->d_mom + 4 * npts,
@@ -3190,7 +3242,8 @@ int fj_ctx_destroy(fj_ctx *c) {
                     c->d_rk, c->d_nodef, c->d_Jr, c->d_Jz, c->d_flag, c->d_list_u, c->d_list_v, c->d_ovf, c->d_cnt,
                     c->d_ctr, c->d_ws, c->d_wsfull, c->d_pR, c->d_pz, c->d_pPhi, c->d_pVe, c->d_out6,
                     c->d_mom, c->d_rhl, c->d_gl, c->d_pw, c->d_pws, c->d_status, c->d_eval, c->d_fpart,
-                    c->d_okey, c->d_operm, c->d_otemp, c->d_fflag, c->d_fJr, c->d_fJz, c->d_fkey};
+                    c->d_okey, c->d_operm, c->d_otemp, c->d_fflag, c->d_fJr, c->d_fJz, c->d_fkey,
+                    c->d_tcost};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     void *sp[] = {c->d_pVc, c->d_X, c->d_T, c->d_bt, c->d_bw, c->d_pos, c->d_loss, c->d_sx16, c->d_sy, c->d_judge_J,
